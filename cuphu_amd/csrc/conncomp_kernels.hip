@@ -1,0 +1,188 @@
+// conncomp_kernels.hip -- connected-component labelling of the unwrapped
+// phase (reference: src/cxx/cuphu/cuphu_conncomp.cu, Laplace criterion).
+//
+// A pixel is valid when !(corr < 0.05) and mask != 0 (:102-114, :324-325).
+// Two valid 4-neighbours join when fabsf(unw_a - unw_b) < pi (:178-218; with
+// no MCF solve every arc is confident, :84-93).  The label of a component is
+// decided by its minimum raster index (the invariant of :16-20), components
+// smaller than max(1, (int)(minfrac*npix)) get 0, and the rest are numbered
+// 1..maxncomps in increasing root order (build_label_map :272-286).
+//
+// GPU scheme: (1) union-find inside a 64x16 tile in LDS, (2) lock-free
+// global union (atomicMin, parent <= self) across tile borders, (3) path
+// compression + per-root counts, (4) keep flags, device scan, relabel.  The
+// output is a pure function of the partition, so it is bit-identical to a
+// sequential union-find.
+#include <hipcub/hipcub.hpp>
+
+#include "common.hpp"
+#include "util.hpp"
+
+namespace cuphu {
+
+constexpr int kCcTW = 64, kCcTH = 16, kCcN = kCcTW * kCcTH;
+
+__device__ __forceinline__ bool cc_valid(const float *corr, const unsigned char *mask, long long i, float th) {
+    bool bad = corr[i] < th;
+    if (mask) bad = bad || (mask[i] == 0);
+    return !bad;
+}
+__device__ __forceinline__ bool cc_join(const float *unw, long long a, long long b) {
+    return fabsf(unw[a] - unw[b]) < kPiF;
+}
+
+__device__ __forceinline__ int lds_find(const int *par, int x) {
+    int p = par[x];
+    while (p != x) { x = p; p = par[x]; }
+    return x;
+}
+__device__ void lds_union(int *par, int a, int b) {
+    for (;;) {
+        a = lds_find(par, a);
+        b = lds_find(par, b);
+        if (a == b) return;
+        if (a < b) { int t = a; a = b; b = t; }   // a > b: link a under b
+        int old = atomicMin(&par[a], b);
+        if (old == a) return;
+        a = old;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_cc_local(const float *unw, const float *corr, const unsigned char *mask,
+                                                  int nrow, int ncol, int stride, float th, uint32_t *lab) {
+    __shared__ int par[kCcN];
+    int tx = blockIdx.x, ty = blockIdx.y;
+    int c0 = tx * kCcTW, r0 = ty * kCcTH;
+    for (int l = threadIdx.x; l < kCcN; l += 256) {
+        int r = r0 + l / kCcTW, c = c0 + l % kCcTW;
+        bool v = r < nrow && c < ncol && cc_valid(corr, mask, (long long)r * stride + c, th);
+        par[l] = v ? l : -1;
+    }
+    __syncthreads();
+    for (int l = threadIdx.x; l < kCcN; l += 256) {
+        if (par[l] < 0) continue;
+        int lr = l / kCcTW, lc = l % kCcTW, r = r0 + lr, c = c0 + lc;
+        long long g = (long long)r * stride + c;
+        if (lc + 1 < kCcTW && par[l + 1] >= 0 && cc_join(unw, g, g + 1)) lds_union(par, l, l + 1);
+        if (lr + 1 < kCcTH && par[l + kCcTW] >= 0 && cc_join(unw, g, g + stride)) lds_union(par, l, l + kCcTW);
+    }
+    __syncthreads();
+    for (int l = threadIdx.x; l < kCcN; l += 256) {
+        int r = r0 + l / kCcTW, c = c0 + l % kCcTW;
+        if (r >= nrow || c >= ncol) continue;
+        long long g = (long long)r * stride + c;
+        uint32_t out = kInvalidLabel;
+        if (par[l] >= 0) {
+            int root = lds_find(par, l);
+            out = (uint32_t)((long long)(r0 + root / kCcTW) * stride + (c0 + root % kCcTW));
+        }
+        lab[g] = out;
+    }
+}
+
+__device__ __forceinline__ uint32_t g_load(const uint32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t g_find(uint32_t *lab, uint32_t x) {
+    uint32_t p = g_load(lab + x);
+    while (p != x) { x = p; p = g_load(lab + x); }
+    return x;
+}
+__device__ void g_union(uint32_t *lab, uint32_t a, uint32_t b) {
+    for (;;) {
+        a = g_find(lab, a);
+        b = g_find(lab, b);
+        if (a == b) return;
+        if (a < b) { uint32_t t = a; a = b; b = t; }
+        uint32_t old = atomicMin(lab + a, b);
+        if (old == a) return;
+        a = old;
+    }
+}
+
+// Unions across tile borders: the right edge of a tile's last column and
+// the down edge of its last row.
+__global__ __launch_bounds__(256) void k_cc_border(const float *unw, int nrow, int ncol, int stride, uint32_t *lab) {
+    long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (long long)nrow * ncol) return;
+    int r = (int)(i / ncol), c = (int)(i % ncol);
+    long long g = (long long)r * stride + c;
+    if (g_load(lab + g) == kInvalidLabel) return;
+    if ((c % kCcTW) == kCcTW - 1 && c + 1 < ncol && g_load(lab + g + 1) != kInvalidLabel && cc_join(unw, g, g + 1))
+        g_union(lab, (uint32_t)g, (uint32_t)(g + 1));
+    if ((r % kCcTH) == kCcTH - 1 && r + 1 < nrow && g_load(lab + g + stride) != kInvalidLabel &&
+        cc_join(unw, g, g + stride))
+        g_union(lab, (uint32_t)g, (uint32_t)(g + stride));
+}
+
+__global__ __launch_bounds__(256) void k_cc_compress(int nrow, int ncol, int stride, uint32_t *lab, uint32_t *cnt) {
+    long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (long long)nrow * ncol) return;
+    long long g = (long long)(i / ncol) * stride + (i % ncol);
+    uint32_t l = lab[g];
+    if (l == kInvalidLabel) return;
+    uint32_t root = g_find(lab, l);
+    lab[g] = root;
+    atomicAdd(cnt + root, 1u);
+}
+
+__global__ void k_cc_fill(uint32_t *p, size_t n, uint32_t v) {
+    size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_cc_keep(const uint32_t *cnt, size_t n, uint32_t min_size, uint32_t *flags) {
+    size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) flags[i] = (cnt[i] >= min_size) ? 1u : 0u;
+}
+
+__global__ __launch_bounds__(256) void k_cc_relabel(int nrow, int ncol, int stride, const uint32_t *rank,
+                                                    const uint32_t *flags, long maxncomps, uint32_t *lab) {
+    long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (long long)nrow * ncol) return;
+    long long g = (long long)(i / ncol) * stride + (i % ncol);
+    uint32_t l = lab[g];
+    uint32_t out = 0u;
+    if (l != kInvalidLabel && flags[l] && (long)rank[l] < maxncomps) out = rank[l] + 1u;
+    lab[g] = out;
+}
+
+__global__ void k_cc_count(const uint32_t *rank, const uint32_t *flags, size_t n, long maxncomps, int *out) {
+    uint32_t total = rank[n - 1] + flags[n - 1];
+    *out = (int)((long)total < maxncomps ? (long)total : (maxncomps > 0 ? maxncomps : 0));
+}
+
+int conncomp_run(const float *unw, const float *corr, const unsigned char *mask, int nrow, int ncol, int stride,
+                 double minfrac, long maxncomps, uint32_t *labels, hipStream_t st, float corr_thresh) {
+    if (nrow <= 0 || ncol <= 0) return 0;
+    size_t n = (size_t)nrow * stride;
+    size_t npx = (size_t)nrow * ncol;
+    CUPHU_REQUIRE(n < 0xFFFFFFF0ull, "conncomp: image too large for 32-bit labels");
+    DevBuf<uint32_t> cnt(n), flags(n), rank(n);
+    DevBuf<int> dcount(1);
+    unsigned g = (unsigned)ceil_div(n, 256), gp = (unsigned)ceil_div(npx, 256);
+    dim3 tiles((unsigned)ceil_div((size_t)ncol, kCcTW), (unsigned)ceil_div((size_t)nrow, kCcTH));
+    hipLaunchKernelGGL(k_cc_fill, dim3(g), dim3(256), 0, st, cnt.get(), n, 0u);
+    if (stride != ncol) hipLaunchKernelGGL(k_cc_fill, dim3(g), dim3(256), 0, st, labels, n, kInvalidLabel);
+    hipLaunchKernelGGL(k_cc_local, tiles, dim3(256), 0, st, unw, corr, mask, nrow, ncol, stride, corr_thresh, labels);
+    hipLaunchKernelGGL(k_cc_border, dim3(gp), dim3(256), 0, st, unw, nrow, ncol, stride, labels);
+    hipLaunchKernelGGL(k_cc_compress, dim3(gp), dim3(256), 0, st, nrow, ncol, stride, labels, cnt.get());
+    int min_size = (int)(minfrac * (double)(int)npx);
+    if (min_size < 1) min_size = 1;
+    hipLaunchKernelGGL(k_cc_keep, dim3(g), dim3(256), 0, st, cnt.get(), n, (uint32_t)min_size, flags.get());
+    CUPHU_HIP_CHECK(hipGetLastError());
+    size_t tmp_bytes = 0;
+    CUPHU_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, flags.get(), rank.get(), (int)n, st));
+    DevBuf<unsigned char> tmp(tmp_bytes + 16);
+    CUPHU_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp.get(), tmp_bytes, flags.get(), rank.get(), (int)n, st));
+    hipLaunchKernelGGL(k_cc_relabel, dim3(gp), dim3(256), 0, st, nrow, ncol, stride, rank.get(), flags.get(),
+                       maxncomps, labels);
+    hipLaunchKernelGGL(k_cc_count, dim3(1), dim3(1), 0, st, rank.get(), flags.get(), n, maxncomps, dcount.get());
+    CUPHU_HIP_CHECK(hipGetLastError());
+    int k = 0;
+    CUPHU_HIP_CHECK(hipMemcpyAsync(&k, dcount.get(), sizeof(int), hipMemcpyDeviceToHost, st));
+    CUPHU_HIP_CHECK(hipStreamSynchronize(st));
+    return k;
+}
+
+}  // namespace cuphu

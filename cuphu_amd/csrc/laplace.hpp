@@ -1,0 +1,90 @@
+// laplace.hpp -- host-side declarations of the Laplace PCG/IRLS machinery.
+#pragma once
+#include "common.hpp"
+
+namespace cuphu {
+
+// Stencil tile geometry: a workgroup of 4 waves covers TW columns (64 lanes x
+// 4 pixels) by 4*RPW rows; each wave owns RPW consecutive rows.
+constexpr int kTileW = 256;
+constexpr int kLdsStride = 264;   // LDS row stride (elements); tile col j -> lds col 4+j
+constexpr int kLdsC0 = 4;
+constexpr int kBlock = 256;
+
+// One instance (a tile or an interferogram) inside the pooled planes.
+struct InstDesc {
+    int nrow, ncol, pitch, tiles_x;
+    int tiles_y, block_begin, row_begin, pad1;
+    long long off;   // element offset of pixel (0,0) in every pixel plane
+};
+
+// Per-instance solver scalars, device resident.
+struct SolveScal {
+    double gamma, delta, gamma_prev, alpha, alpha_prev, beta, gamma0, relres;
+    int iters, active, status, pad;
+};
+
+enum RedMode { RED_START = 0, RED_STEP = 1, RED_RR = 2 };
+
+// Instance source description for ingest (pointers already offset to the
+// instance's (0,0); src_stride = elements per source row).
+struct IngestSrc {
+    const float2 *igram;     // complex64 or NULL
+    const float *phase;      // used when igram is NULL
+    const float *corr;
+    const unsigned char *mask;
+    const float *u0;         // warm start or NULL
+    int src_stride, pad;
+    CostConsts cc;
+};
+
+// Pooled per-group planes.  T = float (FP32 mode) or double (FP64 mode).
+template <class T> struct Planes {
+    float *phi = nullptr;           // wrapped phase (float in both modes)
+    float *corr = nullptr;          // pitched coherence copy
+    unsigned char *mask = nullptr;  // pitched mask copy (1 = valid)
+    short2 *sigsq = nullptr;        // {down, right} arc sigsq
+    T *w2 = nullptr;                // {down, right} arc weights interleaved (2 T per pixel)
+    T *b = nullptr, *dinv = nullptr, *x = nullptr, *r = nullptr, *p = nullptr, *s = nullptr, *wv = nullptr;
+};
+
+struct KernelCfg {
+    int rpw = 4;   // rows per wave (autotuned: 2, 4 or 8)
+};
+
+// ── launchers (laplace_kernels.hip) ─────────────────────────────────────────
+template <class T>
+void launch_ingest(const IngestSrc *d_src, const InstDesc *d_desc, const int *d_rowinst, int nrows_total,
+                   const Planes<T> &pl, int has_u0, int max_pitch, hipStream_t st);
+template <class T>
+void launch_weights(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, int reweight,
+                    T delta2, int rpw, hipStream_t st);
+template <class T>
+void launch_assemble(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, int rpw, hipStream_t st);
+template <class T>
+void launch_start(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, const SolveScal *scal,
+                  int force, double2 *partials, int rpw, hipStream_t st);
+template <class T>
+void launch_cg(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, const SolveScal *scal,
+               double2 *partials, int rpw, hipStream_t st);
+template <class T>
+void launch_rr(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, const SolveScal *scal,
+               int rpw, hipStream_t st);
+void launch_reduce(const InstDesc *d_desc, int ninst, const double2 *partials, SolveScal *scal, int mode,
+                   int max_iter, double tol2, hipStream_t st);
+template <class T>
+void launch_final(const InstDesc *d_desc, const int *d_rowinst, int nrows_total, const Planes<T> &pl,
+                  float *out_pool, int max_pitch, hipStream_t st);
+
+// Plain (reference launch sequence) kernels on one instance.
+template <class T>
+void launch_plain_matvec(const InstDesc &d, const T *w2, const T *x, T *y, hipStream_t st);
+template <class T> void launch_plain_axpy(T *y, const T *x, T alpha, size_t n, hipStream_t st);
+template <class T> void launch_plain_vmul(T *z, const T *r, const T *dinv, size_t n, hipStream_t st);
+template <class T> void launch_plain_pz(T *p, T *z, const T *r, const T *dinv, size_t n, hipStream_t st);
+template <class T> void launch_plain_axpby(T *p, const T *z, T beta, size_t n, hipStream_t st);
+template <class T> void launch_plain_dot(const T *a, const T *b, size_t n, double *partials, double *out,
+                                         hipStream_t st);
+constexpr int kDotParts = 1024;
+
+}  // namespace cuphu

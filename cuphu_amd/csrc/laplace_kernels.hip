@@ -1,0 +1,958 @@
+// laplace_kernels.hip -- gfx950 kernels of the weighted-Laplacian PCG/IRLS
+// solve (reference: src/cxx/cuphu/cuphu_laplace.cu).
+//
+// Layout.  Every instance (a tile or an interferogram) lives in pooled
+// planes with a row pitch that is a multiple of 64 elements; the pad
+// columns and every missing arc hold zero, so a pad pixel is an isolated
+// node with r = u = 0 that drops out of every dot product.  Arc weights are
+// stored per pixel as an interleaved {down, right} pair (the pixel's row arc
+// (r,c)->(r+1,c) and column arc (r,c)->(r,c+1)), so one 32-byte load gives
+// the two outgoing arcs of four pixels.
+//
+// Tiles.  A 256-thread workgroup (4 waves of 64) owns a 256-column x
+// 4*RPW-row tile; each lane owns 4 consecutive columns of RPW rows and
+// moves them with 16-byte loads.  Stencil kernels stage the stencil operand
+// of the tile plus a one-pixel halo in LDS, then apply the 5-point weighted
+// Laplacian from LDS.  Workgroups are renumbered so each XCD receives a
+// contiguous run of tiles (vertically adjacent tiles share halo rows in the
+// same L2).
+//
+// Arithmetic is FP32 (or FP64 in the double instantiation) with -ffp-contract=off;
+// every fused multiply-add is written explicitly.  Inner products are
+// accumulated in FP64 per lane, reduced by a fixed xor-butterfly per wave,
+// in wave order per workgroup, and in block order per instance: the result
+// is deterministic run to run.
+#include "laplace.hpp"
+
+namespace cuphu {
+
+// ── small vector helpers ────────────────────────────────────────────────────
+__device__ __forceinline__ void ld4(const float *p, float (&o)[4]) {
+    float4 t = *reinterpret_cast<const float4 *>(p);
+    o[0] = t.x; o[1] = t.y; o[2] = t.z; o[3] = t.w;
+}
+__device__ __forceinline__ void ld4(const double *p, double (&o)[4]) {
+    double2 a = *reinterpret_cast<const double2 *>(p);
+    double2 b = *reinterpret_cast<const double2 *>(p + 2);
+    o[0] = a.x; o[1] = a.y; o[2] = b.x; o[3] = b.y;
+}
+__device__ __forceinline__ void st4(float *p, const float (&v)[4]) {
+    *reinterpret_cast<float4 *>(p) = make_float4(v[0], v[1], v[2], v[3]);
+}
+__device__ __forceinline__ void st4(double *p, const double (&v)[4]) {
+    *reinterpret_cast<double2 *>(p) = make_double2(v[0], v[1]);
+    *reinterpret_cast<double2 *>(p + 2) = make_double2(v[2], v[3]);
+}
+template <class T> __device__ __forceinline__ void zero4(T (&v)[4]) { v[0] = v[1] = v[2] = v[3] = T(0); }
+
+// Arcs of 4 pixels starting at element e: w2 holds {down, right} pairs.
+__device__ __forceinline__ void ld_arcs(const float *w2, long long e, float (&wd)[4], float (&wr)[4]) {
+    float4 a = *reinterpret_cast<const float4 *>(w2 + 2 * e);
+    float4 b = *reinterpret_cast<const float4 *>(w2 + 2 * e + 4);
+    wd[0] = a.x; wr[0] = a.y; wd[1] = a.z; wr[1] = a.w;
+    wd[2] = b.x; wr[2] = b.y; wd[3] = b.z; wr[3] = b.w;
+}
+__device__ __forceinline__ void ld_arcs(const double *w2, long long e, double (&wd)[4], double (&wr)[4]) {
+    const double2 *q = reinterpret_cast<const double2 *>(w2 + 2 * e);
+    double2 a = q[0], b = q[1], c = q[2], d = q[3];
+    wd[0] = a.x; wr[0] = a.y; wd[1] = b.x; wr[1] = b.y;
+    wd[2] = c.x; wr[2] = c.y; wd[3] = d.x; wr[3] = d.y;
+}
+__device__ __forceinline__ void st_arcs(float *w2, long long e, const float (&wd)[4], const float (&wr)[4]) {
+    float4 *q = reinterpret_cast<float4 *>(w2 + 2 * e);
+    q[0] = make_float4(wd[0], wr[0], wd[1], wr[1]);
+    q[1] = make_float4(wd[2], wr[2], wd[3], wr[3]);
+}
+__device__ __forceinline__ void st_arcs(double *w2, long long e, const double (&wd)[4], const double (&wr)[4]) {
+    double2 *q = reinterpret_cast<double2 *>(w2 + 2 * e);
+    q[0] = make_double2(wd[0], wr[0]); q[1] = make_double2(wd[1], wr[1]);
+    q[2] = make_double2(wd[2], wr[2]); q[3] = make_double2(wd[3], wr[3]);
+}
+
+// Hardware dispatches workgroup b to XCD b % 8.  Give each XCD a contiguous
+// range of logical tiles instead.
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+    const int X = 8;
+    int q = nb / X, r = nb % X, x = b % X, i = b / X;
+    return x * q + (x < r ? x : r) + i;
+}
+
+struct TileInfo {
+    InstDesc d;
+    int inst, blk, tx0, ty0;
+};
+__device__ __forceinline__ TileInfo get_tile(const InstDesc *descs, const int *blk_inst, int th) {
+    TileInfo t;
+    t.blk = xcd_block((int)blockIdx.x, (int)gridDim.x);
+    t.inst = blk_inst[t.blk];
+    t.d = descs[t.inst];
+    int local = t.blk - t.d.block_begin;
+    t.tx0 = (local % t.d.tiles_x) * kTileW;
+    t.ty0 = (local / t.d.tiles_x) * th;
+    return t;
+}
+
+// Halo fill: tile rows -1 and TH (waves 0 and 3, 4 pixels per lane) and
+// tile columns -1 and TW for rows -1..TH (waves 1 and 2, one pixel per
+// lane).  put4(lr, lc, e, ok) / put1(lr, lc, e, ok) store the operand
+// values of pixel element e at LDS (row lr, col lc); ok = false stores 0.
+template <int TH, class P4, class P1>
+__device__ __forceinline__ void fill_halo(const TileInfo &t, int wave, int lane, P4 &&put4, P1 &&put1) {
+    const InstDesc &d = t.d;
+    if (wave == 0 || wave == 3) {
+        int lr = (wave == 0) ? 0 : TH + 1;
+        int row = t.ty0 + lr - 1;
+        int c = t.tx0 + 4 * lane;
+        bool ok = row >= 0 && row < d.nrow && c < d.pitch;
+        put4(lr, kLdsC0 + 4 * lane, d.off + (long long)row * d.pitch + c, ok);
+    } else if (lane < TH + 2) {
+        int lr = lane, row = t.ty0 + lr - 1;
+        int c = (wave == 1) ? t.tx0 - 1 : t.tx0 + kTileW;
+        int lc = (wave == 1) ? kLdsC0 - 1 : kLdsC0 + kTileW;
+        bool ok = row >= 0 && row < d.nrow && c >= 0 && c < d.pitch;
+        put1(lr, lc, d.off + (long long)row * d.pitch + c, ok);
+    }
+}
+
+// Weighted 5-point Laplacian of 4 pixels from an LDS tile (laplace_matvec
+// cuphu_laplace.cu:156-190): y = eps*a, then fma over down, up, right, left.
+template <class T>
+__device__ __forceinline__ void stencil4(const T *lds, int lr, int lane, const T (&wd)[4], const T (&wu)[4],
+                                         const T (&wr)[4], const T (&wl)[4], T (&y)[4]) {
+    const T *rc = lds + (lr + 1) * kLdsStride + kLdsC0 + 4 * lane;
+    T c4[4], u4[4], d4[4];
+    ld4(rc, c4);
+    ld4(rc - kLdsStride, u4);
+    ld4(rc + kLdsStride, d4);
+    T left = rc[-1], right = rc[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        T a = c4[j];
+        T xl = (j == 0) ? left : c4[j - 1];
+        T xr = (j == 3) ? right : c4[j + 1];
+        T yv = Prec<T>::eps * a;
+        yv = Prec<T>::fmaT(wd[j], a - d4[j], yv);
+        yv = Prec<T>::fmaT(wu[j], a - u4[j], yv);
+        yv = Prec<T>::fmaT(wr[j], a - xr, yv);
+        yv = Prec<T>::fmaT(wl[j], a - xl, yv);
+        y[j] = yv;
+    }
+}
+
+// Arcs of a lane's 4 pixels in one row: own down/right arcs and the left
+// arc (the right arc of the previous column: the neighbouring lane's, or a
+// load for lane 0).  Executed by every lane (contains a shuffle).
+template <class T>
+__device__ __forceinline__ void row_arcs(const T *w2, const TileInfo &t, int row, bool cok, int lane,
+                                         T (&wd)[4], T (&wr)[4], T (&wl)[4]) {
+    long long e = t.d.off + (long long)row * t.d.pitch + t.tx0 + 4 * lane;
+    if (cok) ld_arcs(w2, e, wd, wr);
+    else { zero4(wd); zero4(wr); }
+    T from_left = __shfl_up(wr[3], 1, 64);
+    if (lane == 0) from_left = (t.tx0 > 0) ? w2[2 * (e - 1) + 1] : T(0);
+    wl[0] = from_left; wl[1] = wr[0]; wl[2] = wr[1]; wl[3] = wr[2];
+}
+
+// Down arcs of the row above (the up arcs of this row); zero on row 0.
+template <class T>
+__device__ __forceinline__ void up_arcs(const T *w2, const TileInfo &t, int row, bool cok, int lane, T (&wu)[4]) {
+    if (row > 0 && cok) {
+        T dummy[4];
+        ld_arcs(w2, t.d.off + (long long)(row - 1) * t.d.pitch + t.tx0 + 4 * lane, wu, dummy);
+    } else {
+        zero4(wu);
+    }
+}
+
+// Deterministic workgroup sum of two doubles; thread 0 writes *out.
+__device__ __forceinline__ void block_sum2(double a, double b, double2 *out) {
+    __shared__ double sa[4], sb[4];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_xor(a, o, 64);
+        b += __shfl_xor(b, o, 64);
+    }
+    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) { sa[wave] = a; sb[wave] = b; }
+    __syncthreads();
+    if (threadIdx.x == 0) *out = make_double2(((sa[0] + sa[1]) + sa[2]) + sa[3], ((sb[0] + sb[1]) + sb[2]) + sb[3]);
+}
+
+// ═══════════════════════════════════════════════════════════════════════════
+// Ingest: complex64 (or wrapped phase) + coherence + mask -> pitched phi,
+// corr, mask, {down, right} sigsq, initial x.  One block row per image row.
+// phi: atan2 evaluated in FP64 and rounded once, +2pi if negative, then
+// wrapped to [0, 2pi) (cuphu_solver.cpp:1707-1712, cuphu_util.cu:33-46).
+// sigsq: the SNAPHU sigsq of cuphu_cost.cu:105-118 (the only cost field the
+// Laplace weights use), mask -> arc weight as cuphu_solver.cpp:410-426.
+// ═══════════════════════════════════════════════════════════════════════════
+__device__ __forceinline__ short sigsq_arc(const CostConsts &k, float ca, float cb, int w) {
+    if (w == 0) return (short)kLargeShort;
+    double rho = 0.5 * ((double)ca + (double)cb);
+    if (!(rho >= k.thresh)) rho = 0.0;
+    double t = k.sigsqrhoconst * pow(1.0 - rho, k.rhopow);
+    double sigsqrho = (t + k.sigsqcorr) * k.nsc2;
+    double f = sigsqrho / (k.costscale * (double)w);
+    if (f > 32767.0) f = 32767.0;
+    int si = (int)f;
+    if (si < k.sqmin) si = k.sqmin;
+    if (si > 32767) si = 32767;
+    return (short)si;
+}
+
+
+template <class T>
+__global__ __launch_bounds__(256) void k_ingest(const IngestSrc *srcs, const InstDesc *descs, const int *rowinst,
+                                                Planes<T> pl, int has_u0) {
+    int grow = blockIdx.y;
+    int inst = rowinst[grow];
+    const InstDesc d = descs[inst];
+    const IngestSrc &s = srcs[inst];
+    int row = grow - d.row_begin;
+    int c0 = (blockIdx.x * 256 + threadIdx.x) * 4;
+    if (c0 >= d.pitch) return;
+    long long e = d.off + (long long)row * d.pitch + c0;
+    float ph[4], co[4];
+    unsigned char mk[4];
+    short sd[4], sr[4];
+    T x[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        int c = c0 + j;
+        ph[j] = 0.0f; co[j] = 0.0f; mk[j] = 0; sd[j] = 0; sr[j] = 0; x[j] = T(0);
+        if (c < d.ncol) {
+            long long se = (long long)row * s.src_stride + c;
+            if (s.igram) { float2 z = s.igram[se]; ph[j] = phase_of(z.x, z.y); }
+            else ph[j] = s.phase[se];
+            co[j] = s.corr[se];
+            int m = s.mask ? (s.mask[se] != 0) : 1;
+            mk[j] = (unsigned char)m;
+            if (row < d.nrow - 1) {
+                int w = s.mask ? (m && s.mask[se + s.src_stride] != 0) : 1;
+                sd[j] = sigsq_arc(s.cc, co[j], s.corr[se + s.src_stride], w);
+            }
+            if (c < d.ncol - 1) {
+                int w = s.mask ? (m && s.mask[se + 1] != 0) : 1;
+                sr[j] = sigsq_arc(s.cc, co[j], s.corr[se + 1], w);
+            }
+            if (has_u0 && s.u0) x[j] = (T)s.u0[se];
+        }
+    }
+    *reinterpret_cast<float4 *>(pl.phi + e) = make_float4(ph[0], ph[1], ph[2], ph[3]);
+    *reinterpret_cast<float4 *>(pl.corr + e) = make_float4(co[0], co[1], co[2], co[3]);
+    *reinterpret_cast<uchar4 *>(pl.mask + e) = make_uchar4(mk[0], mk[1], mk[2], mk[3]);
+    short2 *sq = pl.sigsq + e;
+    *reinterpret_cast<int4 *>(sq) = make_int4((int)((unsigned short)sd[0] | ((unsigned)(unsigned short)sr[0] << 16)),
+                                              (int)((unsigned short)sd[1] | ((unsigned)(unsigned short)sr[1] << 16)),
+                                              (int)((unsigned short)sd[2] | ((unsigned)(unsigned short)sr[2] << 16)),
+                                              (int)((unsigned short)sd[3] | ((unsigned)(unsigned short)sr[3] << 16)));
+    st4(pl.x + e, x);
+}
+
+// ═══════════════════════════════════════════════════════════════════════════
+// Arc weights: initial w0 = 1/sigsq (laplace_init_weights :198-205) or the
+// IRLS reweight w0 / sqrt(e^2 + delta^2) (laplace_reweight :214-247) for the
+// two outgoing arcs of each pixel.
+// ═══════════════════════════════════════════════════════════════════════════
+template <class T>
+__device__ __forceinline__ T reweight1(short s, float ph, float pt, T uh, T ut, T d2) {
+    T w0 = Prec<T>::weight(s);
+    if (w0 <= T(0)) return T(0);
+    T g = Prec<T>::wrap((T)ph - (T)pt);
+    T e = (uh - ut) - g;
+    return w0 / Prec<T>::sqrtT(Prec<T>::fmaT(e, e, d2));
+}
+
+__device__ __forceinline__ void ld_sigsq4(const short2 *p, short (&sd)[4], short (&sr)[4]) {
+    int4 v = *reinterpret_cast<const int4 *>(p);
+    int a[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) { sd[j] = (short)(a[j] & 0xFFFF); sr[j] = (short)((unsigned)a[j] >> 16); }
+}
+__device__ __forceinline__ void ld4f(const float *p, float (&o)[4]) { ld4(p, o); }
+
+template <class T, int RPW>
+__global__ __launch_bounds__(256) void k_weights(const InstDesc *descs, const int *blk_inst, Planes<T> pl,
+                                                 int reweight, T d2) {
+    constexpr int TH = 4 * RPW;
+    TileInfo t = get_tile(descs, blk_inst, TH);
+    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const InstDesc &d = t.d;
+    int c = t.tx0 + 4 * lane;
+    bool cok = c < d.pitch;
+    for (int k = 0; k < RPW; ++k) {
+        int row = t.ty0 + wave * RPW + k;
+        if (row >= d.nrow) break;
+        long long e = d.off + (long long)row * d.pitch + c;
+        short sd[4] = {0, 0, 0, 0}, sr[4] = {0, 0, 0, 0};
+        T wd[4], wr[4];
+        if (!reweight) {
+            if (cok) ld_sigsq4(pl.sigsq + e, sd, sr);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) { wd[j] = Prec<T>::weight(sd[j]); wr[j] = Prec<T>::weight(sr[j]); }
+        } else {
+            float pa[4] = {0, 0, 0, 0}, pd[4] = {0, 0, 0, 0};
+            T xa[4], xd[4];
+            zero4(xa); zero4(xd);
+            if (cok) {
+                ld_sigsq4(pl.sigsq + e, sd, sr);
+                ld4f(pl.phi + e, pa);
+                ld4(pl.x + e, xa);
+                if (row + 1 < d.nrow) { ld4f(pl.phi + e + d.pitch, pd); ld4(pl.x + e + d.pitch, xd); }
+            }
+            float pr = __shfl_down(pa[0], 1, 64);
+            T xr = __shfl_down(xa[0], 1, 64);
+            if (lane == 63) {
+                bool ok = cok && (c + 4 < d.pitch);
+                pr = ok ? pl.phi[e + 4] : 0.0f;
+                xr = ok ? pl.x[e + 4] : T(0);
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                wd[j] = reweight1<T>(sd[j], pd[j], pa[j], xd[j], xa[j], d2);
+                float phr = (j < 3) ? pa[j + 1] : pr;
+                T xrr = (j < 3) ? xa[j + 1] : xr;
+                wr[j] = reweight1<T>(sr[j], phr, pa[j], xrr, xa[j], d2);
+            }
+        }
+        if (cok) st_arcs(pl.w2, e, wd, wr);
+    }
+}
+
+// ═══════════════════════════════════════════════════════════════════════════
+// Assemble (laplace_assemble :88-148) fused with the initial residual
+// r = b - L x (:431-434) and the Jacobi inverse 1/diag; zeroes p and s.
+// ═══════════════════════════════════════════════════════════════════════════
+template <class T, int RPW>
+__global__ __launch_bounds__(256) void k_assemble(const InstDesc *descs, const int *blk_inst, Planes<T> pl) {
+    constexpr int TH = 4 * RPW;
+    __shared__ float lph[(TH + 2) * kLdsStride];
+    __shared__ T lx[(TH + 2) * kLdsStride];
+    TileInfo t = get_tile(descs, blk_inst, TH);
+    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const InstDesc &d = t.d;
+    int c = t.tx0 + 4 * lane;
+    bool cok = c < d.pitch;
+    for (int k = 0; k < RPW; ++k) {
+        int lr = wave * RPW + k, row = t.ty0 + lr;
+        float ph[4] = {0, 0, 0, 0};
+        T xv[4];
+        zero4(xv);
+        if (row < d.nrow && cok) {
+            long long e = d.off + (long long)row * d.pitch + c;
+            ld4f(pl.phi + e, ph);
+            ld4(pl.x + e, xv);
+        }
+        float *lp = lph + (lr + 1) * kLdsStride + kLdsC0 + 4 * lane;
+        T *lq = lx + (lr + 1) * kLdsStride + kLdsC0 + 4 * lane;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) { lp[j] = ph[j]; lq[j] = xv[j]; }
+    }
+    fill_halo<TH>(
+        t, wave, lane,
+        [&](int lr, int lc, long long e, bool ok) {
+            float ph[4] = {0, 0, 0, 0};
+            T xv[4];
+            zero4(xv);
+            if (ok) { ld4f(pl.phi + e, ph); ld4(pl.x + e, xv); }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) { lph[lr * kLdsStride + lc + j] = ph[j]; lx[lr * kLdsStride + lc + j] = xv[j]; }
+        },
+        [&](int lr, int lc, long long e, bool ok) {
+            lph[lr * kLdsStride + lc] = ok ? pl.phi[e] : 0.0f;
+            lx[lr * kLdsStride + lc] = ok ? pl.x[e] : T(0);
+        });
+    __syncthreads();
+    T wu[4];
+    for (int k = 0; k < RPW; ++k) {
+        int lr = wave * RPW + k, row = t.ty0 + lr;
+        if (row >= d.nrow) break;
+        if (k == 0) up_arcs(pl.w2, t, row, cok, lane, wu);
+        T wd[4], wr[4], wl[4];
+        row_arcs(pl.w2, t, row, cok, lane, wd, wr, wl);
+        const float *pc = lph + (lr + 1) * kLdsStride + kLdsC0 + 4 * lane;
+        T b[4], di[4], y[4], rr[4], z4[4];
+        stencil4(lx, lr, lane, wd, wu, wr, wl, y);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            T pa = (T)pc[j];
+            T bv = T(0), ws = T(0);
+            if (wd[j] > T(0)) { bv = Prec<T>::fmaT(-wd[j], Prec<T>::wrap((T)pc[j + kLdsStride] - pa), bv); ws += wd[j]; }
+            if (wu[j] > T(0)) { bv = Prec<T>::fmaT(wu[j], Prec<T>::wrap(pa - (T)pc[j - kLdsStride]), bv); ws += wu[j]; }
+            if (wr[j] > T(0)) { bv = Prec<T>::fmaT(-wr[j], Prec<T>::wrap((T)pc[j + 1] - pa), bv); ws += wr[j]; }
+            if (wl[j] > T(0)) { bv = Prec<T>::fmaT(wl[j], Prec<T>::wrap(pa - (T)pc[j - 1]), bv); ws += wl[j]; }
+            b[j] = bv;
+            di[j] = T(1) / (ws + Prec<T>::eps);
+            rr[j] = bv - y[j];
+            z4[j] = T(0);
+        }
+        if (cok) {
+            long long e = d.off + (long long)row * d.pitch + c;
+            st4(pl.b + e, b);
+            st4(pl.dinv + e, di);
+            st4(pl.r + e, rr);
+            st4(pl.p + e, z4);
+            st4(pl.s + e, z4);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) wu[j] = wd[j];
+    }
+}
+
+// ═══════════════════════════════════════════════════════════════════════════
+// Start of a solve (and second half of a true-residual replacement):
+// u = r * dinv, wv = L u, partial sums of r.u and wv.u.
+// ═══════════════════════════════════════════════════════════════════════════
+template <class T, int RPW>
+__global__ __launch_bounds__(256) void k_start(const InstDesc *descs, const int *blk_inst, Planes<T> pl,
+                                               const SolveScal *scal, int force, double2 *partials) {
+    constexpr int TH = 4 * RPW;
+    __shared__ T lu[(TH + 2) * kLdsStride];
+    TileInfo t = get_tile(descs, blk_inst, TH);
+    if (!force && !scal[t.inst].active) return;
+    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const InstDesc &d = t.d;
+    int c = t.tx0 + 4 * lane;
+    bool cok = c < d.pitch;
+    T rr[RPW][4];
+    for (int k = 0; k < RPW; ++k) {
+        int lr = wave * RPW + k, row = t.ty0 + lr;
+        T u[4];
+        zero4(u);
+        zero4(rr[k]);
+        if (row < d.nrow && cok) {
+            long long e = d.off + (long long)row * d.pitch + c;
+            T di[4];
+            ld4(pl.r + e, rr[k]);
+            ld4(pl.dinv + e, di);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) u[j] = rr[k][j] * di[j];
+        }
+        T *lq = lu + (lr + 1) * kLdsStride + kLdsC0 + 4 * lane;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) lq[j] = u[j];
+    }
+    fill_halo<TH>(
+        t, wave, lane,
+        [&](int lr, int lc, long long e, bool ok) {
+            T u[4];
+            zero4(u);
+            if (ok) {
+                T r4[4], di[4];
+                ld4(pl.r + e, r4);
+                ld4(pl.dinv + e, di);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) u[j] = r4[j] * di[j];
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) lu[lr * kLdsStride + lc + j] = u[j];
+        },
+        [&](int lr, int lc, long long e, bool ok) { lu[lr * kLdsStride + lc] = ok ? pl.r[e] * pl.dinv[e] : T(0); });
+    __syncthreads();
+    double g = 0.0, dl = 0.0;
+    T wu[4];
+    for (int k = 0; k < RPW; ++k) {
+        int lr = wave * RPW + k, row = t.ty0 + lr;
+        if (row >= d.nrow) break;
+        if (k == 0) up_arcs(pl.w2, t, row, cok, lane, wu);
+        T wd[4], wr[4], wl[4], y[4];
+        row_arcs(pl.w2, t, row, cok, lane, wd, wr, wl);
+        stencil4(lu, lr, lane, wd, wu, wr, wl, y);
+        if (cok) {
+            long long e = d.off + (long long)row * d.pitch + c;
+            st4(pl.wv + e, y);
+            const T *uc = lu + (lr + 1) * kLdsStride + kLdsC0 + 4 * lane;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                g += (double)rr[k][j] * (double)uc[j];
+                dl += (double)y[j] * (double)uc[j];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) wu[j] = wd[j];
+    }
+    block_sum2(g, dl, partials + t.blk);
+}
+
+// ═══════════════════════════════════════════════════════════════════════════
+// One fused PCG iteration in the single-reduction (Chronopoulos-Gear) form.
+// With alpha_i, beta_i from the previous reduction:
+//   p = u + beta p;  s = wv + beta s;  x += alpha p;  r -= alpha s;
+//   u' = r * dinv;   wv = L u';  partial sums of r.u' and wv.u'.
+// u = r * dinv is recomputed, never stored.  The halo values of u' are
+// recomputed from r, s, wv, dinv of the neighbouring pixels with the same
+// operations, so they equal the neighbours' own values bit for bit.
+// ═══════════════════════════════════════════════════════════════════════════
+template <class T, int RPW>
+__global__ __launch_bounds__(256) void k_cg(const InstDesc *descs, const int *blk_inst, Planes<T> pl,
+                                            const SolveScal *scal, double2 *partials) {
+    constexpr int TH = 4 * RPW;
+    __shared__ T lu[(TH + 2) * kLdsStride];
+    TileInfo t = get_tile(descs, blk_inst, TH);
+    const SolveScal *sc = scal + t.inst;
+    if (!sc->active) return;
+    const T al = (T)sc->alpha, be = (T)sc->beta;
+    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const InstDesc &d = t.d;
+    int c = t.tx0 + 4 * lane;
+    bool cok = c < d.pitch;
+    T rr[RPW][4];
+    for (int k = 0; k < RPW; ++k) {
+        int lr = wave * RPW + k, row = t.ty0 + lr;
+        T un[4];
+        zero4(un);
+        zero4(rr[k]);
+        if (row < d.nrow && cok) {
+            long long e = d.off + (long long)row * d.pitch + c;
+            T r4[4], w4[4], s4[4], p4[4], x4[4], di[4];
+            ld4(pl.r + e, r4);
+            ld4(pl.wv + e, w4);
+            ld4(pl.s + e, s4);
+            ld4(pl.p + e, p4);
+            ld4(pl.x + e, x4);
+            ld4(pl.dinv + e, di);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                T uo = r4[j] * di[j];
+                T pn = Prec<T>::fmaT(be, p4[j], uo);
+                T sn = Prec<T>::fmaT(be, s4[j], w4[j]);
+                x4[j] = Prec<T>::fmaT(al, pn, x4[j]);
+                r4[j] = Prec<T>::fmaT(-al, sn, r4[j]);
+                un[j] = r4[j] * di[j];
+                p4[j] = pn;
+                s4[j] = sn;
+                rr[k][j] = r4[j];
+            }
+            st4(pl.p + e, p4);
+            st4(pl.s + e, s4);
+            st4(pl.x + e, x4);
+            st4(pl.r + e, r4);
+        }
+        T *lq = lu + (lr + 1) * kLdsStride + kLdsC0 + 4 * lane;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) lq[j] = un[j];
+    }
+    fill_halo<TH>(
+        t, wave, lane,
+        [&](int lr, int lc, long long e, bool ok) {
+            T un[4];
+            zero4(un);
+            if (ok) {
+                T r4[4], w4[4], s4[4], di[4];
+                ld4(pl.r + e, r4);
+                ld4(pl.wv + e, w4);
+                ld4(pl.s + e, s4);
+                ld4(pl.dinv + e, di);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    T sn = Prec<T>::fmaT(be, s4[j], w4[j]);
+                    un[j] = Prec<T>::fmaT(-al, sn, r4[j]) * di[j];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) lu[lr * kLdsStride + lc + j] = un[j];
+        },
+        [&](int lr, int lc, long long e, bool ok) {
+            T v = T(0);
+            if (ok) {
+                T sn = Prec<T>::fmaT(be, pl.s[e], pl.wv[e]);
+                v = Prec<T>::fmaT(-al, sn, pl.r[e]) * pl.dinv[e];
+            }
+            lu[lr * kLdsStride + lc] = v;
+        });
+    __syncthreads();
+    double g = 0.0, dl = 0.0;
+    T wu[4];
+    for (int k = 0; k < RPW; ++k) {
+        int lr = wave * RPW + k, row = t.ty0 + lr;
+        if (row >= d.nrow) break;
+        if (k == 0) up_arcs(pl.w2, t, row, cok, lane, wu);
+        T wd[4], wr[4], wl[4], y[4];
+        row_arcs(pl.w2, t, row, cok, lane, wd, wr, wl);
+        stencil4(lu, lr, lane, wd, wu, wr, wl, y);
+        if (cok) {
+            long long e = d.off + (long long)row * d.pitch + c;
+            st4(pl.wv + e, y);
+            const T *uc = lu + (lr + 1) * kLdsStride + kLdsC0 + 4 * lane;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                g += (double)rr[k][j] * (double)uc[j];
+                dl += (double)y[j] * (double)uc[j];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) wu[j] = wd[j];
+    }
+    block_sum2(g, dl, partials + t.blk);
+}
+
+// ═══════════════════════════════════════════════════════════════════════════
+// True-residual replacement, first half: r = b - L x and s = L p.
+// ═══════════════════════════════════════════════════════════════════════════
+template <class T, int RPW>
+__global__ __launch_bounds__(256) void k_rr(const InstDesc *descs, const int *blk_inst, Planes<T> pl,
+                                            const SolveScal *scal) {
+    constexpr int TH = 4 * RPW;
+    __shared__ T lx[(TH + 2) * kLdsStride];
+    __shared__ T lp[(TH + 2) * kLdsStride];
+    TileInfo t = get_tile(descs, blk_inst, TH);
+    if (!scal[t.inst].active) return;
+    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const InstDesc &d = t.d;
+    int c = t.tx0 + 4 * lane;
+    bool cok = c < d.pitch;
+    for (int k = 0; k < RPW; ++k) {
+        int lr = wave * RPW + k, row = t.ty0 + lr;
+        T xv[4], pv[4];
+        zero4(xv); zero4(pv);
+        if (row < d.nrow && cok) {
+            long long e = d.off + (long long)row * d.pitch + c;
+            ld4(pl.x + e, xv);
+            ld4(pl.p + e, pv);
+        }
+        int o = (lr + 1) * kLdsStride + kLdsC0 + 4 * lane;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) { lx[o + j] = xv[j]; lp[o + j] = pv[j]; }
+    }
+    fill_halo<TH>(
+        t, wave, lane,
+        [&](int lr, int lc, long long e, bool ok) {
+            T xv[4], pv[4];
+            zero4(xv); zero4(pv);
+            if (ok) { ld4(pl.x + e, xv); ld4(pl.p + e, pv); }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) { lx[lr * kLdsStride + lc + j] = xv[j]; lp[lr * kLdsStride + lc + j] = pv[j]; }
+        },
+        [&](int lr, int lc, long long e, bool ok) {
+            lx[lr * kLdsStride + lc] = ok ? pl.x[e] : T(0);
+            lp[lr * kLdsStride + lc] = ok ? pl.p[e] : T(0);
+        });
+    __syncthreads();
+    T wu[4];
+    for (int k = 0; k < RPW; ++k) {
+        int lr = wave * RPW + k, row = t.ty0 + lr;
+        if (row >= d.nrow) break;
+        if (k == 0) up_arcs(pl.w2, t, row, cok, lane, wu);
+        T wd[4], wr[4], wl[4], y1[4], y2[4];
+        row_arcs(pl.w2, t, row, cok, lane, wd, wr, wl);
+        stencil4(lx, lr, lane, wd, wu, wr, wl, y1);
+        stencil4(lp, lr, lane, wd, wu, wr, wl, y2);
+        if (cok) {
+            long long e = d.off + (long long)row * d.pitch + c;
+            T b4[4], r4[4];
+            ld4(pl.b + e, b4);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) r4[j] = b4[j] - y1[j];
+            st4(pl.r + e, r4);
+            st4(pl.s + e, y2);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) wu[j] = wd[j];
+    }
+}
+
+// ═══════════════════════════════════════════════════════════════════════════
+// Per-instance reduction of the workgroup partial sums (fixed order) and
+// the scalar recurrences, convergence test and status (one workgroup per
+// instance).
+//   start: gamma0 = r.u; alpha = r.u / wv.u; beta = 0  (skip if r.u <= 0,
+//          :450; stop if p.Lp <= 0, :460)
+//   step:  iters++; converged if r.u < tol^2 gamma0 (:478); else
+//          beta = gamma/gamma_prev, alpha = gamma/(delta - beta*gamma/alpha_prev)
+//   rr:    as step with the replaced gamma/delta, without counting.
+// ═══════════════════════════════════════════════════════════════════════════
+__global__ __launch_bounds__(256) void k_reduce(const InstDesc *descs, const double2 *partials, SolveScal *scal,
+                                                int mode, int max_iter, double tol2) {
+    int inst = blockIdx.x;
+    const InstDesc d = descs[inst];
+    SolveScal *sp = scal + inst;
+    if (mode != RED_START && !sp->active) return;
+    int nb = d.tiles_x * d.tiles_y;
+    double a = 0.0, b = 0.0;
+    for (int i = threadIdx.x; i < nb; i += 256) {
+        double2 v = partials[d.block_begin + i];
+        a += v.x;
+        b += v.y;
+    }
+    __shared__ double2 res;
+    block_sum2(a, b, &res);
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    SolveScal s = *sp;
+    double gam = res.x, del = res.y;
+    if (mode == RED_START) {
+        s.iters = 0;
+        s.gamma0 = gam; s.gamma = gam; s.gamma_prev = gam; s.delta = del; s.beta = 0.0; s.relres = 1.0;
+        s.active = 0;
+        if (!(gam > 0.0)) { s.status = (gam == 0.0) ? CUPHU_SOLVE_EMPTY : CUPHU_SOLVE_NONFINITE; s.relres = 0.0; }
+        else if (max_iter <= 0) s.status = CUPHU_SOLVE_MAXITER;
+        else if (!(del > 0.0)) s.status = isfinite(del) ? CUPHU_SOLVE_BREAKDOWN : CUPHU_SOLVE_NONFINITE;
+        else { s.alpha = gam / del; s.alpha_prev = s.alpha; s.active = 1; s.status = CUPHU_SOLVE_MAXITER; }
+    } else {
+        if (mode == RED_STEP) { s.iters += 1; s.gamma_prev = s.gamma; s.alpha_prev = s.alpha; }
+        s.gamma = gam; s.delta = del;
+        s.relres = (gam >= 0.0 && s.gamma0 > 0.0) ? sqrt(gam / s.gamma0) : s.relres;
+        if (!isfinite(gam) || !isfinite(del)) { s.status = CUPHU_SOLVE_NONFINITE; s.active = 0; }
+        else if (gam < tol2 * s.gamma0) { s.status = CUPHU_SOLVE_CONVERGED; s.active = 0; }
+        else if (s.iters >= max_iter) { s.status = CUPHU_SOLVE_MAXITER; s.active = 0; }
+        else {
+            double beta = gam / s.gamma_prev;
+            double denom = del - beta * gam / s.alpha_prev;
+            if (!(denom > 0.0)) { s.status = isfinite(denom) ? CUPHU_SOLVE_BREAKDOWN : CUPHU_SOLVE_NONFINITE; s.active = 0; }
+            else { s.beta = beta; s.alpha = gam / denom; }
+        }
+    }
+    *sp = s;
+}
+
+// ═══════════════════════════════════════════════════════════════════════════
+// Anchor to phi[0,0] (:534-539) and congruence (laplace_congruence :256-266)
+// into a pitched float output plane.
+// ═══════════════════════════════════════════════════════════════════════════
+template <class T>
+__global__ __launch_bounds__(256) void k_final(const InstDesc *descs, const int *rowinst, Planes<T> pl, float *out) {
+    int grow = blockIdx.y;
+    const InstDesc d = descs[rowinst[grow]];
+    int row = grow - d.row_begin;
+    int c0 = (blockIdx.x * 256 + threadIdx.x) * 4;
+    if (c0 >= d.pitch) return;
+    long long e = d.off + (long long)row * d.pitch + c0;
+    T off = (T)pl.phi[d.off] - pl.x[d.off];
+    T xv[4];
+    float ph[4], o[4];
+    ld4(pl.x + e, xv);
+    ld4f(pl.phi + e, ph);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        T v = (off != T(0)) ? xv[j] + off : xv[j];
+        if constexpr (sizeof(T) == 4) {
+            float dd = v - ph[j];
+            o[j] = __builtin_fmaf(kTwoPiF, __builtin_rintf(dd * kInvTwoPiF), ph[j]);
+        } else {
+            double dd = v - (double)ph[j];
+            o[j] = (float)__builtin_fma(kTwoPiD, __builtin_rint(dd / kTwoPiD), (double)ph[j]);
+        }
+    }
+    *reinterpret_cast<float4 *>(out + e) = make_float4(o[0], o[1], o[2], o[3]);
+}
+
+// ═══════════════════════════════════════════════════════════════════════════
+// Plain path: the reference's per-iteration launch sequence
+// (cuphu_laplace.cu:455-484) on one instance, with FP64 partial sums.
+// ═══════════════════════════════════════════════════════════════════════════
+template <class T>
+__global__ __launch_bounds__(256) void k_plain_matvec(InstDesc d, const T *w2, const T *x, T *y) {
+    int row = blockIdx.y;
+    int lane = threadIdx.x & 63;
+    int c = (blockIdx.x * 256 + threadIdx.x) * 4;
+    bool cok = c < d.pitch;
+    long long e = d.off + (long long)row * d.pitch + c;
+    T xa[4], xu[4], xd[4], wd[4], wr[4], wu[4];
+    zero4(xa); zero4(xu); zero4(xd); zero4(wd); zero4(wr); zero4(wu);
+    if (cok) {
+        ld4(x + e, xa);
+        ld_arcs(w2, e, wd, wr);
+        if (row > 0) { T dummy[4]; ld4(x + e - d.pitch, xu); ld_arcs(w2, e - d.pitch, wu, dummy); }
+        if (row + 1 < d.nrow) ld4(x + e + d.pitch, xd);
+    }
+    T xl = __shfl_up(xa[3], 1, 64), wl0 = __shfl_up(wr[3], 1, 64), xr = __shfl_down(xa[0], 1, 64);
+    if (lane == 0) {
+        bool ok = cok && c > 0;
+        xl = ok ? x[e - 1] : T(0);
+        wl0 = ok ? w2[2 * (e - 1) + 1] : T(0);
+    }
+    if (lane == 63) xr = (cok && c + 4 < d.pitch) ? x[e + 4] : T(0);
+    T wl[4] = {wl0, wr[0], wr[1], wr[2]};
+    T yv[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        T a = xa[j];
+        T left = (j == 0) ? xl : xa[j - 1];
+        T right = (j == 3) ? xr : xa[j + 1];
+        T v = Prec<T>::eps * a;
+        v = Prec<T>::fmaT(wd[j], a - xd[j], v);
+        v = Prec<T>::fmaT(wu[j], a - xu[j], v);
+        v = Prec<T>::fmaT(wr[j], a - right, v);
+        v = Prec<T>::fmaT(wl[j], a - left, v);
+        yv[j] = v;
+    }
+    if (cok) st4(y + e, yv);
+}
+
+template <class T> __global__ void k_plain_axpy(T *y, const T *x, T alpha, size_t n) {
+    size_t i = ((size_t)blockIdx.x * 256 + threadIdx.x) * 4;
+    if (i >= n) return;
+    T a[4], b[4];
+    ld4(x + i, a);
+    ld4(y + i, b);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) b[j] = Prec<T>::fmaT(alpha, a[j], b[j]);
+    st4(y + i, b);
+}
+template <class T> __global__ void k_plain_vmul(T *z, const T *r, const T *dinv, size_t n) {
+    size_t i = ((size_t)blockIdx.x * 256 + threadIdx.x) * 4;
+    if (i >= n) return;
+    T a[4], b[4];
+    ld4(r + i, a);
+    ld4(dinv + i, b);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) a[j] = a[j] * b[j];
+    st4(z + i, a);
+}
+template <class T> __global__ void k_plain_pz(T *p, T *z, const T *r, const T *dinv, size_t n) {
+    size_t i = ((size_t)blockIdx.x * 256 + threadIdx.x) * 4;
+    if (i >= n) return;
+    T a[4], b[4];
+    ld4(r + i, a);
+    ld4(dinv + i, b);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) a[j] = a[j] * b[j];
+    st4(z + i, a);
+    st4(p + i, a);
+}
+template <class T> __global__ void k_plain_axpby(T *p, const T *z, T beta, size_t n) {
+    size_t i = ((size_t)blockIdx.x * 256 + threadIdx.x) * 4;
+    if (i >= n) return;
+    T a[4], b[4];
+    ld4(z + i, a);
+    ld4(p + i, b);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) b[j] = Prec<T>::fmaT(beta, b[j], a[j]);
+    st4(p + i, b);
+}
+template <class T> __global__ __launch_bounds__(256) void k_plain_dot_part(const T *a, const T *b, size_t n, double *parts) {
+    double s = 0.0;
+    for (size_t i = ((size_t)blockIdx.x * 256 + threadIdx.x) * 4; i < n; i += (size_t)gridDim.x * 1024) {
+        T x[4], y[4];
+        ld4(a + i, x);
+        ld4(b + i, y);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) s += (double)x[j] * (double)y[j];
+    }
+    __shared__ double2 res;
+    block_sum2(s, 0.0, &res);
+    if (threadIdx.x == 0) parts[blockIdx.x] = res.x;
+}
+__global__ __launch_bounds__(256) void k_plain_dot_final(const double *parts, int nparts, double *out) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += 256) s += parts[i];
+    __shared__ double2 res;
+    block_sum2(s, 0.0, &res);
+    if (threadIdx.x == 0) *out = res.x;
+}
+
+// ── launchers ───────────────────────────────────────────────────────────────
+#define CUPHU_LAUNCH_CHECK() CUPHU_HIP_CHECK(hipGetLastError())
+
+template <class T>
+void launch_ingest(const IngestSrc *d_src, const InstDesc *d_desc, const int *d_rowinst, int nrows_total,
+                   const Planes<T> &pl, int has_u0, int max_pitch, hipStream_t st) {
+    dim3 grid((unsigned)ceil_div((size_t)max_pitch, 1024), (unsigned)nrows_total);
+    hipLaunchKernelGGL(k_ingest<T>, grid, dim3(256), 0, st, d_src, d_desc, d_rowinst, pl, has_u0);
+    CUPHU_LAUNCH_CHECK();
+}
+
+#define CUPHU_RPW_DISPATCH(rpw, KERNEL, ...)                                      \
+    switch (rpw) {                                                                \
+    case 2: hipLaunchKernelGGL((KERNEL<T, 2>), __VA_ARGS__); break;               \
+    case 8: hipLaunchKernelGGL((KERNEL<T, 8>), __VA_ARGS__); break;               \
+    default: hipLaunchKernelGGL((KERNEL<T, 4>), __VA_ARGS__); break;              \
+    }
+
+template <class T>
+void launch_weights(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, int reweight,
+                    T delta2, int rpw, hipStream_t st) {
+    CUPHU_RPW_DISPATCH(rpw, k_weights, dim3(nblk), dim3(256), 0, st, d_desc, d_blk, pl, reweight, delta2);
+    CUPHU_LAUNCH_CHECK();
+}
+template <class T>
+void launch_assemble(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, int rpw, hipStream_t st) {
+    CUPHU_RPW_DISPATCH(rpw, k_assemble, dim3(nblk), dim3(256), 0, st, d_desc, d_blk, pl);
+    CUPHU_LAUNCH_CHECK();
+}
+template <class T>
+void launch_start(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, const SolveScal *scal,
+                  int force, double2 *partials, int rpw, hipStream_t st) {
+    CUPHU_RPW_DISPATCH(rpw, k_start, dim3(nblk), dim3(256), 0, st, d_desc, d_blk, pl, scal, force, partials);
+    CUPHU_LAUNCH_CHECK();
+}
+template <class T>
+void launch_cg(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, const SolveScal *scal,
+               double2 *partials, int rpw, hipStream_t st) {
+    CUPHU_RPW_DISPATCH(rpw, k_cg, dim3(nblk), dim3(256), 0, st, d_desc, d_blk, pl, scal, partials);
+    CUPHU_LAUNCH_CHECK();
+}
+template <class T>
+void launch_rr(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, const SolveScal *scal,
+               int rpw, hipStream_t st) {
+    CUPHU_RPW_DISPATCH(rpw, k_rr, dim3(nblk), dim3(256), 0, st, d_desc, d_blk, pl, scal);
+    CUPHU_LAUNCH_CHECK();
+}
+void launch_reduce(const InstDesc *d_desc, int ninst, const double2 *partials, SolveScal *scal, int mode,
+                   int max_iter, double tol2, hipStream_t st) {
+    hipLaunchKernelGGL(k_reduce, dim3(ninst), dim3(256), 0, st, d_desc, partials, scal, mode, max_iter, tol2);
+    CUPHU_LAUNCH_CHECK();
+}
+template <class T>
+void launch_final(const InstDesc *d_desc, const int *d_rowinst, int nrows_total, const Planes<T> &pl, float *out,
+                  int max_pitch, hipStream_t st) {
+    dim3 grid((unsigned)ceil_div((size_t)max_pitch, 1024), (unsigned)nrows_total);
+    hipLaunchKernelGGL(k_final<T>, grid, dim3(256), 0, st, d_desc, d_rowinst, pl, out);
+    CUPHU_LAUNCH_CHECK();
+}
+
+template <class T> void launch_plain_matvec(const InstDesc &d, const T *w2, const T *x, T *y, hipStream_t st) {
+    dim3 grid((unsigned)ceil_div((size_t)d.pitch, 1024), (unsigned)d.nrow);
+    hipLaunchKernelGGL(k_plain_matvec<T>, grid, dim3(256), 0, st, d, w2, x, y);
+    CUPHU_LAUNCH_CHECK();
+}
+static inline unsigned grid1(size_t n) { return (unsigned)ceil_div(n, 1024); }
+template <class T> void launch_plain_axpy(T *y, const T *x, T alpha, size_t n, hipStream_t st) {
+    hipLaunchKernelGGL(k_plain_axpy<T>, dim3(grid1(n)), dim3(256), 0, st, y, x, alpha, n);
+    CUPHU_LAUNCH_CHECK();
+}
+template <class T> void launch_plain_vmul(T *z, const T *r, const T *dinv, size_t n, hipStream_t st) {
+    hipLaunchKernelGGL(k_plain_vmul<T>, dim3(grid1(n)), dim3(256), 0, st, z, r, dinv, n);
+    CUPHU_LAUNCH_CHECK();
+}
+template <class T> void launch_plain_pz(T *p, T *z, const T *r, const T *dinv, size_t n, hipStream_t st) {
+    hipLaunchKernelGGL(k_plain_pz<T>, dim3(grid1(n)), dim3(256), 0, st, p, z, r, dinv, n);
+    CUPHU_LAUNCH_CHECK();
+}
+template <class T> void launch_plain_axpby(T *p, const T *z, T beta, size_t n, hipStream_t st) {
+    hipLaunchKernelGGL(k_plain_axpby<T>, dim3(grid1(n)), dim3(256), 0, st, p, z, beta, n);
+    CUPHU_LAUNCH_CHECK();
+}
+template <class T>
+void launch_plain_dot(const T *a, const T *b, size_t n, double *partials, double *out, hipStream_t st) {
+    hipLaunchKernelGGL(k_plain_dot_part<T>, dim3(kDotParts), dim3(256), 0, st, a, b, n, partials);
+    CUPHU_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_plain_dot_final, dim3(1), dim3(256), 0, st, partials, kDotParts, out);
+    CUPHU_LAUNCH_CHECK();
+}
+
+#define CUPHU_INSTANTIATE(T)                                                                                    \
+    template void launch_ingest<T>(const IngestSrc *, const InstDesc *, const int *, int, const Planes<T> &, int, \
+                                   int, hipStream_t);                                                            \
+    template void launch_weights<T>(const InstDesc *, const int *, int, const Planes<T> &, int, T, int,          \
+                                    hipStream_t);                                                                \
+    template void launch_assemble<T>(const InstDesc *, const int *, int, const Planes<T> &, int, hipStream_t);   \
+    template void launch_start<T>(const InstDesc *, const int *, int, const Planes<T> &, const SolveScal *, int, \
+                                  double2 *, int, hipStream_t);                                                  \
+    template void launch_cg<T>(const InstDesc *, const int *, int, const Planes<T> &, const SolveScal *,         \
+                               double2 *, int, hipStream_t);                                                     \
+    template void launch_rr<T>(const InstDesc *, const int *, int, const Planes<T> &, const SolveScal *, int,    \
+                               hipStream_t);                                                                     \
+    template void launch_final<T>(const InstDesc *, const int *, int, const Planes<T> &, float *, int,           \
+                                  hipStream_t);                                                                  \
+    template void launch_plain_matvec<T>(const InstDesc &, const T *, const T *, T *, hipStream_t);              \
+    template void launch_plain_axpy<T>(T *, const T *, T, size_t, hipStream_t);                                  \
+    template void launch_plain_vmul<T>(T *, const T *, const T *, size_t, hipStream_t);                          \
+    template void launch_plain_pz<T>(T *, T *, const T *, const T *, size_t, hipStream_t);                       \
+    template void launch_plain_axpby<T>(T *, const T *, T, size_t, hipStream_t);                                \
+    template void launch_plain_dot<T>(const T *, const T *, size_t, double *, double *, hipStream_t);
+
+CUPHU_INSTANTIATE(float)
+CUPHU_INSTANTIATE(double)
+
+}  // namespace cuphu

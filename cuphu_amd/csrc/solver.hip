@@ -1,0 +1,1556 @@
+// solver.hip -- host driver of the Laplace PCG/IRLS path
+// (reference: src/cxx/cuphu/cuphu_laplace.cu:351-552 driver,
+// cuphu_solver.cpp:370-510 solve_tile, :1674-2101 cuphu_unwrap).
+//
+// Batched solver: all instances (tiles of a scene, or independent
+// interferograms) share pooled pitched planes and are solved together, in
+// groups whose working set is bounded by group_bytes_mb.  Per IRLS pass:
+// weights -> assemble (+ r = b - L x) -> start (u, L u, r.u, Lu.u) ->
+// reduce, then chunks of `rr_interval` fused PCG steps (each a tile kernel
+// and a per-instance reduction kernel) closed by a true-residual
+// replacement.  Each chunk is captured once in a HIP graph and replayed;
+// the host reads the per-instance scalars (a few hundred bytes) once per
+// chunk.  Converged instances are masked inside the kernels.
+//
+// Plain solver: the reference's launch sequence for one instance at a
+// time (matvec, dot, axpy, axpy, vmul, dot, axpby with two blocking scalar
+// reads per iteration), on the same planes and kernels' arithmetic.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <type_traits>
+
+#include "laplace.hpp"
+#include "stitch.hpp"
+#include "util.hpp"
+
+namespace cuphu {
+
+static thread_local std::string g_last_error;
+void set_last_error(const std::string &msg) { g_last_error = msg; }
+
+static std::recursive_mutex g_mu;   // serialises calls (shared workspace and stream)
+
+// ── per-device cached resources ─────────────────────────────────────────────
+struct DevRes {
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    std::map<std::string, DevBuf<unsigned char>> bufs;
+    std::map<std::string, PinBuf<unsigned char>> pins;
+    void *get(const std::string &name, size_t bytes) {
+        DevBuf<unsigned char> &b = bufs[name];
+        if (b.n < bytes) b.alloc(bytes);
+        return b.p;
+    }
+    void *pin(const std::string &name, size_t bytes) {
+        PinBuf<unsigned char> &b = pins[name];
+        if (b.n < bytes) b.alloc(bytes);
+        return b.p;
+    }
+};
+static DevRes &devres() {
+    static std::map<int, DevRes> m;
+    int dev = 0;
+    CUPHU_HIP_CHECK(hipGetDevice(&dev));
+    DevRes &r = m[dev];
+    if (!r.stream) {
+        CUPHU_HIP_CHECK(hipStreamCreateWithFlags(&r.stream, hipStreamNonBlocking));
+        CUPHU_HIP_CHECK(hipEventCreateWithFlags(&r.ev_in, hipEventDisableTiming));
+        CUPHU_HIP_CHECK(hipEventCreateWithFlags(&r.ev_out, hipEventDisableTiming));
+    }
+    return r;
+}
+template <class T> static T *ws(DevRes &r, const char *name, size_t count) {
+    return reinterpret_cast<T *>(r.get(name, std::max<size_t>(count, 1) * sizeof(T) + 256));
+}
+
+// Orders the library's work stream after the caller's stream on entry and
+// the caller's stream after the work stream on exit.
+struct StreamJoin {
+    hipStream_t user, work;
+    DevRes &r;
+    StreamJoin(DevRes &res, hipStream_t u) : user(u), work(res.stream), r(res) {
+        CUPHU_HIP_CHECK(hipEventRecord(r.ev_in, user));
+        CUPHU_HIP_CHECK(hipStreamWaitEvent(work, r.ev_in, 0));
+    }
+    ~StreamJoin() {
+        if (hipEventRecord(r.ev_out, work) == hipSuccess) (void)hipStreamWaitEvent(user, r.ev_out, 0);
+    }
+};
+
+// ── options ─────────────────────────────────────────────────────────────────
+static void env_int(const char *k, int *v) { if (const char *e = std::getenv(k)) *v = std::atoi(e); }
+static void env_float(const char *k, float *v) { if (const char *e = std::getenv(k)) *v = (float)std::atof(e); }
+
+void default_options(CuPhuLaplaceOptions *o) {
+    std::memset(o, 0, sizeof(*o));
+    // cuphu_laplace.cu:385-390, cuphu_solver.cpp:468-472
+    o->n_irls = 16;
+    o->delta_max = 0.5f;
+    o->delta_min = 0.01f;
+    o->max_iter = 1000;
+    o->tol = 1e-3;
+    o->solver = CUPHU_SOLVER_BATCHED;
+    o->precision = 32;
+    o->rr_interval = 50;
+    o->use_graph = 1;
+    o->group_bytes_mb = 0;
+    env_int("CUPHU_N_IRLS", &o->n_irls);
+    env_float("CUPHU_IRLS_DELTA_MAX", &o->delta_max);
+    env_float("CUPHU_IRLS_DELTA_MIN", &o->delta_min);
+    env_int("CUPHU_LAPLACE_MAX_ITER", &o->max_iter);
+    if (const char *e = std::getenv("CUPHU_LAPLACE_TOL")) o->tol = std::atof(e);
+    if (std::getenv("CUPHU_LAPLACE_VERBOSE")) o->verbose = 1;
+    if (const char *e = std::getenv("CUPHU_LAPLACE_SOLVER"))
+        o->solver = (std::strcmp(e, "plain") == 0) ? CUPHU_SOLVER_PLAIN : CUPHU_SOLVER_BATCHED;
+    env_int("CUPHU_LAPLACE_PRECISION", &o->precision);
+    env_int("CUPHU_LAPLACE_GROUP_MB", &o->group_bytes_mb);
+    env_int("CUPHU_LAPLACE_GRAPH", &o->use_graph);
+}
+
+static void check_options(const CuPhuLaplaceOptions &o) {
+    CUPHU_REQUIRE(o.n_irls >= 0 && o.n_irls <= CUPHU_MAX_IRLS, "n_irls must be in [0, 32]");
+    CUPHU_REQUIRE(o.precision == 32 || o.precision == 64, "precision must be 32 or 64");
+    CUPHU_REQUIRE(o.solver == CUPHU_SOLVER_PLAIN || o.solver == CUPHU_SOLVER_BATCHED, "unknown solver");
+    CUPHU_REQUIRE(o.tol >= 0.0 && o.max_iter >= 0, "tol and max_iter must be >= 0");
+}
+
+static int rows_per_wave() {
+    int rpw = 4;
+    env_int("CUPHU_LAPLACE_RPW", &rpw);
+    return (rpw == 2 || rpw == 8) ? rpw : 4;
+}
+
+// ── layout ──────────────────────────────────────────────────────────────────
+struct Layout {
+    std::vector<InstDesc> desc;   // global: off, row_begin (block_begin unused)
+    long long total_el = 0;
+    int total_rows = 0, max_pitch = 64;
+};
+static Layout make_layout(const std::vector<std::pair<int, int>> &dims, int th) {
+    Layout L;
+    for (auto &dm : dims) {
+        InstDesc d{};
+        d.nrow = dm.first;
+        d.ncol = dm.second;
+        d.pitch = pitch_of(d.ncol);
+        d.tiles_x = (int)ceil_div((size_t)d.pitch, kTileW);
+        d.tiles_y = (int)ceil_div((size_t)d.nrow, (size_t)th);
+        d.row_begin = L.total_rows;
+        d.off = L.total_el;
+        L.total_el += (long long)d.nrow * d.pitch;
+        L.total_rows += d.nrow;
+        L.max_pitch = std::max(L.max_pitch, d.pitch);
+        L.desc.push_back(d);
+    }
+    return L;
+}
+
+template <class T> struct PoolSet {
+    Planes<T> pl;
+    float *out = nullptr;
+    uint32_t *lab = nullptr;
+};
+template <class T> static PoolSet<T> alloc_pools(DevRes &r, long long n) {
+    PoolSet<T> P;
+    const char *sfx = sizeof(T) == 4 ? "32" : "64";
+    auto nm = [&](const char *b) { return std::string(b) + sfx; };
+    P.pl.phi = ws<float>(r, "phi", n);
+    P.pl.corr = ws<float>(r, "corr", n);
+    P.pl.mask = ws<unsigned char>(r, "mask", n);
+    P.pl.sigsq = ws<short2>(r, "sigsq", n);
+    P.pl.w2 = ws<T>(r, nm("w2").c_str(), 2 * n);
+    P.pl.b = ws<T>(r, nm("b").c_str(), n);
+    P.pl.dinv = ws<T>(r, nm("dinv").c_str(), n);
+    P.pl.x = ws<T>(r, nm("x").c_str(), n);
+    P.pl.r = ws<T>(r, nm("r").c_str(), n);
+    P.pl.p = ws<T>(r, nm("p").c_str(), n);
+    P.pl.s = ws<T>(r, nm("s").c_str(), n);
+    P.pl.wv = ws<T>(r, nm("wv").c_str(), n);
+    P.out = ws<float>(r, "out", n);
+    P.lab = ws<uint32_t>(r, "lab", n);
+    return P;
+}
+
+static void record_pass(CuPhuSolveStats *st, int pass, const SolveScal &s) {
+    if (!st) return;
+    st->iters[pass] = s.iters;
+    st->relres[pass] = s.relres;
+    st->pass_status[pass] = s.status;
+    st->npass = pass + 1;
+    // worst status: nonfinite > breakdown > maxiter > empty > converged
+    auto rank = [](int v) {
+        switch (v) {
+        case CUPHU_SOLVE_NONFINITE: return 4;
+        case CUPHU_SOLVE_BREAKDOWN: return 3;
+        case CUPHU_SOLVE_MAXITER: return 2;
+        case CUPHU_SOLVE_EMPTY: return 1;
+        default: return 0;
+        }
+    };
+    if (pass == 0 || rank(s.status) > rank(st->status)) st->status = s.status;
+}
+
+static float pass_delta(const CuPhuLaplaceOptions &o, int pass) {
+    // delta used to reweight before solve `pass` (after solve pass-1),
+    // cuphu_laplace.cu:516-522.
+    float d = o.delta_max * powf(0.5f, (float)(pass - 1));
+    return std::max(o.delta_min, d);
+}
+
+// ── batched solve of one group ──────────────────────────────────────────────
+struct GroupDev {
+    InstDesc *desc;
+    int *blk;
+    SolveScal *scal;
+    double2 *part;
+    int ninst, nblk;
+};
+
+template <class T>
+static void solve_group_batched(const Planes<T> &pl, const GroupDev &g, const CuPhuLaplaceOptions &o, int rpw,
+                                bool fixed_weights, hipStream_t st, const std::vector<CuPhuSolveStats *> &stats,
+                                std::vector<SolveScal> &h) {
+    const int CH = std::max(1, std::min(o.rr_interval > 0 ? o.rr_interval : 50, std::max(o.max_iter, 1)));
+    const double tol2 = o.tol * o.tol;
+    auto chunk = [&]() {
+        for (int i = 0; i < CH; ++i) {
+            launch_cg<T>(g.desc, g.blk, g.nblk, pl, g.scal, g.part, rpw, st);
+            launch_reduce(g.desc, g.ninst, g.part, g.scal, RED_STEP, o.max_iter, tol2, st);
+        }
+        if (o.rr_interval > 0) {
+            launch_rr<T>(g.desc, g.blk, g.nblk, pl, g.scal, rpw, st);
+            launch_start<T>(g.desc, g.blk, g.nblk, pl, g.scal, 0, g.part, rpw, st);
+            launch_reduce(g.desc, g.ninst, g.part, g.scal, RED_RR, o.max_iter, tol2, st);
+        }
+    };
+    hipGraphExec_t gexec = nullptr;
+    if (o.use_graph) {
+        hipGraph_t graph = nullptr;
+        CUPHU_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+        try {
+            chunk();
+        } catch (...) {
+            hipGraph_t dummy;
+            (void)hipStreamEndCapture(st, &dummy);
+            throw;
+        }
+        CUPHU_HIP_CHECK(hipStreamEndCapture(st, &graph));
+        CUPHU_HIP_CHECK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+        CUPHU_HIP_CHECK(hipGraphDestroy(graph));
+    }
+    h.resize((size_t)g.ninst);
+    const int npass = fixed_weights ? 1 : o.n_irls + 1;
+    try {
+        for (int pass = 0; pass < npass; ++pass) {
+            if (!fixed_weights) {
+                T d2 = T(0);
+                if (pass > 0) {
+                    float df = pass_delta(o, pass);
+                    d2 = (sizeof(T) == 4) ? (T)(df * df) : (T)((double)df * (double)df);
+                }
+                launch_weights<T>(g.desc, g.blk, g.nblk, pl, pass > 0 ? 1 : 0, d2, rpw, st);
+                launch_assemble<T>(g.desc, g.blk, g.nblk, pl, rpw, st);
+            }
+            launch_start<T>(g.desc, g.blk, g.nblk, pl, g.scal, 1, g.part, rpw, st);
+            launch_reduce(g.desc, g.ninst, g.part, g.scal, RED_START, o.max_iter, tol2, st);
+            CUPHU_HIP_CHECK(hipMemcpyAsync(h.data(), g.scal, h.size() * sizeof(SolveScal), hipMemcpyDeviceToHost, st));
+            CUPHU_HIP_CHECK(hipStreamSynchronize(st));
+            int nch = 0;
+            auto any_active = [&]() {
+                for (auto &s : h) if (s.active) return true;
+                return false;
+            };
+            while (any_active()) {
+                if (gexec) CUPHU_HIP_CHECK(hipGraphLaunch(gexec, st));
+                else chunk();
+                CUPHU_HIP_CHECK(hipMemcpyAsync(h.data(), g.scal, h.size() * sizeof(SolveScal), hipMemcpyDeviceToHost, st));
+                CUPHU_HIP_CHECK(hipStreamSynchronize(st));
+                ++nch;
+                if ((long long)nch * CH > (long long)o.max_iter + CH) break;   // cannot happen: reduce caps iterations
+            }
+            if (o.verbose)
+                for (int i = 0; i < g.ninst; ++i)
+                    fprintf(stderr, "[cuphu-pcg] inst %d pass %d iters %d relres %.3e status %d\n", i, pass, h[i].iters,
+                            h[i].relres, h[i].status);
+            for (int i = 0; i < g.ninst; ++i) record_pass(stats[(size_t)i], pass, h[(size_t)i]);
+        }
+    } catch (...) {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        throw;
+    }
+    if (gexec) CUPHU_HIP_CHECK(hipGraphExecDestroy(gexec));
+}
+
+// ── plain solve of one instance (reference launch sequence) ─────────────────
+template <class T>
+static void solve_instance_plain(const Planes<T> &pl, const GroupDev &g, const InstDesc &d, const CuPhuLaplaceOptions &o,
+                                 int rpw, DevRes &res, hipStream_t st, CuPhuSolveStats *stats) {
+    size_t n = (size_t)d.nrow * d.pitch;
+    T *x = pl.x + d.off, *r = pl.r + d.off, *z = pl.wv + d.off, *p = pl.p + d.off, *q = pl.s + d.off;
+    const T *dinv = pl.dinv + d.off;
+    double *parts = ws<double>(res, "dotparts", kDotParts);
+    double *dres = ws<double>(res, "dotres", 1);
+    auto dot = [&](const T *a, const T *b) {
+        launch_plain_dot<T>(a, b, n, parts, dres, st);
+        double v = 0.0;
+        CUPHU_HIP_CHECK(hipMemcpyAsync(&v, dres, sizeof(double), hipMemcpyDeviceToHost, st));
+        CUPHU_HIP_CHECK(hipStreamSynchronize(st));
+        return v;
+    };
+    const double tol2 = o.tol * o.tol;
+    for (int pass = 0; pass <= o.n_irls; ++pass) {
+        T d2 = T(0);
+        if (pass > 0) {
+            float df = pass_delta(o, pass);
+            d2 = (sizeof(T) == 4) ? (T)(df * df) : (T)((double)df * (double)df);
+        }
+        launch_weights<T>(g.desc, g.blk, g.nblk, pl, pass > 0 ? 1 : 0, d2, rpw, st);
+        launch_assemble<T>(g.desc, g.blk, g.nblk, pl, rpw, st);
+        launch_plain_pz<T>(p, z, r, dinv, n, st);
+        double rz = dot(r, z), rz0 = rz, rzl = rz;
+        int iter = 0, status;
+        if (!(rz0 > 0.0)) status = (rz0 == 0.0) ? CUPHU_SOLVE_EMPTY : CUPHU_SOLVE_NONFINITE;
+        else {
+            status = CUPHU_SOLVE_MAXITER;
+            for (; iter < o.max_iter; ++iter) {
+                launch_plain_matvec<T>(d, pl.w2, pl.p, pl.s, st);
+                double pq = dot(p, q);
+                if (!(pq > 0.0)) { status = std::isfinite(pq) ? CUPHU_SOLVE_BREAKDOWN : CUPHU_SOLVE_NONFINITE; break; }
+                T a = (T)(rz / pq);
+                launch_plain_axpy<T>(x, p, a, n, st);
+                launch_plain_axpy<T>(r, q, -a, n, st);
+                launch_plain_vmul<T>(z, r, dinv, n, st);
+                double rzn = dot(r, z);
+                rzl = rzn;
+                if (o.verbose && (iter % 20 == 0))
+                    fprintf(stderr, "[cuphu-plain] pass %d iter %d rel_res %.3e\n", pass, iter, std::sqrt(rzn / rz0));
+                if (rzn < tol2 * rz0) { iter++; status = CUPHU_SOLVE_CONVERGED; break; }
+                T b = (T)(rzn / rz);
+                rz = rzn;
+                launch_plain_axpby<T>(p, z, b, n, st);
+            }
+        }
+        SolveScal s{};
+        s.iters = iter;
+        s.status = status;
+        s.relres = (rz0 > 0.0) ? std::sqrt(std::max(rzl, 0.0) / rz0) : 0.0;
+        record_pass(stats, pass, s);
+    }
+}
+
+// ── batch driver ────────────────────────────────────────────────────────────
+struct BatchOut {
+    Layout lay;
+    std::vector<int> ncomp;
+    float *unw = nullptr;       // pitched pool
+    uint32_t *lab = nullptr;    // pitched pool
+    float *corr = nullptr;      // pitched pool
+    unsigned char *mask = nullptr;
+};
+
+// ingest(planes, layout, d_desc, d_rowinst) fills phi/corr/mask/sigsq and x.
+template <class T, class Ingest>
+static void run_batch(const std::vector<std::pair<int, int>> &dims, const CuPhuLaplaceOptions &o, double minfrac,
+                      long maxncomps, bool do_conncomp, DevRes &res, hipStream_t st, CuPhuSolveStats *stats,
+                      BatchOut &out, Ingest &&ingest) {
+    const int rpw = rows_per_wave(), th = 4 * rpw;
+    Layout L = make_layout(dims, th);
+    int ninst = (int)dims.size();
+    PoolSet<T> P = alloc_pools<T>(res, L.total_el);
+    InstDesc *d_gdesc = ws<InstDesc>(res, "gdesc", (size_t)ninst);
+    int *d_rowinst = ws<int>(res, "rowinst", (size_t)L.total_rows);
+    std::vector<int> rowinst((size_t)L.total_rows);
+    for (int i = 0; i < ninst; ++i)
+        for (int rr = 0; rr < L.desc[(size_t)i].nrow; ++rr) rowinst[(size_t)(L.desc[(size_t)i].row_begin + rr)] = i;
+    CUPHU_HIP_CHECK(hipMemcpyAsync(d_gdesc, L.desc.data(), (size_t)ninst * sizeof(InstDesc), hipMemcpyHostToDevice, st));
+    CUPHU_HIP_CHECK(hipMemcpyAsync(d_rowinst, rowinst.data(), rowinst.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    ingest(P.pl, L, d_gdesc, d_rowinst);
+
+    // groups of consecutive instances bounded by the working-set budget
+    const double bpe = (sizeof(T) == 4) ? 57.0 : 101.0;
+    long long cap = o.group_bytes_mb > 0 ? (long long)((double)o.group_bytes_mb * 1048576.0 / bpe) : (1LL << 62);
+    std::vector<std::pair<int, int>> groups;
+    for (int i = 0; i < ninst;) {
+        int j = i;
+        long long el = 0;
+        while (j < ninst && (j == i || el + (long long)L.desc[(size_t)j].nrow * L.desc[(size_t)j].pitch <= cap)) {
+            el += (long long)L.desc[(size_t)j].nrow * L.desc[(size_t)j].pitch;
+            ++j;
+        }
+        groups.push_back({i, j});
+        i = j;
+    }
+    size_t maxblk = 0, maxinst = 0;
+    for (auto &gr : groups) {
+        size_t nb = 0;
+        for (int i = gr.first; i < gr.second; ++i) nb += (size_t)L.desc[(size_t)i].tiles_x * L.desc[(size_t)i].tiles_y;
+        if (o.solver == CUPHU_SOLVER_PLAIN)
+            for (int i = gr.first; i < gr.second; ++i)
+                maxblk = std::max(maxblk, (size_t)L.desc[(size_t)i].tiles_x * L.desc[(size_t)i].tiles_y);
+        maxblk = std::max(maxblk, nb);
+        maxinst = std::max(maxinst, (size_t)(gr.second - gr.first));
+    }
+    GroupDev g{};
+    g.desc = ws<InstDesc>(res, "grpdesc", maxinst);
+    g.blk = ws<int>(res, "grpblk", maxblk);
+    g.scal = ws<SolveScal>(res, "grpscal", maxinst);
+    g.part = ws<double2>(res, "grppart", maxblk);
+    std::vector<SolveScal> hs;
+    std::vector<CuPhuSolveStats> dummy((size_t)ninst);
+    auto sp = [&](int i) { return stats ? stats + i : &dummy[(size_t)i]; };
+    for (int i = 0; i < ninst; ++i) std::memset(sp(i), 0, sizeof(CuPhuSolveStats));
+
+    auto upload_group = [&](int a, int b) {
+        std::vector<InstDesc> gd;
+        std::vector<int> blk;
+        for (int i = a; i < b; ++i) {
+            InstDesc d = L.desc[(size_t)i];
+            d.block_begin = (int)blk.size();
+            for (int k = 0; k < d.tiles_x * d.tiles_y; ++k) blk.push_back(i - a);
+            gd.push_back(d);
+        }
+        CUPHU_HIP_CHECK(hipMemcpyAsync(g.desc, gd.data(), gd.size() * sizeof(InstDesc), hipMemcpyHostToDevice, st));
+        CUPHU_HIP_CHECK(hipMemcpyAsync(g.blk, blk.data(), blk.size() * sizeof(int), hipMemcpyHostToDevice, st));
+        g.ninst = b - a;
+        g.nblk = (int)blk.size();
+        CUPHU_HIP_CHECK(hipStreamSynchronize(st));   // host vectors die at scope end
+    };
+    if (o.solver == CUPHU_SOLVER_PLAIN) {
+        for (int i = 0; i < ninst; ++i) {
+            upload_group(i, i + 1);
+            solve_instance_plain<T>(P.pl, g, L.desc[(size_t)i], o, rpw, res, st, sp(i));
+        }
+    } else {
+        for (auto &gr : groups) {
+            upload_group(gr.first, gr.second);
+            std::vector<CuPhuSolveStats *> gs;
+            for (int i = gr.first; i < gr.second; ++i) gs.push_back(sp(i));
+            solve_group_batched<T>(P.pl, g, o, rpw, false, st, gs, hs);
+        }
+    }
+    launch_final<T>(d_gdesc, d_rowinst, L.total_rows, P.pl, P.out, L.max_pitch, st);
+    out.ncomp.assign((size_t)ninst, 0);
+    if (do_conncomp)
+        for (int i = 0; i < ninst; ++i) {
+            const InstDesc &d = L.desc[(size_t)i];
+            out.ncomp[(size_t)i] = conncomp_run(P.out + d.off, P.pl.corr + d.off, P.pl.mask + d.off, d.nrow, d.ncol,
+                                                d.pitch, minfrac, maxncomps, P.lab + d.off, st);
+        }
+    out.lay = L;
+    out.unw = P.out;
+    out.lab = P.lab;
+    out.corr = P.pl.corr;
+    out.mask = P.pl.mask;
+}
+
+// Standard ingest from IngestSrc descriptors.
+template <class T>
+static auto src_ingest(const std::vector<IngestSrc> &srcs, bool has_u0, DevRes &res, hipStream_t st) {
+    return [&srcs, has_u0, &res, st](const Planes<T> &pl, const Layout &L, const InstDesc *d_desc, const int *d_rowinst) {
+        IngestSrc *d_src = ws<IngestSrc>(res, "srcs", srcs.size());
+        CUPHU_HIP_CHECK(hipMemcpyAsync(d_src, srcs.data(), srcs.size() * sizeof(IngestSrc), hipMemcpyHostToDevice, st));
+        launch_ingest<T>(d_src, d_desc, d_rowinst, L.total_rows, pl, has_u0 ? 1 : 0, L.max_pitch, st);
+        CUPHU_HIP_CHECK(hipStreamSynchronize(st));
+    };
+}
+
+// ── scene: tiling, batched solve, stitch, merge ─────────────────────────────
+__global__ void k_merge(const float *unw_pool, const uint32_t *lab_pool, const long long *toff, const int *tpitch,
+                        const int *rtab, const int *rloc, const int *ctab, const int *cloc, int ntc, const float *add,
+                        const uint32_t *coff, int nrow, int ncol, float *out, uint32_t *cc) {
+    long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (long long)nrow * ncol) return;
+    int r = (int)(i / ncol), c = (int)(i % ncol);
+    int t = rtab[r] * ntc + ctab[c];
+    long long e = toff[t] + (long long)rloc[r] * tpitch[t] + cloc[c];
+    out[i] = unw_pool[e] + add[t];
+    if (cc) {
+        uint32_t l = lab_pool[e];
+        cc[i] = l ? l + coff[t] : 0u;
+    }
+}
+
+__global__ void k_copy_rows(const float *src, int spitch, float *dst, int nrow, int ncol) {
+    long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (long long)nrow * ncol) return;
+    dst[i] = src[(i / ncol) * spitch + (i % ncol)];
+}
+
+template <class T>
+static void scene_impl(const float2 *d_igram, const float *d_corr, const unsigned char *d_mask,
+                       const unsigned char *d_conf, int nrow, int ncol, double nlooks, const CuPhuParams &prm,
+                       int tile_size, int rovl, int covl, int ntr_req, int ntc_req, const CuPhuLaplaceOptions &o,
+                       float *d_unw, uint32_t *d_cc, CuPhuSolveStats *stats, int *ntiles_out, DevRes &res,
+                       hipStream_t st) {
+    std::vector<int> rf(4096), rs(4096), cf(4096), cs(4096);
+    int ntr = tile_axis(nrow, tile_size, rovl, ntr_req, rf.data(), rs.data(), 4096);
+    int ntc = tile_axis(ncol, tile_size, covl, ntc_req, cf.data(), cs.data(), 4096);
+    CUPHU_REQUIRE(ntr > 0 && ntc > 0 && ntr <= 4096 && ntc <= 4096, "invalid tiling");
+    for (int t = 0; t < ntr; ++t) CUPHU_REQUIRE(rs[(size_t)t] >= 1, "tiling produced an empty tile (rows)");
+    for (int t = 0; t < ntc; ++t) CUPHU_REQUIRE(cs[(size_t)t] >= 1, "tiling produced an empty tile (cols)");
+    int nt = ntr * ntc;
+    if (ntiles_out) *ntiles_out = nt;
+    CostConsts cc = make_cost_consts(prm, nlooks);
+    std::vector<IngestSrc> srcs((size_t)nt);
+    std::vector<std::pair<int, int>> dims((size_t)nt);
+    for (int tr = 0; tr < ntr; ++tr)
+        for (int tc = 0; tc < ntc; ++tc) {
+            IngestSrc &s = srcs[(size_t)(tr * ntc + tc)];
+            size_t o0 = (size_t)rf[(size_t)tr] * ncol + cf[(size_t)tc];
+            std::memset(&s, 0, sizeof(s));
+            s.igram = d_igram + o0;
+            s.corr = d_corr + o0;
+            s.mask = d_mask ? d_mask + o0 : nullptr;
+            s.src_stride = ncol;
+            s.cc = cc;
+            dims[(size_t)(tr * ntc + tc)] = {rs[(size_t)tr], cs[(size_t)tc]};
+        }
+    BatchOut bo;
+    run_batch<T>(dims, o, prm.minconncompfrac, prm.maxncomps, d_cc != nullptr, res, st, stats, bo,
+                 src_ingest<T>(srcs, false, res, st));
+    if (nt == 1) {
+        const InstDesc &d = bo.lay.desc[0];
+        CUPHU_HIP_CHECK(hipMemcpy2DAsync(d_unw, (size_t)ncol * 4, bo.unw + d.off, (size_t)d.pitch * 4, (size_t)ncol * 4,
+                                         (size_t)nrow, hipMemcpyDeviceToDevice, st));
+        if (d_cc)
+            CUPHU_HIP_CHECK(hipMemcpy2DAsync(d_cc, (size_t)ncol * 4, bo.lab + d.off, (size_t)d.pitch * 4,
+                                             (size_t)ncol * 4, (size_t)nrow, hipMemcpyDeviceToDevice, st));
+        CUPHU_HIP_CHECK(hipStreamSynchronize(st));
+        return;
+    }
+    // stitch on the host from the solved tiles (registration + BFS)
+    long long nel = bo.lay.total_el;
+    float *h_unw = reinterpret_cast<float *>(res.pin("h_unw", (size_t)nel * 4));
+    float *h_corr = reinterpret_cast<float *>(res.pin("h_corr", (size_t)nel * 4));
+    unsigned char *h_conf = nullptr;
+    CUPHU_HIP_CHECK(hipMemcpyAsync(h_unw, bo.unw, (size_t)nel * 4, hipMemcpyDeviceToHost, st));
+    CUPHU_HIP_CHECK(hipMemcpyAsync(h_corr, bo.corr, (size_t)nel * 4, hipMemcpyDeviceToHost, st));
+    bool conf_scene = false;
+    if (d_conf) {
+        h_conf = reinterpret_cast<unsigned char *>(res.pin("h_conf", (size_t)nrow * ncol));
+        CUPHU_HIP_CHECK(hipMemcpyAsync(h_conf, d_conf, (size_t)nrow * ncol, hipMemcpyDeviceToHost, st));
+        conf_scene = true;
+    } else if (d_mask) {
+        h_conf = reinterpret_cast<unsigned char *>(res.pin("h_conf", (size_t)nel));
+        CUPHU_HIP_CHECK(hipMemcpyAsync(h_conf, bo.mask, (size_t)nel, hipMemcpyDeviceToHost, st));
+    }
+    CUPHU_HIP_CHECK(hipStreamSynchronize(st));
+    std::vector<TileView> tv((size_t)nt);
+    for (int t = 0; t < nt; ++t) {
+        const InstDesc &d = bo.lay.desc[(size_t)t];
+        TileView &v = tv[(size_t)t];
+        v.nrow = d.nrow; v.ncol = d.ncol; v.stride = d.pitch;
+        v.unw = h_unw + d.off; v.corr = h_corr + d.off;
+        v.conf = nullptr;
+        v.conf_stride = d.pitch;
+        if (h_conf) {
+            if (conf_scene) {
+                v.conf = h_conf + (size_t)rf[(size_t)(t / ntc)] * ncol + cf[(size_t)(t % ntc)];
+                v.conf_stride = ncol;
+            } else {
+                v.conf = h_conf + d.off;
+            }
+        }
+    }
+    std::vector<double> tk = stitch_offsets(ntr, ntc, tv, rovl, covl, true);
+    // merge tables (later tiles win where odd overlaps make trims meet)
+    std::vector<int> rtab((size_t)nrow, 0), rloc((size_t)nrow, 0), ctab((size_t)ncol, 0), cloc((size_t)ncol, 0);
+    for (int tr = 0; tr < ntr; ++tr) {
+        int a, b;
+        trim_range(tr, ntr, rs[(size_t)tr], rovl, &a, &b);
+        for (int r = a; r < b; ++r) { rtab[(size_t)(rf[(size_t)tr] + r)] = tr; rloc[(size_t)(rf[(size_t)tr] + r)] = r; }
+    }
+    for (int tc = 0; tc < ntc; ++tc) {
+        int a, b;
+        trim_range(tc, ntc, cs[(size_t)tc], covl, &a, &b);
+        for (int c = a; c < b; ++c) { ctab[(size_t)(cf[(size_t)tc] + c)] = tc; cloc[(size_t)(cf[(size_t)tc] + c)] = c; }
+    }
+    std::vector<long long> toff((size_t)nt);
+    std::vector<int> tpitch((size_t)nt);
+    std::vector<float> add((size_t)nt);
+    std::vector<uint32_t> coff((size_t)nt);
+    uint32_t off = 0;
+    for (int t = 0; t < nt; ++t) {
+        toff[(size_t)t] = bo.lay.desc[(size_t)t].off;
+        tpitch[(size_t)t] = bo.lay.desc[(size_t)t].pitch;
+        add[(size_t)t] = (float)(tk[(size_t)t] * (2.0 * M_PI));
+        coff[(size_t)t] = off;
+        off += (uint32_t)bo.ncomp[(size_t)t];
+    }
+    long long *d_toff = ws<long long>(res, "m_toff", (size_t)nt);
+    int *d_tpitch = ws<int>(res, "m_tpitch", (size_t)nt);
+    int *d_rtab = ws<int>(res, "m_rtab", (size_t)nrow), *d_rloc = ws<int>(res, "m_rloc", (size_t)nrow);
+    int *d_ctab = ws<int>(res, "m_ctab", (size_t)ncol), *d_cloc = ws<int>(res, "m_cloc", (size_t)ncol);
+    float *d_add = ws<float>(res, "m_add", (size_t)nt);
+    uint32_t *d_coff = ws<uint32_t>(res, "m_coff", (size_t)nt);
+    CUPHU_HIP_CHECK(hipMemcpyAsync(d_toff, toff.data(), (size_t)nt * 8, hipMemcpyHostToDevice, st));
+    CUPHU_HIP_CHECK(hipMemcpyAsync(d_tpitch, tpitch.data(), (size_t)nt * 4, hipMemcpyHostToDevice, st));
+    CUPHU_HIP_CHECK(hipMemcpyAsync(d_rtab, rtab.data(), (size_t)nrow * 4, hipMemcpyHostToDevice, st));
+    CUPHU_HIP_CHECK(hipMemcpyAsync(d_rloc, rloc.data(), (size_t)nrow * 4, hipMemcpyHostToDevice, st));
+    CUPHU_HIP_CHECK(hipMemcpyAsync(d_ctab, ctab.data(), (size_t)ncol * 4, hipMemcpyHostToDevice, st));
+    CUPHU_HIP_CHECK(hipMemcpyAsync(d_cloc, cloc.data(), (size_t)ncol * 4, hipMemcpyHostToDevice, st));
+    CUPHU_HIP_CHECK(hipMemcpyAsync(d_add, add.data(), (size_t)nt * 4, hipMemcpyHostToDevice, st));
+    CUPHU_HIP_CHECK(hipMemcpyAsync(d_coff, coff.data(), (size_t)nt * 4, hipMemcpyHostToDevice, st));
+    long long npx = (long long)nrow * ncol;
+    hipLaunchKernelGGL(k_merge, dim3((unsigned)ceil_div((size_t)npx, 256)), dim3(256), 0, st, bo.unw, bo.lab, d_toff,
+                       d_tpitch, d_rtab, d_rloc, d_ctab, d_cloc, ntc, d_add, d_coff, nrow, ncol, d_unw, d_cc);
+    CUPHU_HIP_CHECK(hipGetLastError());
+    CUPHU_HIP_CHECK(hipStreamSynchronize(st));
+}
+
+// ── fixed-weight WLS (differentiable path) ──────────────────────────────────
+template <class T>
+__global__ void k_wls_ingest(const T *w, const float *phase, const T *rhs, InstDesc d, Planes<T> pl) {
+    long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (long long)d.nrow * d.pitch) return;
+    int r = (int)(i / d.pitch), c = (int)(i % d.pitch);
+    long long e = d.off + i;
+    size_t nra = (size_t)(d.nrow - 1) * d.ncol;
+    T wd = T(0), wr = T(0);
+    float ph = 0.0f;
+    if (c < d.ncol) {
+        ph = phase[(size_t)r * d.ncol + c];
+        if (r < d.nrow - 1) wd = w[(size_t)r * d.ncol + c];
+        if (c < d.ncol - 1) wr = w[nra + (size_t)r * (d.ncol - 1) + c];
+    }
+    pl.w2[2 * e] = wd;
+    pl.w2[2 * e + 1] = wr;
+    pl.phi[e] = ph;
+    pl.x[e] = T(0);
+    (void)rhs;
+}
+// r = b = rhs (mean removed) for the adjoint solve; x = p = s = 0.
+template <class T>
+__global__ void k_wls_rhs(const T *rhs, double mean, InstDesc d, Planes<T> pl) {
+    long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (long long)d.nrow * d.pitch) return;
+    int r = (int)(i / d.pitch), c = (int)(i % d.pitch);
+    long long e = d.off + i;
+    T v = (c < d.ncol) ? (T)((double)rhs[(size_t)r * d.ncol + c] - mean) : T(0);
+    pl.b[e] = v;
+    pl.r[e] = v;
+}
+template <class T>
+__global__ void k_sum_part(const T *v, int nrow, int ncol, int stride, double *parts) {
+    double s = 0.0;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < (long long)nrow * ncol; i += (long long)gridDim.x * 256)
+        s += (double)v[(i / ncol) * stride + (i % ncol)];
+    __shared__ double sh[256];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) parts[blockIdx.x] = sh[0];
+}
+template <class T>
+__global__ void k_out_centered(const T *x, InstDesc d, double mean, T *out) {
+    long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (long long)d.nrow * d.ncol) return;
+    out[i] = (T)((double)x[d.off + (i / d.ncol) * d.pitch + (i % d.ncol)] - mean);
+}
+template <class T>
+__global__ void k_wls_grad(const T *lam, InstDesc d, const T *u, const float *phase, T *gw) {
+    long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    long long nra = (long long)(d.nrow - 1) * d.ncol, na = nra + (long long)d.nrow * (d.ncol - 1);
+    if (i >= na) return;
+    long long t, h;
+    if (i < nra) { t = i; h = i + d.ncol; }
+    else { long long j = i - nra, r = j / (d.ncol - 1), c = j % (d.ncol - 1); t = r * d.ncol + c; h = t + 1; }
+    auto lp = [&](long long k) { return lam[d.off + (k / d.ncol) * d.pitch + (k % d.ncol)]; };
+    T g = Prec<T>::wrap((T)phase[h] - (T)phase[t]);
+    gw[i] = -(lp(h) - lp(t)) * ((u[h] - u[t]) - g);
+}
+
+template <class T>
+static double device_mean(const T *v, int nrow, int ncol, int stride, DevRes &res, hipStream_t st) {
+    const int nb = 256;
+    double *parts = ws<double>(res, "meanparts", nb);
+    hipLaunchKernelGGL(k_sum_part<T>, dim3(nb), dim3(256), 0, st, v, nrow, ncol, stride, parts);
+    CUPHU_HIP_CHECK(hipGetLastError());
+    std::vector<double> h(nb);
+    CUPHU_HIP_CHECK(hipMemcpyAsync(h.data(), parts, nb * sizeof(double), hipMemcpyDeviceToHost, st));
+    CUPHU_HIP_CHECK(hipStreamSynchronize(st));
+    double s = 0.0;
+    for (double x : h) s += x;
+    return s / ((double)nrow * ncol);
+}
+
+template <class T>
+static void wls_impl(const T *d_w, const float *d_phase, const T *d_u, const T *d_gu, int nrow, int ncol, int max_iter,
+                     double tol, T *d_out_u, T *d_gw, T *d_lambda, int *iters_out, double *relres_out, DevRes &res,
+                     hipStream_t st) {
+    CuPhuLaplaceOptions o;
+    default_options(&o);
+    o.n_irls = 0;
+    o.max_iter = max_iter;
+    o.tol = tol;
+    o.solver = CUPHU_SOLVER_BATCHED;
+    const int rpw = rows_per_wave(), th = 4 * rpw;
+    Layout L = make_layout({{nrow, ncol}}, th);
+    PoolSet<T> P = alloc_pools<T>(res, L.total_el);
+    InstDesc d = L.desc[0];
+    unsigned gsz = (unsigned)ceil_div((size_t)d.nrow * d.pitch, 256);
+    hipLaunchKernelGGL(k_wls_ingest<T>, dim3(gsz), dim3(256), 0, st, d_w, d_phase, (const T *)nullptr, d, P.pl);
+    CUPHU_HIP_CHECK(hipGetLastError());
+    GroupDev g{};
+    g.desc = ws<InstDesc>(res, "grpdesc", 1);
+    int nb = d.tiles_x * d.tiles_y;
+    g.blk = ws<int>(res, "grpblk", (size_t)nb);
+    g.scal = ws<SolveScal>(res, "grpscal", 1);
+    g.part = ws<double2>(res, "grppart", (size_t)nb);
+    g.ninst = 1;
+    g.nblk = nb;
+    InstDesc gd = d;
+    gd.block_begin = 0;
+    std::vector<int> blk((size_t)nb, 0);
+    CUPHU_HIP_CHECK(hipMemcpyAsync(g.desc, &gd, sizeof(InstDesc), hipMemcpyHostToDevice, st));
+    CUPHU_HIP_CHECK(hipMemcpyAsync(g.blk, blk.data(), (size_t)nb * sizeof(int), hipMemcpyHostToDevice, st));
+    launch_assemble<T>(g.desc, g.blk, g.nblk, P.pl, rpw, st);
+    if (d_gu) {
+        double m = device_mean<T>(d_gu, nrow, ncol, ncol, res, st);
+        hipLaunchKernelGGL(k_wls_rhs<T>, dim3(gsz), dim3(256), 0, st, d_gu, m, gd, P.pl);
+        CUPHU_HIP_CHECK(hipGetLastError());
+    }
+    std::vector<SolveScal> h;
+    CuPhuSolveStats stt{};
+    std::vector<CuPhuSolveStats *> sp = {&stt};
+    solve_group_batched<T>(P.pl, g, o, rpw, true, st, sp, h);
+    if (iters_out) *iters_out = stt.iters[0];
+    if (relres_out) *relres_out = stt.relres[0];
+    double mx = device_mean<T>(P.pl.x + d.off, nrow, ncol, d.pitch, res, st);
+    unsigned gp = (unsigned)ceil_div((size_t)nrow * ncol, 256);
+    if (d_out_u) {
+        hipLaunchKernelGGL(k_out_centered<T>, dim3(gp), dim3(256), 0, st, P.pl.x, d, mx, d_out_u);
+        CUPHU_HIP_CHECK(hipGetLastError());
+    }
+    if (d_lambda) {
+        hipLaunchKernelGGL(k_out_centered<T>, dim3(gp), dim3(256), 0, st, P.pl.x, d, mx, d_lambda);
+        CUPHU_HIP_CHECK(hipGetLastError());
+    }
+    if (d_gw) {
+        size_t na = (size_t)(nrow - 1) * ncol + (size_t)nrow * (ncol - 1);
+        hipLaunchKernelGGL(k_wls_grad<T>, dim3((unsigned)ceil_div(na, 256)), dim3(256), 0, st, P.pl.x, d, d_u,
+                           d_phase, d_gw);
+        CUPHU_HIP_CHECK(hipGetLastError());
+    }
+    CUPHU_HIP_CHECK(hipStreamSynchronize(st));
+}
+
+// ── flat arc arrays <-> pitched planes ──────────────────────────────────────
+// sig points at the sigsq field of element 0; stride in shorts per element
+// (2 for CuPhuSmoothCost arrays, 4 for CuPhuCost, 1 for a plain short array).
+__global__ void k_costs_to_planes(const short *sig, int stride_shorts, const float *phase, InstDesc d, float *phi,
+                                  float *corr, unsigned char *mask, short2 *sigsq) {
+    long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (long long)d.nrow * d.pitch) return;
+    int r = (int)(i / d.pitch), c = (int)(i % d.pitch);
+    long long e = d.off + i;
+    size_t nra = (size_t)(d.nrow - 1) * d.ncol;
+    short sd = 0, sr = 0;
+    float ph = 0.0f;
+    if (c < d.ncol) {
+        ph = phase[(size_t)r * d.ncol + c];
+        if (r < d.nrow - 1) sd = sig[((size_t)r * d.ncol + c) * stride_shorts];
+        if (c < d.ncol - 1) sr = sig[(nra + (size_t)r * (d.ncol - 1) + c) * stride_shorts];
+    }
+    phi[e] = ph;
+    corr[e] = (c < d.ncol) ? 1.0f : 0.0f;
+    mask[e] = (c < d.ncol) ? 1 : 0;
+    sigsq[e] = make_short2(sd, sr);
+}
+template <class T> __global__ void k_zero(T *p, long long n) {
+    long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = T(0);
+}
+// w2 planes -> flat arcs (row arcs then column arcs)
+template <class T> __global__ void k_planes_to_flat_w(const T *w2, InstDesc d, T *w) {
+    long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    long long nra = (long long)(d.nrow - 1) * d.ncol, na = nra + (long long)d.nrow * (d.ncol - 1);
+    if (i >= na) return;
+    if (i < nra) w[i] = w2[2 * (d.off + (i / d.ncol) * d.pitch + (i % d.ncol))];
+    else {
+        long long j = i - nra;
+        w[i] = w2[2 * (d.off + (j / (d.ncol - 1)) * d.pitch + (j % (d.ncol - 1))) + 1];
+    }
+}
+// flat arcs -> w2 planes (zero where no arc exists)
+template <class T> __global__ void k_flat_w_to_planes(const T *w, InstDesc d, T *w2) {
+    long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (long long)d.nrow * d.pitch) return;
+    int r = (int)(i / d.pitch), c = (int)(i % d.pitch);
+    size_t nra = (size_t)(d.nrow - 1) * d.ncol;
+    T wd = T(0), wr = T(0);
+    if (c < d.ncol) {
+        if (r < d.nrow - 1) wd = w[(size_t)r * d.ncol + c];
+        if (c < d.ncol - 1) wr = w[nra + (size_t)r * (d.ncol - 1) + c];
+    }
+    w2[2 * (d.off + i)] = wd;
+    w2[2 * (d.off + i) + 1] = wr;
+}
+
+// ── complex split planes -> interleaved ─────────────────────────────────────
+__global__ void k_interleave(const float *re, const float *im, float2 *out, size_t n) {
+    size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) out[i] = make_float2(re[i], im[i]);
+}
+
+}  // namespace cuphu
+
+using namespace cuphu;
+
+// ════════════════════════════════════════════════════════════════════════════
+// C ABI
+// ════════════════════════════════════════════════════════════════════════════
+extern "C" {
+
+const char *cuphu_last_error(void) { return g_last_error.c_str(); }
+
+void cuphu_default_laplace_options(CuPhuLaplaceOptions *o) {
+    if (o) default_options(o);
+}
+
+// SNAPHU defaults (cuphu_solver.cpp:148-217 fills these from SNAPHU's DEF_*
+// macros; SNAPHU's header is not available, the values below are SNAPHU's
+// documented defaults).  Only nlooks/ncorrlooks, costscale, nshortcycle,
+// sigsqshortmin, rhosconst1/2, cstd1-3, defothreshfactor, sigsqcorr,
+// defomax, kperpdpsi/kpardpsi and the conncomp fields act on this path.
+void cuphu_default_params(CuPhuParams *p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->rhosconst1 = 1.3;
+    p->rhosconst2 = 0.14;
+    p->cstd1 = 0.4;
+    p->cstd2 = 0.35;
+    p->cstd3 = 0.06;
+    p->defothreshfactor = 1.2;
+    p->sigsqcorr = 0.05;
+    p->nlooks = 23.8;
+    p->ncorrlooks = 23.8;
+    p->costscale = 100.0;
+    p->nshortcycle = 200.0;
+    p->sigsqshortmin = 1;
+    p->kperpdpsi = 7;
+    p->kpardpsi = 7;
+    p->defomax = 1.2;
+    p->minconncompfrac = 0.01;
+    p->conncompthresh = 300;
+    p->maxncomps = 32;
+    // network-flow / topography fields (unused by the Laplace path)
+    p->p = -1.0;
+    p->maxcost = 1000.0;
+    p->costscaleambight = 80.0;
+    p->dnomincangle = 0.01;
+    p->initmaxflow = 9999;
+    p->arcmaxflowconst = 3;
+    p->maxflow = 4;
+    p->cs2scalefactor = 8;
+    p->orbitradius = 7153000.0;
+    p->earthradius = 6378000.0;
+    p->baseline = 150.0;
+    p->baselineangle = 0.25 * M_PI;
+    p->transmitmode = 1;
+    p->nlooksrange = 1;
+    p->nlooksaz = 5;
+    p->nlooksother = 1;
+    p->ncorrlooksrange = 3;
+    p->ncorrlooksaz = 15;
+    p->nearrange = 831000.0;
+    p->dr = 8.0;
+    p->da = 20.0;
+    p->rangeres = 10.0;
+    p->azres = 6.0;
+    p->lambda = 0.0565647;
+    p->kds = 0.02;
+    p->specularexp = 8.0;
+    p->dzrcritfactor = 2.0;
+    p->dzeimin = -4.0;
+    p->laywidth = 16;
+    p->layminei = 1.25;
+    p->sloperatiofactor = 1.18;
+    p->sigsqei = 100.0;
+    p->dzlaypeak = -2.0;
+    p->azdzfactor = 0.99;
+    p->dzeifactor = 4.0;
+    p->dzeiweight = 0.5;
+    p->dzlayfactor = 1.0;
+    p->layconst = 0.9;
+    p->layfalloffconst = 2.0;
+    p->sigsqlayfactor = 0.1;
+    p->defoazdzfactor = 1.0;
+    p->defolayconst = 0.9;
+    p->initdzr = 2048.0;
+    p->initdzstep = 100.0;
+    p->threshold = 0.001;
+}
+
+// cuphu_solver.cpp:219-238
+void cuphu_default_tile_params(CuPhuTileParams *tp) {
+    if (!tp) return;
+    std::memset(tp, 0, sizeof(*tp));
+    tp->ntilerow = 1;
+    tp->ntilecol = 1;
+    tp->rowovrlp = 0;
+    tp->colovrlp = 0;
+    tp->tilecostthresh = 500;
+    tp->minregionsize = 100;
+    tp->nproc = 1;
+    tp->ngpustreams = 2;
+    tp->laplace_neighbor_feedback_feather = 200;
+}
+
+// cuphu_solver.cpp:240-249
+void cuphu_default_bridge_params(CuPhuBridgeParams *bp) {
+    if (!bp) return;
+    std::memset(bp, 0, sizeof(*bp));
+    bp->radius = 500;
+    bp->min_num_pixel = 14;
+    bp->erosion_size = 2;
+    bp->max_boundary_samples = 4096;
+    bp->ramp_type = CUPHU_RAMP_NONE;
+    bp->ramp_max_num_sample = 1000000;
+}
+
+int cuphu_gpu_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int cuphu_gpu_name(int device_id, char *buf, int buflen) {
+    return guarded([&]() {
+        CUPHU_REQUIRE(buf && buflen > 0, "buffer required");
+        hipDeviceProp_t prop;
+        CUPHU_HIP_CHECK(hipGetDeviceProperties(&prop, device_id));
+        std::snprintf(buf, (size_t)buflen, "%s", prop.name);
+        return 0;
+    });
+}
+
+int cuphu_gpu_info(int device_id, long long *total_mem, int *cu_count, char *arch_buf, int arch_buflen) {
+    return guarded([&]() {
+        hipDeviceProp_t prop;
+        CUPHU_HIP_CHECK(hipGetDeviceProperties(&prop, device_id));
+        if (total_mem) *total_mem = (long long)prop.totalGlobalMem;
+        if (cu_count) *cu_count = prop.multiProcessorCount;
+        if (arch_buf && arch_buflen > 0) std::snprintf(arch_buf, (size_t)arch_buflen, "%s", prop.gcnArchName);
+        return 0;
+    });
+}
+
+static CuPhuLaplaceOptions opts_or_default(const CuPhuLaplaceOptions *o) {
+    CuPhuLaplaceOptions r;
+    if (o) r = *o;
+    else default_options(&r);
+    check_options(r);
+    return r;
+}
+
+int cuphu_laplace_batch_gpu(const CuPhuInstance *inst, int ninst, CuPhuCostMode cost_mode, const CuPhuParams *base,
+                            const CuPhuLaplaceOptions *opt, CuPhuSolveStats *stats, cuphu_stream_t stream) {
+    return guarded([&]() {
+        std::lock_guard<std::recursive_mutex> lk(g_mu);
+        CUPHU_REQUIRE(inst && ninst > 0 && base, "instances and params required");
+        CUPHU_REQUIRE(cost_mode >= CUPHU_COST_SMOOTH && cost_mode <= CUPHU_COST_TOPO, "invalid cost mode");
+        CuPhuLaplaceOptions o = opts_or_default(opt);
+        DevRes &res = devres();
+        StreamJoin join(res, (hipStream_t)stream);
+        hipStream_t st = res.stream;
+        std::vector<IngestSrc> srcs((size_t)ninst);
+        std::vector<std::pair<int, int>> dims((size_t)ninst);
+        bool has_u0 = false, want_cc = false;
+        for (int i = 0; i < ninst; ++i) {
+            const CuPhuInstance &I = inst[i];
+            CUPHU_REQUIRE(I.nrow >= 1 && I.ncol >= 1, "instance dimensions must be >= 1");
+            CUPHU_REQUIRE(I.d_igram || I.d_phase, "instance needs d_igram or d_phase");
+            CUPHU_REQUIRE(I.d_corr && I.d_unw, "instance needs d_corr and d_unw");
+            CUPHU_REQUIRE(I.nlooks >= 1.0, "nlooks must be >= 1");
+            IngestSrc &s = srcs[(size_t)i];
+            std::memset(&s, 0, sizeof(s));
+            s.igram = reinterpret_cast<const float2 *>(I.d_igram);
+            s.phase = I.d_phase;
+            s.corr = I.d_corr;
+            s.mask = I.d_mask;
+            s.u0 = I.d_u0;
+            s.src_stride = I.ncol;
+            s.cc = make_cost_consts(*base, I.nlooks);
+            has_u0 = has_u0 || I.d_u0;
+            want_cc = want_cc || I.d_conncomp;
+            dims[(size_t)i] = {I.nrow, I.ncol};
+        }
+        BatchOut bo;
+        if (o.precision == 64)
+            run_batch<double>(dims, o, base->minconncompfrac, base->maxncomps, want_cc, res, st, stats, bo,
+                              src_ingest<double>(srcs, has_u0, res, st));
+        else
+            run_batch<float>(dims, o, base->minconncompfrac, base->maxncomps, want_cc, res, st, stats, bo,
+                             src_ingest<float>(srcs, has_u0, res, st));
+        for (int i = 0; i < ninst; ++i) {
+            const InstDesc &d = bo.lay.desc[(size_t)i];
+            const CuPhuInstance &I = inst[i];
+            CUPHU_HIP_CHECK(hipMemcpy2DAsync(I.d_unw, (size_t)I.ncol * 4, bo.unw + d.off, (size_t)d.pitch * 4,
+                                             (size_t)I.ncol * 4, (size_t)I.nrow, hipMemcpyDeviceToDevice, st));
+            if (I.d_conncomp)
+                CUPHU_HIP_CHECK(hipMemcpy2DAsync(I.d_conncomp, (size_t)I.ncol * 4, bo.lab + d.off, (size_t)d.pitch * 4,
+                                                 (size_t)I.ncol * 4, (size_t)I.nrow, hipMemcpyDeviceToDevice, st));
+        }
+        CUPHU_HIP_CHECK(hipStreamSynchronize(st));
+        return 0;
+    });
+}
+
+int cuphu_unwrap_scene_gpu(const float *d_igram, const float *d_corr, const unsigned char *d_mask,
+                           const unsigned char *d_orig_mask, int nrow, int ncol, double nlooks,
+                           CuPhuCostMode cost_mode, const CuPhuParams *params, int tile_size, int overlap,
+                           int ntilerow, int ntilecol, const CuPhuLaplaceOptions *opt, float *d_unw,
+                           uint32_t *d_conncomp, CuPhuSolveStats *stats, int *ntiles_out, cuphu_stream_t stream) {
+    return guarded([&]() {
+        std::lock_guard<std::recursive_mutex> lk(g_mu);
+        CUPHU_REQUIRE(d_igram && d_corr && d_unw && params, "igram, corr, unw and params required");
+        CUPHU_REQUIRE(nrow >= 1 && ncol >= 1, "dimensions must be >= 1");
+        CUPHU_REQUIRE(nlooks >= 1.0, "nlooks must be >= 1");
+        CUPHU_REQUIRE(cost_mode >= CUPHU_COST_SMOOTH && cost_mode <= CUPHU_COST_TOPO, "invalid cost mode");
+        CUPHU_REQUIRE(overlap >= 0 && tile_size >= 0, "tile_size and overlap must be >= 0");
+        CUPHU_REQUIRE(tile_size == 0 || tile_size > overlap, "tile_size must exceed overlap");
+        CuPhuLaplaceOptions o = opts_or_default(opt);
+        DevRes &res = devres();
+        StreamJoin join(res, (hipStream_t)stream);
+        const unsigned char *conf = d_orig_mask;
+        if (o.precision == 64)
+            scene_impl<double>(reinterpret_cast<const float2 *>(d_igram), d_corr, d_mask, conf, nrow, ncol, nlooks,
+                               *params, tile_size, overlap, overlap, ntilerow, ntilecol, o, d_unw, d_conncomp, stats,
+                               ntiles_out, res, res.stream);
+        else
+            scene_impl<float>(reinterpret_cast<const float2 *>(d_igram), d_corr, d_mask, conf, nrow, ncol, nlooks,
+                              *params, tile_size, overlap, overlap, ntilerow, ntilecol, o, d_unw, d_conncomp, stats,
+                              ntiles_out, res, res.stream);
+        return 0;
+    });
+}
+
+int cuphu_unwrap_ex(const float *igram_r, const float *igram_i, const float *igram_c64, const float *corr,
+                    const unsigned char *mask, int nrow, int ncol, CuPhuCostMode cost_mode, CuPhuInitMethod init_meth,
+                    const CuPhuParams *params, const CuPhuTileParams *tile, const CuPhuBridgeParams *bridge,
+                    const unsigned char *orig_mask, int gpu_id, const CuPhuLaplaceOptions *opt, CuPhuSolveStats *stats,
+                    CuPhuResult *result) {
+    return guarded([&]() {
+        std::lock_guard<std::recursive_mutex> lk(g_mu);
+        CUPHU_REQUIRE(result && corr && params && tile, "null argument");
+        CUPHU_REQUIRE(igram_c64 || (igram_r && igram_i), "igram required");
+        CUPHU_REQUIRE(nrow >= 1 && ncol >= 1, "dimensions must be >= 1");
+        result->unw = nullptr;
+        result->conncomp = nullptr;
+        result->nrow = nrow;
+        result->ncol = ncol;
+        if (init_meth != CUPHU_INIT_LAPLACE)
+            throw Error(CUPHU_ERR_UNSUPPORTED,
+                        "init 'mcf'/'mst' needs SNAPHU's CPU network-flow solver, which is not part of this library");
+        CUPHU_REQUIRE(cost_mode >= CUPHU_COST_SMOOTH && cost_mode <= CUPHU_COST_TOPO, "invalid cost mode");
+        if (bridge && bridge->enabled) throw Error(CUPHU_ERR_UNSUPPORTED, "phase bridging is not implemented");
+        if (tile->single_tile_reoptimize && tile->ntilerow * tile->ntilecol > 1)
+            throw Error(CUPHU_ERR_UNSUPPORTED, "single_tile_reoptimize needs the CPU network-flow solver");
+        if (tile->laplace_neighbor_feedback && tile->ntilerow * tile->ntilecol > 1)
+            throw Error(CUPHU_ERR_UNSUPPORTED, "laplace_neighbor_feedback is not implemented");
+        if (tile->fix_cycle_spikes) throw Error(CUPHU_ERR_UNSUPPORTED, "fix_cycle_spikes is not implemented");
+        CUPHU_REQUIRE(tile->ntilerow >= 1 && tile->ntilecol >= 1, "ntilerow/ntilecol must be >= 1");
+        CUPHU_REQUIRE(tile->rowovrlp >= 0 && tile->colovrlp >= 0, "tile overlaps must be >= 0");
+        CUPHU_REQUIRE(params->ncorrlooks >= 1.0, "nlooks must be >= 1");
+        CuPhuLaplaceOptions o = opts_or_default(opt);
+        DeviceGuard dg(gpu_id);
+        DevRes &res = devres();
+        hipStream_t st = res.stream;
+        size_t n = (size_t)nrow * ncol;
+        result->unw = (float *)std::malloc(n * sizeof(float));
+        result->conncomp = (uint32_t *)std::malloc(n * sizeof(uint32_t));
+        if (!result->unw || !result->conncomp) {
+            std::free(result->unw); std::free(result->conncomp);
+            result->unw = nullptr; result->conncomp = nullptr;
+            return (int)CUPHU_ERR_NOMEM;
+        }
+        try {
+            DevBuf<float> d_corr(n), d_unw(n);
+            DevBuf<float2> d_ig(n);
+            DevBuf<uint32_t> d_cc(n);
+            DevBuf<unsigned char> d_mask(mask ? n : 0), d_orig(orig_mask ? n : 0);
+            if (igram_c64) {
+                CUPHU_HIP_CHECK(hipMemcpyAsync(d_ig.get(), igram_c64, n * 8, hipMemcpyHostToDevice, st));
+            } else {
+                DevBuf<float> d_re(n), d_im(n);
+                CUPHU_HIP_CHECK(hipMemcpyAsync(d_re.get(), igram_r, n * 4, hipMemcpyHostToDevice, st));
+                CUPHU_HIP_CHECK(hipMemcpyAsync(d_im.get(), igram_i, n * 4, hipMemcpyHostToDevice, st));
+                hipLaunchKernelGGL(k_interleave, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st, d_re.get(),
+                                   d_im.get(), d_ig.get(), n);
+                CUPHU_HIP_CHECK(hipGetLastError());
+                CUPHU_HIP_CHECK(hipStreamSynchronize(st));
+            }
+            CUPHU_HIP_CHECK(hipMemcpyAsync(d_corr.get(), corr, n * 4, hipMemcpyHostToDevice, st));
+            if (mask) CUPHU_HIP_CHECK(hipMemcpyAsync(d_mask.get(), mask, n, hipMemcpyHostToDevice, st));
+            if (orig_mask) CUPHU_HIP_CHECK(hipMemcpyAsync(d_orig.get(), orig_mask, n, hipMemcpyHostToDevice, st));
+            // tile-stitching confidence mask: orig_mask, else mask (cuphu_solver.cpp:1696)
+            const unsigned char *conf = orig_mask ? d_orig.get() : nullptr;
+            int single = (tile->ntilerow == 1 && tile->ntilecol == 1);
+            int rov = single ? 0 : tile->rowovrlp, cov = single ? 0 : tile->colovrlp;
+            if (o.precision == 64)
+                scene_impl<double>(d_ig.get(), d_corr.get(), mask ? d_mask.get() : nullptr, conf, nrow, ncol,
+                                   params->ncorrlooks, *params, 0, rov, cov, tile->ntilerow, tile->ntilecol, o,
+                                   d_unw.get(), d_cc.get(), stats, nullptr, res, st);
+            else
+                scene_impl<float>(d_ig.get(), d_corr.get(), mask ? d_mask.get() : nullptr, conf, nrow, ncol,
+                                  params->ncorrlooks, *params, 0, rov, cov, tile->ntilerow, tile->ntilecol, o,
+                                  d_unw.get(), d_cc.get(), stats, nullptr, res, st);
+            CUPHU_HIP_CHECK(hipMemcpyAsync(result->unw, d_unw.get(), n * 4, hipMemcpyDeviceToHost, st));
+            CUPHU_HIP_CHECK(hipMemcpyAsync(result->conncomp, d_cc.get(), n * 4, hipMemcpyDeviceToHost, st));
+            CUPHU_HIP_CHECK(hipStreamSynchronize(st));
+        } catch (...) {
+            std::free(result->unw); std::free(result->conncomp);
+            result->unw = nullptr; result->conncomp = nullptr;
+            throw;
+        }
+        return 0;
+    });
+}
+
+int cuphu_unwrap(const float *igram_r, const float *igram_i, const float *corr, const float *mag,
+                 const unsigned char *mask, int nrow, int ncol, CuPhuCostMode cost_mode, CuPhuInitMethod init_meth,
+                 const CuPhuParams *params, const CuPhuTileParams *tile, const CuPhuBridgeParams *bridge,
+                 const unsigned char *orig_mask, int gpu_id, CuPhuResult *result) {
+    (void)mag;   // the Laplace branch never reads the magnitude (cuphu_solver.cpp:458-510)
+    if (!igram_r || !igram_i) {
+        set_last_error("igram_r and igram_i required");
+        return CUPHU_ERR_INVALID;
+    }
+    return cuphu_unwrap_ex(igram_r, igram_i, nullptr, corr, mask, nrow, ncol, cost_mode, init_meth, params, tile, bridge,
+                           orig_mask, gpu_id, nullptr, nullptr, result);
+}
+
+int cuphu_regrow_conncomp(const float *unw, const float *corr, const unsigned char *mask, int nrow, int ncol,
+                          double corr_thresh, double minfrac, long maxncomps, int gpu_id, uint32_t *labels) {
+    return guarded([&]() {
+        std::lock_guard<std::recursive_mutex> lk(g_mu);
+        CUPHU_REQUIRE(unw && corr && labels && nrow >= 1 && ncol >= 1, "invalid argument");
+        DeviceGuard dg(gpu_id);
+        DevRes &res = devres();
+        hipStream_t st = res.stream;
+        size_t n = (size_t)nrow * ncol;
+        DevBuf<float> du(n), dc(n);
+        DevBuf<unsigned char> dm(mask ? n : 0);
+        DevBuf<uint32_t> dl(n);
+        CUPHU_HIP_CHECK(hipMemcpyAsync(du.get(), unw, n * 4, hipMemcpyHostToDevice, st));
+        CUPHU_HIP_CHECK(hipMemcpyAsync(dc.get(), corr, n * 4, hipMemcpyHostToDevice, st));
+        if (mask) CUPHU_HIP_CHECK(hipMemcpyAsync(dm.get(), mask, n, hipMemcpyHostToDevice, st));
+        conncomp_run(du.get(), dc.get(), mask ? dm.get() : nullptr, nrow, ncol, ncol, minfrac, maxncomps, dl.get(), st,
+                     (float)corr_thresh);
+        CUPHU_HIP_CHECK(hipMemcpyAsync(labels, dl.get(), n * 4, hipMemcpyDeviceToHost, st));
+        CUPHU_HIP_CHECK(hipStreamSynchronize(st));
+        return 0;
+    });
+}
+
+int cuphu_build_costs_gpu(const float *igram_r, const float *igram_i, const float *corr, const float *mag,
+                          const short *weights, int nrow, int ncol, CuPhuCostMode cost_mode, const CuPhuParams *params,
+                          int gpu_id, void **costs_out, size_t *cost_elem_sz) {
+    (void)mag;
+    return guarded([&]() {
+        std::lock_guard<std::recursive_mutex> lk(g_mu);
+        CUPHU_REQUIRE(igram_r && igram_i && corr && params && costs_out && cost_elem_sz, "null argument");
+        CUPHU_REQUIRE(nrow >= 1 && ncol >= 1, "dimensions must be >= 1");
+        CUPHU_REQUIRE(cost_mode >= CUPHU_COST_SMOOTH && cost_mode <= CUPHU_COST_TOPO, "invalid cost mode");
+        *costs_out = nullptr;
+        DeviceGuard dg(gpu_id);
+        DevRes &res = devres();
+        hipStream_t st = res.stream;
+        size_t n = (size_t)nrow * ncol, na = (size_t)(nrow - 1) * ncol + (size_t)nrow * (ncol - 1);
+        int defo = cost_mode != CUPHU_COST_SMOOTH;
+        size_t esz = defo ? 8 : 4;
+        DevBuf<float> d_re(n), d_im(n), d_corr(n), d_ph(n);
+        DevBuf<float2> d_ig(n);
+        DevBuf<short> d_w(weights ? na : 0), d_out(na * esz / 2 + 1);
+        CUPHU_HIP_CHECK(hipMemcpyAsync(d_re.get(), igram_r, n * 4, hipMemcpyHostToDevice, st));
+        CUPHU_HIP_CHECK(hipMemcpyAsync(d_im.get(), igram_i, n * 4, hipMemcpyHostToDevice, st));
+        CUPHU_HIP_CHECK(hipMemcpyAsync(d_corr.get(), corr, n * 4, hipMemcpyHostToDevice, st));
+        if (weights && na) CUPHU_HIP_CHECK(hipMemcpyAsync(d_w.get(), weights, na * 2, hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_interleave, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st, d_re.get(), d_im.get(),
+                           d_ig.get(), n);
+        CUPHU_HIP_CHECK(hipGetLastError());
+        phase_from_igram(d_ig.get(), d_ph.get(), n, st);
+        CostConsts k = make_cost_consts(*params, params->ncorrlooks);
+        util_costs(d_ph.get(), d_corr.get(), weights ? d_w.get() : nullptr, nrow, ncol, k, params->kperpdpsi,
+                   params->kpardpsi, defo, d_out.get(), st);
+        void *h = std::malloc(na * esz + 1);
+        if (!h) return (int)CUPHU_ERR_NOMEM;
+        CUPHU_HIP_CHECK(hipMemcpyAsync(h, d_out.get(), na * esz, hipMemcpyDeviceToHost, st));
+        CUPHU_HIP_CHECK(hipStreamSynchronize(st));
+        *costs_out = h;
+        *cost_elem_sz = esz;
+        return (int)esz;
+    });
+}
+
+static int device_costs(const float *d_phase, const float *d_corr, const short *d_roww, const short *d_colw, int nrow,
+                        int ncol, const CuPhuParams *params, int kperp, int kpar, int defo, void **out,
+                        cuphu_stream_t stream) {
+    return guarded([&]() {
+        std::lock_guard<std::recursive_mutex> lk(g_mu);
+        CUPHU_REQUIRE(d_phase && d_corr && params && out, "null argument");
+        CUPHU_REQUIRE(nrow >= 1 && ncol >= 1, "dimensions must be >= 1");
+        CUPHU_REQUIRE((d_roww == nullptr) == (d_colw == nullptr), "row and column weights go together");
+        DevRes &res = devres();
+        StreamJoin join(res, (hipStream_t)stream);
+        hipStream_t st = res.stream;
+        size_t nra = (size_t)(nrow - 1) * ncol, nca = (size_t)nrow * (ncol - 1), na = nra + nca;
+        size_t esz = defo ? 8 : 4;
+        DevBuf<short> w;
+        if (d_roww) {
+            w.alloc(na + 1);
+            if (nra) CUPHU_HIP_CHECK(hipMemcpyAsync(w.get(), d_roww, nra * 2, hipMemcpyDeviceToDevice, st));
+            if (nca) CUPHU_HIP_CHECK(hipMemcpyAsync(w.get() + nra, d_colw, nca * 2, hipMemcpyDeviceToDevice, st));
+        }
+        void *d = nullptr;
+        CUPHU_HIP_CHECK(hipMalloc(&d, na * esz + 8));
+        try {
+            CostConsts k = make_cost_consts(*params, params->ncorrlooks);
+            util_costs(d_phase, d_corr, d_roww ? w.get() : nullptr, nrow, ncol, k, kperp, kpar, defo, (short *)d, st);
+        } catch (...) {
+            (void)hipFree(d);
+            throw;
+        }
+        *out = d;
+        return 0;
+    });
+}
+
+int cuphu_build_smooth_costs_gpu(const float *d_phase, const float *d_corr, const short *d_roww, const short *d_colw,
+                                 int nrow, int ncol, const CuPhuParams *params, int kperpdpsi, int kpardpsi,
+                                 CuPhuSmoothCost **d_costs_out, cuphu_stream_t stream) {
+    return device_costs(d_phase, d_corr, d_roww, d_colw, nrow, ncol, params, kperpdpsi, kpardpsi, 0,
+                        reinterpret_cast<void **>(d_costs_out), stream);
+}
+int cuphu_build_defo_costs_gpu(const float *d_phase, const float *d_corr, const short *d_roww, const short *d_colw,
+                               int nrow, int ncol, const CuPhuParams *params, int kperpdpsi, int kpardpsi,
+                               CuPhuCost **d_costs_out, cuphu_stream_t stream) {
+    return device_costs(d_phase, d_corr, d_roww, d_colw, nrow, ncol, params, kperpdpsi, kpardpsi, 1,
+                        reinterpret_cast<void **>(d_costs_out), stream);
+}
+
+int cuphu_wrap_phase(float *d_phase, int n, cuphu_stream_t stream) {
+    return guarded([&]() {
+        CUPHU_REQUIRE(d_phase && n >= 0, "invalid argument");
+        util_wrap_phase(d_phase, (size_t)n, (hipStream_t)stream);
+        return 0;
+    });
+}
+int cuphu_phase_diffs(const float *d_phase, float *d_rng, float *d_az, int nrow, int ncol, cuphu_stream_t stream) {
+    return guarded([&]() {
+        CUPHU_REQUIRE(d_phase && d_rng && d_az && nrow >= 1 && ncol >= 1, "invalid argument");
+        util_phase_diffs(d_phase, d_rng, d_az, nrow, ncol, (hipStream_t)stream);
+        return 0;
+    });
+}
+int cuphu_boxcar2d(const float *d_src, float *d_dst, int nrow, int ncol, int kr, int kc, cuphu_stream_t stream) {
+    return guarded([&]() {
+        CUPHU_REQUIRE(d_src && d_dst && nrow >= 1 && ncol >= 1 && kr >= 0 && kc >= 0, "invalid argument");
+        DevBuf<float> tmp((size_t)(nrow + 2 * kr) * ncol);
+        util_boxcar(d_src, d_dst, tmp.get(), nrow, ncol, kr, kc, (hipStream_t)stream);
+        CUPHU_HIP_CHECK(hipStreamSynchronize((hipStream_t)stream));
+        return 0;
+    });
+}
+int cuphu_residues_gpu(const float *d_phase, int nrow, int ncol, signed char *d_res, cuphu_stream_t stream) {
+    return guarded([&]() {
+        CUPHU_REQUIRE(d_phase && d_res && nrow >= 1 && ncol >= 1, "invalid argument");
+        util_residues(d_phase, nrow, ncol, d_res, (hipStream_t)stream);
+        return 0;
+    });
+}
+int cuphu_integrate_phase_gpu(const float *d_phase, const short *d_hflows, const short *d_vflows, int nrow, int ncol,
+                              int gpu_id, float *d_unw) {
+    return guarded([&]() {
+        CUPHU_REQUIRE(d_phase && d_hflows && d_vflows && d_unw && nrow >= 1 && ncol >= 1, "invalid argument");
+        DeviceGuard dg(gpu_id);
+        util_integrate(d_phase, d_hflows, d_vflows, nrow, ncol, d_unw, nullptr);
+        return 0;
+    });
+}
+int cuphu_conncomp_gpu(const float *d_unw, const float *d_corr, const unsigned char *d_mask, int nrow, int ncol,
+                       const short *d_poscost, const short *d_negcost, const CuPhuParams *params, int gpu_id,
+                       uint32_t *d_labels_out) {
+    return guarded([&]() {
+        CUPHU_REQUIRE(d_unw && d_corr && params && d_labels_out && nrow >= 1 && ncol >= 1, "invalid argument");
+        if (d_poscost || d_negcost)
+            throw Error(CUPHU_ERR_UNSUPPORTED, "incremental-cost gating needs an MCF solve (not in this library)");
+        DeviceGuard dg(gpu_id);
+        conncomp_run(d_unw, d_corr, d_mask, nrow, ncol, ncol, params->minconncompfrac, params->maxncomps, d_labels_out,
+                     nullptr);
+        return 0;
+    });
+}
+
+int cuphu_laplace_unwrap_gpu(const CuPhuSmoothCost *d_costs, const float *d_phase, int nrow, int ncol,
+                             double nshortcycle, int max_iter, float tol, int verbose, float *d_unw,
+                             cuphu_stream_t stream) {
+    (void)nshortcycle;   // unused by the reference as well (cuphu_laplace.cu:365)
+    return guarded([&]() {
+        std::lock_guard<std::recursive_mutex> lk(g_mu);
+        CUPHU_REQUIRE(d_costs && d_phase && d_unw && nrow >= 1 && ncol >= 1, "invalid argument");
+        CuPhuLaplaceOptions o;
+        default_options(&o);
+        o.max_iter = max_iter;
+        o.tol = tol;
+        o.verbose = verbose;
+        check_options(o);
+        DevRes &res = devres();
+        StreamJoin join(res, (hipStream_t)stream);
+        hipStream_t st = res.stream;
+        auto ing = [&](auto &pl, const Layout &L, const InstDesc *, const int *) {
+            const InstDesc &d = L.desc[0];
+            unsigned gsz = (unsigned)ceil_div((size_t)d.nrow * d.pitch, 256);
+            hipLaunchKernelGGL(k_costs_to_planes, dim3(gsz), dim3(256), 0, st, (const short *)d_costs + 1, 2, d_phase, d,
+                               pl.phi, pl.corr, pl.mask, pl.sigsq);
+            using TT = std::remove_pointer_t<decltype(pl.x)>;
+            hipLaunchKernelGGL(k_zero<TT>, dim3(gsz), dim3(256), 0, st, pl.x + d.off, (long long)d.nrow * d.pitch);
+            CUPHU_HIP_CHECK(hipGetLastError());
+        };
+        BatchOut bo;
+        std::vector<std::pair<int, int>> dims = {{nrow, ncol}};
+        if (o.precision == 64) run_batch<double>(dims, o, 0.0, 0, false, res, st, nullptr, bo, ing);
+        else run_batch<float>(dims, o, 0.0, 0, false, res, st, nullptr, bo, ing);
+        const InstDesc &d = bo.lay.desc[0];
+        CUPHU_HIP_CHECK(hipMemcpy2DAsync(d_unw, (size_t)ncol * 4, bo.unw + d.off, (size_t)d.pitch * 4, (size_t)ncol * 4,
+                                         (size_t)nrow, hipMemcpyDeviceToDevice, st));
+        CUPHU_HIP_CHECK(hipStreamSynchronize(st));
+        return 0;
+    });
+}
+
+int cuphu_wls_solve_gpu(const void *d_w, const void *d_phase, int nrow, int ncol, int precision, int max_iter,
+                        double tol, void *d_u, int *iters_out, double *relres_out, cuphu_stream_t stream) {
+    return guarded([&]() {
+        std::lock_guard<std::recursive_mutex> lk(g_mu);
+        CUPHU_REQUIRE(d_w && d_phase && d_u && nrow >= 2 && ncol >= 2, "invalid argument");
+        CUPHU_REQUIRE(precision == 32 || precision == 64, "precision must be 32 or 64");
+        DevRes &res = devres();
+        StreamJoin join(res, (hipStream_t)stream);
+        if (precision == 64)
+            wls_impl<double>((const double *)d_w, (const float *)d_phase, nullptr, nullptr, nrow, ncol, max_iter, tol,
+                             (double *)d_u, nullptr, nullptr, iters_out, relres_out, res, res.stream);
+        else
+            wls_impl<float>((const float *)d_w, (const float *)d_phase, nullptr, nullptr, nrow, ncol, max_iter, tol,
+                            (float *)d_u, nullptr, nullptr, iters_out, relres_out, res, res.stream);
+        return 0;
+    });
+}
+
+int cuphu_wls_adjoint_gpu(const void *d_w, const void *d_phase, const void *d_u, const void *d_gu, int nrow, int ncol,
+                          int precision, int max_iter, double tol, void *d_gw, void *d_lambda, int *iters_out,
+                          double *relres_out, cuphu_stream_t stream) {
+    return guarded([&]() {
+        std::lock_guard<std::recursive_mutex> lk(g_mu);
+        CUPHU_REQUIRE(d_w && d_phase && d_u && d_gu && d_gw && nrow >= 2 && ncol >= 2, "invalid argument");
+        CUPHU_REQUIRE(precision == 32 || precision == 64, "precision must be 32 or 64");
+        DevRes &res = devres();
+        StreamJoin join(res, (hipStream_t)stream);
+        if (precision == 64)
+            wls_impl<double>((const double *)d_w, (const float *)d_phase, (const double *)d_u, (const double *)d_gu, nrow,
+                             ncol, max_iter, tol, nullptr, (double *)d_gw, (double *)d_lambda, iters_out, relres_out,
+                             res, res.stream);
+        else
+            wls_impl<float>((const float *)d_w, (const float *)d_phase, (const float *)d_u, (const float *)d_gu, nrow,
+                            ncol, max_iter, tol, nullptr, (float *)d_gw, (float *)d_lambda, iters_out, relres_out, res,
+                            res.stream);
+        return 0;
+    });
+}
+
+// ── test hooks ──────────────────────────────────────────────────────────────
+namespace {
+struct HookInst {
+    Layout L;
+    PoolSet<float> P;
+    GroupDev g{};
+    InstDesc d{};
+};
+void hook_setup(HookInst &h, DevRes &res, int nrow, int ncol, int rpw, hipStream_t st) {
+    h.L = make_layout({{nrow, ncol}}, 4 * rpw);
+    h.P = alloc_pools<float>(res, h.L.total_el);
+    h.d = h.L.desc[0];
+    int nb = h.d.tiles_x * h.d.tiles_y;
+    h.g.desc = ws<InstDesc>(res, "grpdesc", 1);
+    h.g.blk = ws<int>(res, "grpblk", (size_t)nb);
+    h.g.scal = ws<SolveScal>(res, "grpscal", 1);
+    h.g.part = ws<double2>(res, "grppart", (size_t)nb);
+    h.g.ninst = 1;
+    h.g.nblk = nb;
+    std::vector<int> blk((size_t)nb, 0);
+    CUPHU_HIP_CHECK(hipMemcpyAsync(h.g.desc, &h.d, sizeof(InstDesc), hipMemcpyHostToDevice, st));
+    CUPHU_HIP_CHECK(hipMemcpyAsync(h.g.blk, blk.data(), (size_t)nb * sizeof(int), hipMemcpyHostToDevice, st));
+    size_t nel = (size_t)h.L.total_el;
+    unsigned g = (unsigned)ceil_div(2 * nel, 256);
+    hipLaunchKernelGGL(k_zero<float>, dim3(g), dim3(256), 0, st, h.P.pl.w2, (long long)(2 * nel));
+    for (float *p : {h.P.pl.b, h.P.pl.dinv, h.P.pl.x, h.P.pl.r, h.P.pl.p, h.P.pl.s, h.P.pl.wv, h.P.pl.phi})
+        hipLaunchKernelGGL(k_zero<float>, dim3(g), dim3(256), 0, st, p, (long long)nel);
+    CUPHU_HIP_CHECK(hipGetLastError());
+    CUPHU_HIP_CHECK(hipStreamSynchronize(st));
+}
+void to_plane(const float *host, float *dev, const InstDesc &d, hipStream_t st) {
+    CUPHU_HIP_CHECK(hipMemcpy2DAsync(dev + d.off, (size_t)d.pitch * 4, host, (size_t)d.ncol * 4, (size_t)d.ncol * 4,
+                                     (size_t)d.nrow, hipMemcpyHostToDevice, st));
+}
+void from_plane(const float *dev, float *host, const InstDesc &d, hipStream_t st) {
+    CUPHU_HIP_CHECK(hipMemcpy2DAsync(host, (size_t)d.ncol * 4, dev + d.off, (size_t)d.pitch * 4, (size_t)d.ncol * 4,
+                                     (size_t)d.nrow, hipMemcpyDeviceToHost, st));
+}
+}  // namespace
+
+int cuphu_test_ingest(const float *igram_c64, const float *corr, const unsigned char *mask, int nrow, int ncol,
+                      const CuPhuParams *params, float *phase_out, short *sigsq_out) {
+    return guarded([&]() {
+        std::lock_guard<std::recursive_mutex> lk(g_mu);
+        CUPHU_REQUIRE(igram_c64 && corr && params && phase_out && sigsq_out && nrow >= 1 && ncol >= 1, "null argument");
+        DevRes &res = devres();
+        hipStream_t st = res.stream;
+        size_t n = (size_t)nrow * ncol;
+        DevBuf<float2> dig(n);
+        DevBuf<float> dc(n);
+        DevBuf<unsigned char> dm(mask ? n : 0);
+        CUPHU_HIP_CHECK(hipMemcpyAsync(dig.get(), igram_c64, n * 8, hipMemcpyHostToDevice, st));
+        CUPHU_HIP_CHECK(hipMemcpyAsync(dc.get(), corr, n * 4, hipMemcpyHostToDevice, st));
+        if (mask) CUPHU_HIP_CHECK(hipMemcpyAsync(dm.get(), mask, n, hipMemcpyHostToDevice, st));
+        std::vector<IngestSrc> srcs(1);
+        std::memset(&srcs[0], 0, sizeof(IngestSrc));
+        srcs[0].igram = dig.get();
+        srcs[0].corr = dc.get();
+        srcs[0].mask = mask ? dm.get() : nullptr;
+        srcs[0].src_stride = ncol;
+        srcs[0].cc = make_cost_consts(*params, params->ncorrlooks);
+        Layout L = make_layout({{nrow, ncol}}, 16);
+        PoolSet<float> P = alloc_pools<float>(res, L.total_el);
+        InstDesc *d_desc = ws<InstDesc>(res, "gdesc", 1);
+        int *d_rowinst = ws<int>(res, "rowinst", (size_t)nrow);
+        std::vector<int> ri((size_t)nrow, 0);
+        CUPHU_HIP_CHECK(hipMemcpyAsync(d_desc, L.desc.data(), sizeof(InstDesc), hipMemcpyHostToDevice, st));
+        CUPHU_HIP_CHECK(hipMemcpyAsync(d_rowinst, ri.data(), ri.size() * 4, hipMemcpyHostToDevice, st));
+        auto ing = src_ingest<float>(srcs, false, res, st);
+        ing(P.pl, L, d_desc, d_rowinst);
+        const InstDesc &d = L.desc[0];
+        from_plane(P.pl.phi, phase_out, d, st);
+        std::vector<short2> sq((size_t)nrow * d.pitch);
+        CUPHU_HIP_CHECK(hipMemcpyAsync(sq.data(), P.pl.sigsq + d.off, sq.size() * sizeof(short2), hipMemcpyDeviceToHost, st));
+        CUPHU_HIP_CHECK(hipStreamSynchronize(st));
+        size_t nra = (size_t)(nrow - 1) * ncol;
+        for (int r = 0; r < nrow; ++r)
+            for (int c = 0; c < ncol; ++c) {
+                const short2 &v = sq[(size_t)r * d.pitch + c];
+                if (r < nrow - 1) sigsq_out[(size_t)r * ncol + c] = v.x;
+                if (c < ncol - 1) sigsq_out[nra + (size_t)r * (ncol - 1) + c] = v.y;
+            }
+        return 0;
+    });
+}
+
+int cuphu_test_laplace_ops(const float *phase, const short *sigsq, const float *x, int nrow, int ncol, float delta,
+                           float *w0, float *b, float *dinv, float *r, float *lx, float *wrw, float *cong) {
+    return guarded([&]() {
+        std::lock_guard<std::recursive_mutex> lk(g_mu);
+        CUPHU_REQUIRE(phase && sigsq && x && nrow >= 1 && ncol >= 1, "null argument");
+        DevRes &res = devres();
+        hipStream_t st = res.stream;
+        const int rpw = rows_per_wave();
+        HookInst h;
+        hook_setup(h, res, nrow, ncol, rpw, st);
+        const InstDesc &d = h.d;
+        size_t na = (size_t)(nrow - 1) * ncol + (size_t)nrow * (ncol - 1);
+        DevBuf<short> dsig(na + 1);
+        DevBuf<float> dph((size_t)nrow * ncol), dflat(na + 1);
+        CUPHU_HIP_CHECK(hipMemcpyAsync(dsig.get(), sigsq, na * 2, hipMemcpyHostToDevice, st));
+        CUPHU_HIP_CHECK(hipMemcpyAsync(dph.get(), phase, (size_t)nrow * ncol * 4, hipMemcpyHostToDevice, st));
+        unsigned gp = (unsigned)ceil_div((size_t)nrow * d.pitch, 256), ga = (unsigned)ceil_div(na + 1, 256);
+        hipLaunchKernelGGL(k_costs_to_planes, dim3(gp), dim3(256), 0, st, (const short *)dsig.get(), 1,
+                           (const float *)dph.get(), d, h.P.pl.phi, h.P.pl.corr, h.P.pl.mask, h.P.pl.sigsq);
+        CUPHU_HIP_CHECK(hipGetLastError());
+        to_plane(x, h.P.pl.x, d, st);
+        launch_weights<float>(h.g.desc, h.g.blk, h.g.nblk, h.P.pl, 0, 0.0f, rpw, st);
+        if (w0 && na) {
+            hipLaunchKernelGGL(k_planes_to_flat_w<float>, dim3(ga), dim3(256), 0, st, h.P.pl.w2, d, dflat.get());
+            CUPHU_HIP_CHECK(hipMemcpyAsync(w0, dflat.get(), na * 4, hipMemcpyDeviceToHost, st));
+        }
+        launch_assemble<float>(h.g.desc, h.g.blk, h.g.nblk, h.P.pl, rpw, st);
+        if (b) from_plane(h.P.pl.b, b, d, st);
+        if (dinv) from_plane(h.P.pl.dinv, dinv, d, st);
+        if (r) from_plane(h.P.pl.r, r, d, st);
+        if (lx) {
+            launch_plain_matvec<float>(d, h.P.pl.w2, h.P.pl.x, h.P.pl.wv, st);
+            from_plane(h.P.pl.wv, lx, d, st);
+        }
+        if (wrw && na) {
+            launch_weights<float>(h.g.desc, h.g.blk, h.g.nblk, h.P.pl, 1, delta * delta, rpw, st);
+            hipLaunchKernelGGL(k_planes_to_flat_w<float>, dim3(ga), dim3(256), 0, st, h.P.pl.w2, d, dflat.get());
+            CUPHU_HIP_CHECK(hipMemcpyAsync(wrw, dflat.get(), na * 4, hipMemcpyDeviceToHost, st));
+        }
+        if (cong) {
+            std::vector<int> ri((size_t)nrow, 0);
+            int *d_rowinst = ws<int>(res, "rowinst", (size_t)nrow);
+            InstDesc *d_gdesc = ws<InstDesc>(res, "gdesc", 1);
+            CUPHU_HIP_CHECK(hipMemcpyAsync(d_rowinst, ri.data(), ri.size() * 4, hipMemcpyHostToDevice, st));
+            CUPHU_HIP_CHECK(hipMemcpyAsync(d_gdesc, &d, sizeof(InstDesc), hipMemcpyHostToDevice, st));
+            launch_final<float>(d_gdesc, d_rowinst, nrow, h.P.pl, h.P.out, d.pitch, st);
+            from_plane(h.P.out, cong, d, st);
+        }
+        CUPHU_HIP_CHECK(hipStreamSynchronize(st));
+        return 0;
+    });
+}
+
+int cuphu_test_cg_step(const float *w, const float *dinv, int nrow, int ncol, double alpha, double beta, int rpw,
+                       float *x, float *r, float *p, float *s, float *wv, double *gamma, double *delta) {
+    return guarded([&]() {
+        std::lock_guard<std::recursive_mutex> lk(g_mu);
+        CUPHU_REQUIRE(w && dinv && x && r && p && s && wv && gamma && delta && nrow >= 1 && ncol >= 1, "null argument");
+        CUPHU_REQUIRE(rpw == 2 || rpw == 4 || rpw == 8, "rpw must be 2, 4 or 8");
+        DevRes &res = devres();
+        hipStream_t st = res.stream;
+        HookInst h;
+        hook_setup(h, res, nrow, ncol, rpw, st);
+        const InstDesc &d = h.d;
+        size_t na = (size_t)(nrow - 1) * ncol + (size_t)nrow * (ncol - 1);
+        DevBuf<float> dw(na + 1);
+        if (na) CUPHU_HIP_CHECK(hipMemcpyAsync(dw.get(), w, na * 4, hipMemcpyHostToDevice, st));
+        unsigned gp = (unsigned)ceil_div((size_t)nrow * d.pitch, 256);
+        hipLaunchKernelGGL(k_flat_w_to_planes<float>, dim3(gp), dim3(256), 0, st, (const float *)dw.get(), d,
+                           h.P.pl.w2);
+        CUPHU_HIP_CHECK(hipGetLastError());
+        to_plane(dinv, h.P.pl.dinv, d, st);
+        to_plane(x, h.P.pl.x, d, st);
+        to_plane(r, h.P.pl.r, d, st);
+        to_plane(p, h.P.pl.p, d, st);
+        to_plane(s, h.P.pl.s, d, st);
+        to_plane(wv, h.P.pl.wv, d, st);
+        SolveScal sc{};
+        sc.alpha = alpha;
+        sc.beta = beta;
+        sc.active = 1;
+        CUPHU_HIP_CHECK(hipMemcpyAsync(h.g.scal, &sc, sizeof(sc), hipMemcpyHostToDevice, st));
+        launch_cg<float>(h.g.desc, h.g.blk, h.g.nblk, h.P.pl, h.g.scal, h.g.part, rpw, st);
+        std::vector<double2> parts((size_t)h.g.nblk);
+        CUPHU_HIP_CHECK(hipMemcpyAsync(parts.data(), h.g.part, parts.size() * sizeof(double2), hipMemcpyDeviceToHost, st));
+        from_plane(h.P.pl.x, x, d, st);
+        from_plane(h.P.pl.r, r, d, st);
+        from_plane(h.P.pl.p, p, d, st);
+        from_plane(h.P.pl.s, s, d, st);
+        from_plane(h.P.pl.wv, wv, d, st);
+        CUPHU_HIP_CHECK(hipStreamSynchronize(st));
+        double ga = 0.0, de = 0.0;
+        for (auto &v : parts) { ga += v.x; de += v.y; }
+        *gamma = ga;
+        *delta = de;
+        return 0;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,25 @@
+// util.hpp -- launchers of util_kernels.hip and conncomp_kernels.hip.
+#pragma once
+#include "common.hpp"
+
+namespace cuphu {
+
+void util_wrap_phase(float *d, size_t n, hipStream_t st);
+void phase_from_igram(const float2 *ig, float *ph, size_t n, hipStream_t st);
+void util_phase_diffs(const float *ph, float *rng, float *az, int nrow, int ncol, hipStream_t st);
+// tmp must hold (nrow + 2*kr) * ncol floats.
+void util_boxcar(const float *src, float *dst, float *tmp, int nrow, int ncol, int kr, int kc, hipStream_t st);
+// out: flat row-then-col arcs, 2 shorts per arc (smooth) or 4 (defo).
+void util_costs(const float *ph, const float *corr, const short *weights, int nrow, int ncol, const CostConsts &k,
+                int kperp, int kpar, int defo, short *out, hipStream_t st);
+void util_residues(const float *ph, int nrow, int ncol, signed char *res, hipStream_t st);
+void util_integrate(const float *ph, const short *hflows, const short *vflows, int nrow, int ncol, float *unw,
+                    hipStream_t st);
+
+// Connected components of one image (stride = elements per row of the unw,
+// corr, mask and label arrays).  Returns the number of labels assigned
+// (labels are 1..K).  Scratch is allocated internally.
+int conncomp_run(const float *unw, const float *corr, const unsigned char *mask, int nrow, int ncol, int stride,
+                 double minfrac, long maxncomps, uint32_t *labels, hipStream_t st, float corr_thresh = 0.05f);
+
+}  // namespace cuphu

@@ -1,0 +1,230 @@
+"""Kernel-level parity with the oracle (GPU).
+
+Integer, byte and index results are compared bit for bit; FP32 kernels
+whose operation order is fixed (ingest wrap, diffs, boxcar, weights,
+assemble, both stencils, reweight, congruence, the fused PCG step) are
+compared bit for bit as well.  Stated exceptions: the FP64 atan2 of the
+GPU math library vs the host's may round to a neighbouring float in rare
+cases (<= 1 ulp, counted), SNAPHU sigsq/offset may differ by +-1 where a
+pow() result lands on a truncation boundary (counted), and FP64 reductions
+differ in summation order (relative tolerance 1e-12).
+"""
+
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle as O
+from cuphu_amd import _capi, synth
+from tests import gpu_util as G
+
+pytestmark = pytest.mark.gpu
+TWOPI = 2 * np.pi
+SHAPES = [(1, 7), (5, 1), (2, 2), (37, 301), (64, 256), (130, 257), (300, 70)]
+
+
+def _rand_igram(rng, nrow, ncol):
+    ig = (rng.standard_normal((nrow, ncol)) + 1j * rng.standard_normal((nrow, ncol))).astype(np.complex64)
+    ig.ravel()[::17] = 0
+    ig.ravel()[5::23] = -1.0 + 0j
+    ig.ravel()[7::29] = -1.0 - 1e-30j
+    return ig
+
+
+def _ulps(a, b):
+    return np.abs(a.view(np.int32).astype(np.int64) - b.view(np.int32).astype(np.int64))
+
+
+@pytest.mark.parametrize("shape", SHAPES + [(257, 513)])
+def test_ingest_phase_and_sigsq(gpu_ready, shape):
+    rng = np.random.default_rng(sum(shape))
+    nrow, ncol = shape
+    ig = _rand_igram(rng, nrow, ncol)
+    corr = rng.uniform(0, 1, shape).astype(np.float32)
+    corr.ravel()[::13] = 1.0
+    mask = (rng.uniform(size=shape) > 0.1).astype(np.uint8)
+    for nl in (1.0, 5.0, 23.8):
+        prm = _capi.default_params()
+        prm.nlooks = prm.ncorrlooks = nl
+        ph = np.empty(shape, np.float32)
+        sq = np.empty(G.narcs(nrow, ncol) + 1, np.int16)
+        _capi.check(_capi.lib.cuphu_test_ingest(ig.ctypes.data, corr.ctypes.data, mask.ctypes.data, nrow, ncol,
+                                                C.byref(prm), ph.ctypes.data, sq.ctypes.data), "ingest")
+        ref_ph = O.wrapped_phase(ig, correctly_rounded=True)
+        u = _ulps(ph, ref_ph)
+        assert u.max() <= 1 and (u > 0).sum() <= 1 + ph.size // 100000
+        ref_sq = G.sigsq_from_mask(ph, corr, nl, mask)[:, 1]
+        d = np.abs(sq[:-1].astype(np.int32) - ref_sq)
+        assert d.max() <= 1 and (d > 0).sum() <= 1 + d.size // 100000, f"sigsq mismatches {(d > 0).sum()}"
+
+
+@pytest.mark.parametrize("defo", [False, True])
+def test_build_costs_matches_oracle(gpu_ready, defo):
+    s = synth.make_scene(96, 130, nbumps=3, npatches=1, nlooks=5.0, seed=3, patch=30)
+    from cuphu_amd._ext import _cuphu_ext
+    raw = _cuphu_ext.build_costs(s.igram, s.corr, 5.0, cost="defo" if defo else "smooth")
+    ne = 4 if defo else 2
+    got = raw.view(np.int16).reshape(-1, ne)
+    ph = O.wrapped_phase(s.igram, correctly_rounded=True)
+    ref = O.costs(ph, s.corr, 5.0, defo=defo)
+    d = np.abs(got.astype(np.int32) - ref)
+    print("cost +-1 count:", int((d > 0).sum()), "of", d.size)
+    assert d.max() <= 1 and (d > 0).sum() <= 2
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("rpw", [2, 4, 8])
+def test_laplace_ops_bitexact(gpu_ready, monkeypatch, shape, rpw):
+    monkeypatch.setenv("CUPHU_LAPLACE_RPW", str(rpw))
+    rng = np.random.default_rng(shape[0] * 1000 + shape[1])
+    nrow, ncol = shape
+    na = G.narcs(nrow, ncol)
+    phi = rng.uniform(0, TWOPI, shape).astype(np.float32)
+    sig = rng.integers(1, 300, na).astype(np.int16)
+    if na:
+        sig[::7] = 32000
+        sig[3::11] = 0
+    x = (rng.standard_normal(shape) * 20).astype(np.float32)
+    delta = np.float32(0.0625)
+    outs = {k: np.empty(shape, np.float32) for k in ("b", "dinv", "r", "lx", "cong")}
+    w0 = np.empty(na + 1, np.float32)
+    wrw = np.empty(na + 1, np.float32)
+    _capi.check(_capi.lib.cuphu_test_laplace_ops(
+        phi.ctypes.data, sig.ctypes.data, x.ctypes.data, nrow, ncol, C.c_float(delta), w0.ctypes.data,
+        outs["b"].ctypes.data, outs["dinv"].ctypes.data, outs["r"].ctypes.data, outs["lx"].ctypes.data,
+        wrw.ctypes.data, outs["cong"].ctypes.data), "laplace_ops")
+    ref_w0 = O.arc_weights(sig)
+    np.testing.assert_array_equal(w0[:na], ref_w0)
+    b, diag = O.assemble(ref_w0, phi)
+    np.testing.assert_array_equal(outs["b"], b)
+    np.testing.assert_array_equal(outs["dinv"], (np.float32(1) / diag).astype(np.float32))
+    lx = O.matvec(ref_w0, x)
+    np.testing.assert_array_equal(outs["lx"], lx)
+    np.testing.assert_array_equal(outs["r"], b - lx)
+    np.testing.assert_array_equal(wrw[:na], O.reweight(sig, phi, x, float(delta)))
+    xa = x + (phi.ravel()[0] - x.ravel()[0]) if phi.ravel()[0] != x.ravel()[0] else x
+    np.testing.assert_array_equal(outs["cong"], O.congruence(phi, xa.astype(np.float32)))
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("rpw", [2, 4, 8])
+def test_fused_pcg_step_bitexact(gpu_ready, shape, rpw):
+    rng = np.random.default_rng(shape[0] + 7 * shape[1] + rpw)
+    nrow, ncol = shape
+    na = G.narcs(nrow, ncol)
+    w = rng.uniform(0.001, 0.3, na + 1).astype(np.float32)
+    if na:
+        w[::9] = 0
+    dinv = rng.uniform(0.5, 50, shape).astype(np.float32)
+    vecs = [rng.standard_normal(shape).astype(np.float32) for _ in range(5)]
+    alpha, beta = 0.37, 0.81
+    ref = O.cgcg_step_f32(w[:na], dinv, alpha, beta, *vecs)
+    got = [v.copy() for v in vecs]
+    g, d = C.c_double(), C.c_double()
+    _capi.check(_capi.lib.cuphu_test_cg_step(w.ctypes.data, dinv.ctypes.data, nrow, ncol, alpha, beta, rpw,
+                                             *[v.ctypes.data for v in got], C.byref(g), C.byref(d)), "cg_step")
+    for a, e in zip(got, ref[:5]):
+        np.testing.assert_array_equal(a, e)
+    assert g.value == pytest.approx(ref[5], rel=1e-12, abs=1e-300)
+    assert d.value == pytest.approx(ref[6], rel=1e-12, abs=1e-300)
+
+
+@pytest.mark.parametrize("shape", [(2, 2), (40, 70), (257, 300)])
+def test_wrap_diffs_boxcar_residues(gpu_ready, shape):
+    torch = G.torch_mod()
+    rng = np.random.default_rng(11)
+    raw = rng.uniform(-20, 20, shape).astype(np.float32)
+    nrow, ncol = shape
+    d = G.dev(raw)
+    _capi.check(_capi.lib.cuphu_wrap_phase(G.p(d), raw.size, G.stream()), "wrap")
+    ph = G.host(d)
+    ref = (raw - np.float32(TWOPI) * np.floor(raw / np.float32(TWOPI))).astype(np.float32)
+    np.testing.assert_array_equal(ph, ref)
+    drng = G.empty((nrow, ncol - 1), torch.float32)
+    daz = G.empty((nrow - 1, ncol), torch.float32)
+    _capi.check(_capi.lib.cuphu_phase_diffs(G.p(d), G.p(drng), G.p(daz), nrow, ncol, G.stream()), "diffs")
+    er, ea = O.phase_diffs(ph)
+    np.testing.assert_array_equal(G.host(drng), er)
+    np.testing.assert_array_equal(G.host(daz), ea)
+    for kr, kc in [(3, 3), (1, 4), (0, 0)]:
+        out = G.empty(shape, torch.float32)
+        _capi.check(_capi.lib.cuphu_boxcar2d(G.p(d), G.p(out), nrow, ncol, kr, kc, G.stream()), "boxcar")
+        np.testing.assert_array_equal(G.host(out), O.boxcar2d(ph, kr, kc))
+    dres = G.empty((nrow - 1, ncol - 1), torch.int8)
+    _capi.check(_capi.lib.cuphu_residues_gpu(G.p(d), nrow, ncol, G.p(dres), G.stream()), "residues")
+    np.testing.assert_array_equal(G.host(dres), O.residues(ph))
+
+
+def test_residues_vortex(gpu_ready):
+    torch = G.torch_mod()
+    r, c = np.mgrid[0:64, 0:80].astype(np.float64)
+    ph = np.mod(np.arctan2(r - 30.5, c - 41.5), TWOPI).astype(np.float32)
+    d = G.dev(ph)
+    dres = G.empty((63, 79), torch.int8)
+    _capi.check(_capi.lib.cuphu_residues_gpu(G.p(d), 64, 80, G.p(dres), G.stream()), "residues")
+    res = G.host(dres)
+    assert np.argwhere(res != 0).tolist() == [[30, 41]]
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+@pytest.mark.parametrize("shape", [(23, 31), (100, 300), (515, 260)])
+def test_conncomp_bitexact(gpu_ready, seed, shape):
+    torch = G.torch_mod()
+    rng = np.random.default_rng(seed)
+    unw = np.cumsum(rng.normal(0, 1.3, shape), axis=1).astype(np.float32)
+    corr = rng.uniform(0, 1, shape).astype(np.float32)
+    corr[rng.uniform(size=shape) < 0.1] = 0.01
+    mask = (rng.uniform(size=shape) > 0.05).astype(np.uint8)
+    for minfrac, maxn in [(0.0, 100000), (0.001, 32), (0.01, 2)]:
+        prm = _capi.default_params()
+        prm.minconncompfrac = minfrac
+        prm.maxncomps = maxn
+        lab = G.empty(shape, torch.int32)
+        _capi.check(_capi.lib.cuphu_conncomp_gpu(G.p(G.dev(unw)), G.p(G.dev(corr)), G.p(G.dev(mask)), shape[0],
+                                                 shape[1], None, None, C.byref(prm), 0, G.p(lab)), "conncomp")
+        np.testing.assert_array_equal(G.host(lab).view(np.uint32), O.conncomp(unw, corr, mask, minfrac, maxn))
+
+
+def test_conncomp_single_giant_component(gpu_ready):
+    torch = G.torch_mod()
+    shape = (1100, 1300)
+    unw = np.zeros(shape, np.float32)
+    corr = np.ones(shape, np.float32)
+    prm = _capi.default_params()
+    lab = G.empty(shape, torch.int32)
+    _capi.check(_capi.lib.cuphu_conncomp_gpu(G.p(G.dev(unw)), G.p(G.dev(corr)), None, shape[0], shape[1], None,
+                                             None, C.byref(prm), 0, G.p(lab)), "conncomp")
+    assert (G.host(lab) == 1).all()
+
+
+def test_regrow_conncomp_reference_semantics(gpu_ready):
+    import cuphu_amd as cuphu
+    unw = np.zeros((20, 30), np.float32)
+    unw[:, 15:] += 10.0
+    corr = np.full((20, 30), 0.8, np.float32)
+    out = cuphu.regrow_conncomp(unw, corr, cost_thresh=0.1)
+    assert sorted(set(np.unique(out)) - {0}) == [1, 2] and out.dtype == np.uint32
+    corr2 = corr.copy()
+    corr2[:, 5] = 0.09          # below cost_thresh 0.1 but above the 0.05 floor
+    o2 = cuphu.regrow_conncomp(np.zeros((20, 30), np.float32), corr2, cost_thresh=0.1, min_conncomp_frac=0.0)
+    assert (o2[:, 5] == 0).all() and o2[0, 0] != o2[0, 10]
+
+
+def test_integrate_phase(gpu_ready):
+    torch = G.torch_mod()
+    rng = np.random.default_rng(13)
+    nrow, ncol = 300, 700
+    u = np.cumsum(np.cumsum(rng.normal(0, 1.6, (nrow, ncol)), 0), 1) * 0.05
+    phi = np.mod(u, TWOPI).astype(np.float32)
+    p64 = phi.astype(np.float64)
+    hfl = np.rint(((p64[1:] - p64[:-1] - TWOPI * np.rint((p64[1:] - p64[:-1]) / TWOPI)) - np.diff(u, axis=0)) / TWOPI)
+    vfl = np.rint((np.diff(u, axis=1) - (np.diff(p64, axis=1) - TWOPI * np.rint(np.diff(p64, axis=1) / TWOPI))) / TWOPI)
+    hfl, vfl = hfl.astype(np.int16), vfl.astype(np.int16)
+    out = G.empty((nrow, ncol), torch.float32)
+    _capi.check(_capi.lib.cuphu_integrate_phase_gpu(G.p(G.dev(phi)), G.p(G.dev(hfl)), G.p(G.dev(vfl)), nrow, ncol,
+                                                    0, G.p(out)), "integrate")
+    got = G.host(out)
+    ref = O.integrate_phase(phi, hfl, vfl)
+    assert _ulps(got, ref).max() <= 1
+    np.testing.assert_allclose(got, u - u[0, 0] + p64[0, 0], atol=1e-2)

@@ -1,0 +1,258 @@
+"""Solver-level parity (GPU): PCG/IRLS against the oracle, the reference's
+unwrap() semantics with init='laplace', batching and tiling properties.
+
+Tolerances: the plain solver runs the oracle's FP32 operation sequence and
+differs only in FP64 reduction order; the batched solver is the
+single-reduction recurrence of the same Krylov method.  Both are compared
+to the FP64 oracle by the 2-pi ambiguity count k = rint((u - phi)/2pi):
+>= 99.9 % of pixels must agree, and the FP64 modes must agree with the FP64
+oracle to 1e-6 rad on converged solves.
+"""
+
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import cuphu_amd as cuphu
+import oracle as O
+from cuphu_amd import _capi, synth
+from tests import gpu_util as G
+
+pytestmark = pytest.mark.gpu
+TWOPI = 2 * np.pi
+
+
+def amb(u, phi):
+    return np.rint((u.astype(np.float64) - phi.astype(np.float64)) / TWOPI)
+
+
+def _oracle_solve(s, f64=True, mask=None, **opts):
+    ph = O.wrapped_phase(s.igram, correctly_rounded=True)
+    sig = G.sigsq_from_mask(ph, s.corr, s.nlooks, mask)[:, 1]
+    fn = O.pcg_irls_f64 if f64 else O.pcg_irls_f32
+    u, st = fn(sig, ph, **opts)
+    return u, st, ph
+
+
+SMALL = [synth.make_scene(70, 90, nbumps=2, npatches=1, nlooks=10.0, seed=21, patch=15),
+         synth.make_scene(96, 64, nbumps=3, npatches=0, nlooks=5.0, seed=22)]
+
+
+@pytest.mark.parametrize("k", [0, 1])
+def test_plain_solver_matches_fp32_oracle(gpu_ready, k):
+    s = SMALL[k]
+    opts = dict(n_irls=3, max_iter=300, tol=1e-4)
+    o = _capi.default_options(solver="plain", **opts)
+    u, cc, st = G.batch_solve([s], o)
+    uo, sto, ph = _oracle_solve(s, f64=False, **opts)
+    agree = (amb(u[0], ph) == amb(uo, ph)).mean()
+    print("plain vs oracle32 agreement", agree, st[0]["iters"], sto["iters"])
+    assert agree >= 0.999
+    assert np.abs(u[0] - uo).max() < 1e-2
+
+
+@pytest.mark.parametrize("solver", ["plain", "batched"])
+def test_fp64_modes_match_fp64_oracle(gpu_ready, solver):
+    s = SMALL[0]
+    opts = dict(n_irls=2, max_iter=4000, tol=1e-12)
+    o = _capi.default_options(solver=solver, precision=64, **opts)
+    u, _, st = G.batch_solve([s], o)
+    uo, sto, ph = _oracle_solve(s, f64=True, **opts)
+    assert all(x == 0 for x in st[0]["pass_status"]), st[0]
+    np.testing.assert_allclose(u[0], uo, atol=1e-5)
+
+
+@pytest.mark.parametrize("k", [0, 1])
+def test_batched_fp32_agrees_with_fp64_oracle(gpu_ready, k):
+    s = SMALL[k]
+    opts = dict(n_irls=5, max_iter=1000, tol=1e-6)
+    u, cc, st = G.batch_solve([s], _capi.default_options(solver="batched", **opts))
+    uo, sto, ph = _oracle_solve(s, f64=True, **opts)
+    agree = (amb(u[0], ph) == amb(uo, ph)).mean()
+    print("batched vs oracle64", agree, st[0]["iters"], sto["iters"], np.abs(u[0] - uo).max())
+    assert agree >= 0.999
+
+
+def test_batch_equals_solo_bitwise(gpu_ready):
+    scenes = [synth.make_scene(40 + 13 * i, 50 + 29 * i, nbumps=2, npatches=0, nlooks=[1, 5, 10, 20][i % 4],
+                               seed=100 + i) for i in range(5)]
+    o = _capi.default_options(n_irls=3, max_iter=200, tol=1e-5)
+    ub, cb, sb = G.batch_solve(scenes, o)
+    for i, s in enumerate(scenes):
+        us, cs, ss = G.batch_solve([s], o)
+        np.testing.assert_array_equal(ub[i], us[0])
+        np.testing.assert_array_equal(cb[i], cs[0])
+        assert sb[i]["iters"] == ss[0]["iters"]
+    og = _capi.default_options(n_irls=3, max_iter=200, tol=1e-5, group_bytes_mb=1)
+    ug, cg, sg = G.batch_solve(scenes, og)
+    for i in range(len(scenes)):
+        np.testing.assert_array_equal(ug[i], ub[i])
+
+
+def test_graph_and_eager_launches_identical(gpu_ready):
+    s = SMALL[1]
+    u1, c1, s1 = G.batch_solve([s], _capi.default_options(n_irls=2, max_iter=150, use_graph=1))
+    u2, c2, s2 = G.batch_solve([s], _capi.default_options(n_irls=2, max_iter=150, use_graph=0))
+    u3, c3, s3 = G.batch_solve([s], _capi.default_options(n_irls=2, max_iter=150, use_graph=1))
+    np.testing.assert_array_equal(u1[0], u2[0])
+    np.testing.assert_array_equal(u1[0], u3[0])
+    assert s1[0]["iters"] == s2[0]["iters"] == s3[0]["iters"]
+
+
+def test_conncomp_of_batched_output_matches_oracle(gpu_ready):
+    s = SMALL[0]
+    mask = np.ones(s.corr.shape, np.uint8)
+    mask[10:20, 30:45] = 0
+    u, cc, _ = G.batch_solve([s], _capi.default_options(n_irls=2, max_iter=200), masks=[mask])
+    prm = _capi.default_params()
+    np.testing.assert_array_equal(cc[0], O.conncomp(u[0], s.corr, mask, prm.minconncompfrac, prm.maxncomps))
+
+
+def test_all_masked_instance(gpu_ready):
+    s = SMALL[1]
+    mask = np.zeros(s.corr.shape, np.uint8)
+    u, cc, st = G.batch_solve([s], _capi.default_options(n_irls=2), masks=[mask])
+    ph = O.wrapped_phase(s.igram, correctly_rounded=True)
+    assert st[0]["pass_status"] == [3, 3, 3]
+    exp = ph + np.float32(TWOPI) * np.rint((ph[0, 0] - ph) / np.float32(TWOPI))
+    np.testing.assert_allclose(u[0], exp, atol=1e-5)
+    assert (cc[0] == 0).all()
+
+
+def test_phase_input_instance_equals_igram_instance(gpu_ready):
+    s = SMALL[1]
+    ph = O.wrapped_phase(s.igram, correctly_rounded=True)
+    o = _capi.default_options(n_irls=1, max_iter=100)
+    u1, _, _ = G.batch_solve([s], o)
+    u2, _, _ = G.batch_solve([s], o, phases=[ph])
+    np.testing.assert_array_equal(u1[0], u2[0])
+
+
+# ── unwrap() with init='laplace' (reference test semantics) ─────────────────
+def test_ramp_256_closed_form(gpu_ready):
+    s = synth.ramp_256()
+    unw, cc = cuphu.unwrap(s.igram, s.corr, 1.0, init="laplace", ntiles=(1, 1))
+    assert np.abs(unw - s.truth).max() < 1e-3
+    assert (cc == 1).all()
+
+
+def _ramp6090():
+    unw_true = 0.05 * np.arange(90)[None, :] * np.ones((60, 1)) + 10.0
+    return np.exp(1j * unw_true).astype(np.complex64), np.full((60, 90), 0.8, np.float32)
+
+
+def _wrapped(ig):
+    p = np.angle(ig)
+    return np.where(p < 0, p + 2 * np.pi, p)
+
+
+def test_reference_pixel_default_and_none(gpu_ready):
+    ig, corr = _ramp6090()
+    unw, _ = cuphu.unwrap(ig, corr, nlooks=10.0, init="laplace")
+    assert unw[0, 0] == pytest.approx(_wrapped(ig)[0, 0], abs=1e-4)
+    raw, _ = cuphu.unwrap(ig, corr, nlooks=10.0, init="laplace", reference_pixel=None)
+    n = (unw - raw)[0, 0] / (2 * np.pi)
+    assert abs(n - round(n)) < 1e-3
+    rp, rc = 30, 45
+    u2, _ = cuphu.unwrap(ig, corr, nlooks=10.0, init="laplace", reference_pixel=(rp, rc))
+    assert u2[rp, rc] == pytest.approx(_wrapped(ig)[rp, rc], abs=1e-4)
+    mask = np.ones((60, 90), np.uint8)
+    mask[0, 0] = 0
+    u3, cc3 = cuphu.unwrap(ig, corr, nlooks=10.0, init="laplace", mask=mask)
+    assert np.isfinite(u3).all() and cc3[0, 0] == 0
+
+
+def test_outputs_shape_dtype_finite_and_prealloc(gpu_ready):
+    r = np.arange(50, dtype=np.float32)
+    c = np.arange(60, dtype=np.float32)
+    ig = np.exp(1j * (0.1 * r[:, None] + 0.2 * c[None, :])).astype(np.complex64)
+    corr = np.full((50, 60), 0.9, np.float32)
+    ubuf = np.zeros((50, 60), np.float32)
+    cbuf = np.zeros((50, 60), np.uint32)
+    ou, oc = cuphu.unwrap(ig, corr, 10.0, init="laplace", unw=ubuf, conncomp=cbuf)
+    assert ou is ubuf and oc is cbuf and np.isfinite(ubuf).all()
+    assert ubuf.dtype == np.float32 and cbuf.dtype == np.uint32
+    wrapped_back = np.angle(np.exp(1j * ubuf))
+    d = np.abs(np.angle(np.exp(1j * (wrapped_back - np.angle(ig)))))
+    assert float(np.percentile(d, 99)) < 0.1
+
+
+@pytest.mark.parametrize("ntiles", [(1, 1), (2, 2), (2, 3)])
+def test_tiled_matches_single(gpu_ready, ntiles):
+    r = np.arange(50, dtype=np.float32)
+    c = np.arange(60, dtype=np.float32)
+    ig = np.exp(1j * (0.1 * r[:, None] + 0.2 * c[None, :])).astype(np.complex64)
+    corr = np.full((50, 60), 0.9, np.float32)
+    ref, _ = cuphu.unwrap(ig, corr, 10.0, init="laplace", ntiles=(1, 1))
+    til, _ = cuphu.unwrap(ig, corr, 10.0, init="laplace", ntiles=ntiles, tile_overlap=4)
+    assert float(np.percentile(np.abs(ref - til), 95)) < 0.5
+
+
+def test_mask_pad_distance_restores_original_mask(gpu_ready):
+    unw_true = 0.05 * np.arange(100)[None, :] * np.ones((100, 1))
+    ig = np.exp(1j * unw_true).astype(np.complex64)
+    corr = np.full((100, 100), 0.8, np.float32)
+    mask = np.ones((100, 100), np.uint8)
+    mask[:, 40:60] = 0
+    for pad in [0, 5, 20, 50]:
+        _, cc = cuphu.unwrap(ig, corr, nlooks=10.0, init="laplace", mask=mask, mask_pad_distance=pad)
+        assert (cc[:, 40:60] == 0).all() and (cc[:, :40] != 0).any()
+    a = cuphu.unwrap(ig, corr, nlooks=10.0, init="laplace", mask=mask)
+    b = cuphu.unwrap(ig, corr, nlooks=10.0, init="laplace", mask=mask, mask_pad_distance=0)
+    np.testing.assert_array_equal(a[0], b[0])
+    np.testing.assert_array_equal(a[1], b[1])
+
+
+def test_nan_input_does_not_crash(gpu_ready):
+    ig = np.full((50, 60), np.nan + 0j, dtype=np.complex64)
+    unw, _ = cuphu.unwrap(ig, np.full((50, 60), 0.9, np.float32), nlooks=10.0, init="laplace")
+    assert np.isfinite(unw).all()
+
+
+def test_decorrelated_strip_gives_two_components(gpu_ready):
+    corr = np.full((20, 30), 0.8, np.float32)
+    corr[:, 14:16] = 0.0
+    unw = (0.05 * np.arange(30)[None, :] * np.ones((20, 1))).astype(np.float32)
+    ig = np.exp(1j * unw).astype(np.complex64)
+    _, cc = cuphu.unwrap(ig, corr, nlooks=10.0, cost="smooth", init="laplace")
+    counts = np.bincount(cc.ravel())[1:]
+    assert sorted(counts.tolist()) == [20 * 14, 20 * 14]
+
+
+def test_scene_api_equals_batch_plus_oracle_stitch(gpu_ready):
+    torch = G.torch_mod()
+    s = synth.make_scene(300, 410, nbumps=3, npatches=1, nlooks=10.0, seed=77, patch=40)
+    nrow, ncol = s.corr.shape
+    o = _capi.default_options(n_irls=2, max_iter=150)
+    ts, ov = 160, 32
+    rf, rs = O.tile_axis(nrow, ts, ov)
+    cf, cs = O.tile_axis(ncol, ts, ov)
+    tiles = []
+    for a in range(len(rf)):
+        for b in range(len(cf)):
+            sl = (slice(rf[a], rf[a] + rs[a]), slice(cf[b], cf[b] + cs[b]))
+            tiles.append(synth.Scene(np.ascontiguousarray(s.igram[sl]), np.ascontiguousarray(s.corr[sl]),
+                                     s.truth[sl], s.nlooks, 0))
+    tu, tc, _ = G.batch_solve(tiles, o)
+    eu, ec, _ = O.stitch(nrow, ncol, rf, rs, cf, cs, ov, ov, tu, tc, [t.corr for t in tiles], None)
+    dig = G.dev(s.igram.view(np.float32).reshape(nrow, 2 * ncol))
+    dc = G.dev(s.corr)
+    du = G.empty((nrow, ncol), torch.float32)
+    dl = G.empty((nrow, ncol), torch.int32)
+    nt = C.c_int()
+    prm = _capi.default_params()
+    _capi.check(_capi.lib.cuphu_unwrap_scene_gpu(G.p(dig), G.p(dc), None, None, nrow, ncol, 10.0, 0, C.byref(prm),
+                                                 ts, ov, 0, 0, C.byref(o), G.p(du), G.p(dl), None, C.byref(nt),
+                                                 G.stream()), "scene")
+    assert nt.value == len(rf) * len(cf)
+    np.testing.assert_array_equal(G.host(du), eu)
+    np.testing.assert_array_equal(G.host(dl).view(np.uint32), ec)
+
+
+def test_repeat_runs_are_deterministic(gpu_ready):
+    s = SMALL[0]
+    o = _capi.default_options(n_irls=2, max_iter=300)
+    a, _, _ = G.batch_solve([s], o)
+    b, _, _ = G.batch_solve([s], o)
+    np.testing.assert_array_equal(a[0], b[0])

@@ -1,0 +1,564 @@
+"""Pins the CPU oracle (CPU only).
+
+Each oracle function is checked against something that does not restate
+it: closed forms, dense numpy linear algebra, scipy, brute force, or the
+reference's own known answers.  A plausible slip (dropped term, wrong sign
+or index, transposed operand) fails at least one check here.
+"""
+
+import numpy as np
+import pytest
+from scipy import ndimage
+from scipy.sparse import coo_matrix
+from scipy.sparse.csgraph import connected_components
+
+import oracle as O
+from cuphu_amd import synth
+
+TWOPI = 2.0 * np.pi
+
+
+def arc_list(nrow, ncol):
+    """(tail, head) pixel indices of the flat arc layout: row arcs first."""
+    idx = np.arange(nrow * ncol).reshape(nrow, ncol)
+    t = np.concatenate([idx[:-1, :].ravel(), idx[:, :-1].ravel()])
+    h = np.concatenate([idx[1:, :].ravel(), idx[:, 1:].ravel()])
+    return t, h
+
+
+def dense_laplacian(w, nrow, ncol, eps=1e-12):
+    t, h = arc_list(nrow, ncol)
+    n = nrow * ncol
+    L = np.zeros((n, n))
+    for a, (i, j) in enumerate(zip(t, h)):
+        L[i, i] += w[a]
+        L[j, j] += w[a]
+        L[i, j] -= w[a]
+        L[j, i] -= w[a]
+    return L + eps * np.eye(n)
+
+
+def wrap_pi(x):
+    return x - TWOPI * np.rint(x / TWOPI)
+
+
+# ── a1 wrapped phase ────────────────────────────────────────────────────────
+def test_wrapped_phase_closed_form_and_range():
+    th = np.linspace(-np.pi + 1e-3, np.pi, 4001)
+    ig = np.exp(1j * th).astype(np.complex64)[None, :]
+    phi = O.wrapped_phase(ig)
+    assert phi.min() >= 0.0 and phi.max() < TWOPI
+    expect = np.mod(np.angle(ig.astype(np.complex128)), TWOPI)
+    d = np.abs(phi - expect)
+    d = np.minimum(d, TWOPI - d)
+    assert d.max() < 1e-6
+
+
+def test_wrapped_phase_library_atan2f_within_one_ulp():
+    rng = np.random.default_rng(0)
+    ig = (rng.standard_normal((64, 64)) + 1j * rng.standard_normal((64, 64))).astype(np.complex64)
+    a = O.wrapped_phase(ig, correctly_rounded=True)
+    b = O.wrapped_phase(ig, correctly_rounded=False)
+    ulps = np.abs(a.view(np.int32).astype(np.int64) - b.view(np.int32).astype(np.int64))
+    assert ulps.max() <= 1
+
+
+def test_wrapped_phase_special_values():
+    ig = np.array([[0 + 0j, -1 - 1e-30j, 1 + 0j, -1e-30 + 1j]], np.complex64)
+    phi = O.wrapped_phase(ig)
+    assert phi[0, 0] == 0.0
+    assert phi[0, 2] == 0.0
+    assert abs(phi[0, 3] - np.pi / 2) < 1e-6
+    # a hair below the negative real axis wraps to (just under) pi, not -pi
+    assert 0.0 <= phi[0, 1] < TWOPI
+
+
+# ── a2 wrapped differences ──────────────────────────────────────────────────
+def test_phase_diffs_signs_and_values():
+    nrow, ncol = 9, 11
+    r, c = np.mgrid[0:nrow, 0:ncol].astype(np.float64)
+    u = 0.3 * c - 0.7 * r + 5.0        # row slope -0.7 rad/px, col slope 0.3
+    phi = np.mod(u, TWOPI).astype(np.float32)
+    rng, az = O.phase_diffs(phi)
+    assert rng.shape == (nrow, ncol - 1) and az.shape == (nrow - 1, ncol)
+    np.testing.assert_allclose(rng, 0.3 / TWOPI, atol=2e-6)        # right - left
+    np.testing.assert_allclose(az, 0.7 / TWOPI, atol=2e-6)         # top - bottom
+    big = np.mod(2.9 * c, TWOPI).astype(np.float32)               # 2.9 rad/px < pi
+    rng2, _ = O.phase_diffs(big)
+    np.testing.assert_allclose(rng2, 2.9 / TWOPI, atol=3e-6)
+    assert rng2.min() >= -0.5 and rng2.max() < 0.5
+
+
+# ── a4 boxcar ───────────────────────────────────────────────────────────────
+@pytest.mark.parametrize("shape,kr,kc", [((40, 53), 3, 3), ((17, 9), 1, 4), ((30, 30), 0, 2), ((6, 5), 3, 3)])
+def test_boxcar_matches_scipy_mirror(shape, kr, kc):
+    rng = np.random.default_rng(1)
+    src = rng.standard_normal(shape).astype(np.float32)
+    got = O.boxcar2d(src, kr, kc)
+    ref = ndimage.uniform_filter(src.astype(np.float64), size=(2 * kr + 1, 2 * kc + 1), mode="mirror")
+    np.testing.assert_allclose(got, ref, atol=2e-6)
+
+
+# ── a5 costs ────────────────────────────────────────────────────────────────
+def _flat_phase(nrow, ncol):
+    return np.zeros((nrow, ncol), np.float32)
+
+
+def test_cost_rho_zero_and_full_coherence():
+    phi = _flat_phase(8, 9)
+    # nlooks = 1: threshold 1.2*(1.3+0.14) = 1.728 > any rho -> rho = 0 -> pow = 1
+    c = O.costs(phi, np.full((8, 9), 0.9, np.float32), 1.0)
+    assert (c[:, 1] == int((1 / 6 + 0.05) * 200 * 200 / 100)).all()   # 86
+    # nlooks = 10, corr = 1: rho = 1 -> (1-rho)^p = 0 -> sigsq = 0.05*200^2/100 = 20
+    c = O.costs(phi, np.ones((8, 9), np.float32), 10.0)
+    assert (c[:, 1] == 20).all()
+
+
+@pytest.mark.parametrize("cstd1,rho,expect", [(0.5, 0.5, 53), (1.0, 0.5, 36), (0.5, 0.75, 36)])
+def test_cost_power_law_with_chosen_exponent(cstd1, rho, expect):
+    # cstd2 = cstd3 = 0 makes rhopow = 2*cstd1 (1 or 2): sigsq = ((1-rho)^p/6 + 0.05)*400, truncated
+    phi = _flat_phase(4, 4)
+    corr = np.full((4, 4), rho, np.float32)
+    c = O.costs(phi, corr, 10.0, cstd1=cstd1, cstd2=0.0, cstd3=0.0)
+    p = 2 * cstd1
+    assert int(((1 - rho) ** p / 6 + 0.05) * 400) == expect
+    assert (c[:, 1] == expect).all()
+
+
+def test_cost_threshold_masked_clamp_floor():
+    phi = _flat_phase(5, 6)
+    nl = 10.0
+    thresh = 1.2 * (1.3 / nl + 0.14)
+    lo = np.full((5, 6), np.float32(thresh) - np.float32(1e-3), np.float32)
+    assert (O.costs(phi, lo, nl)[:, 1] == 86).all()        # below threshold -> rho = 0
+    w = np.ones(O.narcs(5, 6), np.int16)
+    w[3] = 0
+    c = O.costs(phi, np.full((5, 6), 0.8, np.float32), nl, weights=w)
+    assert tuple(c[3]) == (16000, 32000)
+    cd = O.costs(phi, np.full((5, 6), 0.8, np.float32), nl, weights=w, defo=True)
+    assert tuple(cd[3]) == (16000, 32000, 32000, 0)
+    big = O.costs(phi, np.full((5, 6), 0.5, np.float32), nl, costscale=1e-4)
+    assert (big[:, 1] == 32767).all()                       # short clamp
+    tiny = O.costs(phi, np.full((5, 6), 0.5, np.float32), nl, costscale=1e6)
+    assert (tiny[:, 1] == 1).all()                          # sigsqshortmin floor
+
+
+def test_cost_offsets_truncate_smooth_round_defo():
+    # one arc with dpsi = -0.0135 cycles and zero smoothed gradient except
+    # itself: use a 1x2 image so the boxcar averages a single sample.
+    nsc = 200.0
+    d = -0.0135
+    phi = np.array([[0.0, np.mod(d * TWOPI, TWOPI)]], np.float32)
+    corr = np.full((1, 2), 0.9, np.float32)
+    rng, _ = O.phase_diffs(phi)
+    dp = float(rng[0, 0])
+    # rho = 0.9 > threshold at nlooks 10 -> offset = nsc*(dp - avg) with avg = dp -> 0
+    cs = O.costs(phi, corr, 10.0)
+    assert cs[0, 0] == 0
+    # rho = 0 (nlooks 1) -> offset = nsc*(dp - dp/2) = nsc*dp/2 = -1.35
+    cs = O.costs(phi, corr, 1.0)
+    cd = O.costs(phi, corr, 1.0, defo=True)
+    v = nsc * (dp - 0.5 * dp)
+    assert cs[0, 0] == int(v) == -1                          # truncation toward zero
+    assert cd[0, 0] == int(np.rint(v)) == -1
+    assert cd[0, 2] == 240 and cd[0, 3] == int(np.sqrt(240 * cd[0, 1]))
+    phi2 = np.array([[0.0, np.mod(-0.02 * TWOPI, TWOPI)]], np.float32)
+    v2 = nsc * 0.5 * float(O.phase_diffs(phi2)[0][0, 0])
+    assert O.costs(phi2, corr, 1.0)[0, 0] == int(v2) == -1
+    assert O.costs(phi2, corr, 1.0, defo=True)[0, 0] == int(np.rint(v2)) == -2
+
+
+def test_cost_windows_are_transposed_for_azimuth():
+    rng0 = np.random.default_rng(3)
+    phi = rng0.uniform(0, TWOPI, (12, 15)).astype(np.float32)
+    corr = np.full((12, 15), 0.9, np.float32)
+    kperp, kpar = 3, 7
+    c = O.costs(phi, corr, 1.0, kperp=kperp, kpar=kpar)   # rho = 0: offset = nsc*(dp - avg/2)
+    r_, a_ = O.phase_diffs(phi)
+    ar = ndimage.uniform_filter(r_.astype(np.float64), size=(2 * (kperp // 2) + 1, 2 * (kpar // 2) + 1), mode="mirror")
+    aa = ndimage.uniform_filter(a_.astype(np.float64), size=(2 * (kpar // 2) + 1, 2 * (kperp // 2) + 1), mode="mirror")
+    nra = 11 * 15
+    exp_row = np.trunc(200 * (a_.astype(np.float64) - 0.5 * aa)).ravel()
+    exp_col = np.trunc(200 * (r_.astype(np.float64) - 0.5 * ar)).ravel()
+    assert np.abs(c[:nra, 0] - exp_row).max() <= 1
+    assert np.abs(c[nra:, 0] - exp_col).max() <= 1
+    assert (c[:nra, 0] == exp_row).mean() > 0.99 and (c[nra:, 0] == exp_col).mean() > 0.99
+
+
+# ── a3 residues ─────────────────────────────────────────────────────────────
+def test_residue_vortex_and_conservative_field():
+    nrow, ncol = 24, 30
+    r, c = np.mgrid[0:nrow, 0:ncol].astype(np.float64)
+    vort = np.mod(np.arctan2(r - 10.5, c - 12.5), TWOPI).astype(np.float32)
+    res = O.residues(vort)
+    nz = np.argwhere(res != 0)
+    assert nz.tolist() == [[10, 12]]
+    assert abs(int(res[10, 12])) == 1
+    anti = np.mod(-np.arctan2(r - 10.5, c - 12.5), TWOPI).astype(np.float32)
+    assert O.residues(anti)[10, 12] == -res[10, 12]
+    smooth = np.mod(0.4 * c + 0.2 * r + 0.5 * np.sin(r / 3), TWOPI).astype(np.float32)
+    assert not O.residues(smooth).any()
+
+
+def test_residue_equals_minus_curl_of_integer_flows():
+    rng = np.random.default_rng(5)
+    u = np.cumsum(np.cumsum(rng.normal(0, 1.4, (20, 25)), 0), 1) * 0.3
+    phi = np.mod(u, TWOPI).astype(np.float32)
+    rngd, az = O.phase_diffs(phi)
+    phi64 = phi.astype(np.float64)
+    # integer flows: how many cycles the true gradient differs from the wrapped one
+    n_r = np.rint((np.diff(phi64, axis=1) / TWOPI) - rngd)         # (right - left) arcs
+    n_a = np.rint(((phi64[:-1] - phi64[1:]) / TWOPI) - az)          # (top - bottom) arcs
+    curl = (n_r[:-1, :] - n_a[:, 1:] - n_r[1:, :] + n_a[:, :-1])
+    np.testing.assert_array_equal(O.residues(phi), -curl.astype(np.int8))
+
+
+# ── a7/a8 assemble + matvec ─────────────────────────────────────────────────
+def test_assemble_and_matvec_match_dense_laplacian():
+    rng = np.random.default_rng(7)
+    nrow, ncol = 7, 9
+    na = O.narcs(nrow, ncol)
+    w = rng.uniform(0.01, 0.2, na).astype(np.float32)
+    w[::7] = 0.0
+    phi = rng.uniform(0, TWOPI, (nrow, ncol)).astype(np.float32)
+    x = rng.standard_normal((nrow, ncol)).astype(np.float32)
+    L = dense_laplacian(w.astype(np.float64), nrow, ncol)
+    y = O.matvec(w, x)
+    np.testing.assert_allclose(y.ravel(), L @ x.ravel().astype(np.float64), rtol=1e-5, atol=1e-6)
+    b, diag = O.assemble(w, phi)
+    t, h = arc_list(nrow, ncol)
+    g = wrap_pi(phi.ravel()[h].astype(np.float64) - phi.ravel()[t])
+    bb = np.zeros(nrow * ncol)
+    np.add.at(bb, h, w * g)
+    np.add.at(bb, t, -w * g)
+    np.testing.assert_allclose(b.ravel(), bb, rtol=1e-5, atol=1e-6)
+    np.testing.assert_allclose(diag.ravel(), np.diag(L), rtol=1e-6)
+    assert abs(float(b.astype(np.float64).sum())) < 1e-4
+
+
+def test_reweight_indices_and_formula():
+    rng = np.random.default_rng(8)
+    nrow, ncol = 6, 8
+    na = O.narcs(nrow, ncol)
+    sig = rng.integers(1, 200, na).astype(np.int16)
+    sig[::5] = 32000
+    phi = rng.uniform(0, TWOPI, (nrow, ncol)).astype(np.float32)
+    u = (phi + TWOPI * rng.integers(-2, 3, (nrow, ncol))).astype(np.float32) + rng.normal(0, 0.2, (nrow, ncol)).astype(np.float32)
+    w = O.reweight(sig, phi, u, 0.125)
+    t, h = arc_list(nrow, ncol)
+    g = wrap_pi(phi.ravel()[h].astype(np.float64) - phi.ravel()[t])
+    e = (u.ravel()[h].astype(np.float64) - u.ravel()[t]) - g
+    w0 = np.where(sig >= 32000, 0.0, 1.0 / sig)
+    np.testing.assert_allclose(w, w0 / np.sqrt(e * e + 0.125 ** 2), rtol=2e-6)
+
+
+# ── a10 PCG/IRLS ────────────────────────────────────────────────────────────
+def _anchor_congruence(u, phi):
+    u = u + (float(phi.ravel()[0]) - u.ravel()[0])
+    return phi + TWOPI * np.rint((u - phi) / TWOPI)
+
+
+def test_pcg_single_solve_matches_dense_solution():
+    s = synth.make_scene(14, 17, nbumps=2, npatches=0, nlooks=5.0, seed=11)
+    phi = O.wrapped_phase(s.igram)
+    sig = O.costs(phi, s.corr, 5.0)[:, 1]
+    u, st = O.pcg_irls_f64(sig, phi, n_irls=0, max_iter=5000, tol=1e-13)
+    assert st["status"][0] == 0
+    w = np.where((sig <= 0) | (sig >= 32000), 0.0, 1.0 / sig.astype(np.float64))
+    L = dense_laplacian(w, 14, 17)
+    t, h = arc_list(14, 17)
+    g = wrap_pi(phi.ravel()[h].astype(np.float64) - phi.ravel()[t])
+    b = np.zeros(14 * 17)
+    np.add.at(b, h, w * g)
+    np.add.at(b, t, -w * g)
+    x = np.linalg.lstsq(L, b, rcond=None)[0].reshape(14, 17)
+    ref = _anchor_congruence(x, phi.astype(np.float64))
+    np.testing.assert_allclose(u, ref, atol=1e-6)
+
+
+def test_irls_schedule_matches_dense_irls():
+    s = synth.make_scene(12, 13, nbumps=3, npatches=0, nlooks=10.0, seed=12)
+    phi = O.wrapped_phase(s.igram).astype(np.float64)
+    sig = O.costs(phi.astype(np.float32), s.corr, 10.0)[:, 1]
+    n_irls, dmax, dmin = 3, 0.5, 0.1
+    u, _ = O.pcg_irls_f64(sig, phi.astype(np.float32), n_irls=n_irls, delta_max=dmax, delta_min=dmin,
+                          max_iter=5000, tol=1e-14)
+    t, h = arc_list(12, 13)
+    g = wrap_pi(phi.ravel()[h] - phi.ravel()[t])
+    w0 = np.where((sig <= 0) | (sig >= 32000), 0.0, 1.0 / sig.astype(np.float64))
+    w = w0.copy()
+    x = np.zeros(12 * 13)
+    for outer in range(n_irls + 1):
+        L = dense_laplacian(w, 12, 13)
+        b = np.zeros(12 * 13)
+        np.add.at(b, h, w * g)
+        np.add.at(b, t, -w * g)
+        x = np.linalg.solve(L, b)
+        if outer < n_irls:
+            d = max(dmin, dmax * 0.5 ** outer)
+            e = (x[h] - x[t]) - g
+            w = w0 / np.sqrt(e * e + d * d)
+    ref = _anchor_congruence(x.reshape(12, 13), phi)
+    np.testing.assert_allclose(u, ref, atol=1e-5)
+
+
+def test_ramp_closed_form_both_precisions():
+    s = synth.ramp_256()
+    phi = O.wrapped_phase(s.igram)
+    sig = O.costs(phi, s.corr, 1.0)[:, 1]
+    assert (sig == 86).all()
+    u64, _ = O.pcg_irls_f64(sig, phi, n_irls=0, max_iter=3000, tol=1e-10)
+    assert np.abs(u64 - s.truth).max() < 1e-5
+    amb = np.rint((u64 - phi) / TWOPI)
+    np.testing.assert_array_equal(amb, np.rint((s.truth - phi) / TWOPI))
+
+
+def test_all_masked_scene_skips_solve():
+    phi = np.random.default_rng(2).uniform(0, TWOPI, (6, 7)).astype(np.float32)
+    sig = np.full(O.narcs(6, 7), 32000, np.int16)
+    u, st = O.pcg_irls_f32(sig, phi, n_irls=2)
+    assert st["status"] == [3, 3, 3] and st["iters"] == [0, 0, 0]
+    exp = phi + np.float32(TWOPI) * np.rint((phi[0, 0] - phi) / np.float32(TWOPI))
+    np.testing.assert_allclose(u, exp, atol=1e-5)
+
+
+def test_cgcg_recurrences_reproduce_cg_iterates():
+    rng = np.random.default_rng(9)
+    nrow, ncol = 10, 12
+    na = O.narcs(nrow, ncol)
+    w = rng.uniform(0.02, 0.1, na).astype(np.float32)
+    phi = rng.uniform(0, TWOPI, (nrow, ncol)).astype(np.float32)
+    b, diag = O.assemble(w, phi)
+    dinv = (1.0 / diag).astype(np.float32)
+    # single-reduction recurrences driven by the oracle step
+    x = np.zeros_like(b)
+    r = b.copy()
+    p = np.zeros_like(b)
+    s = np.zeros_like(b)
+    u0 = r * dinv
+    wv = O.matvec(w, u0)
+    gam = float((r.astype(np.float64) * u0).sum())
+    dl = float((wv.astype(np.float64) * u0).sum())
+    alpha, beta = gam / dl, 0.0
+    for _ in range(6):
+        x, r, p, s, wv, gn, dn = O.cgcg_step_f32(w, dinv, alpha, beta, x, r, p, s, wv)
+        beta = gn / gam
+        alpha_prev, alpha = alpha, gn / (dn - beta * gn / alpha)
+        gam = gn
+    # textbook Jacobi-PCG, dense float64, same number of updates
+    L = dense_laplacian(w.astype(np.float64), nrow, ncol)
+    bb = b.ravel().astype(np.float64)
+    D = dinv.ravel().astype(np.float64)
+    xx = np.zeros_like(bb)
+    rr = bb.copy()
+    zz = rr * D
+    pp = zz.copy()
+    rz = rr @ zz
+    for _ in range(6):
+        q = L @ pp
+        a = rz / (pp @ q)
+        xx += a * pp
+        rr -= a * q
+        zz = rr * D
+        rzn = rr @ zz
+        pp = zz + (rzn / rz) * pp
+        rz = rzn
+    np.testing.assert_allclose(x.ravel(), xx, rtol=2e-3, atol=2e-4 * np.abs(xx).max())
+
+
+# ── a14 conncomp ────────────────────────────────────────────────────────────
+def _scipy_components(unw, corr, mask, minfrac, maxn):
+    nrow, ncol = unw.shape
+    valid = ~(corr < 0.05)
+    if mask is not None:
+        valid &= mask.astype(bool)
+    h_ok = valid[:, :-1] & valid[:, 1:] & (np.abs(unw[:, :-1] - unw[:, 1:]) < np.float32(np.pi))
+    v_ok = valid[:-1, :] & valid[1:, :] & (np.abs(unw[:-1, :] - unw[1:, :]) < np.float32(np.pi))
+    idx = np.arange(nrow * ncol).reshape(nrow, ncol)
+    ri = np.concatenate([idx[:, :-1][h_ok], idx[:-1, :][v_ok]])
+    ci = np.concatenate([idx[:, 1:][h_ok], idx[1:, :][v_ok]])
+    g = coo_matrix((np.ones(ri.size), (ri, ci)), shape=(nrow * ncol, nrow * ncol))
+    _, lab = connected_components(g, directed=False)
+    lab = lab.reshape(nrow, ncol)
+    min_size = max(1, int(minfrac * nrow * ncol))
+    out = np.zeros((nrow, ncol), np.uint32)
+    nxt = 1
+    seen = {}
+    for i in range(nrow * ncol):          # components by increasing minimum index
+        rr, cc = divmod(i, ncol)
+        if not valid[rr, cc]:
+            continue
+        l = lab[rr, cc]
+        if l in seen:
+            continue
+        members = (lab == l) & valid
+        seen[l] = 0
+        if members.sum() >= min_size and nxt - 1 < maxn:
+            seen[l] = nxt
+            nxt += 1
+    for l, v in seen.items():
+        out[(lab == l) & valid] = v
+    return out
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_conncomp_matches_scipy(seed):
+    rng = np.random.default_rng(seed)
+    unw = np.cumsum(rng.normal(0, 1.4, (23, 31)), axis=1).astype(np.float32)
+    corr = rng.uniform(0, 1, (23, 31)).astype(np.float32)
+    corr[rng.uniform(0, 1, corr.shape) < 0.1] = 0.01
+    mask = (rng.uniform(0, 1, corr.shape) > 0.05).astype(np.uint8)
+    for minfrac, maxn in [(0.0, 1000), (0.01, 32), (0.02, 3)]:
+        got = O.conncomp(unw, corr, mask, minfrac, maxn)
+        np.testing.assert_array_equal(got, _scipy_components(unw, corr, mask, minfrac, maxn))
+
+
+def test_conncomp_reference_known_answers():
+    # test_conncomp.py:32-81 fixtures
+    unw = np.zeros((20, 30), np.float32)
+    unw[:, 15:] += 10.0
+    corr = np.full((20, 30), 0.8, np.float32)
+    out = O.conncomp(unw, corr, None, 0.01, 32)
+    assert sorted(set(np.unique(out)) - {0}) == [1, 2]
+    assert (O.conncomp(np.zeros((10, 10), np.float32), corr[:10, :10], np.zeros((10, 10), np.uint8), 0.0, 32) == 0).all()
+    u2 = np.zeros((20, 20), np.float32)
+    u2[10, 10] = 100.0
+    assert O.conncomp(u2, np.full((20, 20), 0.9, np.float32), None, 0.05, 32)[10, 10] == 0
+    u3 = np.zeros((20, 30), np.float32)
+    u3[:, 10:20] += 20.0
+    u3[:, 20:] += 40.0
+    o3 = O.conncomp(u3, np.full((20, 30), 0.9, np.float32), None, 0.0, 1)
+    assert 1 in o3 and 2 not in o3
+
+
+# ── a13 integration ─────────────────────────────────────────────────────────
+def test_integrate_phase_recovers_field_from_integer_flows():
+    rng = np.random.default_rng(13)
+    nrow, ncol = 15, 19
+    u = np.cumsum(np.cumsum(rng.normal(0, 1.6, (nrow, ncol)), 0), 1) * 0.4
+    phi = np.mod(u, TWOPI).astype(np.float32)
+    p64 = phi.astype(np.float64)
+    wr_r = wrap_pi(np.diff(p64, axis=0))
+    wr_c = wrap_pi(np.diff(p64, axis=1))
+    hfl = np.rint((wr_r - np.diff(u, axis=0)) / TWOPI).astype(np.int16)   # downward subtracts
+    vfl = np.rint((np.diff(u, axis=1) - wr_c) / TWOPI).astype(np.int16)
+    got = O.integrate_phase(phi, hfl, vfl)
+    np.testing.assert_allclose(got, u - u[0, 0] + p64[0, 0], atol=2e-4)
+    zero = O.integrate_phase(phi, np.zeros_like(hfl), np.zeros_like(vfl))
+    smooth = np.mod(0.3 * np.arange(ncol)[None, :] + 0.1 * np.arange(nrow)[:, None], TWOPI).astype(np.float32)
+    z2 = O.integrate_phase(smooth, np.zeros_like(hfl), np.zeros_like(vfl))
+    exp = 0.3 * np.arange(ncol)[None, :] + 0.1 * np.arange(nrow)[:, None]
+    np.testing.assert_allclose(z2, exp, atol=2e-5)
+    assert zero.shape == phi.shape
+
+
+# ── a15 tiling + stitch ─────────────────────────────────────────────────────
+def test_tile_axis_known_answers():
+    assert O.tile_axis(8192, 2048, 256) == ([0, 1792, 3584, 5376, 7168], [2048, 2048, 2048, 2048, 1024])
+    assert O.tile_axis(100, 0, 4, 3) == ([0, 32, 64], [36, 36, 36])
+    assert O.tile_axis(500, 0, 0, 1) == ([0], [500])
+    rng = np.random.default_rng(0)
+    for _ in range(200):
+        n = int(rng.integers(50, 5000))
+        nt = int(rng.integers(1, 9))
+        ov = int(rng.integers(0, 40))
+        f, s = O.tile_axis(n, 0, ov, nt)
+        assert f[0] == 0 and f[-1] + s[-1] == n
+        for k in range(nt - 1):
+            assert f[k] + s[k] - f[k + 1] == ov          # exact overlap between neighbours
+        ts = int(rng.integers(ov + 16, ov + 600))
+        f, s = O.tile_axis(n, ts, ov)
+        assert f[0] == 0 and f[-1] + s[-1] == n and all(x <= ts for x in s)
+        for k in range(len(f) - 1):
+            assert f[k] + s[k] - f[k + 1] == ov
+
+
+def _wmedian_sort(v, w):
+    order = np.argsort(v, kind="stable")
+    cw = np.cumsum(w[order].astype(np.float64))
+    i = int(np.searchsorted(cw, cw[-1] / 2.0, side="left"))
+    return float(v[order][i])
+
+
+def test_stitch_two_tiles_fractional_and_integer_offsets():
+    rng = np.random.default_rng(21)
+    nrow, ncol, ov = 12, 30, 6
+    rf, rs = [0], [nrow]
+    cf, cs = O.tile_axis(ncol, 0, ov, 2)
+    base = np.cumsum(rng.normal(0, 0.3, (nrow, ncol)), axis=1).astype(np.float32)
+    corr = rng.uniform(0.04, 1.0, (nrow, ncol)).astype(np.float32)
+    tiles_u = [base[:, cf[0]:cf[0] + cs[0]].copy(), base[:, cf[1]:cf[1] + cs[1]] + np.float32(-7.3)]
+    tiles_c = [corr[:, cf[0]:cf[0] + cs[0]].copy(), corr[:, cf[1]:cf[1] + cs[1]].copy()]
+    cc = [np.ones((nrow, cs[0]), np.uint32), np.where(rng.uniform(size=(nrow, cs[1])) < 0.2, 0, 2).astype(np.uint32)]
+    unw, lab, tk = O.stitch(nrow, ncol, rf, rs, cf, cs, 0, ov, tiles_u, cc, tiles_c, laplace=True)
+    ovl_l = tiles_u[0][:, -ov:]
+    ovl_r = tiles_u[1][:, :ov]
+    ok = (tiles_c[0][:, -ov:] >= 0.05) & (tiles_c[1][:, :ov] >= 0.05)
+    dv = (ovl_l - ovl_r)[ok]
+    wv = np.minimum(tiles_c[0][:, -ov:], tiles_c[1][:, :ov])[ok]
+    assert tk[1] == pytest.approx(_wmedian_sort(dv, wv) / TWOPI, abs=1e-12)
+    assert tk[1] == pytest.approx(7.3 / TWOPI, abs=1e-5)
+    np.testing.assert_allclose(unw, base, atol=1e-4)
+    # labels: tile 1 offset by tile 0's max (1); zeros stay zero
+    right = lab[:, cf[1] + ov // 2:]
+    assert set(np.unique(right)) <= {0, 3}
+    unw_i, _, tki = O.stitch(nrow, ncol, rf, rs, cf, cs, 0, ov, tiles_u, cc, tiles_c, laplace=False)
+    assert tki[1] == round(7.3 / TWOPI)
+
+
+def test_weighted_median_ties_and_selection():
+    rng = np.random.default_rng(4)
+    for trial in range(30):
+        n = int(rng.integers(1, 400))
+        v = rng.integers(-5, 6, n).astype(np.float32) * np.float32(0.7)
+        w = rng.uniform(0.05, 1.0, n).astype(np.float32)
+        nrow, ncol, ov = n, 8, 1
+        u0 = np.zeros((n, 4), np.float32)
+        u1 = np.zeros((n, 5), np.float32)
+        u0[:, -1] = v
+        c0 = np.ones((n, 4), np.float32)
+        c1 = np.ones((n, 5), np.float32)
+        c0[:, -1] = w
+        c1[:, 0] = w
+        cc = [np.zeros((n, 4), np.uint32), np.zeros((n, 5), np.uint32)]
+        _, _, tk = O.stitch(nrow, ncol, [0], [n], [0, 3], [4, 5], 0, ov, [u0, u1], cc, [c0, c1])
+        assert tk[1] == pytest.approx(_wmedian_sort(v, w) / TWOPI, abs=1e-12)
+
+
+# ── WLS (differentiable path) ───────────────────────────────────────────────
+def test_wls_solve_and_adjoint_against_dense_and_finite_differences():
+    rng = np.random.default_rng(31)
+    nrow, ncol = 6, 7
+    na = O.narcs(nrow, ncol)
+    w = rng.uniform(0.05, 0.5, na)
+    phi = rng.uniform(0, TWOPI, (nrow, ncol))
+    truth = rng.normal(0, 1, (nrow, ncol))
+    t, h = arc_list(nrow, ncol)
+    g = wrap_pi(phi.ravel()[h] - phi.ravel()[t])
+
+    def dense_u(wv):
+        L = dense_laplacian(wv, nrow, ncol)
+        b = np.zeros(nrow * ncol)
+        np.add.at(b, h, wv * g)
+        np.add.at(b, t, -wv * g)
+        x = np.linalg.solve(L, b)
+        return x - x.mean()
+
+    u, _ = O.wls_solve_f64(w, phi)
+    np.testing.assert_allclose(u.ravel(), dense_u(w), atol=1e-9)
+    loss = lambda uu: 0.5 * np.sum((uu - (truth.ravel() - truth.mean())) ** 2)  # noqa: E731
+    gu = (u.ravel() - (truth.ravel() - truth.mean())).reshape(nrow, ncol)
+    gw, _ = O.wls_adjoint_f64(w, phi, u, gu)
+    for a in rng.choice(na, 12, replace=False):
+        e = np.zeros(na)
+        e[a] = 1e-6
+        fd = (loss(dense_u(w + e)) - loss(dense_u(w - e))) / 2e-6
+        assert gw[a] == pytest.approx(fd, rel=1e-5, abs=1e-9)
+
+
+def test_synth_generator_frozen():
+    s = synth.make_scene(64, 80, nbumps=3, npatches=1, nlooks=5.0, seed=7, patch=20)
+    assert synth.checksum(s.igram, s.corr) == GOLDEN_SYNTH_SHA
+
+
+GOLDEN_SYNTH_SHA = "062084a27175bc59f7dc1561622450e223c4eca21a96c51ca6d308ee015ac156"
