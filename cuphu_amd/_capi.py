@@ -85,6 +85,11 @@ class CuPhuSolveStats(C.Structure):
                 "relres": list(self.relres[:n]), "pass_status": list(self.pass_status[:n])}
 
 
+class CuPhuKernelProfile(C.Structure):
+    _fields_ = [("cg_samples", C.c_longlong), ("cg_ms_sum", _d), ("cg_bytes_sum", _d), ("cg_px_sum", _d),
+                ("cg_launches", C.c_longlong)]
+
+
 class CuPhuInstance(C.Structure):
     _fields_ = [("nrow", _i), ("ncol", _i), ("d_igram", C.c_void_p), ("d_phase", C.c_void_p),
                 ("d_corr", C.c_void_p), ("d_mask", C.c_void_p), ("d_u0", C.c_void_p),
@@ -138,6 +143,8 @@ _SIGS = {
     "cuphu_wls_solve_gpu": (_i, [_P, _P, _i, _i, _i, _i, _d, _P, C.POINTER(_i), C.POINTER(_d), _P]),
     "cuphu_wls_adjoint_gpu": (_i, [_P, _P, _P, _P, _i, _i, _i, _i, _d, _P, _P, C.POINTER(_i),
                                    C.POINTER(_d), _P]),
+    "cuphu_set_profiling": (_i, [_i]),
+    "cuphu_get_profile": (_i, [C.POINTER(CuPhuKernelProfile)]),
     "cuphu_test_ingest": (_i, [_P, _P, _P, _i, _i, C.POINTER(CuPhuParams), _P, _P]),
     "cuphu_test_laplace_ops": (_i, [_P, _P, _P, _i, _i, C.c_float, _P, _P, _P, _P, _P, _P, _P]),
     "cuphu_test_cg_step": (_i, [_P, _P, _i, _i, _d, _d, _i, _P, _P, _P, _P, _P, C.POINTER(_d),
@@ -152,6 +159,9 @@ def _load() -> C.CDLL:
         raise ImportError(
             f"libcuphu_hip.so not found at {LIB_PATH}; build it with "
             "`python -c 'import __graft_entry__ as g; g.build()'` (hipcc, gfx950)")
+    from cuphu_amd._runtime import ensure_single_runtime
+
+    ensure_single_runtime()
     handle = C.CDLL(LIB_PATH)
     for name, (res, args) in _SIGS.items():
         fn = getattr(handle, name)

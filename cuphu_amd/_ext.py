@@ -4,6 +4,10 @@ There is no fallback: without the built HIP extension the package does not
 import.
 """
 
+from cuphu_amd._runtime import ensure_single_runtime
+
+ensure_single_runtime()
+
 try:
     import cuphu_amd._cuphu_ext as _cuphu_ext
 except ImportError as e:  # pragma: no cover - exercised only on broken installs

@@ -46,6 +46,10 @@ template <class T> struct Planes {
     short2 *sigsq = nullptr;        // {down, right} arc sigsq
     T *w2 = nullptr;                // {down, right} arc weights interleaved (2 T per pixel)
     T *b = nullptr, *dinv = nullptr, *x = nullptr, *r = nullptr, *p = nullptr, *s = nullptr, *wv = nullptr;
+    // Second set of the three vectors the fused PCG step reads at halo
+    // pixels: step k reads set (k % 2) and writes the other, so no
+    // workgroup overwrites a value a neighbouring workgroup still reads.
+    T *r2 = nullptr, *s2 = nullptr, *wv2 = nullptr;
 };
 
 struct KernelCfg {
@@ -64,9 +68,10 @@ void launch_assemble(const InstDesc *d_desc, const int *d_blk, int nblk, const P
 template <class T>
 void launch_start(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, const SolveScal *scal,
                   int force, double2 *partials, int rpw, hipStream_t st);
+// parity 0 reads r/s/wv and writes r2/s2/wv2; parity 1 the reverse.
 template <class T>
 void launch_cg(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, const SolveScal *scal,
-               double2 *partials, int rpw, hipStream_t st);
+               double2 *partials, int rpw, int parity, hipStream_t st);
 template <class T>
 void launch_rr(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, const SolveScal *scal,
                int rpw, hipStream_t st);

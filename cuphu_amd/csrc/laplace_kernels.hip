@@ -203,13 +203,14 @@ __device__ __forceinline__ short sigsq_arc(const CostConsts &k, float ca, float 
 
 template <class T>
 __global__ __launch_bounds__(256) void k_ingest(const IngestSrc *srcs, const InstDesc *descs, const int *rowinst,
-                                                Planes<T> pl, int has_u0) {
-    int grow = blockIdx.y;
+                                                Planes<T> pl, int has_u0, int nxb) {
+    // 1-D grid: block b covers row b / nxb, column block b % nxb
+    int grow = (int)(blockIdx.x / (unsigned)nxb);
     int inst = rowinst[grow];
     const InstDesc d = descs[inst];
     const IngestSrc &s = srcs[inst];
     int row = grow - d.row_begin;
-    int c0 = (blockIdx.x * 256 + threadIdx.x) * 4;
+    int c0 = ((int)(blockIdx.x % (unsigned)nxb) * 256 + threadIdx.x) * 4;
     if (c0 >= d.pitch) return;
     long long e = d.off + (long long)row * d.pitch + c0;
     float ph[4], co[4];
@@ -481,17 +482,23 @@ __global__ __launch_bounds__(256) void k_start(const InstDesc *descs, const int 
 //   u' = r * dinv;   wv = L u';  partial sums of r.u' and wv.u'.
 // u = r * dinv is recomputed, never stored.  The halo values of u' are
 // recomputed from r, s, wv, dinv of the neighbouring pixels with the same
-// operations, so they equal the neighbours' own values bit for bit.
+// operations, so they equal the neighbours' own values bit for bit.  Those
+// three vectors are therefore read from one plane set and written to the
+// other (parity), so a neighbour's old values are never overwritten while
+// this workgroup still reads them; x and p are only read at the own pixel
+// and stay in place.
 // ═══════════════════════════════════════════════════════════════════════════
 template <class T, int RPW>
 __global__ __launch_bounds__(256) void k_cg(const InstDesc *descs, const int *blk_inst, Planes<T> pl,
-                                            const SolveScal *scal, double2 *partials) {
+                                            const SolveScal *scal, double2 *partials, int parity) {
     constexpr int TH = 4 * RPW;
     __shared__ T lu[(TH + 2) * kLdsStride];
     TileInfo t = get_tile(descs, blk_inst, TH);
     const SolveScal *sc = scal + t.inst;
     if (!sc->active) return;
     const T al = (T)sc->alpha, be = (T)sc->beta;
+    const T *r_in = parity ? pl.r2 : pl.r, *s_in = parity ? pl.s2 : pl.s, *wv_in = parity ? pl.wv2 : pl.wv;
+    T *r_out = parity ? pl.r : pl.r2, *s_out = parity ? pl.s : pl.s2, *wv_out = parity ? pl.wv : pl.wv2;
     int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const InstDesc &d = t.d;
     int c = t.tx0 + 4 * lane;
@@ -505,9 +512,9 @@ __global__ __launch_bounds__(256) void k_cg(const InstDesc *descs, const int *bl
         if (row < d.nrow && cok) {
             long long e = d.off + (long long)row * d.pitch + c;
             T r4[4], w4[4], s4[4], p4[4], x4[4], di[4];
-            ld4(pl.r + e, r4);
-            ld4(pl.wv + e, w4);
-            ld4(pl.s + e, s4);
+            ld4(r_in + e, r4);
+            ld4(wv_in + e, w4);
+            ld4(s_in + e, s4);
             ld4(pl.p + e, p4);
             ld4(pl.x + e, x4);
             ld4(pl.dinv + e, di);
@@ -524,9 +531,9 @@ __global__ __launch_bounds__(256) void k_cg(const InstDesc *descs, const int *bl
                 rr[k][j] = r4[j];
             }
             st4(pl.p + e, p4);
-            st4(pl.s + e, s4);
+            st4(s_out + e, s4);
             st4(pl.x + e, x4);
-            st4(pl.r + e, r4);
+            st4(r_out + e, r4);
         }
         T *lq = lu + (lr + 1) * kLdsStride + kLdsC0 + 4 * lane;
 #pragma unroll
@@ -539,9 +546,9 @@ __global__ __launch_bounds__(256) void k_cg(const InstDesc *descs, const int *bl
             zero4(un);
             if (ok) {
                 T r4[4], w4[4], s4[4], di[4];
-                ld4(pl.r + e, r4);
-                ld4(pl.wv + e, w4);
-                ld4(pl.s + e, s4);
+                ld4(r_in + e, r4);
+                ld4(wv_in + e, w4);
+                ld4(s_in + e, s4);
                 ld4(pl.dinv + e, di);
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
@@ -555,8 +562,8 @@ __global__ __launch_bounds__(256) void k_cg(const InstDesc *descs, const int *bl
         [&](int lr, int lc, long long e, bool ok) {
             T v = T(0);
             if (ok) {
-                T sn = Prec<T>::fmaT(be, pl.s[e], pl.wv[e]);
-                v = Prec<T>::fmaT(-al, sn, pl.r[e]) * pl.dinv[e];
+                T sn = Prec<T>::fmaT(be, s_in[e], wv_in[e]);
+                v = Prec<T>::fmaT(-al, sn, r_in[e]) * pl.dinv[e];
             }
             lu[lr * kLdsStride + lc] = v;
         });
@@ -572,7 +579,7 @@ __global__ __launch_bounds__(256) void k_cg(const InstDesc *descs, const int *bl
         stencil4(lu, lr, lane, wd, wu, wr, wl, y);
         if (cok) {
             long long e = d.off + (long long)row * d.pitch + c;
-            st4(pl.wv + e, y);
+            st4(wv_out + e, y);
             const T *uc = lu + (lr + 1) * kLdsStride + kLdsC0 + 4 * lane;
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
@@ -710,11 +717,12 @@ __global__ __launch_bounds__(256) void k_reduce(const InstDesc *descs, const dou
 // into a pitched float output plane.
 // ═══════════════════════════════════════════════════════════════════════════
 template <class T>
-__global__ __launch_bounds__(256) void k_final(const InstDesc *descs, const int *rowinst, Planes<T> pl, float *out) {
-    int grow = blockIdx.y;
+__global__ __launch_bounds__(256) void k_final(const InstDesc *descs, const int *rowinst, Planes<T> pl, float *out,
+                                               int nxb) {
+    int grow = (int)(blockIdx.x / (unsigned)nxb);
     const InstDesc d = descs[rowinst[grow]];
     int row = grow - d.row_begin;
-    int c0 = (blockIdx.x * 256 + threadIdx.x) * 4;
+    int c0 = ((int)(blockIdx.x % (unsigned)nxb) * 256 + threadIdx.x) * 4;
     if (c0 >= d.pitch) return;
     long long e = d.off + (long long)row * d.pitch + c0;
     T off = (T)pl.phi[d.off] - pl.x[d.off];
@@ -741,10 +749,10 @@ __global__ __launch_bounds__(256) void k_final(const InstDesc *descs, const int 
 // (cuphu_laplace.cu:455-484) on one instance, with FP64 partial sums.
 // ═══════════════════════════════════════════════════════════════════════════
 template <class T>
-__global__ __launch_bounds__(256) void k_plain_matvec(InstDesc d, const T *w2, const T *x, T *y) {
-    int row = blockIdx.y;
+__global__ __launch_bounds__(256) void k_plain_matvec(InstDesc d, const T *w2, const T *x, T *y, int nxb) {
+    int row = (int)(blockIdx.x / (unsigned)nxb);
     int lane = threadIdx.x & 63;
-    int c = (blockIdx.x * 256 + threadIdx.x) * 4;
+    int c = ((int)(blockIdx.x % (unsigned)nxb) * 256 + threadIdx.x) * 4;
     bool cok = c < d.pitch;
     long long e = d.off + (long long)row * d.pitch + c;
     T xa[4], xu[4], xd[4], wd[4], wr[4], wu[4];
@@ -847,8 +855,9 @@ __global__ __launch_bounds__(256) void k_plain_dot_final(const double *parts, in
 template <class T>
 void launch_ingest(const IngestSrc *d_src, const InstDesc *d_desc, const int *d_rowinst, int nrows_total,
                    const Planes<T> &pl, int has_u0, int max_pitch, hipStream_t st) {
-    dim3 grid((unsigned)ceil_div((size_t)max_pitch, 1024), (unsigned)nrows_total);
-    hipLaunchKernelGGL(k_ingest<T>, grid, dim3(256), 0, st, d_src, d_desc, d_rowinst, pl, has_u0);
+    int nxb = (int)ceil_div((size_t)max_pitch, 1024);
+    dim3 grid((unsigned)((size_t)nxb * (size_t)nrows_total));
+    hipLaunchKernelGGL(k_ingest<T>, grid, dim3(256), 0, st, d_src, d_desc, d_rowinst, pl, has_u0, nxb);
     CUPHU_LAUNCH_CHECK();
 }
 
@@ -878,8 +887,8 @@ void launch_start(const InstDesc *d_desc, const int *d_blk, int nblk, const Plan
 }
 template <class T>
 void launch_cg(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, const SolveScal *scal,
-               double2 *partials, int rpw, hipStream_t st) {
-    CUPHU_RPW_DISPATCH(rpw, k_cg, dim3(nblk), dim3(256), 0, st, d_desc, d_blk, pl, scal, partials);
+               double2 *partials, int rpw, int parity, hipStream_t st) {
+    CUPHU_RPW_DISPATCH(rpw, k_cg, dim3(nblk), dim3(256), 0, st, d_desc, d_blk, pl, scal, partials, parity);
     CUPHU_LAUNCH_CHECK();
 }
 template <class T>
@@ -896,14 +905,16 @@ void launch_reduce(const InstDesc *d_desc, int ninst, const double2 *partials, S
 template <class T>
 void launch_final(const InstDesc *d_desc, const int *d_rowinst, int nrows_total, const Planes<T> &pl, float *out,
                   int max_pitch, hipStream_t st) {
-    dim3 grid((unsigned)ceil_div((size_t)max_pitch, 1024), (unsigned)nrows_total);
-    hipLaunchKernelGGL(k_final<T>, grid, dim3(256), 0, st, d_desc, d_rowinst, pl, out);
+    int nxb = (int)ceil_div((size_t)max_pitch, 1024);
+    dim3 grid((unsigned)((size_t)nxb * (size_t)nrows_total));
+    hipLaunchKernelGGL(k_final<T>, grid, dim3(256), 0, st, d_desc, d_rowinst, pl, out, nxb);
     CUPHU_LAUNCH_CHECK();
 }
 
 template <class T> void launch_plain_matvec(const InstDesc &d, const T *w2, const T *x, T *y, hipStream_t st) {
-    dim3 grid((unsigned)ceil_div((size_t)d.pitch, 1024), (unsigned)d.nrow);
-    hipLaunchKernelGGL(k_plain_matvec<T>, grid, dim3(256), 0, st, d, w2, x, y);
+    int nxb = (int)ceil_div((size_t)d.pitch, 1024);
+    dim3 grid((unsigned)((size_t)nxb * (size_t)d.nrow));
+    hipLaunchKernelGGL(k_plain_matvec<T>, grid, dim3(256), 0, st, d, w2, x, y, nxb);
     CUPHU_LAUNCH_CHECK();
 }
 static inline unsigned grid1(size_t n) { return (unsigned)ceil_div(n, 1024); }
@@ -940,7 +951,7 @@ void launch_plain_dot(const T *a, const T *b, size_t n, double *partials, double
     template void launch_start<T>(const InstDesc *, const int *, int, const Planes<T> &, const SolveScal *, int, \
                                   double2 *, int, hipStream_t);                                                  \
     template void launch_cg<T>(const InstDesc *, const int *, int, const Planes<T> &, const SolveScal *,         \
-                               double2 *, int, hipStream_t);                                                     \
+                               double2 *, int, int, hipStream_t);                                                \
     template void launch_rr<T>(const InstDesc *, const int *, int, const Planes<T> &, const SolveScal *, int,    \
                                hipStream_t);                                                                     \
     template void launch_final<T>(const InstDesc *, const int *, int, const Planes<T> &, float *, int,           \

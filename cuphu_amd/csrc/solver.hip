@@ -172,6 +172,9 @@ template <class T> static PoolSet<T> alloc_pools(DevRes &r, long long n) {
     P.pl.p = ws<T>(r, nm("p").c_str(), n);
     P.pl.s = ws<T>(r, nm("s").c_str(), n);
     P.pl.wv = ws<T>(r, nm("wv").c_str(), n);
+    P.pl.r2 = ws<T>(r, nm("r2").c_str(), n);
+    P.pl.s2 = ws<T>(r, nm("s2").c_str(), n);
+    P.pl.wv2 = ws<T>(r, nm("wv2").c_str(), n);
     P.out = ws<float>(r, "out", n);
     P.lab = ws<uint32_t>(r, "lab", n);
     return P;
@@ -210,17 +213,38 @@ struct GroupDev {
     SolveScal *scal;
     double2 *part;
     int ninst, nblk;
+    std::vector<long long> px;   // host: logical pixels per instance
 };
+
+// Live timing of the fused PCG step (cuphu_set_profiling).
+struct KernelProf {
+    bool on = false;
+    hipEvent_t a = nullptr, b = nullptr;
+    CuPhuKernelProfile acc{};
+};
+static KernelProf g_prof;
 
 template <class T>
 static void solve_group_batched(const Planes<T> &pl, const GroupDev &g, const CuPhuLaplaceOptions &o, int rpw,
                                 bool fixed_weights, hipStream_t st, const std::vector<CuPhuSolveStats *> &stats,
                                 std::vector<SolveScal> &h) {
-    const int CH = std::max(1, std::min(o.rr_interval > 0 ? o.rr_interval : 50, std::max(o.max_iter, 1)));
+    // Chunk length: even (the fused step alternates plane sets, so an even
+    // chunk returns the state to the first set for the residual
+    // replacement), at least 2.
+    int ch = o.rr_interval > 0 ? o.rr_interval : 50;
+    ch = std::max(2, std::min(ch, std::max(o.max_iter, 2)));
+    const int CH = ch + (ch & 1);
     const double tol2 = o.tol * o.tol;
+    const bool prof = g_prof.on;
+    if (prof && !g_prof.a) {
+        CUPHU_HIP_CHECK(hipEventCreate(&g_prof.a));
+        CUPHU_HIP_CHECK(hipEventCreate(&g_prof.b));
+    }
+    // With profiling on, the first two steps of a chunk run outside the
+    // graph (the first between two events); the captured part holds CH-2.
     auto chunk = [&]() {
-        for (int i = 0; i < CH; ++i) {
-            launch_cg<T>(g.desc, g.blk, g.nblk, pl, g.scal, g.part, rpw, st);
+        for (int i = prof ? 2 : 0; i < CH; ++i) {
+            launch_cg<T>(g.desc, g.blk, g.nblk, pl, g.scal, g.part, rpw, i & 1, st);
             launch_reduce(g.desc, g.ninst, g.part, g.scal, RED_STEP, o.max_iter, tol2, st);
         }
         if (o.rr_interval > 0) {
@@ -267,10 +291,30 @@ static void solve_group_batched(const Planes<T> &pl, const GroupDev &g, const Cu
                 return false;
             };
             while (any_active()) {
+                double active_px = 0.0;
+                if (prof) {
+                    for (int i = 0; i < g.ninst; ++i)
+                        if (h[(size_t)i].active) active_px += (double)g.px[(size_t)i];
+                    CUPHU_HIP_CHECK(hipEventRecord(g_prof.a, st));
+                    launch_cg<T>(g.desc, g.blk, g.nblk, pl, g.scal, g.part, rpw, 0, st);
+                    CUPHU_HIP_CHECK(hipEventRecord(g_prof.b, st));
+                    launch_reduce(g.desc, g.ninst, g.part, g.scal, RED_STEP, o.max_iter, tol2, st);
+                    launch_cg<T>(g.desc, g.blk, g.nblk, pl, g.scal, g.part, rpw, 1, st);
+                    launch_reduce(g.desc, g.ninst, g.part, g.scal, RED_STEP, o.max_iter, tol2, st);
+                }
                 if (gexec) CUPHU_HIP_CHECK(hipGraphLaunch(gexec, st));
                 else chunk();
                 CUPHU_HIP_CHECK(hipMemcpyAsync(h.data(), g.scal, h.size() * sizeof(SolveScal), hipMemcpyDeviceToHost, st));
                 CUPHU_HIP_CHECK(hipStreamSynchronize(st));
+                if (prof) {
+                    float ms = 0.0f;
+                    CUPHU_HIP_CHECK(hipEventElapsedTime(&ms, g_prof.a, g_prof.b));
+                    g_prof.acc.cg_samples += 1;
+                    g_prof.acc.cg_ms_sum += ms;
+                    g_prof.acc.cg_px_sum += active_px;
+                    g_prof.acc.cg_bytes_sum += active_px * (sizeof(T) == 4 ? 52.0 : 104.0);
+                    g_prof.acc.cg_launches += CH;
+                }
                 ++nch;
                 if ((long long)nch * CH > (long long)o.max_iter + CH) break;   // cannot happen: reduce caps iterations
             }
@@ -419,6 +463,8 @@ static void run_batch(const std::vector<std::pair<int, int>> &dims, const CuPhuL
         CUPHU_HIP_CHECK(hipMemcpyAsync(g.blk, blk.data(), blk.size() * sizeof(int), hipMemcpyHostToDevice, st));
         g.ninst = b - a;
         g.nblk = (int)blk.size();
+        g.px.clear();
+        for (int i = a; i < b; ++i) g.px.push_back((long long)L.desc[(size_t)i].nrow * L.desc[(size_t)i].ncol);
         CUPHU_HIP_CHECK(hipStreamSynchronize(st));   // host vectors die at scope end
     };
     if (o.solver == CUPHU_SOLVER_PLAIN) {
@@ -708,6 +754,7 @@ static void wls_impl(const T *d_w, const float *d_phase, const T *d_u, const T *
     g.part = ws<double2>(res, "grppart", (size_t)nb);
     g.ninst = 1;
     g.nblk = nb;
+    g.px = {(long long)nrow * ncol};
     InstDesc gd = d;
     gd.block_begin = 0;
     std::vector<int> blk((size_t)nb, 0);
@@ -916,6 +963,20 @@ void cuphu_default_bridge_params(CuPhuBridgeParams *bp) {
     bp->max_boundary_samples = 4096;
     bp->ramp_type = CUPHU_RAMP_NONE;
     bp->ramp_max_num_sample = 1000000;
+}
+
+int cuphu_set_profiling(int enabled) {
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    g_prof.on = enabled != 0;
+    g_prof.acc = CuPhuKernelProfile{};
+    return 0;
+}
+
+int cuphu_get_profile(CuPhuKernelProfile *out) {
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    if (!out) return CUPHU_ERR_INVALID;
+    *out = g_prof.acc;
+    return 0;
 }
 
 int cuphu_gpu_count(void) {
@@ -1393,7 +1454,8 @@ void hook_setup(HookInst &h, DevRes &res, int nrow, int ncol, int rpw, hipStream
     size_t nel = (size_t)h.L.total_el;
     unsigned g = (unsigned)ceil_div(2 * nel, 256);
     hipLaunchKernelGGL(k_zero<float>, dim3(g), dim3(256), 0, st, h.P.pl.w2, (long long)(2 * nel));
-    for (float *p : {h.P.pl.b, h.P.pl.dinv, h.P.pl.x, h.P.pl.r, h.P.pl.p, h.P.pl.s, h.P.pl.wv, h.P.pl.phi})
+    for (float *p : {h.P.pl.b, h.P.pl.dinv, h.P.pl.x, h.P.pl.r, h.P.pl.p, h.P.pl.s, h.P.pl.wv, h.P.pl.phi,
+                     h.P.pl.r2, h.P.pl.s2, h.P.pl.wv2})
         hipLaunchKernelGGL(k_zero<float>, dim3(g), dim3(256), 0, st, p, (long long)nel);
     CUPHU_HIP_CHECK(hipGetLastError());
     CUPHU_HIP_CHECK(hipStreamSynchronize(st));
@@ -1536,14 +1598,15 @@ int cuphu_test_cg_step(const float *w, const float *dinv, int nrow, int ncol, do
         sc.beta = beta;
         sc.active = 1;
         CUPHU_HIP_CHECK(hipMemcpyAsync(h.g.scal, &sc, sizeof(sc), hipMemcpyHostToDevice, st));
-        launch_cg<float>(h.g.desc, h.g.blk, h.g.nblk, h.P.pl, h.g.scal, h.g.part, rpw, st);
+        // parity 0: reads r/s/wv, writes r2/s2/wv2
+        launch_cg<float>(h.g.desc, h.g.blk, h.g.nblk, h.P.pl, h.g.scal, h.g.part, rpw, 0, st);
         std::vector<double2> parts((size_t)h.g.nblk);
         CUPHU_HIP_CHECK(hipMemcpyAsync(parts.data(), h.g.part, parts.size() * sizeof(double2), hipMemcpyDeviceToHost, st));
         from_plane(h.P.pl.x, x, d, st);
-        from_plane(h.P.pl.r, r, d, st);
+        from_plane(h.P.pl.r2, r, d, st);
         from_plane(h.P.pl.p, p, d, st);
-        from_plane(h.P.pl.s, s, d, st);
-        from_plane(h.P.pl.wv, wv, d, st);
+        from_plane(h.P.pl.s2, s, d, st);
+        from_plane(h.P.pl.wv2, wv, d, st);
         CUPHU_HIP_CHECK(hipStreamSynchronize(st));
         double ga = 0.0, de = 0.0;
         for (auto &v : parts) { ga += v.x; de += v.y; }

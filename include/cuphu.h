@@ -418,6 +418,22 @@ int cuphu_wls_adjoint_gpu(const void *d_w, const void *d_phase, const void *d_u,
                           int *iters_out, double *relres_out,
                           cuphu_stream_t stream);
 
+/* Live timing of the fused PCG step kernel.  While enabled, the first
+ * PCG step of every iteration chunk is launched outside the chunk's HIP
+ * graph between two HIP events on the solver's stream; its duration and
+ * algorithmic bytes (52 B per active pixel in FP32, 104 B in FP64: x, r, p,
+ * s, L u and 1/diag read, x, r, p, s, L u written, two arc weights read)
+ * are accumulated.  cuphu_set_profiling resets the counters. */
+typedef struct CuPhuKernelProfile {
+    long long cg_samples;     /* timed launches                            */
+    double    cg_ms_sum;      /* their summed duration (ms)                */
+    double    cg_bytes_sum;   /* their summed algorithmic bytes            */
+    double    cg_px_sum;      /* their summed active pixels                */
+    long long cg_launches;    /* all fused PCG launches while enabled      */
+} CuPhuKernelProfile;
+int cuphu_set_profiling(int enabled);
+int cuphu_get_profile(CuPhuKernelProfile *out);
+
 /* ── test hooks (like the reference's private _*_test bindings,
  *    cuphu_py.cu:608-633): run single kernels on host arrays, FP32. ─────── */
 

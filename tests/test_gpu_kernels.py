@@ -176,13 +176,14 @@ def test_conncomp_bitexact(gpu_ready, seed, shape):
     corr = rng.uniform(0, 1, shape).astype(np.float32)
     corr[rng.uniform(size=shape) < 0.1] = 0.01
     mask = (rng.uniform(size=shape) > 0.05).astype(np.uint8)
+    du, dc, dm = G.dev(unw), G.dev(corr), G.dev(mask)   # keep the tensors alive during the call
     for minfrac, maxn in [(0.0, 100000), (0.001, 32), (0.01, 2)]:
         prm = _capi.default_params()
         prm.minconncompfrac = minfrac
         prm.maxncomps = maxn
         lab = G.empty(shape, torch.int32)
-        _capi.check(_capi.lib.cuphu_conncomp_gpu(G.p(G.dev(unw)), G.p(G.dev(corr)), G.p(G.dev(mask)), shape[0],
-                                                 shape[1], None, None, C.byref(prm), 0, G.p(lab)), "conncomp")
+        _capi.check(_capi.lib.cuphu_conncomp_gpu(G.p(du), G.p(dc), G.p(dm), shape[0], shape[1], None, None,
+                                                 C.byref(prm), 0, G.p(lab)), "conncomp")
         np.testing.assert_array_equal(G.host(lab).view(np.uint32), O.conncomp(unw, corr, mask, minfrac, maxn))
 
 
@@ -193,8 +194,9 @@ def test_conncomp_single_giant_component(gpu_ready):
     corr = np.ones(shape, np.float32)
     prm = _capi.default_params()
     lab = G.empty(shape, torch.int32)
-    _capi.check(_capi.lib.cuphu_conncomp_gpu(G.p(G.dev(unw)), G.p(G.dev(corr)), None, shape[0], shape[1], None,
-                                             None, C.byref(prm), 0, G.p(lab)), "conncomp")
+    du, dc = G.dev(unw), G.dev(corr)
+    _capi.check(_capi.lib.cuphu_conncomp_gpu(G.p(du), G.p(dc), None, shape[0], shape[1], None, None, C.byref(prm),
+                                             0, G.p(lab)), "conncomp")
     assert (G.host(lab) == 1).all()
 
 
@@ -222,8 +224,9 @@ def test_integrate_phase(gpu_ready):
     vfl = np.rint((np.diff(u, axis=1) - (np.diff(p64, axis=1) - TWOPI * np.rint(np.diff(p64, axis=1) / TWOPI))) / TWOPI)
     hfl, vfl = hfl.astype(np.int16), vfl.astype(np.int16)
     out = G.empty((nrow, ncol), torch.float32)
-    _capi.check(_capi.lib.cuphu_integrate_phase_gpu(G.p(G.dev(phi)), G.p(G.dev(hfl)), G.p(G.dev(vfl)), nrow, ncol,
-                                                    0, G.p(out)), "integrate")
+    dp, dh, dv = G.dev(phi), G.dev(hfl), G.dev(vfl)
+    _capi.check(_capi.lib.cuphu_integrate_phase_gpu(G.p(dp), G.p(dh), G.p(dv), nrow, ncol, 0, G.p(out)),
+                "integrate")
     got = G.host(out)
     ref = O.integrate_phase(phi, hfl, vfl)
     assert _ulps(got, ref).max() <= 1
