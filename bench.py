@@ -10,15 +10,17 @@ tiles in one batched launch sequence, tile stitching and conncomp.  Inputs
 are resident in HBM before the timed region; a step is one full unwrap.
 
 Reported beside the headline Mpixel/s:
-* roofline: the fused PCG step kernel's algorithmic bytes (52 B per active
-  pixel) over its duration, timed live with HIP events on the solver's
-  stream during the timed steps;
+* roofline: the PCG direction+stencil kernel (p = z + beta p, q = L p, p.q;
+  28 B per active pixel) -- algorithmic bytes over its duration, timed live
+  with HIP events on the solver's stream during the timed steps; the PCG
+  update kernel (x, r, r.z; 28 B per active pixel) is reported beside it;
 * plain: the reference launch sequence (per-tile loop, 7 launches and 2
   blocking scalar reads per iteration) on the same scene, and the ratio;
 * accuracy: 2-pi ambiguity agreement of the FP32 result with the FP64 mode
   of the same solver (same tiles and schedule), and unwrapped-phase error;
 * cpu_baseline: the FP64 CPU oracle (same algorithm and schedule,
-  OpenMP over all host cores) on a 1024x1024 crop of the scene.
+  OpenMP over the host cores this process may use, at most 16) on a
+  1024x1024 crop of the scene.
 
 Usage: python bench.py [--gpus 1] [--steps 2] [--warmup 1]
 """
@@ -39,7 +41,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Mpixel/s unwrapped (Laplace PCG/IRLS) + stencil HBM GB/s; batched vs plain"
 HBM_PEAK_GBPS = 8000.0
-BYTES_PER_PX_CG = 52.0
+BYTES_PER_PX_PQ = 28.0
+BYTES_PER_PX_XR = 28.0
 
 
 def log(msg):
@@ -127,9 +130,11 @@ def main() -> int:
     tile_px = None
     log(f"batched: {mpx:.2f} Mpx/s ({ms_per_step:.0f} ms/step)")
 
-    cg_ms = prof.cg_ms_sum / max(prof.cg_samples, 1)
-    cg_bytes = prof.cg_bytes_sum / max(prof.cg_samples, 1)
-    achieved = (prof.cg_bytes_sum / 1e9) / (prof.cg_ms_sum / 1e3) if prof.cg_ms_sum > 0 else None
+    ns = max(prof.samples, 1)
+    pq_ms, xr_ms = prof.pq_ms_sum / ns, prof.xr_ms_sum / ns
+    pq_bytes, xr_bytes = prof.pq_bytes_sum / ns, prof.xr_bytes_sum / ns
+    achieved = (prof.pq_bytes_sum / 1e9) / (prof.pq_ms_sum / 1e3) if prof.pq_ms_sum > 0 else None
+    xr_gbs = (prof.xr_bytes_sum / 1e9) / (prof.xr_ms_sum / 1e3) if prof.xr_ms_sum > 0 else None
 
     out = {
         "metric": METRIC,
@@ -152,16 +157,20 @@ def main() -> int:
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_cg (fused single-reduction PCG step)",
+            "kernel": "k_pq (PCG direction + weighted 5-point stencil + p.q)",
             "achieved": round(achieved, 1) if achieved else None,
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 4) if achieved else None,
             "traffic": None,
-            "bytes_per_px": BYTES_PER_PX_CG,
-            "avg_launch_ms": round(cg_ms, 4),
-            "avg_bytes_per_launch": cg_bytes,
-            "timed_launches": prof.cg_samples,
+            "bytes_per_px": BYTES_PER_PX_PQ,
+            "avg_launch_ms": round(pq_ms, 4),
+            "avg_bytes_per_launch": pq_bytes,
+            "timed_launches": prof.samples,
+            "update_kernel": {"kernel": "k_xr (x, r update + r.z)", "bytes_per_px": BYTES_PER_PX_XR,
+                              "avg_launch_ms": round(xr_ms, 4),
+                              "achieved": round(xr_gbs, 1) if xr_gbs else None,
+                              "frac": round(xr_gbs / HBM_PEAK_GBPS, 4) if xr_gbs else None},
         },
         "pcg_iters_per_tile": iters[:3] + (["..."] if len(iters) > 3 else []),
         "pcg_iters_total": int(sum(sum(i) for i in iters)),
@@ -206,7 +215,11 @@ def main() -> int:
         import oracle as O
 
         n = min(args.cpu_sample, nrow, ncol)
-        ncore = os.cpu_count() or 1
+        try:
+            ncore = len(os.sched_getaffinity(0))
+        except AttributeError:
+            ncore = os.cpu_count() or 1
+        ncore = max(1, min(ncore, 16))
         O.set_threads(ncore)
         ig = np.ascontiguousarray(scene.igram[:n, :n])
         co = np.ascontiguousarray(scene.corr[:n, :n])

@@ -86,8 +86,8 @@ class CuPhuSolveStats(C.Structure):
 
 
 class CuPhuKernelProfile(C.Structure):
-    _fields_ = [("cg_samples", C.c_longlong), ("cg_ms_sum", _d), ("cg_bytes_sum", _d), ("cg_px_sum", _d),
-                ("cg_launches", C.c_longlong)]
+    _fields_ = [("samples", C.c_longlong), ("px_sum", _d), ("pq_ms_sum", _d), ("pq_bytes_sum", _d),
+                ("xr_ms_sum", _d), ("xr_bytes_sum", _d), ("iterations", C.c_longlong)]
 
 
 class CuPhuInstance(C.Structure):
@@ -147,8 +147,8 @@ _SIGS = {
     "cuphu_get_profile": (_i, [C.POINTER(CuPhuKernelProfile)]),
     "cuphu_test_ingest": (_i, [_P, _P, _P, _i, _i, C.POINTER(CuPhuParams), _P, _P]),
     "cuphu_test_laplace_ops": (_i, [_P, _P, _P, _i, _i, C.c_float, _P, _P, _P, _P, _P, _P, _P]),
-    "cuphu_test_cg_step": (_i, [_P, _P, _i, _i, _d, _d, _i, _P, _P, _P, _P, _P, C.POINTER(_d),
-                                C.POINTER(_d)]),
+    "cuphu_test_pcg_step": (_i, [_P, _P, _i, _i, _d, _d, _i, _P, _P, _P, _P, C.POINTER(_d), C.POINTER(_d),
+                                 C.POINTER(_d)]),
 }
 
 EXPORTED = tuple(_SIGS)

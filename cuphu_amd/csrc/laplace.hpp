@@ -21,10 +21,10 @@ struct InstDesc {
 // Per-instance solver scalars, device resident.
 struct SolveScal {
     double gamma, delta, gamma_prev, alpha, alpha_prev, beta, gamma0, relres;
-    int iters, active, status, pad;
+    int iters, active, status, replace;   // replace: the last residual replacement was taken
 };
 
-enum RedMode { RED_START = 0, RED_STEP = 1, RED_RR = 2 };
+enum RedMode { RED_START = 0, RED_PQ = 1, RED_RZ = 2, RED_RR = 3 };
 
 // Instance source description for ingest (pointers already offset to the
 // instance's (0,0); src_stride = elements per source row).
@@ -45,11 +45,12 @@ template <class T> struct Planes {
     unsigned char *mask = nullptr;  // pitched mask copy (1 = valid)
     short2 *sigsq = nullptr;        // {down, right} arc sigsq
     T *w2 = nullptr;                // {down, right} arc weights interleaved (2 T per pixel)
+    // s holds q = L p; wv is the plain solver's z.
     T *b = nullptr, *dinv = nullptr, *x = nullptr, *r = nullptr, *p = nullptr, *s = nullptr, *wv = nullptr;
-    // Second set of the three vectors the fused PCG step reads at halo
-    // pixels: step k reads set (k % 2) and writes the other, so no
-    // workgroup overwrites a value a neighbouring workgroup still reads.
-    T *r2 = nullptr, *s2 = nullptr, *wv2 = nullptr;
+    // Second direction plane: the fused direction kernel of step k reads the
+    // old direction from one plane and writes the new one to the other, so
+    // no workgroup overwrites a value a neighbouring workgroup still reads.
+    T *p2 = nullptr;
 };
 
 struct KernelCfg {
@@ -65,16 +66,24 @@ void launch_weights(const InstDesc *d_desc, const int *d_blk, int nblk, const Pl
                     T delta2, int rpw, hipStream_t st);
 template <class T>
 void launch_assemble(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, int rpw, hipStream_t st);
+// start: partial r.z (force: also for inactive instances)
 template <class T>
-void launch_start(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, const SolveScal *scal,
-                  int force, double2 *partials, int rpw, hipStream_t st);
-// parity 0 reads r/s/wv and writes r2/s2/wv2; parity 1 the reverse.
+void launch_rz(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, const SolveScal *scal,
+               int force, double2 *partials, int rpw, hipStream_t st);
+// parity 0 reads direction p and writes p2; parity 1 the reverse.
 template <class T>
-void launch_cg(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, const SolveScal *scal,
+void launch_pq(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, const SolveScal *scal,
                double2 *partials, int rpw, int parity, hipStream_t st);
 template <class T>
+void launch_xr(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, const SolveScal *scal,
+               double2 *partials, int rpw, int parity, hipStream_t st);
+// true residual into p2 (+ partial r_t.z_t), then r := p2 where replaced
+template <class T>
 void launch_rr(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, const SolveScal *scal,
-               int rpw, hipStream_t st);
+               double2 *partials, int rpw, hipStream_t st);
+template <class T>
+void launch_rr_commit(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, const SolveScal *scal,
+                      int rpw, hipStream_t st);
 void launch_reduce(const InstDesc *d_desc, int ninst, const double2 *partials, SolveScal *scal, int mode,
                    int max_iter, double tol2, hipStream_t st);
 template <class T>

@@ -401,174 +401,115 @@ __global__ __launch_bounds__(256) void k_assemble(const InstDesc *descs, const i
 }
 
 // ═══════════════════════════════════════════════════════════════════════════
-// Start of a solve (and second half of a true-residual replacement):
-// u = r * dinv, wv = L u, partial sums of r.u and wv.u.
+// Start of a solve: partial sums of r.z with z = r * dinv, the Jacobi-
+// preconditioned residual (cuphu_laplace.cu:437-446).  Elementwise over the
+// stencil tiles, so its partial sums line up with the other kernels'.
 // ═══════════════════════════════════════════════════════════════════════════
 template <class T, int RPW>
-__global__ __launch_bounds__(256) void k_start(const InstDesc *descs, const int *blk_inst, Planes<T> pl,
-                                               const SolveScal *scal, int force, double2 *partials) {
+__global__ __launch_bounds__(256) void k_rz(const InstDesc *descs, const int *blk_inst, Planes<T> pl,
+                                            const SolveScal *scal, int force, double2 *partials) {
     constexpr int TH = 4 * RPW;
-    __shared__ T lu[(TH + 2) * kLdsStride];
     TileInfo t = get_tile(descs, blk_inst, TH);
     if (!force && !scal[t.inst].active) return;
     int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const InstDesc &d = t.d;
     int c = t.tx0 + 4 * lane;
-    bool cok = c < d.pitch;
-    T rr[RPW][4];
-    for (int k = 0; k < RPW; ++k) {
-        int lr = wave * RPW + k, row = t.ty0 + lr;
-        T u[4];
-        zero4(u);
-        zero4(rr[k]);
-        if (row < d.nrow && cok) {
+    double g = 0.0;
+    if (c < d.pitch) {
+        for (int k = 0; k < RPW; ++k) {
+            int row = t.ty0 + wave * RPW + k;
+            if (row >= d.nrow) break;
             long long e = d.off + (long long)row * d.pitch + c;
-            T di[4];
-            ld4(pl.r + e, rr[k]);
+            T r4[4], di[4];
+            ld4(pl.r + e, r4);
             ld4(pl.dinv + e, di);
 #pragma unroll
-            for (int j = 0; j < 4; ++j) u[j] = rr[k][j] * di[j];
+            for (int j = 0; j < 4; ++j) g += (double)r4[j] * (double)(r4[j] * di[j]);
         }
-        T *lq = lu + (lr + 1) * kLdsStride + kLdsC0 + 4 * lane;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) lq[j] = u[j];
     }
-    fill_halo<TH>(
-        t, wave, lane,
-        [&](int lr, int lc, long long e, bool ok) {
-            T u[4];
-            zero4(u);
-            if (ok) {
-                T r4[4], di[4];
-                ld4(pl.r + e, r4);
-                ld4(pl.dinv + e, di);
-#pragma unroll
-                for (int j = 0; j < 4; ++j) u[j] = r4[j] * di[j];
-            }
-#pragma unroll
-            for (int j = 0; j < 4; ++j) lu[lr * kLdsStride + lc + j] = u[j];
-        },
-        [&](int lr, int lc, long long e, bool ok) { lu[lr * kLdsStride + lc] = ok ? pl.r[e] * pl.dinv[e] : T(0); });
-    __syncthreads();
-    double g = 0.0, dl = 0.0;
-    T wu[4];
-    for (int k = 0; k < RPW; ++k) {
-        int lr = wave * RPW + k, row = t.ty0 + lr;
-        if (row >= d.nrow) break;
-        if (k == 0) up_arcs(pl.w2, t, row, cok, lane, wu);
-        T wd[4], wr[4], wl[4], y[4];
-        row_arcs(pl.w2, t, row, cok, lane, wd, wr, wl);
-        stencil4(lu, lr, lane, wd, wu, wr, wl, y);
-        if (cok) {
-            long long e = d.off + (long long)row * d.pitch + c;
-            st4(pl.wv + e, y);
-            const T *uc = lu + (lr + 1) * kLdsStride + kLdsC0 + 4 * lane;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                g += (double)rr[k][j] * (double)uc[j];
-                dl += (double)y[j] * (double)uc[j];
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < 4; ++j) wu[j] = wd[j];
-    }
-    block_sum2(g, dl, partials + t.blk);
+    block_sum2(g, 0.0, partials + t.blk);
 }
 
 // ═══════════════════════════════════════════════════════════════════════════
-// One fused PCG iteration in the single-reduction (Chronopoulos-Gear) form.
-// With alpha_i, beta_i from the previous reduction:
-//   p = u + beta p;  s = wv + beta s;  x += alpha p;  r -= alpha s;
-//   u' = r * dinv;   wv = L u';  partial sums of r.u' and wv.u'.
-// u = r * dinv is recomputed, never stored.  The halo values of u' are
-// recomputed from r, s, wv, dinv of the neighbouring pixels with the same
-// operations, so they equal the neighbours' own values bit for bit.  Those
-// three vectors are therefore read from one plane set and written to the
-// other (parity), so a neighbour's old values are never overwritten while
-// this workgroup still reads them; x and p are only read at the own pixel
-// and stay in place.
+// PCG direction and operator, fused (cuphu_laplace.cu:493-498 then
+// :452-458):
+//   p = z + beta p   (z = r * dinv; p = z on a solve's first step, beta = 0)
+//   q = L p,  partial sums of p.q.
+// The halo values of p are recomputed from r, dinv and the old direction of
+// the neighbouring pixels with the same operations, so they equal the values
+// the neighbouring workgroup writes bit for bit.  The old direction is read
+// from one plane and the new one written to the other (parity 0: p -> p2,
+// parity 1: p2 -> p), so no workgroup overwrites a value that a neighbour
+// still reads.  q is stored in the s plane.
 // ═══════════════════════════════════════════════════════════════════════════
 template <class T, int RPW>
-__global__ __launch_bounds__(256) void k_cg(const InstDesc *descs, const int *blk_inst, Planes<T> pl,
+__global__ __launch_bounds__(256) void k_pq(const InstDesc *descs, const int *blk_inst, Planes<T> pl,
                                             const SolveScal *scal, double2 *partials, int parity) {
     constexpr int TH = 4 * RPW;
-    __shared__ T lu[(TH + 2) * kLdsStride];
+    __shared__ T lp[(TH + 2) * kLdsStride];
     TileInfo t = get_tile(descs, blk_inst, TH);
     const SolveScal *sc = scal + t.inst;
     if (!sc->active) return;
-    const T al = (T)sc->alpha, be = (T)sc->beta;
-    const T *r_in = parity ? pl.r2 : pl.r, *s_in = parity ? pl.s2 : pl.s, *wv_in = parity ? pl.wv2 : pl.wv;
-    T *r_out = parity ? pl.r : pl.r2, *s_out = parity ? pl.s : pl.s2, *wv_out = parity ? pl.wv : pl.wv2;
+    const T be = (T)sc->beta;
+    const bool keep = sc->beta != 0.0;   // first step of a solve: p = z
+    const T *p_in = parity ? pl.p2 : pl.p;
+    T *p_out = parity ? pl.p : pl.p2;
     int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const InstDesc &d = t.d;
     int c = t.tx0 + 4 * lane;
     bool cok = c < d.pitch;
-    T rr[RPW][4];
     for (int k = 0; k < RPW; ++k) {
         int lr = wave * RPW + k, row = t.ty0 + lr;
-        T un[4];
-        zero4(un);
-        zero4(rr[k]);
+        T pn[4];
+        zero4(pn);
         if (row < d.nrow && cok) {
             long long e = d.off + (long long)row * d.pitch + c;
-            T r4[4], w4[4], s4[4], p4[4], x4[4], di[4];
-            ld4(r_in + e, r4);
-            ld4(wv_in + e, w4);
-            ld4(s_in + e, s4);
-            ld4(pl.p + e, p4);
-            ld4(pl.x + e, x4);
+            T r4[4], di[4], po[4];
+            ld4(pl.r + e, r4);
             ld4(pl.dinv + e, di);
+            if (keep) ld4(p_in + e, po);
+            else zero4(po);
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                T uo = r4[j] * di[j];
-                T pn = Prec<T>::fmaT(be, p4[j], uo);
-                T sn = Prec<T>::fmaT(be, s4[j], w4[j]);
-                x4[j] = Prec<T>::fmaT(al, pn, x4[j]);
-                r4[j] = Prec<T>::fmaT(-al, sn, r4[j]);
-                un[j] = r4[j] * di[j];
-                p4[j] = pn;
-                s4[j] = sn;
-                rr[k][j] = r4[j];
+                T z = r4[j] * di[j];
+                pn[j] = keep ? Prec<T>::fmaT(be, po[j], z) : z;
             }
-            st4(pl.p + e, p4);
-            st4(s_out + e, s4);
-            st4(pl.x + e, x4);
-            st4(r_out + e, r4);
+            st4(p_out + e, pn);
         }
-        T *lq = lu + (lr + 1) * kLdsStride + kLdsC0 + 4 * lane;
+        T *lq = lp + (lr + 1) * kLdsStride + kLdsC0 + 4 * lane;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) lq[j] = un[j];
+        for (int j = 0; j < 4; ++j) lq[j] = pn[j];
     }
     fill_halo<TH>(
         t, wave, lane,
         [&](int lr, int lc, long long e, bool ok) {
-            T un[4];
-            zero4(un);
+            T pn[4];
+            zero4(pn);
             if (ok) {
-                T r4[4], w4[4], s4[4], di[4];
-                ld4(r_in + e, r4);
-                ld4(wv_in + e, w4);
-                ld4(s_in + e, s4);
+                T r4[4], di[4], po[4];
+                ld4(pl.r + e, r4);
                 ld4(pl.dinv + e, di);
+                if (keep) ld4(p_in + e, po);
+                else zero4(po);
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
-                    T sn = Prec<T>::fmaT(be, s4[j], w4[j]);
-                    un[j] = Prec<T>::fmaT(-al, sn, r4[j]) * di[j];
+                    T z = r4[j] * di[j];
+                    pn[j] = keep ? Prec<T>::fmaT(be, po[j], z) : z;
                 }
             }
 #pragma unroll
-            for (int j = 0; j < 4; ++j) lu[lr * kLdsStride + lc + j] = un[j];
+            for (int j = 0; j < 4; ++j) lp[lr * kLdsStride + lc + j] = pn[j];
         },
         [&](int lr, int lc, long long e, bool ok) {
             T v = T(0);
             if (ok) {
-                T sn = Prec<T>::fmaT(be, s_in[e], wv_in[e]);
-                v = Prec<T>::fmaT(-al, sn, r_in[e]) * pl.dinv[e];
+                T z = pl.r[e] * pl.dinv[e];
+                v = keep ? Prec<T>::fmaT(be, p_in[e], z) : z;
             }
-            lu[lr * kLdsStride + lc] = v;
+            lp[lr * kLdsStride + lc] = v;
         });
     __syncthreads();
-    double g = 0.0, dl = 0.0;
+    double pq = 0.0;
     T wu[4];
     for (int k = 0; k < RPW; ++k) {
         int lr = wave * RPW + k, row = t.ty0 + lr;
@@ -576,32 +517,71 @@ __global__ __launch_bounds__(256) void k_cg(const InstDesc *descs, const int *bl
         if (k == 0) up_arcs(pl.w2, t, row, cok, lane, wu);
         T wd[4], wr[4], wl[4], y[4];
         row_arcs(pl.w2, t, row, cok, lane, wd, wr, wl);
-        stencil4(lu, lr, lane, wd, wu, wr, wl, y);
+        stencil4(lp, lr, lane, wd, wu, wr, wl, y);
         if (cok) {
             long long e = d.off + (long long)row * d.pitch + c;
-            st4(wv_out + e, y);
-            const T *uc = lu + (lr + 1) * kLdsStride + kLdsC0 + 4 * lane;
+            st4(pl.s + e, y);
+            const T *pc = lp + (lr + 1) * kLdsStride + kLdsC0 + 4 * lane;
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                g += (double)rr[k][j] * (double)uc[j];
-                dl += (double)y[j] * (double)uc[j];
-            }
+            for (int j = 0; j < 4; ++j) pq += (double)pc[j] * (double)y[j];
         }
 #pragma unroll
         for (int j = 0; j < 4; ++j) wu[j] = wd[j];
     }
-    block_sum2(g, dl, partials + t.blk);
+    block_sum2(pq, 0.0, partials + t.blk);
 }
 
 // ═══════════════════════════════════════════════════════════════════════════
-// True-residual replacement, first half: r = b - L x and s = L p.
+// PCG update (cuphu_laplace.cu:466-476): x += alpha p, r -= alpha q,
+// z = r * dinv, partial sums of r.z.  p is the direction k_pq of the same
+// parity wrote.
+// ═══════════════════════════════════════════════════════════════════════════
+template <class T, int RPW>
+__global__ __launch_bounds__(256) void k_xr(const InstDesc *descs, const int *blk_inst, Planes<T> pl,
+                                            const SolveScal *scal, double2 *partials, int parity) {
+    constexpr int TH = 4 * RPW;
+    TileInfo t = get_tile(descs, blk_inst, TH);
+    const SolveScal *sc = scal + t.inst;
+    if (!sc->active) return;
+    const T al = (T)sc->alpha;
+    const T *pd = parity ? pl.p : pl.p2;
+    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const InstDesc &d = t.d;
+    int c = t.tx0 + 4 * lane;
+    double g = 0.0;
+    if (c < d.pitch) {
+        for (int k = 0; k < RPW; ++k) {
+            int row = t.ty0 + wave * RPW + k;
+            if (row >= d.nrow) break;
+            long long e = d.off + (long long)row * d.pitch + c;
+            T x4[4], p4[4], r4[4], q4[4], di[4];
+            ld4(pl.x + e, x4);
+            ld4(pd + e, p4);
+            ld4(pl.r + e, r4);
+            ld4(pl.s + e, q4);
+            ld4(pl.dinv + e, di);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                x4[j] = Prec<T>::fmaT(al, p4[j], x4[j]);
+                r4[j] = Prec<T>::fmaT(-al, q4[j], r4[j]);
+                g += (double)r4[j] * (double)(r4[j] * di[j]);
+            }
+            st4(pl.x + e, x4);
+            st4(pl.r + e, r4);
+        }
+    }
+    block_sum2(g, 0.0, partials + t.blk);
+}
+
+// ═══════════════════════════════════════════════════════════════════════════
+// True residual r_t = b - L x into the p2 plane (free at the end of an even
+// chunk: the current direction is in p), with partial sums of r_t.z_t.
 // ═══════════════════════════════════════════════════════════════════════════
 template <class T, int RPW>
 __global__ __launch_bounds__(256) void k_rr(const InstDesc *descs, const int *blk_inst, Planes<T> pl,
-                                            const SolveScal *scal) {
+                                            const SolveScal *scal, double2 *partials) {
     constexpr int TH = 4 * RPW;
     __shared__ T lx[(TH + 2) * kLdsStride];
-    __shared__ T lp[(TH + 2) * kLdsStride];
     TileInfo t = get_tile(descs, blk_inst, TH);
     if (!scal[t.inst].active) return;
     int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -610,63 +590,84 @@ __global__ __launch_bounds__(256) void k_rr(const InstDesc *descs, const int *bl
     bool cok = c < d.pitch;
     for (int k = 0; k < RPW; ++k) {
         int lr = wave * RPW + k, row = t.ty0 + lr;
-        T xv[4], pv[4];
-        zero4(xv); zero4(pv);
-        if (row < d.nrow && cok) {
-            long long e = d.off + (long long)row * d.pitch + c;
-            ld4(pl.x + e, xv);
-            ld4(pl.p + e, pv);
-        }
-        int o = (lr + 1) * kLdsStride + kLdsC0 + 4 * lane;
+        T xv[4];
+        zero4(xv);
+        if (row < d.nrow && cok) ld4(pl.x + d.off + (long long)row * d.pitch + c, xv);
+        T *lq = lx + (lr + 1) * kLdsStride + kLdsC0 + 4 * lane;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) { lx[o + j] = xv[j]; lp[o + j] = pv[j]; }
+        for (int j = 0; j < 4; ++j) lq[j] = xv[j];
     }
     fill_halo<TH>(
         t, wave, lane,
         [&](int lr, int lc, long long e, bool ok) {
-            T xv[4], pv[4];
-            zero4(xv); zero4(pv);
-            if (ok) { ld4(pl.x + e, xv); ld4(pl.p + e, pv); }
+            T xv[4];
+            zero4(xv);
+            if (ok) ld4(pl.x + e, xv);
 #pragma unroll
-            for (int j = 0; j < 4; ++j) { lx[lr * kLdsStride + lc + j] = xv[j]; lp[lr * kLdsStride + lc + j] = pv[j]; }
+            for (int j = 0; j < 4; ++j) lx[lr * kLdsStride + lc + j] = xv[j];
         },
-        [&](int lr, int lc, long long e, bool ok) {
-            lx[lr * kLdsStride + lc] = ok ? pl.x[e] : T(0);
-            lp[lr * kLdsStride + lc] = ok ? pl.p[e] : T(0);
-        });
+        [&](int lr, int lc, long long e, bool ok) { lx[lr * kLdsStride + lc] = ok ? pl.x[e] : T(0); });
     __syncthreads();
+    double g = 0.0;
     T wu[4];
     for (int k = 0; k < RPW; ++k) {
         int lr = wave * RPW + k, row = t.ty0 + lr;
         if (row >= d.nrow) break;
         if (k == 0) up_arcs(pl.w2, t, row, cok, lane, wu);
-        T wd[4], wr[4], wl[4], y1[4], y2[4];
+        T wd[4], wr[4], wl[4], y[4];
         row_arcs(pl.w2, t, row, cok, lane, wd, wr, wl);
-        stencil4(lx, lr, lane, wd, wu, wr, wl, y1);
-        stencil4(lp, lr, lane, wd, wu, wr, wl, y2);
+        stencil4(lx, lr, lane, wd, wu, wr, wl, y);
         if (cok) {
             long long e = d.off + (long long)row * d.pitch + c;
-            T b4[4], r4[4];
+            T b4[4], di[4], r4[4];
             ld4(pl.b + e, b4);
+            ld4(pl.dinv + e, di);
 #pragma unroll
-            for (int j = 0; j < 4; ++j) r4[j] = b4[j] - y1[j];
-            st4(pl.r + e, r4);
-            st4(pl.s + e, y2);
+            for (int j = 0; j < 4; ++j) {
+                r4[j] = b4[j] - y[j];
+                g += (double)r4[j] * (double)(r4[j] * di[j]);
+            }
+            st4(pl.p2 + e, r4);
         }
 #pragma unroll
         for (int j = 0; j < 4; ++j) wu[j] = wd[j];
+    }
+    block_sum2(g, 0.0, partials + t.blk);
+}
+
+// Copy the true residual over r where the reduction chose to replace it.
+template <class T, int RPW>
+__global__ __launch_bounds__(256) void k_rr_commit(const InstDesc *descs, const int *blk_inst, Planes<T> pl,
+                                                   const SolveScal *scal) {
+    constexpr int TH = 4 * RPW;
+    TileInfo t = get_tile(descs, blk_inst, TH);
+    if (!scal[t.inst].active || !scal[t.inst].replace) return;
+    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const InstDesc &d = t.d;
+    int c = t.tx0 + 4 * lane;
+    if (c >= d.pitch) return;
+    for (int k = 0; k < RPW; ++k) {
+        int row = t.ty0 + wave * RPW + k;
+        if (row >= d.nrow) break;
+        long long e = d.off + (long long)row * d.pitch + c;
+        T v[4];
+        ld4(pl.p2 + e, v);
+        st4(pl.r + e, v);
     }
 }
 
 // ═══════════════════════════════════════════════════════════════════════════
 // Per-instance reduction of the workgroup partial sums (fixed order) and
-// the scalar recurrences, convergence test and status (one workgroup per
-// instance).
-//   start: gamma0 = r.u; alpha = r.u / wv.u; beta = 0  (skip if r.u <= 0,
-//          :450; stop if p.Lp <= 0, :460)
-//   step:  iters++; converged if r.u < tol^2 gamma0 (:478); else
-//          beta = gamma/gamma_prev, alpha = gamma/(delta - beta*gamma/alpha_prev)
-//   rr:    as step with the replaced gamma/delta, without counting.
+// the scalar recurrences of the reference loop (one workgroup per
+// instance, cuphu_laplace.cu:446-499):
+//   start: rz0 = r.z; stop if rz0 is not positive (:446-450)
+//   pq:    stop on p.q <= 0 (:460); alpha = rz / p.q (:462)
+//   rz:    iteration done; converged if rz' < tol^2 rz0 (:478);
+//          beta = rz' / rz (:493)
+//   rr:    true residual r_t.z_t: converged if below tol^2 rz0; otherwise
+//          r := r_t (beta = r_t.z_t / rz_prev) unless r_t.z_t exceeds the
+//          recursive rz by more than 4x, i.e. the recursion has gone below
+//          the accuracy the arithmetic attains and continues unreplaced.
 // ═══════════════════════════════════════════════════════════════════════════
 __global__ __launch_bounds__(256) void k_reduce(const InstDesc *descs, const double2 *partials, SolveScal *scal,
                                                 int mode, int max_iter, double tol2) {
@@ -675,38 +676,43 @@ __global__ __launch_bounds__(256) void k_reduce(const InstDesc *descs, const dou
     SolveScal *sp = scal + inst;
     if (mode != RED_START && !sp->active) return;
     int nb = d.tiles_x * d.tiles_y;
-    double a = 0.0, b = 0.0;
-    for (int i = threadIdx.x; i < nb; i += 256) {
-        double2 v = partials[d.block_begin + i];
-        a += v.x;
-        b += v.y;
-    }
+    double a = 0.0;
+    for (int i = threadIdx.x; i < nb; i += 256) a += partials[d.block_begin + i].x;
     __shared__ double2 res;
-    block_sum2(a, b, &res);
+    block_sum2(a, 0.0, &res);
     __syncthreads();
     if (threadIdx.x != 0) return;
     SolveScal s = *sp;
-    double gam = res.x, del = res.y;
+    const double v = res.x;
+    s.replace = 0;
     if (mode == RED_START) {
         s.iters = 0;
-        s.gamma0 = gam; s.gamma = gam; s.gamma_prev = gam; s.delta = del; s.beta = 0.0; s.relres = 1.0;
+        s.gamma0 = v; s.gamma = v; s.gamma_prev = v; s.delta = 0.0; s.alpha = 0.0; s.beta = 0.0;
+        s.relres = 1.0;
         s.active = 0;
-        if (!(gam > 0.0)) { s.status = (gam == 0.0) ? CUPHU_SOLVE_EMPTY : CUPHU_SOLVE_NONFINITE; s.relres = 0.0; }
+        if (!(v > 0.0)) { s.status = (v == 0.0) ? CUPHU_SOLVE_EMPTY : CUPHU_SOLVE_NONFINITE; s.relres = 0.0; }
         else if (max_iter <= 0) s.status = CUPHU_SOLVE_MAXITER;
-        else if (!(del > 0.0)) s.status = isfinite(del) ? CUPHU_SOLVE_BREAKDOWN : CUPHU_SOLVE_NONFINITE;
-        else { s.alpha = gam / del; s.alpha_prev = s.alpha; s.active = 1; s.status = CUPHU_SOLVE_MAXITER; }
-    } else {
-        if (mode == RED_STEP) { s.iters += 1; s.gamma_prev = s.gamma; s.alpha_prev = s.alpha; }
-        s.gamma = gam; s.delta = del;
-        s.relres = (gam >= 0.0 && s.gamma0 > 0.0) ? sqrt(gam / s.gamma0) : s.relres;
-        if (!isfinite(gam) || !isfinite(del)) { s.status = CUPHU_SOLVE_NONFINITE; s.active = 0; }
-        else if (gam < tol2 * s.gamma0) { s.status = CUPHU_SOLVE_CONVERGED; s.active = 0; }
-        else if (s.iters >= max_iter) { s.status = CUPHU_SOLVE_MAXITER; s.active = 0; }
-        else {
-            double beta = gam / s.gamma_prev;
-            double denom = del - beta * gam / s.alpha_prev;
-            if (!(denom > 0.0)) { s.status = isfinite(denom) ? CUPHU_SOLVE_BREAKDOWN : CUPHU_SOLVE_NONFINITE; s.active = 0; }
-            else { s.beta = beta; s.alpha = gam / denom; }
+        else { s.active = 1; s.status = CUPHU_SOLVE_MAXITER; }
+    } else if (mode == RED_PQ) {
+        s.delta = v;
+        if (!(v > 0.0)) { s.status = isfinite(v) ? CUPHU_SOLVE_BREAKDOWN : CUPHU_SOLVE_NONFINITE; s.active = 0; }
+        else s.alpha = s.gamma / v;
+    } else if (mode == RED_RZ) {
+        s.iters += 1;
+        s.relres = (s.gamma0 > 0.0) ? sqrt(fabs(v) / s.gamma0) : 0.0;
+        if (v < tol2 * s.gamma0) { s.status = CUPHU_SOLVE_CONVERGED; s.active = 0; s.gamma = v; }
+        else if (s.iters >= max_iter) { s.status = CUPHU_SOLVE_MAXITER; s.active = 0; s.gamma = v; }
+        else { s.beta = v / s.gamma; s.gamma_prev = s.gamma; s.gamma = v; }
+    } else {   // RED_RR
+        if (!isfinite(v)) { s.status = CUPHU_SOLVE_NONFINITE; s.active = 0; }
+        else if (v < tol2 * s.gamma0) {
+            s.status = CUPHU_SOLVE_CONVERGED; s.active = 0; s.gamma = v;
+            s.relres = (s.gamma0 > 0.0) ? sqrt(v / s.gamma0) : 0.0;
+        } else if (v <= 4.0 * s.gamma) {
+            s.replace = 1;
+            s.beta = v / s.gamma_prev;
+            s.gamma = v;
+            s.relres = sqrt(v / s.gamma0);
         }
     }
     *sp = s;
@@ -880,21 +886,33 @@ void launch_assemble(const InstDesc *d_desc, const int *d_blk, int nblk, const P
     CUPHU_LAUNCH_CHECK();
 }
 template <class T>
-void launch_start(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, const SolveScal *scal,
-                  int force, double2 *partials, int rpw, hipStream_t st) {
-    CUPHU_RPW_DISPATCH(rpw, k_start, dim3(nblk), dim3(256), 0, st, d_desc, d_blk, pl, scal, force, partials);
+void launch_rz(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, const SolveScal *scal,
+               int force, double2 *partials, int rpw, hipStream_t st) {
+    CUPHU_RPW_DISPATCH(rpw, k_rz, dim3(nblk), dim3(256), 0, st, d_desc, d_blk, pl, scal, force, partials);
     CUPHU_LAUNCH_CHECK();
 }
 template <class T>
-void launch_cg(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, const SolveScal *scal,
+void launch_pq(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, const SolveScal *scal,
                double2 *partials, int rpw, int parity, hipStream_t st) {
-    CUPHU_RPW_DISPATCH(rpw, k_cg, dim3(nblk), dim3(256), 0, st, d_desc, d_blk, pl, scal, partials, parity);
+    CUPHU_RPW_DISPATCH(rpw, k_pq, dim3(nblk), dim3(256), 0, st, d_desc, d_blk, pl, scal, partials, parity);
+    CUPHU_LAUNCH_CHECK();
+}
+template <class T>
+void launch_xr(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, const SolveScal *scal,
+               double2 *partials, int rpw, int parity, hipStream_t st) {
+    CUPHU_RPW_DISPATCH(rpw, k_xr, dim3(nblk), dim3(256), 0, st, d_desc, d_blk, pl, scal, partials, parity);
     CUPHU_LAUNCH_CHECK();
 }
 template <class T>
 void launch_rr(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, const SolveScal *scal,
-               int rpw, hipStream_t st) {
-    CUPHU_RPW_DISPATCH(rpw, k_rr, dim3(nblk), dim3(256), 0, st, d_desc, d_blk, pl, scal);
+               double2 *partials, int rpw, hipStream_t st) {
+    CUPHU_RPW_DISPATCH(rpw, k_rr, dim3(nblk), dim3(256), 0, st, d_desc, d_blk, pl, scal, partials);
+    CUPHU_LAUNCH_CHECK();
+}
+template <class T>
+void launch_rr_commit(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, const SolveScal *scal,
+                      int rpw, hipStream_t st) {
+    CUPHU_RPW_DISPATCH(rpw, k_rr_commit, dim3(nblk), dim3(256), 0, st, d_desc, d_blk, pl, scal);
     CUPHU_LAUNCH_CHECK();
 }
 void launch_reduce(const InstDesc *d_desc, int ninst, const double2 *partials, SolveScal *scal, int mode,
@@ -948,12 +966,16 @@ void launch_plain_dot(const T *a, const T *b, size_t n, double *partials, double
     template void launch_weights<T>(const InstDesc *, const int *, int, const Planes<T> &, int, T, int,          \
                                     hipStream_t);                                                                \
     template void launch_assemble<T>(const InstDesc *, const int *, int, const Planes<T> &, int, hipStream_t);   \
-    template void launch_start<T>(const InstDesc *, const int *, int, const Planes<T> &, const SolveScal *, int, \
-                                  double2 *, int, hipStream_t);                                                  \
-    template void launch_cg<T>(const InstDesc *, const int *, int, const Planes<T> &, const SolveScal *,         \
+    template void launch_rz<T>(const InstDesc *, const int *, int, const Planes<T> &, const SolveScal *, int,    \
+                               double2 *, int, hipStream_t);                                                     \
+    template void launch_pq<T>(const InstDesc *, const int *, int, const Planes<T> &, const SolveScal *,         \
                                double2 *, int, int, hipStream_t);                                                \
-    template void launch_rr<T>(const InstDesc *, const int *, int, const Planes<T> &, const SolveScal *, int,    \
-                               hipStream_t);                                                                     \
+    template void launch_xr<T>(const InstDesc *, const int *, int, const Planes<T> &, const SolveScal *,         \
+                               double2 *, int, int, hipStream_t);                                                \
+    template void launch_rr<T>(const InstDesc *, const int *, int, const Planes<T> &, const SolveScal *,         \
+                               double2 *, int, hipStream_t);                                                     \
+    template void launch_rr_commit<T>(const InstDesc *, const int *, int, const Planes<T> &, const SolveScal *,  \
+                                      int, hipStream_t);                                                         \
     template void launch_final<T>(const InstDesc *, const int *, int, const Planes<T> &, float *, int,           \
                                   hipStream_t);                                                                  \
     template void launch_plain_matvec<T>(const InstDesc &, const T *, const T *, T *, hipStream_t);              \

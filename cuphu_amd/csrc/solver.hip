@@ -172,9 +172,7 @@ template <class T> static PoolSet<T> alloc_pools(DevRes &r, long long n) {
     P.pl.p = ws<T>(r, nm("p").c_str(), n);
     P.pl.s = ws<T>(r, nm("s").c_str(), n);
     P.pl.wv = ws<T>(r, nm("wv").c_str(), n);
-    P.pl.r2 = ws<T>(r, nm("r2").c_str(), n);
-    P.pl.s2 = ws<T>(r, nm("s2").c_str(), n);
-    P.pl.wv2 = ws<T>(r, nm("wv2").c_str(), n);
+    P.pl.p2 = ws<T>(r, nm("p2").c_str(), n);
     P.out = ws<float>(r, "out", n);
     P.lab = ws<uint32_t>(r, "lab", n);
     return P;
@@ -216,41 +214,49 @@ struct GroupDev {
     std::vector<long long> px;   // host: logical pixels per instance
 };
 
-// Live timing of the fused PCG step (cuphu_set_profiling).
+// Live timing of the two PCG kernels (cuphu_set_profiling).
 struct KernelProf {
     bool on = false;
-    hipEvent_t a = nullptr, b = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     CuPhuKernelProfile acc{};
 };
 static KernelProf g_prof;
+
+// Algorithmic HBM bytes per active pixel of the two PCG kernels.
+//   direction + operator: reads r, dinv, p_old and the {down, right} arc
+//   pair, writes p_new and q: 7 values.
+//   update: reads x, p, r, q, dinv, writes x and r: 7 values.
+constexpr double kPqValuesPerPx = 7.0, kXrValuesPerPx = 7.0;
 
 template <class T>
 static void solve_group_batched(const Planes<T> &pl, const GroupDev &g, const CuPhuLaplaceOptions &o, int rpw,
                                 bool fixed_weights, hipStream_t st, const std::vector<CuPhuSolveStats *> &stats,
                                 std::vector<SolveScal> &h) {
-    // Chunk length: even (the fused step alternates plane sets, so an even
-    // chunk returns the state to the first set for the residual
-    // replacement), at least 2.
+    // Chunk length: even (the direction kernel alternates its two planes, so
+    // an even chunk leaves the current direction in p and p2 free for the
+    // true residual), at least 2.
     int ch = o.rr_interval > 0 ? o.rr_interval : 50;
     ch = std::max(2, std::min(ch, std::max(o.max_iter, 2)));
     const int CH = ch + (ch & 1);
     const double tol2 = o.tol * o.tol;
     const bool prof = g_prof.on;
-    if (prof && !g_prof.a) {
-        CUPHU_HIP_CHECK(hipEventCreate(&g_prof.a));
-        CUPHU_HIP_CHECK(hipEventCreate(&g_prof.b));
-    }
+    if (prof && !g_prof.ev[0])
+        for (auto &e : g_prof.ev) CUPHU_HIP_CHECK(hipEventCreate(&e));
+    auto step = [&](int parity) {
+        launch_pq<T>(g.desc, g.blk, g.nblk, pl, g.scal, g.part, rpw, parity, st);
+        launch_reduce(g.desc, g.ninst, g.part, g.scal, RED_PQ, o.max_iter, tol2, st);
+        launch_xr<T>(g.desc, g.blk, g.nblk, pl, g.scal, g.part, rpw, parity, st);
+        launch_reduce(g.desc, g.ninst, g.part, g.scal, RED_RZ, o.max_iter, tol2, st);
+    };
     // With profiling on, the first two steps of a chunk run outside the
-    // graph (the first between two events); the captured part holds CH-2.
+    // graph (the first with events around each kernel); the captured part
+    // holds CH-2.
     auto chunk = [&]() {
-        for (int i = prof ? 2 : 0; i < CH; ++i) {
-            launch_cg<T>(g.desc, g.blk, g.nblk, pl, g.scal, g.part, rpw, i & 1, st);
-            launch_reduce(g.desc, g.ninst, g.part, g.scal, RED_STEP, o.max_iter, tol2, st);
-        }
+        for (int i = prof ? 2 : 0; i < CH; ++i) step(i & 1);
         if (o.rr_interval > 0) {
-            launch_rr<T>(g.desc, g.blk, g.nblk, pl, g.scal, rpw, st);
-            launch_start<T>(g.desc, g.blk, g.nblk, pl, g.scal, 0, g.part, rpw, st);
+            launch_rr<T>(g.desc, g.blk, g.nblk, pl, g.scal, g.part, rpw, st);
             launch_reduce(g.desc, g.ninst, g.part, g.scal, RED_RR, o.max_iter, tol2, st);
+            launch_rr_commit<T>(g.desc, g.blk, g.nblk, pl, g.scal, rpw, st);
         }
     };
     hipGraphExec_t gexec = nullptr;
@@ -281,7 +287,7 @@ static void solve_group_batched(const Planes<T> &pl, const GroupDev &g, const Cu
                 launch_weights<T>(g.desc, g.blk, g.nblk, pl, pass > 0 ? 1 : 0, d2, rpw, st);
                 launch_assemble<T>(g.desc, g.blk, g.nblk, pl, rpw, st);
             }
-            launch_start<T>(g.desc, g.blk, g.nblk, pl, g.scal, 1, g.part, rpw, st);
+            launch_rz<T>(g.desc, g.blk, g.nblk, pl, g.scal, 1, g.part, rpw, st);
             launch_reduce(g.desc, g.ninst, g.part, g.scal, RED_START, o.max_iter, tol2, st);
             CUPHU_HIP_CHECK(hipMemcpyAsync(h.data(), g.scal, h.size() * sizeof(SolveScal), hipMemcpyDeviceToHost, st));
             CUPHU_HIP_CHECK(hipStreamSynchronize(st));
@@ -295,25 +301,32 @@ static void solve_group_batched(const Planes<T> &pl, const GroupDev &g, const Cu
                 if (prof) {
                     for (int i = 0; i < g.ninst; ++i)
                         if (h[(size_t)i].active) active_px += (double)g.px[(size_t)i];
-                    CUPHU_HIP_CHECK(hipEventRecord(g_prof.a, st));
-                    launch_cg<T>(g.desc, g.blk, g.nblk, pl, g.scal, g.part, rpw, 0, st);
-                    CUPHU_HIP_CHECK(hipEventRecord(g_prof.b, st));
-                    launch_reduce(g.desc, g.ninst, g.part, g.scal, RED_STEP, o.max_iter, tol2, st);
-                    launch_cg<T>(g.desc, g.blk, g.nblk, pl, g.scal, g.part, rpw, 1, st);
-                    launch_reduce(g.desc, g.ninst, g.part, g.scal, RED_STEP, o.max_iter, tol2, st);
+                    CUPHU_HIP_CHECK(hipEventRecord(g_prof.ev[0], st));
+                    launch_pq<T>(g.desc, g.blk, g.nblk, pl, g.scal, g.part, rpw, 0, st);
+                    CUPHU_HIP_CHECK(hipEventRecord(g_prof.ev[1], st));
+                    launch_reduce(g.desc, g.ninst, g.part, g.scal, RED_PQ, o.max_iter, tol2, st);
+                    CUPHU_HIP_CHECK(hipEventRecord(g_prof.ev[2], st));
+                    launch_xr<T>(g.desc, g.blk, g.nblk, pl, g.scal, g.part, rpw, 0, st);
+                    CUPHU_HIP_CHECK(hipEventRecord(g_prof.ev[3], st));
+                    launch_reduce(g.desc, g.ninst, g.part, g.scal, RED_RZ, o.max_iter, tol2, st);
+                    step(1);
                 }
                 if (gexec) CUPHU_HIP_CHECK(hipGraphLaunch(gexec, st));
                 else chunk();
                 CUPHU_HIP_CHECK(hipMemcpyAsync(h.data(), g.scal, h.size() * sizeof(SolveScal), hipMemcpyDeviceToHost, st));
                 CUPHU_HIP_CHECK(hipStreamSynchronize(st));
                 if (prof) {
-                    float ms = 0.0f;
-                    CUPHU_HIP_CHECK(hipEventElapsedTime(&ms, g_prof.a, g_prof.b));
-                    g_prof.acc.cg_samples += 1;
-                    g_prof.acc.cg_ms_sum += ms;
-                    g_prof.acc.cg_px_sum += active_px;
-                    g_prof.acc.cg_bytes_sum += active_px * (sizeof(T) == 4 ? 52.0 : 104.0);
-                    g_prof.acc.cg_launches += CH;
+                    float ms_pq = 0.0f, ms_xr = 0.0f;
+                    CUPHU_HIP_CHECK(hipEventElapsedTime(&ms_pq, g_prof.ev[0], g_prof.ev[1]));
+                    CUPHU_HIP_CHECK(hipEventElapsedTime(&ms_xr, g_prof.ev[2], g_prof.ev[3]));
+                    CuPhuKernelProfile &a = g_prof.acc;
+                    a.samples += 1;
+                    a.px_sum += active_px;
+                    a.pq_ms_sum += ms_pq;
+                    a.pq_bytes_sum += active_px * kPqValuesPerPx * sizeof(T);
+                    a.xr_ms_sum += ms_xr;
+                    a.xr_bytes_sum += active_px * kXrValuesPerPx * sizeof(T);
+                    a.iterations += CH;
                 }
                 ++nch;
                 if ((long long)nch * CH > (long long)o.max_iter + CH) break;   // cannot happen: reduce caps iterations
@@ -417,7 +430,8 @@ static void run_batch(const std::vector<std::pair<int, int>> &dims, const CuPhuL
     ingest(P.pl, L, d_gdesc, d_rowinst);
 
     // groups of consecutive instances bounded by the working-set budget
-    const double bpe = (sizeof(T) == 4) ? 57.0 : 101.0;
+    // bytes per pixel one PCG iteration touches: w2 pair + x, r, p, p2, q, dinv
+    const double bpe = 8.0 * sizeof(T);
     long long cap = o.group_bytes_mb > 0 ? (long long)((double)o.group_bytes_mb * 1048576.0 / bpe) : (1LL << 62);
     std::vector<std::pair<int, int>> groups;
     for (int i = 0; i < ninst;) {
@@ -1455,7 +1469,7 @@ void hook_setup(HookInst &h, DevRes &res, int nrow, int ncol, int rpw, hipStream
     unsigned g = (unsigned)ceil_div(2 * nel, 256);
     hipLaunchKernelGGL(k_zero<float>, dim3(g), dim3(256), 0, st, h.P.pl.w2, (long long)(2 * nel));
     for (float *p : {h.P.pl.b, h.P.pl.dinv, h.P.pl.x, h.P.pl.r, h.P.pl.p, h.P.pl.s, h.P.pl.wv, h.P.pl.phi,
-                     h.P.pl.r2, h.P.pl.s2, h.P.pl.wv2})
+                     h.P.pl.p2})
         hipLaunchKernelGGL(k_zero<float>, dim3(g), dim3(256), 0, st, p, (long long)nel);
     CUPHU_HIP_CHECK(hipGetLastError());
     CUPHU_HIP_CHECK(hipStreamSynchronize(st));
@@ -1569,11 +1583,11 @@ int cuphu_test_laplace_ops(const float *phase, const short *sigsq, const float *
     });
 }
 
-int cuphu_test_cg_step(const float *w, const float *dinv, int nrow, int ncol, double alpha, double beta, int rpw,
-                       float *x, float *r, float *p, float *s, float *wv, double *gamma, double *delta) {
+int cuphu_test_pcg_step(const float *w, const float *dinv, int nrow, int ncol, double gamma, double beta, int rpw,
+                        float *x, float *r, float *p, float *q, double *pq, double *alpha, double *rz) {
     return guarded([&]() {
         std::lock_guard<std::recursive_mutex> lk(g_mu);
-        CUPHU_REQUIRE(w && dinv && x && r && p && s && wv && gamma && delta && nrow >= 1 && ncol >= 1, "null argument");
+        CUPHU_REQUIRE(w && dinv && x && r && p && q && pq && alpha && rz && nrow >= 1 && ncol >= 1, "null argument");
         CUPHU_REQUIRE(rpw == 2 || rpw == 4 || rpw == 8, "rpw must be 2, 4 or 8");
         DevRes &res = devres();
         hipStream_t st = res.stream;
@@ -1591,27 +1605,28 @@ int cuphu_test_cg_step(const float *w, const float *dinv, int nrow, int ncol, do
         to_plane(x, h.P.pl.x, d, st);
         to_plane(r, h.P.pl.r, d, st);
         to_plane(p, h.P.pl.p, d, st);
-        to_plane(s, h.P.pl.s, d, st);
-        to_plane(wv, h.P.pl.wv, d, st);
         SolveScal sc{};
-        sc.alpha = alpha;
+        sc.gamma = gamma;
+        sc.gamma0 = gamma;
         sc.beta = beta;
         sc.active = 1;
+        sc.status = CUPHU_SOLVE_MAXITER;
         CUPHU_HIP_CHECK(hipMemcpyAsync(h.g.scal, &sc, sizeof(sc), hipMemcpyHostToDevice, st));
-        // parity 0: reads r/s/wv, writes r2/s2/wv2
-        launch_cg<float>(h.g.desc, h.g.blk, h.g.nblk, h.P.pl, h.g.scal, h.g.part, rpw, 0, st);
-        std::vector<double2> parts((size_t)h.g.nblk);
-        CUPHU_HIP_CHECK(hipMemcpyAsync(parts.data(), h.g.part, parts.size() * sizeof(double2), hipMemcpyDeviceToHost, st));
+        // parity 0: direction p -> p2, q -> s
+        launch_pq<float>(h.g.desc, h.g.blk, h.g.nblk, h.P.pl, h.g.scal, h.g.part, rpw, 0, st);
+        launch_reduce(h.g.desc, 1, h.g.part, h.g.scal, RED_PQ, 1 << 30, 0.0, st);
+        launch_xr<float>(h.g.desc, h.g.blk, h.g.nblk, h.P.pl, h.g.scal, h.g.part, rpw, 0, st);
+        launch_reduce(h.g.desc, 1, h.g.part, h.g.scal, RED_RZ, 1 << 30, 0.0, st);
         from_plane(h.P.pl.x, x, d, st);
-        from_plane(h.P.pl.r2, r, d, st);
-        from_plane(h.P.pl.p, p, d, st);
-        from_plane(h.P.pl.s2, s, d, st);
-        from_plane(h.P.pl.wv2, wv, d, st);
+        from_plane(h.P.pl.r, r, d, st);
+        from_plane(h.P.pl.p2, p, d, st);
+        from_plane(h.P.pl.s, q, d, st);
+        CUPHU_HIP_CHECK(hipMemcpyAsync(&sc, h.g.scal, sizeof(sc), hipMemcpyDeviceToHost, st));
         CUPHU_HIP_CHECK(hipStreamSynchronize(st));
-        double ga = 0.0, de = 0.0;
-        for (auto &v : parts) { ga += v.x; de += v.y; }
-        *gamma = ga;
-        *delta = de;
+        CUPHU_REQUIRE(sc.active == 1, "PCG step stopped (p.q not positive)");
+        *pq = sc.delta;
+        *alpha = sc.alpha;
+        *rz = sc.gamma;
         return 0;
     });
 }

@@ -418,18 +418,24 @@ int cuphu_wls_adjoint_gpu(const void *d_w, const void *d_phase, const void *d_u,
                           int *iters_out, double *relres_out,
                           cuphu_stream_t stream);
 
-/* Live timing of the fused PCG step kernel.  While enabled, the first
- * PCG step of every iteration chunk is launched outside the chunk's HIP
- * graph between two HIP events on the solver's stream; its duration and
- * algorithmic bytes (52 B per active pixel in FP32, 104 B in FP64: x, r, p,
- * s, L u and 1/diag read, x, r, p, s, L u written, two arc weights read)
- * are accumulated.  cuphu_set_profiling resets the counters. */
+/* Live timing of the two PCG kernels.  While enabled, the first PCG
+ * iteration of every iteration chunk is launched outside the chunk's HIP
+ * graph with HIP events around each kernel on the solver's stream; the
+ * durations and algorithmic bytes are accumulated per kernel:
+ *   pq: p = z + beta p, q = L p, p.q  -- reads r, 1/diag, p_old and two arc
+ *       weights, writes p_new and q (28 B per active pixel in FP32, 56 B in
+ *       FP64);
+ *   xr: x += alpha p, r -= alpha q, r.z -- reads x, p, r, q, 1/diag, writes
+ *       x and r (28 B / 56 B per active pixel).
+ * cuphu_set_profiling resets the counters. */
 typedef struct CuPhuKernelProfile {
-    long long cg_samples;     /* timed launches                            */
-    double    cg_ms_sum;      /* their summed duration (ms)                */
-    double    cg_bytes_sum;   /* their summed algorithmic bytes            */
-    double    cg_px_sum;      /* their summed active pixels                */
-    long long cg_launches;    /* all fused PCG launches while enabled      */
+    long long samples;        /* timed iterations                          */
+    double    px_sum;         /* their summed active pixels                */
+    double    pq_ms_sum;      /* summed duration of the pq kernel (ms)     */
+    double    pq_bytes_sum;   /* its summed algorithmic bytes              */
+    double    xr_ms_sum;      /* summed duration of the xr kernel (ms)     */
+    double    xr_bytes_sum;   /* its summed algorithmic bytes              */
+    long long iterations;     /* all PCG iterations launched while enabled */
 } CuPhuKernelProfile;
 int cuphu_set_profiling(int enabled);
 int cuphu_get_profile(CuPhuKernelProfile *out);
@@ -453,13 +459,15 @@ int cuphu_test_laplace_ops(const float *phase, const short *sigsq, const float *
                            float *w0, float *b, float *dinv, float *r, float *lx,
                            float *wrw, float *cong);
 
-/* One fused single-reduction PCG step with given alpha/beta (FP32):
- * updates x, r, p, s, wv in place and returns gamma = r.u', delta = wv.u'.
- * w is the flat arc layout [narcs]; rpw = rows per wave (2, 4, 8). */
-int cuphu_test_cg_step(const float *w, const float *dinv, int nrow, int ncol,
-                       double alpha, double beta, int rpw,
-                       float *x, float *r, float *p, float *s, float *wv,
-                       double *gamma, double *delta);
+/* One batched-path PCG iteration (FP32) through its two kernels and
+ * reductions: p = r*dinv + beta*p (p = r*dinv when beta == 0), q = L p,
+ * pq = p.q, alpha = gamma / pq, x += alpha p, r -= alpha q, rz = r.(r*dinv).
+ * Updates x, r, p, q in place; returns pq, alpha and rz.  w is the flat arc
+ * layout [narcs]; rpw = rows per wave (2, 4, 8). */
+int cuphu_test_pcg_step(const float *w, const float *dinv, int nrow, int ncol,
+                        double gamma, double beta, int rpw,
+                        float *x, float *r, float *p, float *q,
+                        double *pq, double *alpha, double *rz);
 
 #ifdef __cplusplus
 } /* extern "C" */

@@ -198,13 +198,17 @@ def pcg_irls_f64(sigsq, phi, *, u0=None, **opts):
     return u, st.as_dict()
 
 
-def cgcg_step_f32(w, dinv, alpha, beta, x, r, p, s, wv):
+def pcg_step_f32(w, dinv, gamma, beta, x, r, p, alpha=None):
+    """One Jacobi-PCG iteration (direction update first).  Returns
+    (x, r, p, q, pq, alpha, rz, status); status 2 = p.q <= 0."""
     w, dinv = _f32(w), _f32(dinv)
-    x, r, p, s, wv = (_f32(a).copy() for a in (x, r, p, s, wv))
-    g, d = C.c_double(), C.c_double()
-    _lib.o_cgcg_step_f32(_p(w), _p(dinv), C.c_int(x.shape[0]), C.c_int(x.shape[1]), C.c_double(alpha),
-                         C.c_double(beta), _p(x), _p(r), _p(p), _p(s), _p(wv), C.byref(g), C.byref(d))
-    return x, r, p, s, wv, g.value, d.value
+    x, r, p = (_f32(a).copy() for a in (x, r, p))
+    q = np.zeros_like(x)
+    pq, al, rz = C.c_double(), C.c_double(float("nan")), C.c_double(float("nan"))
+    st = _lib.o_pcg_step_f32(_p(w), _p(dinv), C.c_int(x.shape[0]), C.c_int(x.shape[1]), C.c_double(gamma),
+                             C.c_double(beta), C.c_double(float("nan") if alpha is None else alpha), _p(x), _p(r),
+                             _p(p), _p(q), C.byref(pq), C.byref(al), C.byref(rz))
+    return x, r, p, q, pq.value, al.value, rz.value, st
 
 
 def conncomp(unw, corr, mask=None, minfrac=0.01, maxncomps=32) -> np.ndarray:
@@ -293,10 +297,10 @@ def _sig(fn, res, args):
 
 for _n in ("o_set_threads", "o_wrapped_phase", "o_phase_diffs", "o_boxcar2d", "o_costs", "o_residues",
            "o_arc_weights", "o_assemble", "o_matvec", "o_reweight", "o_congruence", "o_pcg_irls_f32",
-           "o_pcg_irls_f64", "o_cgcg_step_f32", "o_conncomp", "o_integrate_phase", "o_tile_axis", "o_stitch",
+           "o_pcg_irls_f64", "o_pcg_step_f32", "o_conncomp", "o_integrate_phase", "o_tile_axis", "o_stitch",
            "o_wls_solve_f64", "o_wls_adjoint_f64"):
     getattr(_lib, _n).restype = C.c_int if _n not in ("o_set_threads", "o_wrapped_phase", "o_phase_diffs",
                                                        "o_costs", "o_residues", "o_arc_weights",
                                                        "o_assemble", "o_matvec", "o_reweight",
-                                                       "o_congruence", "o_cgcg_step_f32",
+                                                       "o_congruence",
                                                        "o_integrate_phase") else None

@@ -530,29 +530,38 @@ int o_pcg_irls_f64(const short *sigsq, size_t stride, const float *phi, int nrow
     return 0;
 }
 
-/* ── fused single-reduction PCG step (Chronopoulos-Gear recurrences) ─────────
- * The vector part of one iteration of the batched HIP solver, FP32 op order:
- * u = r*dinv; p = fma(beta, p, u); s = fma(beta, s, wv); x = fma(alpha, p, x);
- * r = fma(-alpha, s, r); u' = r*dinv; wv = A u' (o_matvec order);
- * gamma = sum r*u', delta = sum wv*u' (FP64, sequential). */
-void o_cgcg_step_f32(const float *w, const float *dinv, int nrow, int ncol, double alpha, double beta,
-                     float *x, float *r, float *p, float *s, float *wv, double *gamma, double *delta) {
+/* ── one Jacobi-PCG iteration, FP32 vectors (the loop body of
+ * o_pcg_irls_f32 above, cuphu_laplace.cu:452-498, with the direction update
+ * of :493-498 moved to the front, as the batched HIP solver runs it):
+ * p = fmaf((float)beta, p, r*dinv), or p = r*dinv when beta == 0 (first
+ * step); q = A p (o_matvec order); pq = sum p*q; alpha = gamma / pq, or
+ * alpha_in when it is finite; x = fmaf((float)alpha, p, x);
+ * r = fmaf(-(float)alpha, q, r); rz = sum r*(r*dinv).  FP64 sequential sums.
+ * Returns 0, or 2 (no update of x, r) when pq <= 0. */
+int o_pcg_step_f32(const float *w, const float *dinv, int nrow, int ncol, double gamma, double beta,
+                   double alpha_in, float *x, float *r, float *p, float *q, double *pq, double *alpha,
+                   double *rz) {
     size_t n = (size_t)nrow * ncol;
-    float af = (float)alpha, bf = (float)beta;
-    float *un = (float *)malloc(n * sizeof(float));
+    float bf = (float)beta;
     for (size_t i = 0; i < n; ++i) {
-        float u = r[i] * dinv[i];
-        p[i] = fmaf(bf, p[i], u);
-        s[i] = fmaf(bf, s[i], wv[i]);
-        x[i] = fmaf(af, p[i], x[i]);
-        r[i] = fmaf(-af, s[i], r[i]);
-        un[i] = r[i] * dinv[i];
+        float z = r[i] * dinv[i];
+        p[i] = (beta != 0.0) ? fmaf(bf, p[i], z) : z;
     }
-    o_matvec(w, un, nrow, ncol, wv);
-    double g = 0.0, d = 0.0;
-    for (size_t i = 0; i < n; ++i) { g += (double)r[i] * (double)un[i]; d += (double)wv[i] * (double)un[i]; }
-    *gamma = g; *delta = d;
-    free(un);
+    o_matvec(w, p, nrow, ncol, q);
+    double s = dot_f32(p, q, n);
+    *pq = s;
+    if (!(s > 0.0)) return 2;
+    double a = isfinite(alpha_in) ? alpha_in : gamma / s;
+    *alpha = a;
+    float af = (float)a;
+    double g = 0.0;
+    for (size_t i = 0; i < n; ++i) {
+        x[i] = fmaf(af, p[i], x[i]);
+        r[i] = fmaf(-af, q[i], r[i]);
+        g += (double)r[i] * (double)(r[i] * dinv[i]);
+    }
+    *rz = g;
+    return 0;
 }
 
 /* ── a14: connected components (cuphu_conncomp_gpu cuphu_conncomp.cu:289-428)

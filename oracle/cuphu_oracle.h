@@ -42,9 +42,10 @@ typedef struct OLapStats {
                              const OLapOptions *o, const float *u0, float *unw, OLapStats *st);
 /* a10 */ int o_pcg_irls_f64(const short *sigsq, size_t stride, const float *phi, int nrow, int ncol,
                              const OLapOptions *o, const float *u0, double *unw, OLapStats *st);
-/* one fused single-reduction PCG step (vector part), FP32 op order */
-void o_cgcg_step_f32(const float *w, const float *dinv, int nrow, int ncol, double alpha, double beta,
-                     float *x, float *r, float *p, float *s, float *wv, double *gamma, double *delta);
+/* one Jacobi-PCG iteration in the batched solver's order, FP32 vectors */
+int o_pcg_step_f32(const float *w, const float *dinv, int nrow, int ncol, double gamma, double beta,
+                   double alpha_in, float *x, float *r, float *p, float *q, double *pq, double *alpha,
+                   double *rz);
 /* a14 */ int o_conncomp(const float *unw, const float *corr, const unsigned char *mask, int nrow,
                          int ncol, double minfrac, long maxncomps, uint32_t *labels);
 /* a13 */ void o_integrate_phase(const float *phi, const short *hflows, const short *vflows,

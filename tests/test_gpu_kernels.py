@@ -109,7 +109,11 @@ def test_laplace_ops_bitexact(gpu_ready, monkeypatch, shape, rpw):
 
 @pytest.mark.parametrize("shape", SHAPES)
 @pytest.mark.parametrize("rpw", [2, 4, 8])
-def test_fused_pcg_step_bitexact(gpu_ready, shape, rpw):
+@pytest.mark.parametrize("beta", [0.0, 0.81])
+def test_pcg_step_kernels_bitexact(gpu_ready, shape, rpw, beta):
+    """Direction+operator kernel and update kernel against the oracle step:
+    p, q bit-exact; pq, alpha, rz to FP64 summation-order tolerance; x, r
+    bit-exact given the kernel's alpha."""
     rng = np.random.default_rng(shape[0] + 7 * shape[1] + rpw)
     nrow, ncol = shape
     na = G.narcs(nrow, ncol)
@@ -117,17 +121,22 @@ def test_fused_pcg_step_bitexact(gpu_ready, shape, rpw):
     if na:
         w[::9] = 0
     dinv = rng.uniform(0.5, 50, shape).astype(np.float32)
-    vecs = [rng.standard_normal(shape).astype(np.float32) for _ in range(5)]
-    alpha, beta = 0.37, 0.81
-    ref = O.cgcg_step_f32(w[:na], dinv, alpha, beta, *vecs)
-    got = [v.copy() for v in vecs]
-    g, d = C.c_double(), C.c_double()
-    _capi.check(_capi.lib.cuphu_test_cg_step(w.ctypes.data, dinv.ctypes.data, nrow, ncol, alpha, beta, rpw,
-                                             *[v.ctypes.data for v in got], C.byref(g), C.byref(d)), "cg_step")
-    for a, e in zip(got, ref[:5]):
-        np.testing.assert_array_equal(a, e)
-    assert g.value == pytest.approx(ref[5], rel=1e-12, abs=1e-300)
-    assert d.value == pytest.approx(ref[6], rel=1e-12, abs=1e-300)
+    x, r, p = (rng.standard_normal(shape).astype(np.float32) for _ in range(3))
+    gamma = float((r.astype(np.float64) * (r * dinv)).sum())
+    got = [v.copy() for v in (x, r, p)] + [np.zeros(shape, np.float32)]
+    pq, al, rz = C.c_double(), C.c_double(), C.c_double()
+    _capi.check(_capi.lib.cuphu_test_pcg_step(w.ctypes.data, dinv.ctypes.data, nrow, ncol, gamma, beta, rpw,
+                                              *[v.ctypes.data for v in got], C.byref(pq), C.byref(al),
+                                              C.byref(rz)), "pcg_step")
+    ref = O.pcg_step_f32(w[:na], dinv, gamma, beta, x, r, p, alpha=al.value)
+    assert ref[7] == 0
+    np.testing.assert_array_equal(got[2], ref[2])   # p
+    np.testing.assert_array_equal(got[3], ref[3])   # q
+    assert pq.value == pytest.approx(ref[4], rel=1e-12)
+    assert al.value == pytest.approx(gamma / ref[4], rel=1e-12)
+    np.testing.assert_array_equal(got[0], ref[0])   # x
+    np.testing.assert_array_equal(got[1], ref[1])   # r
+    assert rz.value == pytest.approx(ref[6], rel=1e-12)
 
 
 @pytest.mark.parametrize("shape", [(2, 2), (40, 70), (257, 300)])

@@ -322,7 +322,9 @@ def test_all_masked_scene_skips_solve():
     np.testing.assert_allclose(u, exp, atol=1e-5)
 
 
-def test_cgcg_recurrences_reproduce_cg_iterates():
+def test_pcg_step_reproduces_dense_pcg_iterates():
+    """o_pcg_step_f32 driven 6 times against textbook Jacobi-PCG in dense
+    float64 (a dropped term, sign or index in any of the updates fails)."""
     rng = np.random.default_rng(9)
     nrow, ncol = 10, 12
     na = O.narcs(nrow, ncol)
@@ -330,22 +332,19 @@ def test_cgcg_recurrences_reproduce_cg_iterates():
     phi = rng.uniform(0, TWOPI, (nrow, ncol)).astype(np.float32)
     b, diag = O.assemble(w, phi)
     dinv = (1.0 / diag).astype(np.float32)
-    # single-reduction recurrences driven by the oracle step
     x = np.zeros_like(b)
     r = b.copy()
     p = np.zeros_like(b)
-    s = np.zeros_like(b)
-    u0 = r * dinv
-    wv = O.matvec(w, u0)
-    gam = float((r.astype(np.float64) * u0).sum())
-    dl = float((wv.astype(np.float64) * u0).sum())
-    alpha, beta = gam / dl, 0.0
+    gam = float((r.astype(np.float64) * (r * dinv)).sum())
+    beta = 0.0
+    pqs = []
     for _ in range(6):
-        x, r, p, s, wv, gn, dn = O.cgcg_step_f32(w, dinv, alpha, beta, x, r, p, s, wv)
-        beta = gn / gam
-        alpha_prev, alpha = alpha, gn / (dn - beta * gn / alpha)
-        gam = gn
-    # textbook Jacobi-PCG, dense float64, same number of updates
+        x, r, p, q, pq, alpha, rzn, st = O.pcg_step_f32(w, dinv, gam, beta, x, r, p)
+        assert st == 0
+        np.testing.assert_array_equal(q, O.matvec(w, p))
+        pqs.append(pq)
+        beta = rzn / gam
+        gam = rzn
     L = dense_laplacian(w.astype(np.float64), nrow, ncol)
     bb = b.ravel().astype(np.float64)
     D = dinv.ravel().astype(np.float64)
@@ -354,9 +353,11 @@ def test_cgcg_recurrences_reproduce_cg_iterates():
     zz = rr * D
     pp = zz.copy()
     rz = rr @ zz
+    dpqs = []
     for _ in range(6):
         q = L @ pp
-        a = rz / (pp @ q)
+        dpqs.append(pp @ q)
+        a = rz / dpqs[-1]
         xx += a * pp
         rr -= a * q
         zz = rr * D
@@ -364,6 +365,18 @@ def test_cgcg_recurrences_reproduce_cg_iterates():
         pp = zz + (rzn / rz) * pp
         rz = rzn
     np.testing.assert_allclose(x.ravel(), xx, rtol=2e-3, atol=2e-4 * np.abs(xx).max())
+    np.testing.assert_allclose(pqs, dpqs, rtol=1e-3)
+    # the step's rz is the r.z of its returned residual
+    assert gam == pytest.approx(float((r.astype(np.float64) * (r * dinv)).sum()), rel=1e-12)
+
+
+def test_pcg_step_reports_breakdown():
+    nrow, ncol = 3, 4
+    w = np.zeros(O.narcs(nrow, ncol), np.float32)
+    dinv = np.ones((nrow, ncol), np.float32)
+    z = np.zeros((nrow, ncol), np.float32)
+    *_, st = O.pcg_step_f32(w, dinv, 1.0, 0.0, z, z, z)
+    assert st == 2   # p = r*dinv = 0 -> p.q = 0
 
 
 # ── a14 conncomp ────────────────────────────────────────────────────────────
