@@ -43,6 +43,11 @@ METRIC = "Mpixel/s unwrapped (Laplace PCG/IRLS) + stencil HBM GB/s; batched vs p
 HBM_PEAK_GBPS = 8000.0
 BYTES_PER_PX_PQ = 28.0
 BYTES_PER_PX_XR = 28.0
+# HBM bytes per active pixel actually moved by k_pq at this workload: rocprofv3
+# FETCH_SIZE (x2.000, calibrated on a copy of known size) + WRITE_SIZE
+# (x1.000) per launch / active pixels per launch
+# (profiles/pcg_kernels_scene8192_hbm_traffic.md).  Re-measure after kernel changes.
+TRAFFIC_PER_PX_PQ = 2770.6e6 / 84.93e6
 
 
 def log(msg):
@@ -60,6 +65,11 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=1024)
     ap.add_argument("--group-mb", type=int, default=0)
+    # counter runs: fewer iterations (same launch shape), and a copy kernel of
+    # known size to calibrate FETCH_SIZE / WRITE_SIZE
+    ap.add_argument("--max-iter", type=int, default=1000)
+    ap.add_argument("--n-irls", type=int, default=5)
+    ap.add_argument("--calib-copy", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (tiles would be sharded across "
@@ -94,7 +104,16 @@ def main() -> int:
     prm = _capi.default_params()
     prm.nlooks = prm.ncorrlooks = scene.nlooks
     cost = _capi.COST["defo"]
-    base_opts = dict(n_irls=5, tol=1e-6, max_iter=1000, rr_interval=50, group_bytes_mb=args.group_mb)
+    base_opts = dict(n_irls=args.n_irls, tol=1e-6, max_iter=args.max_iter, rr_interval=50,
+                     group_bytes_mb=args.group_mb)
+    if args.calib_copy:
+        # 512 MiB read + 512 MiB written per launch (torch.neg: one elementwise pass)
+        src = torch.ones(128 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.neg(src, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
     torch.cuda.synchronize()
     stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
 
@@ -151,8 +170,9 @@ def main() -> int:
         "data": "synthetic (seeded ramp+bumps+coherence generator, cuphu_amd/synth.py)",
         "config": {
             "workload": f"{args.workload}: {nrow}x{ncol}, tiles {tile}/{ovl} overlap, defo cost, nlooks "
-                        f"{scene.nlooks:g}, 5 IRLS re-solves, tol 1e-6, max_iter 1000, batched FP32/FP64-reduction PCG",
-            "tiles": len(st), "n_irls": 5, "tol": 1e-6, "max_iter": 1000, "rr_interval": 50,
+                        f"{scene.nlooks:g}, {args.n_irls} IRLS re-solves, tol 1e-6, max_iter {args.max_iter}, "
+                        f"batched FP32/FP64-reduction PCG",
+            "tiles": len(st), "n_irls": args.n_irls, "tol": 1e-6, "max_iter": args.max_iter, "rr_interval": 50,
             "gpu": name.value.decode(),
         },
         "roofline": {
@@ -162,7 +182,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 4) if achieved else None,
-            "traffic": None,
+            "traffic": round(pq_bytes / BYTES_PER_PX_PQ * TRAFFIC_PER_PX_PQ) if args.workload == "scene8192" else None,
             "bytes_per_px": BYTES_PER_PX_PQ,
             "avg_launch_ms": round(pq_ms, 4),
             "avg_bytes_per_launch": pq_bytes,
@@ -227,11 +247,12 @@ def main() -> int:
         sig = O.costs(ph, co, scene.nlooks, defo=True)[:, 1]
         log(f"CPU oracle on {n}x{n} with {ncore} threads")
         t0 = time.perf_counter()
-        _, cst = O.pcg_irls_f64(sig, ph, n_irls=5, tol=1e-6, max_iter=1000)
+        _, cst = O.pcg_irls_f64(sig, ph, n_irls=args.n_irls, tol=1e-6, max_iter=args.max_iter)
         tc = time.perf_counter() - t0
         out["cpu_baseline"] = {
             "value": round(n * n / tc / 1e6, 5), "unit": "Mpixel/s", "cores": ncore, "kind": "port",
-            "sample": f"FP64 C oracle (OpenMP) PCG/IRLS, same schedule (6 solves, tol 1e-6, cap 1000), on the "
+            "sample": f"FP64 C oracle (OpenMP) PCG/IRLS, same schedule ({args.n_irls + 1} solves, tol 1e-6, cap "
+                      f"{args.max_iter}), on the "
                       f"{n}x{n} top-left crop of the scene; iterations {cst['iters']}; {tc:.1f} s; SNAPHU's CPU MCF "
                       f"is not available (source absent) and is not timed",
         }

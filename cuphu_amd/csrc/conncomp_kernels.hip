@@ -8,11 +8,14 @@
 // smaller than max(1, (int)(minfrac*npix)) get 0, and the rest are numbered
 // 1..maxncomps in increasing root order (build_label_map :272-286).
 //
-// GPU scheme: (1) union-find inside a 64x16 tile in LDS, (2) lock-free
-// global union (atomicMin, parent <= self) across tile borders, (3) path
-// compression + per-root counts, (4) keep flags, device scan, relabel.  The
-// output is a pure function of the partition, so it is bit-identical to a
-// sequential union-find.
+// GPU scheme: (1) union-find inside a 64x16 tile in LDS; every pixel points
+// at its tile-local root and the local roots are flagged, (2) lock-free
+// global union (atomicMin, parent <= self) across tile borders, (3) the
+// local roots (a few per tile) find their final root with path halving,
+// (4) every pixel takes its local root's final root (two loads) and the
+// per-root sizes are counted with one atomic per distinct root per wave,
+// (5) keep flags, device scan, relabel.  The output is a pure function of
+// the partition, so it is bit-identical to a sequential union-find.
 #include <hipcub/hipcub.hpp>
 
 #include "common.hpp"
@@ -49,7 +52,8 @@ __device__ void lds_union(int *par, int a, int b) {
 }
 
 __global__ __launch_bounds__(256) void k_cc_local(const float *unw, const float *corr, const unsigned char *mask,
-                                                  int nrow, int ncol, int stride, float th, uint32_t *lab) {
+                                                  int nrow, int ncol, int stride, float th, uint32_t *lab,
+                                                  uint32_t *rootflag) {
     __shared__ int par[kCcN];
     int tx = blockIdx.x, ty = blockIdx.y;
     int c0 = tx * kCcTW, r0 = ty * kCcTH;
@@ -71,12 +75,14 @@ __global__ __launch_bounds__(256) void k_cc_local(const float *unw, const float 
         int r = r0 + l / kCcTW, c = c0 + l % kCcTW;
         if (r >= nrow || c >= ncol) continue;
         long long g = (long long)r * stride + c;
-        uint32_t out = kInvalidLabel;
+        uint32_t out = kInvalidLabel, isroot = 0u;
         if (par[l] >= 0) {
             int root = lds_find(par, l);
             out = (uint32_t)((long long)(r0 + root / kCcTW) * stride + (c0 + root % kCcTW));
+            isroot = (root == l) ? 1u : 0u;
         }
         lab[g] = out;
+        rootflag[g] = isroot;
     }
 }
 
@@ -115,15 +121,56 @@ __global__ __launch_bounds__(256) void k_cc_border(const float *unw, int nrow, i
         g_union(lab, (uint32_t)g, (uint32_t)(g + stride));
 }
 
-__global__ __launch_bounds__(256) void k_cc_compress(int nrow, int ncol, int stride, uint32_t *lab, uint32_t *cnt) {
+// Final root of every tile-local root.  The forest no longer changes except
+// for the halving stores, each of which points a node at one of its own
+// ancestors, so every find still ends at the true root.  Along any path the
+// indices decrease toward the root, which is the minimum of its component,
+// so the stores are atomicMin: an entry only moves closer to the root, and a
+// late halving store from another thread cannot overwrite a final root.
+__global__ __launch_bounds__(256) void k_cc_flatten(int nrow, int ncol, int stride, const uint32_t *rootflag,
+                                                    uint32_t *lab) {
     long long i = (long long)blockIdx.x * 256 + threadIdx.x;
     if (i >= (long long)nrow * ncol) return;
     long long g = (long long)(i / ncol) * stride + (i % ncol);
-    uint32_t l = lab[g];
-    if (l == kInvalidLabel) return;
-    uint32_t root = g_find(lab, l);
-    lab[g] = root;
-    atomicAdd(cnt + root, 1u);
+    if (!rootflag[g]) return;
+    uint32_t x = (uint32_t)g;
+    uint32_t p = g_load(lab + x);
+    while (p != x) {
+        uint32_t gp = g_load(lab + p);
+        if (gp != p) atomicMin(lab + x, gp);
+        x = p;
+        p = gp;
+    }
+    atomicMin(lab + g, x);
+}
+
+// lab[g] := final root (the local root's entry after k_cc_flatten; only
+// local-root entries are read, and a local root rewrites its entry with the
+// value it already holds) and per-root pixel counts, aggregated per wave.
+__global__ __launch_bounds__(256) void k_cc_compress(int nrow, int ncol, int stride, uint32_t *lab, uint32_t *cnt) {
+    long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    bool pending = false;
+    uint32_t root = 0u;
+    if (i < (long long)nrow * ncol) {
+        long long g = (long long)(i / ncol) * stride + (i % ncol);
+        uint32_t l = lab[g];
+        if (l != kInvalidLabel) {
+            root = lab[l];
+            lab[g] = root;
+            pending = true;
+        }
+    }
+    const int lane = threadIdx.x & 63;
+    for (;;) {
+        unsigned long long m = __ballot(pending);
+        if (m == 0ull) break;
+        int leader = __ffsll((long long)m) - 1;
+        uint32_t lr = __shfl(root, leader, 64);
+        bool mine = pending && root == lr;
+        unsigned long long mm = __ballot(mine);
+        if (lane == leader) atomicAdd(cnt + lr, (uint32_t)__popcll(mm));
+        if (mine) pending = false;
+    }
 }
 
 __global__ void k_cc_fill(uint32_t *p, size_t n, uint32_t v) {
@@ -164,8 +211,10 @@ int conncomp_run(const float *unw, const float *corr, const unsigned char *mask,
     dim3 tiles((unsigned)ceil_div((size_t)ncol, kCcTW), (unsigned)ceil_div((size_t)nrow, kCcTH));
     hipLaunchKernelGGL(k_cc_fill, dim3(g), dim3(256), 0, st, cnt.get(), n, 0u);
     if (stride != ncol) hipLaunchKernelGGL(k_cc_fill, dim3(g), dim3(256), 0, st, labels, n, kInvalidLabel);
-    hipLaunchKernelGGL(k_cc_local, tiles, dim3(256), 0, st, unw, corr, mask, nrow, ncol, stride, corr_thresh, labels);
+    hipLaunchKernelGGL(k_cc_local, tiles, dim3(256), 0, st, unw, corr, mask, nrow, ncol, stride, corr_thresh, labels,
+                       flags.get());
     hipLaunchKernelGGL(k_cc_border, dim3(gp), dim3(256), 0, st, unw, nrow, ncol, stride, labels);
+    hipLaunchKernelGGL(k_cc_flatten, dim3(gp), dim3(256), 0, st, nrow, ncol, stride, flags.get(), labels);
     hipLaunchKernelGGL(k_cc_compress, dim3(gp), dim3(256), 0, st, nrow, ncol, stride, labels, cnt.get());
     int min_size = (int)(minfrac * (double)(int)npx);
     if (min_size < 1) min_size = 1;
