@@ -10,10 +10,11 @@ tiles in one batched launch sequence, tile stitching and conncomp.  Inputs
 are resident in HBM before the timed region; a step is one full unwrap.
 
 Reported beside the headline Mpixel/s:
-* roofline: the PCG direction+stencil kernel (p = z + beta p, q = L p, p.q;
-  28 B per active pixel) -- algorithmic bytes over its duration, timed live
-  with HIP events on the solver's stream during the timed steps; the PCG
-  update kernel (x, r, r.z; 28 B per active pixel) is reported beside it;
+* roofline: the fused PCG iteration kernel k_it (x, r, p, q updated and
+  q = L p formed in one pass; 44 B per active pixel) -- algorithmic bytes
+  over its duration, timed live with HIP events on the solver's stream
+  during the timed steps (with --solver batched: the direction+stencil
+  kernel k_pq, 28 B/px, with the update kernel k_xr beside it);
 * plain: the reference launch sequence (per-tile loop, 7 launches and 2
   blocking scalar reads per iteration) on the same scene, and the ratio;
 * accuracy: 2-pi ambiguity agreement of the FP32 result with the FP64 mode
@@ -41,13 +42,16 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Mpixel/s unwrapped (Laplace PCG/IRLS) + stencil HBM GB/s; batched vs plain"
 HBM_PEAK_GBPS = 8000.0
+BYTES_PER_PX_IT = 44.0
 BYTES_PER_PX_PQ = 28.0
 BYTES_PER_PX_XR = 28.0
-# HBM bytes per active pixel actually moved by k_pq at this workload: rocprofv3
-# FETCH_SIZE (x2.000, calibrated on a copy of known size) + WRITE_SIZE
-# (x1.000) per launch / active pixels per launch
-# (profiles/pcg_kernels_scene8192_hbm_traffic.md).  Re-measure after kernel changes.
+# HBM bytes per active pixel actually moved per launch at this workload:
+# rocprofv3 FETCH_SIZE (x2.000, calibrated on a copy of known size) +
+# WRITE_SIZE (x1.000) per launch / active pixels per launch
+# (profiles/k_it_scene8192_hbm_traffic.md, profiles/k_pq_k_xr_scene8192_hbm_traffic.md).
+# Re-measure after kernel changes.
 TRAFFIC_PER_PX_PQ = 2770.6e6 / 84.93e6
+TRAFFIC_PER_PX_IT = 47.28
 
 
 def log(msg):
@@ -70,6 +74,7 @@ def main() -> int:
     ap.add_argument("--max-iter", type=int, default=1000)
     ap.add_argument("--n-irls", type=int, default=5)
     ap.add_argument("--calib-copy", action="store_true")
+    ap.add_argument("--solver", default="fused", choices=["fused", "batched"])
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (tiles would be sharded across "
@@ -105,7 +110,7 @@ def main() -> int:
     prm.nlooks = prm.ncorrlooks = scene.nlooks
     cost = _capi.COST["defo"]
     base_opts = dict(n_irls=args.n_irls, tol=1e-6, max_iter=args.max_iter, rr_interval=50,
-                     group_bytes_mb=args.group_mb)
+                     group_bytes_mb=args.group_mb, solver=args.solver)
     if args.calib_copy:
         # 512 MiB read + 512 MiB written per launch (torch.neg: one elementwise pass)
         src = torch.ones(128 << 20, dtype=torch.float32, device="cuda")
@@ -150,10 +155,23 @@ def main() -> int:
     log(f"batched: {mpx:.2f} Mpx/s ({ms_per_step:.0f} ms/step)")
 
     ns = max(prof.samples, 1)
-    pq_ms, xr_ms = prof.pq_ms_sum / ns, prof.xr_ms_sum / ns
-    pq_bytes, xr_bytes = prof.pq_bytes_sum / ns, prof.xr_bytes_sum / ns
-    achieved = (prof.pq_bytes_sum / 1e9) / (prof.pq_ms_sum / 1e3) if prof.pq_ms_sum > 0 else None
-    xr_gbs = (prof.xr_bytes_sum / 1e9) / (prof.xr_ms_sum / 1e3) if prof.xr_ms_sum > 0 else None
+    gbs = lambda b, ms: (b / 1e9) / (ms / 1e3) if ms > 0 else None
+    if args.solver == "fused":
+        kname, bpp, tpp = "k_it (fused PCG iteration: x, r, p, q updates + weighted 5-point stencil + 4 sums)", \
+            BYTES_PER_PX_IT, TRAFFIC_PER_PX_IT
+        k_ms, k_bytes = prof.it_ms_sum / ns, prof.it_bytes_sum / ns
+        achieved = gbs(prof.it_bytes_sum, prof.it_ms_sum)
+        extra = {}
+    else:
+        kname, bpp, tpp = "k_pq (PCG direction + weighted 5-point stencil + p.q)", BYTES_PER_PX_PQ, TRAFFIC_PER_PX_PQ
+        k_ms, k_bytes = prof.pq_ms_sum / ns, prof.pq_bytes_sum / ns
+        achieved = gbs(prof.pq_bytes_sum, prof.pq_ms_sum)
+        xr_gbs = gbs(prof.xr_bytes_sum, prof.xr_ms_sum)
+        extra = {"update_kernel": {"kernel": "k_xr (x, r update + r.z)", "bytes_per_px": BYTES_PER_PX_XR,
+                                   "avg_launch_ms": round(prof.xr_ms_sum / ns, 4),
+                                   "achieved": round(xr_gbs, 1) if xr_gbs else None,
+                                   "frac": round(xr_gbs / HBM_PEAK_GBPS, 4) if xr_gbs else None}}
+    traffic = round(k_bytes / bpp * tpp) if (tpp and args.workload == "scene8192") else None
 
     out = {
         "metric": METRIC,
@@ -171,26 +189,24 @@ def main() -> int:
         "config": {
             "workload": f"{args.workload}: {nrow}x{ncol}, tiles {tile}/{ovl} overlap, defo cost, nlooks "
                         f"{scene.nlooks:g}, {args.n_irls} IRLS re-solves, tol 1e-6, max_iter {args.max_iter}, "
-                        f"batched FP32/FP64-reduction PCG",
+                        f"{args.solver} FP32/FP64-reduction PCG",
             "tiles": len(st), "n_irls": args.n_irls, "tol": 1e-6, "max_iter": args.max_iter, "rr_interval": 50,
+            "solver": args.solver,
             "gpu": name.value.decode(),
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_pq (PCG direction + weighted 5-point stencil + p.q)",
+            "kernel": kname,
             "achieved": round(achieved, 1) if achieved else None,
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 4) if achieved else None,
-            "traffic": round(pq_bytes / BYTES_PER_PX_PQ * TRAFFIC_PER_PX_PQ) if args.workload == "scene8192" else None,
-            "bytes_per_px": BYTES_PER_PX_PQ,
-            "avg_launch_ms": round(pq_ms, 4),
-            "avg_bytes_per_launch": pq_bytes,
+            "traffic": traffic,
+            "bytes_per_px": bpp,
+            "avg_launch_ms": round(k_ms, 4),
+            "avg_bytes_per_launch": k_bytes,
             "timed_launches": prof.samples,
-            "update_kernel": {"kernel": "k_xr (x, r update + r.z)", "bytes_per_px": BYTES_PER_PX_XR,
-                              "avg_launch_ms": round(xr_ms, 4),
-                              "achieved": round(xr_gbs, 1) if xr_gbs else None,
-                              "frac": round(xr_gbs / HBM_PEAK_GBPS, 4) if xr_gbs else None},
+            **extra,
         },
         "pcg_iters_per_tile": iters[:3] + (["..."] if len(iters) > 3 else []),
         "pcg_iters_total": int(sum(sum(i) for i in iters)),

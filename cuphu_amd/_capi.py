@@ -14,14 +14,14 @@ import os
 __all__ = [
     "lib", "CuPhuParams", "CuPhuTileParams", "CuPhuBridgeParams", "CuPhuResult",
     "CuPhuLaplaceOptions", "CuPhuSolveStats", "CuPhuInstance", "check", "LIB_PATH",
-    "SOLVER_PLAIN", "SOLVER_BATCHED", "COST", "INIT", "EXPORTED",
+    "SOLVER_PLAIN", "SOLVER_BATCHED", "SOLVER_FUSED", "COST", "INIT", "EXPORTED",
 ]
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libcuphu_hip.so")
 
 COST = {"smooth": 0, "defo": 1, "topo": 2}
 INIT = {"mst": 0, "mcf": 1, "laplace": 2}
-SOLVER_PLAIN, SOLVER_BATCHED = 0, 1
+SOLVER_PLAIN, SOLVER_BATCHED, SOLVER_FUSED = 0, 1, 2
 MAX_IRLS = 32
 
 OK, ERR_NOMEM, ERR_INVALID, ERR_UNSUPPORTED, ERR_HIP, ERR_INTERNAL = 0, -1, -2, -3, -4, -5
@@ -87,7 +87,8 @@ class CuPhuSolveStats(C.Structure):
 
 class CuPhuKernelProfile(C.Structure):
     _fields_ = [("samples", C.c_longlong), ("px_sum", _d), ("pq_ms_sum", _d), ("pq_bytes_sum", _d),
-                ("xr_ms_sum", _d), ("xr_bytes_sum", _d), ("iterations", C.c_longlong)]
+                ("xr_ms_sum", _d), ("xr_bytes_sum", _d), ("it_ms_sum", _d), ("it_bytes_sum", _d),
+                ("iterations", C.c_longlong)]
 
 
 class CuPhuInstance(C.Structure):
@@ -149,6 +150,7 @@ _SIGS = {
     "cuphu_test_laplace_ops": (_i, [_P, _P, _P, _i, _i, C.c_float, _P, _P, _P, _P, _P, _P, _P]),
     "cuphu_test_pcg_step": (_i, [_P, _P, _i, _i, _d, _d, _i, _P, _P, _P, _P, C.POINTER(_d), C.POINTER(_d),
                                  C.POINTER(_d)]),
+    "cuphu_test_pcg_fused_step": (_i, [_P, _P, _i, _i, _d, _d, _i, _P, _i, _P, _P, _P, _P, _P]),
 }
 
 EXPORTED = tuple(_SIGS)
@@ -199,7 +201,7 @@ def default_options(**over) -> CuPhuLaplaceOptions:
     lib.cuphu_default_laplace_options(C.byref(o))
     for k, v in over.items():
         if k == "solver" and isinstance(v, str):
-            v = SOLVER_PLAIN if v == "plain" else SOLVER_BATCHED
+            v = {"plain": SOLVER_PLAIN, "batched": SOLVER_BATCHED, "fused": SOLVER_FUSED}[v]
         if not hasattr(o, k):
             raise ValueError(f"unknown laplace option {k!r}")
         setattr(o, k, v)

@@ -8,7 +8,7 @@ module to the gfx950 library.  Only ``init='laplace'`` is implemented;
 ``'mcf'``/``'mst'`` need SNAPHU's CPU network-flow solver and raise.
 
 One keyword is added: ``laplace_options`` (dict; keys n_irls, delta_max,
-delta_min, max_iter, tol, solver ('plain'|'batched'), precision (32|64),
+delta_min, max_iter, tol, solver ('plain'|'batched'|'fused'), precision (32|64),
 rr_interval, use_graph, group_bytes_mb, verbose).  Unset keys follow the
 reference defaults and its CUPHU_* environment overrides.
 """

@@ -152,7 +152,8 @@ py::tuple py_unwrap_arrays(py::array_t<std::complex<float>, py::array::c_style> 
                 std::string s = v.cast<std::string>();
                 if (s == "plain") opt.solver = CUPHU_SOLVER_PLAIN;
                 else if (s == "batched") opt.solver = CUPHU_SOLVER_BATCHED;
-                else throw py::value_error("solver must be 'plain' or 'batched'");
+                else if (s == "fused") opt.solver = CUPHU_SOLVER_FUSED;
+                else throw py::value_error("solver must be 'plain', 'batched' or 'fused'");
             } else if (k == "precision") opt.precision = v.cast<int>();
             else if (k == "rr_interval") opt.rr_interval = v.cast<int>();
             else if (k == "use_graph") opt.use_graph = v.cast<bool>() ? 1 : 0;

@@ -24,7 +24,7 @@ struct SolveScal {
     int iters, active, status, replace;   // replace: the last residual replacement was taken
 };
 
-enum RedMode { RED_START = 0, RED_PQ = 1, RED_RZ = 2, RED_RR = 3 };
+enum RedMode { RED_START = 0, RED_PQ = 1, RED_RZ = 2, RED_RR = 3, RED_IT = 4 };
 
 // Instance source description for ingest (pointers already offset to the
 // instance's (0,0); src_stride = elements per source row).
@@ -47,10 +47,12 @@ template <class T> struct Planes {
     T *w2 = nullptr;                // {down, right} arc weights interleaved (2 T per pixel)
     // s holds q = L p; wv is the plain solver's z.
     T *b = nullptr, *dinv = nullptr, *x = nullptr, *r = nullptr, *p = nullptr, *s = nullptr, *wv = nullptr;
-    // Second direction plane: the fused direction kernel of step k reads the
-    // old direction from one plane and writes the new one to the other, so
-    // no workgroup overwrites a value a neighbouring workgroup still reads.
-    T *p2 = nullptr;
+    // Second plane set: a stencil kernel that reads a vector at halo pixels
+    // reads it from one set and writes its update to the other, so no
+    // workgroup overwrites a value a neighbouring workgroup still reads.
+    // Two-kernel solver: direction p/p2.  Fused solver: (r, p, s) and
+    // (r2, p2, q2); wv then holds the true residual of a replacement.
+    T *p2 = nullptr, *r2 = nullptr, *q2 = nullptr;
 };
 
 struct KernelCfg {
@@ -86,6 +88,18 @@ void launch_rr_commit(const InstDesc *d_desc, const int *d_blk, int nblk, const 
                       int rpw, hipStream_t st);
 void launch_reduce(const InstDesc *d_desc, int ninst, const double2 *partials, SolveScal *scal, int mode,
                    int max_iter, double tol2, hipStream_t st);
+// Fused solver: one stencil kernel per PCG iteration (parity 0 reads plane
+// set 0 and writes set 1; start = 1 forms the first direction from set 0),
+// the true residual of a replacement, and the reduction (modes RED_START,
+// RED_IT, RED_RR).
+template <class T>
+void launch_it(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, const SolveScal *scal,
+               double4 *partials, int rpw, int parity, int start, hipStream_t st);
+template <class T>
+void launch_rr_it(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, const SolveScal *scal,
+                  double4 *partials, int rpw, hipStream_t st);
+void launch_reduce_it(const InstDesc *d_desc, int ninst, const double4 *partials, SolveScal *scal, int mode,
+                      int max_iter, double tol2, hipStream_t st);
 template <class T>
 void launch_final(const InstDesc *d_desc, const int *d_rowinst, int nrows_total, const Planes<T> &pl,
                   float *out_pool, int max_pitch, hipStream_t st);

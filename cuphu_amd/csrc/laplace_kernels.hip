@@ -452,49 +452,52 @@ __global__ __launch_bounds__(256) void k_pq(const InstDesc *descs, const int *bl
     if (!sc->active) return;
     const T be = (T)sc->beta;
     const bool keep = sc->beta != 0.0;   // first step of a solve: p = z
-    const T *p_in = parity ? pl.p2 : pl.p;
-    T *p_out = parity ? pl.p : pl.p2;
+    const T *__restrict__ p_in = parity ? pl.p2 : pl.p;
+    T *__restrict__ p_out = parity ? pl.p : pl.p2;
+    const T *__restrict__ rg = pl.r;
+    const T *__restrict__ dg = pl.dinv;
+    const T *__restrict__ w2 = pl.w2;
     int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const InstDesc &d = t.d;
-    int c = t.tx0 + 4 * lane;
-    bool cok = c < d.pitch;
+    const int c = t.tx0 + 4 * lane;
+    const bool cok = c < d.pitch;
+    const int row0 = t.ty0 + wave * RPW;
+    // Every load of the own rows is issued before any use: r, 1/diag, the old
+    // direction, the {down, right} arcs, the up arcs of the first row and,
+    // for lane 0, the left arc from the column before the tile.
+    T r4[RPW][4], di[RPW][4], po[RPW][4], wd[RPW][4], wr[RPW][4], wu[4], wlo[RPW];
+#pragma unroll
     for (int k = 0; k < RPW; ++k) {
-        int lr = wave * RPW + k, row = t.ty0 + lr;
-        T pn[4];
-        zero4(pn);
-        if (row < d.nrow && cok) {
+        zero4(r4[k]); zero4(di[k]); zero4(po[k]); zero4(wd[k]); zero4(wr[k]);
+        wlo[k] = T(0);
+        const int row = row0 + k;
+        if (row < d.nrow) {
             long long e = d.off + (long long)row * d.pitch + c;
-            T r4[4], di[4], po[4];
-            ld4(pl.r + e, r4);
-            ld4(pl.dinv + e, di);
-            if (keep) ld4(p_in + e, po);
-            else zero4(po);
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                T z = r4[j] * di[j];
-                pn[j] = keep ? Prec<T>::fmaT(be, po[j], z) : z;
+            if (cok) {
+                ld4(rg + e, r4[k]);
+                ld4(dg + e, di[k]);
+                if (keep) ld4(p_in + e, po[k]);
+                ld_arcs(w2, e, wd[k], wr[k]);
             }
-            st4(p_out + e, pn);
+            if (lane == 0 && t.tx0 > 0) wlo[k] = w2[2 * (e - 1) + 1];
         }
-        T *lq = lp + (lr + 1) * kLdsStride + kLdsC0 + 4 * lane;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) lq[j] = pn[j];
     }
+    up_arcs(w2, t, row0, cok && row0 < d.nrow, lane, wu);
     fill_halo<TH>(
         t, wave, lane,
         [&](int lr, int lc, long long e, bool ok) {
             T pn[4];
             zero4(pn);
             if (ok) {
-                T r4[4], di[4], po[4];
-                ld4(pl.r + e, r4);
-                ld4(pl.dinv + e, di);
-                if (keep) ld4(p_in + e, po);
-                else zero4(po);
+                T a[4], b[4], o[4];
+                ld4(rg + e, a);
+                ld4(dg + e, b);
+                if (keep) ld4(p_in + e, o);
+                else zero4(o);
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
-                    T z = r4[j] * di[j];
-                    pn[j] = keep ? Prec<T>::fmaT(be, po[j], z) : z;
+                    T z = a[j] * b[j];
+                    pn[j] = keep ? Prec<T>::fmaT(be, o[j], z) : z;
                 }
             }
 #pragma unroll
@@ -503,30 +506,42 @@ __global__ __launch_bounds__(256) void k_pq(const InstDesc *descs, const int *bl
         [&](int lr, int lc, long long e, bool ok) {
             T v = T(0);
             if (ok) {
-                T z = pl.r[e] * pl.dinv[e];
+                T z = rg[e] * dg[e];
                 v = keep ? Prec<T>::fmaT(be, p_in[e], z) : z;
             }
             lp[lr * kLdsStride + lc] = v;
         });
+#pragma unroll
+    for (int k = 0; k < RPW; ++k) {
+        const int lr = wave * RPW + k, row = row0 + k;
+        T pn[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            T z = r4[k][j] * di[k][j];
+            pn[j] = keep ? Prec<T>::fmaT(be, po[k][j], z) : z;
+        }
+        if (row < d.nrow && cok) st4(p_out + d.off + (long long)row * d.pitch + c, pn);
+        T *lq = lp + (lr + 1) * kLdsStride + kLdsC0 + 4 * lane;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) lq[j] = pn[j];
+    }
     __syncthreads();
     double pq = 0.0;
-    T wu[4];
+#pragma unroll
     for (int k = 0; k < RPW; ++k) {
-        int lr = wave * RPW + k, row = t.ty0 + lr;
+        const int lr = wave * RPW + k, row = row0 + k;
         if (row >= d.nrow) break;
-        if (k == 0) up_arcs(pl.w2, t, row, cok, lane, wu);
-        T wd[4], wr[4], wl[4], y[4];
-        row_arcs(pl.w2, t, row, cok, lane, wd, wr, wl);
-        stencil4(lp, lr, lane, wd, wu, wr, wl, y);
+        T wl[4], y[4];
+        T from_left = __shfl_up(wr[k][3], 1, 64);
+        wl[0] = (lane == 0) ? wlo[k] : from_left;
+        wl[1] = wr[k][0]; wl[2] = wr[k][1]; wl[3] = wr[k][2];
+        stencil4(lp, lr, lane, wd[k], (k == 0) ? wu : wd[k > 0 ? k - 1 : 0], wr[k], wl, y);
         if (cok) {
-            long long e = d.off + (long long)row * d.pitch + c;
-            st4(pl.s + e, y);
+            st4(pl.s + d.off + (long long)row * d.pitch + c, y);
             const T *pc = lp + (lr + 1) * kLdsStride + kLdsC0 + 4 * lane;
 #pragma unroll
             for (int j = 0; j < 4; ++j) pq += (double)pc[j] * (double)y[j];
         }
-#pragma unroll
-        for (int j = 0; j < 4; ++j) wu[j] = wd[j];
     }
     block_sum2(pq, 0.0, partials + t.blk);
 }
@@ -534,7 +549,7 @@ __global__ __launch_bounds__(256) void k_pq(const InstDesc *descs, const int *bl
 // ═══════════════════════════════════════════════════════════════════════════
 // PCG update (cuphu_laplace.cu:466-476): x += alpha p, r -= alpha q,
 // z = r * dinv, partial sums of r.z.  p is the direction k_pq of the same
-// parity wrote.
+// parity wrote.  All loads of a lane's rows are issued before the first use.
 // ═══════════════════════════════════════════════════════════════════════════
 template <class T, int RPW>
 __global__ __launch_bounds__(256) void k_xr(const InstDesc *descs, const int *blk_inst, Planes<T> pl,
@@ -544,30 +559,42 @@ __global__ __launch_bounds__(256) void k_xr(const InstDesc *descs, const int *bl
     const SolveScal *sc = scal + t.inst;
     if (!sc->active) return;
     const T al = (T)sc->alpha;
-    const T *pd = parity ? pl.p : pl.p2;
+    const T *__restrict__ pd = parity ? pl.p : pl.p2;
+    T *__restrict__ xg = pl.x;
+    T *__restrict__ rg = pl.r;
+    const T *__restrict__ qg = pl.s;
+    const T *__restrict__ dg = pl.dinv;
     int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const InstDesc &d = t.d;
-    int c = t.tx0 + 4 * lane;
+    const int c = t.tx0 + 4 * lane;
+    const int row0 = t.ty0 + wave * RPW;
     double g = 0.0;
     if (c < d.pitch) {
-        for (int k = 0; k < RPW; ++k) {
-            int row = t.ty0 + wave * RPW + k;
-            if (row >= d.nrow) break;
-            long long e = d.off + (long long)row * d.pitch + c;
-            T x4[4], p4[4], r4[4], q4[4], di[4];
-            ld4(pl.x + e, x4);
-            ld4(pd + e, p4);
-            ld4(pl.r + e, r4);
-            ld4(pl.s + e, q4);
-            ld4(pl.dinv + e, di);
+        T x4[RPW][4], p4[RPW][4], r4[RPW][4], q4[RPW][4], di[RPW][4];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                x4[j] = Prec<T>::fmaT(al, p4[j], x4[j]);
-                r4[j] = Prec<T>::fmaT(-al, q4[j], r4[j]);
-                g += (double)r4[j] * (double)(r4[j] * di[j]);
+        for (int k = 0; k < RPW; ++k) {
+            if (row0 + k < d.nrow) {
+                long long e = d.off + (long long)(row0 + k) * d.pitch + c;
+                ld4(xg + e, x4[k]);
+                ld4(pd + e, p4[k]);
+                ld4(rg + e, r4[k]);
+                ld4(qg + e, q4[k]);
+                ld4(dg + e, di[k]);
             }
-            st4(pl.x + e, x4);
-            st4(pl.r + e, r4);
+        }
+#pragma unroll
+        for (int k = 0; k < RPW; ++k) {
+            if (row0 + k < d.nrow) {
+                long long e = d.off + (long long)(row0 + k) * d.pitch + c;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    x4[k][j] = Prec<T>::fmaT(al, p4[k][j], x4[k][j]);
+                    r4[k][j] = Prec<T>::fmaT(-al, q4[k][j], r4[k][j]);
+                    g += (double)r4[k][j] * (double)(r4[k][j] * di[k][j]);
+                }
+                st4(xg + e, x4[k]);
+                st4(rg + e, r4[k]);
+            }
         }
     }
     block_sum2(g, 0.0, partials + t.blk);
@@ -654,6 +681,335 @@ __global__ __launch_bounds__(256) void k_rr_commit(const InstDesc *descs, const 
         ld4(pl.p2 + e, v);
         st4(pl.r + e, v);
     }
+}
+
+// ═══════════════════════════════════════════════════════════════════════════
+// Fused PCG iteration: one stencil kernel and one reduction per iteration.
+//
+// The reference iteration j (cuphu_laplace.cu:452-498) is
+//   q_j = A p_j; alpha_j = rz_j / p_j.q_j; x += alpha_j p_j;
+//   r_{j+1} = r_j - alpha_j q_j; z_{j+1} = r_{j+1} * dinv;
+//   rz_{j+1} = r_{j+1}.z_{j+1}; beta_j = rz_{j+1} / rz_j;
+//   p_{j+1} = z_{j+1} + beta_j p_j.
+// k_it fuses the vector updates of iteration j with the operator of
+// iteration j+1: from (x, r_j, p_j, q_j) and (alpha_j, beta_j) it forms
+// x_{j+1}, r_{j+1}, z_{j+1}, p_{j+1}, q_{j+1} = A p_{j+1} and the partial
+// sums of p.q, r.z, q.(dinv r) and q.(dinv q) of step j+1.  beta_j is needed
+// before r_{j+1}.z_{j+1} is known, so the reduction after step j takes it
+// from the expansion
+//   rz_{j+1} = rz_j - 2 alpha_j q_j.(dinv r_j) + alpha_j^2 q_j.(dinv q_j),
+// which equals the reference's sum up to the rounding of r_{j+1}; alpha, the
+// stopping test (rz_{j+1} < tol^2 rz_0) and the breakdown test (p.q <= 0)
+// use the sums of the vectors actually formed, as in the reference.
+//
+// Vector operations are the reference's, in the oracle's FP32 order: x =
+// fma(a, p, x); r = fma(-a, q, r); z = r * dinv; p = fma(b, p, z).
+// r, p and q are read at halo pixels, so they alternate between two plane
+// sets (parity 0: (r, p, s) -> (r2, p2, q2); parity 1: the reverse); x is
+// only touched at its own pixel.  start = 1 forms p_0 = z_0 from r in set 0
+// (no update) and writes set 1.  A residual replacement (replace flag set by
+// the reduction) takes r_{j+1} from the wv plane instead of r_j - a q_j.
+// ═══════════════════════════════════════════════════════════════════════════
+__device__ __forceinline__ void block_sum4(double a, double b, double c, double e, double4 *out) {
+    __shared__ double s4[4][4];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_xor(a, o, 64);
+        b += __shfl_xor(b, o, 64);
+        c += __shfl_xor(c, o, 64);
+        e += __shfl_xor(e, o, 64);
+    }
+    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) { s4[0][wave] = a; s4[1][wave] = b; s4[2][wave] = c; s4[3][wave] = e; }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        *out = make_double4(((s4[0][0] + s4[0][1]) + s4[0][2]) + s4[0][3], ((s4[1][0] + s4[1][1]) + s4[1][2]) + s4[1][3],
+                            ((s4[2][0] + s4[2][1]) + s4[2][2]) + s4[2][3], ((s4[3][0] + s4[3][1]) + s4[3][2]) + s4[3][3]);
+}
+
+template <class T>
+struct ItSets {
+    const T *r_in, *p_in, *q_in;
+    T *r_out, *p_out, *q_out;
+};
+template <class T>
+__device__ __forceinline__ ItSets<T> it_sets(const Planes<T> &pl, int parity) {
+    ItSets<T> s;
+    s.r_in = parity ? pl.r2 : pl.r;
+    s.p_in = parity ? pl.p2 : pl.p;
+    s.q_in = parity ? pl.q2 : pl.s;
+    s.r_out = parity ? pl.r : pl.r2;
+    s.p_out = parity ? pl.p : pl.p2;
+    s.q_out = parity ? pl.s : pl.q2;
+    return s;
+}
+
+template <class T, int RPW>
+__global__ __launch_bounds__(256) void k_it(const InstDesc *descs, const int *blk_inst, Planes<T> pl,
+                                            const SolveScal *scal, double4 *partials, int parity, int start) {
+    constexpr int TH = 4 * RPW;
+    __shared__ T lp[(TH + 2) * kLdsStride];
+    TileInfo t = get_tile(descs, blk_inst, TH);
+    const SolveScal *sc = scal + t.inst;
+    if (!start && !sc->active) return;
+    const bool upd = !start;                        // x += a p, r -= a q
+    const bool keep = !start && sc->beta != 0.0;    // p = z + b p (else p = z)
+    const bool rep = !start && sc->replace != 0;    // r from the true-residual plane
+    const T al = upd ? (T)sc->alpha : T(0);
+    const T be = (T)sc->beta;
+    const ItSets<T> S = it_sets(pl, parity);
+    const T *__restrict__ r_in = rep ? pl.wv : S.r_in;
+    const T *__restrict__ p_in = S.p_in;
+    const T *__restrict__ q_in = S.q_in;
+    const T *__restrict__ dg = pl.dinv;
+    const T *__restrict__ w2 = pl.w2;
+    T *__restrict__ xg = pl.x;
+    const bool need_p = upd || keep, need_q = upd && !rep;
+    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const InstDesc &d = t.d;
+    const int c = t.tx0 + 4 * lane;
+    const bool cok = c < d.pitch;
+    const int row0 = t.ty0 + wave * RPW;
+    // the pixel update shared by own and halo pixels
+    auto step1 = [&](T r, T q, T p, T di, T &rn, T &pn) {
+        rn = (upd && !rep) ? Prec<T>::fmaT(-al, q, r) : r;
+        T z = rn * di;
+        pn = keep ? Prec<T>::fmaT(be, p, z) : z;
+    };
+    T x4[RPW][4], r4[RPW][4], p4[RPW][4], q4[RPW][4], di[RPW][4], wd[RPW][4], wr[RPW][4], wu[4], wlo[RPW];
+#pragma unroll
+    for (int k = 0; k < RPW; ++k) {
+        zero4(x4[k]); zero4(r4[k]); zero4(p4[k]); zero4(q4[k]); zero4(di[k]); zero4(wd[k]); zero4(wr[k]);
+        wlo[k] = T(0);
+        const int row = row0 + k;
+        if (row < d.nrow) {
+            long long e = d.off + (long long)row * d.pitch + c;
+            if (cok) {
+                if (upd) ld4(xg + e, x4[k]);
+                ld4(r_in + e, r4[k]);
+                if (need_p) ld4(p_in + e, p4[k]);
+                if (need_q) ld4(q_in + e, q4[k]);
+                ld4(dg + e, di[k]);
+                ld_arcs(w2, e, wd[k], wr[k]);
+            }
+            if (lane == 0 && t.tx0 > 0) wlo[k] = w2[2 * (e - 1) + 1];
+        }
+    }
+    up_arcs(w2, t, row0, cok && row0 < d.nrow, lane, wu);
+    fill_halo<TH>(
+        t, wave, lane,
+        [&](int lr, int lc, long long e, bool ok) {
+            T pn[4];
+            zero4(pn);
+            if (ok) {
+                T a[4], b[4], q[4], p[4];
+                ld4(r_in + e, a);
+                ld4(dg + e, b);
+                if (need_q) ld4(q_in + e, q); else zero4(q);
+                if (keep) ld4(p_in + e, p); else zero4(p);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) { T rn; step1(a[j], q[j], p[j], b[j], rn, pn[j]); }
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) lp[lr * kLdsStride + lc + j] = pn[j];
+        },
+        [&](int lr, int lc, long long e, bool ok) {
+            T v = T(0);
+            if (ok) {
+                T rn;
+                step1(r_in[e], need_q ? q_in[e] : T(0), keep ? p_in[e] : T(0), dg[e], rn, v);
+            }
+            lp[lr * kLdsStride + lc] = v;
+        });
+    // own pixels: updates, stores, LDS
+#pragma unroll
+    for (int k = 0; k < RPW; ++k) {
+        const int lr = wave * RPW + k, row = row0 + k;
+        T pn[4], rn[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (upd) x4[k][j] = Prec<T>::fmaT(al, p4[k][j], x4[k][j]);
+            step1(r4[k][j], q4[k][j], p4[k][j], di[k][j], rn[j], pn[j]);
+            r4[k][j] = rn[j];
+        }
+        if (row < d.nrow && cok) {
+            long long e = d.off + (long long)row * d.pitch + c;
+            if (upd) st4(xg + e, x4[k]);
+            st4(S.r_out + e, rn);
+            st4(S.p_out + e, pn);
+        }
+        T *lq = lp + (lr + 1) * kLdsStride + kLdsC0 + 4 * lane;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) lq[j] = pn[j];
+    }
+    __syncthreads();
+    double s_pq = 0.0, s_rz = 0.0, s_qr = 0.0, s_qq = 0.0;
+#pragma unroll
+    for (int k = 0; k < RPW; ++k) {
+        const int lr = wave * RPW + k, row = row0 + k;
+        if (row >= d.nrow) break;
+        T wl[4], y[4];
+        T from_left = __shfl_up(wr[k][3], 1, 64);
+        wl[0] = (lane == 0) ? wlo[k] : from_left;
+        wl[1] = wr[k][0]; wl[2] = wr[k][1]; wl[3] = wr[k][2];
+        stencil4(lp, lr, lane, wd[k], (k == 0) ? wu : wd[k > 0 ? k - 1 : 0], wr[k], wl, y);
+        if (cok) {
+            st4(S.q_out + d.off + (long long)row * d.pitch + c, y);
+            const T *pc = lp + (lr + 1) * kLdsStride + kLdsC0 + 4 * lane;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                double rd = (double)r4[k][j], dd = (double)di[k][j], qd = (double)y[j];
+                s_pq += (double)pc[j] * qd;
+                s_rz += rd * (double)(r4[k][j] * di[k][j]);
+                double qdi = qd * dd;
+                s_qr += qdi * rd;
+                s_qq += qdi * qd;
+            }
+        }
+    }
+    block_sum4(s_pq, s_rz, s_qr, s_qq, partials + t.blk);
+}
+
+// True residual for the fused solver, at the end of an even chunk (state in
+// set 1 with alpha_j pending): x' = x + alpha_j p_j on own and halo pixels,
+// r_t = b - A x' into the wv plane, partial sums of r_t.(dinv r_t) and of
+// r'.(dinv r') for the recursive r' = r_j - alpha_j q_j.
+template <class T, int RPW>
+__global__ __launch_bounds__(256) void k_rr_it(const InstDesc *descs, const int *blk_inst, Planes<T> pl,
+                                               const SolveScal *scal, double4 *partials) {
+    constexpr int TH = 4 * RPW;
+    __shared__ T lx[(TH + 2) * kLdsStride];
+    TileInfo t = get_tile(descs, blk_inst, TH);
+    const SolveScal *sc = scal + t.inst;
+    if (!sc->active) return;
+    const T al = (T)sc->alpha;
+    const T *__restrict__ pg = pl.p2;
+    const T *__restrict__ xg = pl.x;
+    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const InstDesc &d = t.d;
+    const int c = t.tx0 + 4 * lane;
+    const bool cok = c < d.pitch;
+    for (int k = 0; k < RPW; ++k) {
+        int lr = wave * RPW + k, row = t.ty0 + lr;
+        T xv[4], pv[4];
+        zero4(xv); zero4(pv);
+        if (row < d.nrow && cok) {
+            long long e = d.off + (long long)row * d.pitch + c;
+            ld4(xg + e, xv);
+            ld4(pg + e, pv);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) lx[(lr + 1) * kLdsStride + kLdsC0 + 4 * lane + j] = Prec<T>::fmaT(al, pv[j], xv[j]);
+    }
+    fill_halo<TH>(
+        t, wave, lane,
+        [&](int lr, int lc, long long e, bool ok) {
+            T xv[4], pv[4];
+            zero4(xv); zero4(pv);
+            if (ok) { ld4(xg + e, xv); ld4(pg + e, pv); }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) lx[lr * kLdsStride + lc + j] = Prec<T>::fmaT(al, pv[j], xv[j]);
+        },
+        [&](int lr, int lc, long long e, bool ok) {
+            lx[lr * kLdsStride + lc] = ok ? Prec<T>::fmaT(al, pg[e], xg[e]) : T(0);
+        });
+    __syncthreads();
+    double s_t = 0.0, s_r = 0.0;
+    T wu[4];
+    for (int k = 0; k < RPW; ++k) {
+        int lr = wave * RPW + k, row = t.ty0 + lr;
+        if (row >= d.nrow) break;
+        if (k == 0) up_arcs(pl.w2, t, row, cok, lane, wu);
+        T wd[4], wr[4], wl[4], y[4];
+        row_arcs(pl.w2, t, row, cok, lane, wd, wr, wl);
+        stencil4(lx, lr, lane, wd, wu, wr, wl, y);
+        if (cok) {
+            long long e = d.off + (long long)row * d.pitch + c;
+            T b4[4], di[4], rt[4], r4[4], q4[4];
+            ld4(pl.b + e, b4);
+            ld4(pl.dinv + e, di);
+            ld4(pl.r2 + e, r4);
+            ld4(pl.q2 + e, q4);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                rt[j] = b4[j] - y[j];
+                T rr = Prec<T>::fmaT(-al, q4[j], r4[j]);
+                s_t += (double)rt[j] * (double)(rt[j] * di[j]);
+                s_r += (double)rr * (double)(rr * di[j]);
+            }
+            st4(pl.wv + e, rt);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) wu[j] = wd[j];
+    }
+    block_sum4(s_t, s_r, 0.0, 0.0, partials + t.blk);
+}
+
+// Reduction and scalar recurrences of the fused solver (one workgroup per
+// instance).
+//   start: rz_0; stop if not positive (:446-450) or if max_iter <= 0; else
+//          as `it` without counting an iteration.
+//   it:    one iteration done (iters++); converged if rz < tol^2 rz_0
+//          (:478); max_iter; p.q <= 0 breakdown (:460); alpha = rz / p.q
+//          (:462); beta from the rz expansion (see k_it).
+//   rr:    replace r by the true residual unless it exceeds the recursive
+//          one by more than 4x (the recursion is then below the accuracy the
+//          arithmetic attains); beta = (chosen rz) / rz_j exactly.
+__global__ __launch_bounds__(256) void k_reduce_it(const InstDesc *descs, const double4 *partials, SolveScal *scal,
+                                                   int mode, int max_iter, double tol2) {
+    int inst = blockIdx.x;
+    const InstDesc d = descs[inst];
+    SolveScal *sp = scal + inst;
+    if (mode != RED_START && !sp->active) return;
+    int nb = d.tiles_x * d.tiles_y;
+    double a = 0.0, b = 0.0, c = 0.0, e = 0.0;
+    for (int i = threadIdx.x; i < nb; i += 256) {
+        double4 v = partials[d.block_begin + i];
+        a += v.x; b += v.y; c += v.z; e += v.w;
+    }
+    __shared__ double4 res;
+    block_sum4(a, b, c, e, &res);
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    SolveScal s = *sp;
+    const double4 v = res;
+    s.replace = 0;
+    if (mode == RED_RR) {
+        const double rt = v.x, rr = v.y;
+        if (isfinite(rt) && rt <= 4.0 * rr) { s.replace = 1; s.beta = rt / s.gamma; }
+        else if (isfinite(rr)) s.beta = rr / s.gamma;
+        *sp = s;
+        return;
+    }
+    const double pq = v.x, rz = v.y, qr = v.z, qq = v.w;
+    if (mode == RED_START) {
+        s.iters = 0; s.gamma0 = rz; s.gamma_prev = rz; s.relres = 1.0; s.active = 0;
+        s.alpha = 0.0; s.beta = 0.0; s.delta = pq; s.gamma = rz;
+        if (!(rz > 0.0)) { s.status = (rz == 0.0) ? CUPHU_SOLVE_EMPTY : CUPHU_SOLVE_NONFINITE; s.relres = 0.0; *sp = s; return; }
+        if (max_iter <= 0) { s.status = CUPHU_SOLVE_MAXITER; *sp = s; return; }
+    } else {
+        s.iters += 1;
+        s.relres = (s.gamma0 > 0.0) ? sqrt(fabs(rz) / s.gamma0) : 0.0;
+        s.gamma_prev = s.gamma;
+        s.gamma = rz;
+        s.delta = pq;
+        if (rz < tol2 * s.gamma0) { s.status = CUPHU_SOLVE_CONVERGED; s.active = 0; *sp = s; return; }
+        if (s.iters >= max_iter) { s.status = CUPHU_SOLVE_MAXITER; s.active = 0; *sp = s; return; }
+    }
+    if (!(pq > 0.0)) {
+        s.status = isfinite(pq) ? CUPHU_SOLVE_BREAKDOWN : CUPHU_SOLVE_NONFINITE;
+        s.active = 0;
+        *sp = s;
+        return;
+    }
+    const double al = rz / pq;
+    double rzn = rz - 2.0 * al * qr + al * al * qq;
+    if (!(rzn > 0.0)) rzn = 0.0;
+    s.alpha = al;
+    s.beta = rzn / rz;
+    s.status = CUPHU_SOLVE_MAXITER;
+    s.active = 1;
+    *sp = s;
 }
 
 // ═══════════════════════════════════════════════════════════════════════════
@@ -915,6 +1271,23 @@ void launch_rr_commit(const InstDesc *d_desc, const int *d_blk, int nblk, const 
     CUPHU_RPW_DISPATCH(rpw, k_rr_commit, dim3(nblk), dim3(256), 0, st, d_desc, d_blk, pl, scal);
     CUPHU_LAUNCH_CHECK();
 }
+template <class T>
+void launch_it(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, const SolveScal *scal,
+               double4 *partials, int rpw, int parity, int start, hipStream_t st) {
+    CUPHU_RPW_DISPATCH(rpw, k_it, dim3(nblk), dim3(256), 0, st, d_desc, d_blk, pl, scal, partials, parity, start);
+    CUPHU_LAUNCH_CHECK();
+}
+template <class T>
+void launch_rr_it(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, const SolveScal *scal,
+                  double4 *partials, int rpw, hipStream_t st) {
+    CUPHU_RPW_DISPATCH(rpw, k_rr_it, dim3(nblk), dim3(256), 0, st, d_desc, d_blk, pl, scal, partials);
+    CUPHU_LAUNCH_CHECK();
+}
+void launch_reduce_it(const InstDesc *d_desc, int ninst, const double4 *partials, SolveScal *scal, int mode,
+                      int max_iter, double tol2, hipStream_t st) {
+    hipLaunchKernelGGL(k_reduce_it, dim3(ninst), dim3(256), 0, st, d_desc, partials, scal, mode, max_iter, tol2);
+    CUPHU_LAUNCH_CHECK();
+}
 void launch_reduce(const InstDesc *d_desc, int ninst, const double2 *partials, SolveScal *scal, int mode,
                    int max_iter, double tol2, hipStream_t st) {
     hipLaunchKernelGGL(k_reduce, dim3(ninst), dim3(256), 0, st, d_desc, partials, scal, mode, max_iter, tol2);
@@ -976,6 +1349,10 @@ void launch_plain_dot(const T *a, const T *b, size_t n, double *partials, double
                                double2 *, int, hipStream_t);                                                     \
     template void launch_rr_commit<T>(const InstDesc *, const int *, int, const Planes<T> &, const SolveScal *,  \
                                       int, hipStream_t);                                                         \
+    template void launch_it<T>(const InstDesc *, const int *, int, const Planes<T> &, const SolveScal *,         \
+                               double4 *, int, int, int, hipStream_t);                                           \
+    template void launch_rr_it<T>(const InstDesc *, const int *, int, const Planes<T> &, const SolveScal *,      \
+                                  double4 *, int, hipStream_t);                                                  \
     template void launch_final<T>(const InstDesc *, const int *, int, const Planes<T> &, float *, int,           \
                                   hipStream_t);                                                                  \
     template void launch_plain_matvec<T>(const InstDesc &, const T *, const T *, T *, hipStream_t);              \

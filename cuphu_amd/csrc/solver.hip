@@ -95,7 +95,7 @@ void default_options(CuPhuLaplaceOptions *o) {
     o->delta_min = 0.01f;
     o->max_iter = 1000;
     o->tol = 1e-3;
-    o->solver = CUPHU_SOLVER_BATCHED;
+    o->solver = CUPHU_SOLVER_FUSED;
     o->precision = 32;
     o->rr_interval = 50;
     o->use_graph = 1;
@@ -107,7 +107,9 @@ void default_options(CuPhuLaplaceOptions *o) {
     if (const char *e = std::getenv("CUPHU_LAPLACE_TOL")) o->tol = std::atof(e);
     if (std::getenv("CUPHU_LAPLACE_VERBOSE")) o->verbose = 1;
     if (const char *e = std::getenv("CUPHU_LAPLACE_SOLVER"))
-        o->solver = (std::strcmp(e, "plain") == 0) ? CUPHU_SOLVER_PLAIN : CUPHU_SOLVER_BATCHED;
+        o->solver = (std::strcmp(e, "plain") == 0)     ? CUPHU_SOLVER_PLAIN
+                    : (std::strcmp(e, "batched") == 0) ? CUPHU_SOLVER_BATCHED
+                                                       : CUPHU_SOLVER_FUSED;
     env_int("CUPHU_LAPLACE_PRECISION", &o->precision);
     env_int("CUPHU_LAPLACE_GROUP_MB", &o->group_bytes_mb);
     env_int("CUPHU_LAPLACE_GRAPH", &o->use_graph);
@@ -116,7 +118,8 @@ void default_options(CuPhuLaplaceOptions *o) {
 static void check_options(const CuPhuLaplaceOptions &o) {
     CUPHU_REQUIRE(o.n_irls >= 0 && o.n_irls <= CUPHU_MAX_IRLS, "n_irls must be in [0, 32]");
     CUPHU_REQUIRE(o.precision == 32 || o.precision == 64, "precision must be 32 or 64");
-    CUPHU_REQUIRE(o.solver == CUPHU_SOLVER_PLAIN || o.solver == CUPHU_SOLVER_BATCHED, "unknown solver");
+    CUPHU_REQUIRE(o.solver == CUPHU_SOLVER_PLAIN || o.solver == CUPHU_SOLVER_BATCHED || o.solver == CUPHU_SOLVER_FUSED,
+                  "unknown solver");
     CUPHU_REQUIRE(o.tol >= 0.0 && o.max_iter >= 0, "tol and max_iter must be >= 0");
 }
 
@@ -173,6 +176,8 @@ template <class T> static PoolSet<T> alloc_pools(DevRes &r, long long n) {
     P.pl.s = ws<T>(r, nm("s").c_str(), n);
     P.pl.wv = ws<T>(r, nm("wv").c_str(), n);
     P.pl.p2 = ws<T>(r, nm("p2").c_str(), n);
+    P.pl.r2 = ws<T>(r, nm("r2").c_str(), n);
+    P.pl.q2 = ws<T>(r, nm("q2").c_str(), n);
     P.out = ws<float>(r, "out", n);
     P.lab = ws<uint32_t>(r, "lab", n);
     return P;
@@ -210,6 +215,7 @@ struct GroupDev {
     int *blk;
     SolveScal *scal;
     double2 *part;
+    double4 *part4;   // fused solver's four partial sums per workgroup
     int ninst, nblk;
     std::vector<long long> px;   // host: logical pixels per instance
 };
@@ -249,7 +255,8 @@ static void solve_group_batched(const Planes<T> &pl, const GroupDev &g, const Cu
         launch_reduce(g.desc, g.ninst, g.part, g.scal, RED_RZ, o.max_iter, tol2, st);
     };
     // With profiling on, the first two steps of a chunk run outside the
-    // graph (the first with events around each kernel); the captured part
+    // graph: step 0 keeps the GPU busy so that step 1, with events around
+    // each kernel, runs back to back like the graph's launches; the graph
     // holds CH-2.
     auto chunk = [&]() {
         for (int i = prof ? 2 : 0; i < CH; ++i) step(i & 1);
@@ -301,15 +308,15 @@ static void solve_group_batched(const Planes<T> &pl, const GroupDev &g, const Cu
                 if (prof) {
                     for (int i = 0; i < g.ninst; ++i)
                         if (h[(size_t)i].active) active_px += (double)g.px[(size_t)i];
+                    step(0);
                     CUPHU_HIP_CHECK(hipEventRecord(g_prof.ev[0], st));
-                    launch_pq<T>(g.desc, g.blk, g.nblk, pl, g.scal, g.part, rpw, 0, st);
+                    launch_pq<T>(g.desc, g.blk, g.nblk, pl, g.scal, g.part, rpw, 1, st);
                     CUPHU_HIP_CHECK(hipEventRecord(g_prof.ev[1], st));
                     launch_reduce(g.desc, g.ninst, g.part, g.scal, RED_PQ, o.max_iter, tol2, st);
                     CUPHU_HIP_CHECK(hipEventRecord(g_prof.ev[2], st));
-                    launch_xr<T>(g.desc, g.blk, g.nblk, pl, g.scal, g.part, rpw, 0, st);
+                    launch_xr<T>(g.desc, g.blk, g.nblk, pl, g.scal, g.part, rpw, 1, st);
                     CUPHU_HIP_CHECK(hipEventRecord(g_prof.ev[3], st));
                     launch_reduce(g.desc, g.ninst, g.part, g.scal, RED_RZ, o.max_iter, tol2, st);
-                    step(1);
                 }
                 if (gexec) CUPHU_HIP_CHECK(hipGraphLaunch(gexec, st));
                 else chunk();
@@ -342,6 +349,125 @@ static void solve_group_batched(const Planes<T> &pl, const GroupDev &g, const Cu
         throw;
     }
     if (gexec) CUPHU_HIP_CHECK(hipGraphExecDestroy(gexec));
+}
+
+// Algorithmic HBM bytes per active pixel of the fused iteration kernel:
+// reads x, r, p, q, 1/diag and the {down, right} arc pair, writes x, r, p, q.
+constexpr double kItValuesPerPx = 11.0;
+
+// Fused solver: one stencil kernel and one reduction per PCG iteration.
+template <class T>
+static void solve_group_fused(const Planes<T> &pl, const GroupDev &g, const CuPhuLaplaceOptions &o, int rpw,
+                              bool fixed_weights, hipStream_t st, const std::vector<CuPhuSolveStats *> &stats,
+                              std::vector<SolveScal> &h) {
+    // Chunk length: even, so a chunk (parities 1, 0, 1, ..., 0) leaves the
+    // state in plane set 1 where the start kernel put it.
+    int ch = o.rr_interval > 0 ? o.rr_interval : 50;
+    ch = std::max(2, std::min(ch, std::max(o.max_iter, 2)));
+    const int CH = ch + (ch & 1);
+    const double tol2 = o.tol * o.tol;
+    const bool prof = g_prof.on;
+    if (prof && !g_prof.ev[0])
+        for (auto &e : g_prof.ev) CUPHU_HIP_CHECK(hipEventCreate(&e));
+    auto step = [&](int i) {
+        launch_it<T>(g.desc, g.blk, g.nblk, pl, g.scal, g.part4, rpw, 1 - (i & 1), 0, st);
+        launch_reduce_it(g.desc, g.ninst, g.part4, g.scal, RED_IT, o.max_iter, tol2, st);
+    };
+    // With profiling on, the first two steps of a chunk run outside the
+    // graph: step 0 keeps the GPU busy so that step 1, timed between two
+    // events, runs back to back like the graph's launches (a launch onto an
+    // idle GPU runs faster than the steady state); the graph holds CH-2.
+    auto chunk = [&]() {
+        for (int i = prof ? 2 : 0; i < CH; ++i) step(i);
+        if (o.rr_interval > 0) {
+            launch_rr_it<T>(g.desc, g.blk, g.nblk, pl, g.scal, g.part4, rpw, st);
+            launch_reduce_it(g.desc, g.ninst, g.part4, g.scal, RED_RR, o.max_iter, tol2, st);
+        }
+    };
+    hipGraphExec_t gexec = nullptr;
+    if (o.use_graph) {
+        hipGraph_t graph = nullptr;
+        CUPHU_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+        try {
+            chunk();
+        } catch (...) {
+            hipGraph_t dummy;
+            (void)hipStreamEndCapture(st, &dummy);
+            throw;
+        }
+        CUPHU_HIP_CHECK(hipStreamEndCapture(st, &graph));
+        CUPHU_HIP_CHECK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+        CUPHU_HIP_CHECK(hipGraphDestroy(graph));
+    }
+    h.resize((size_t)g.ninst);
+    const int npass = fixed_weights ? 1 : o.n_irls + 1;
+    try {
+        for (int pass = 0; pass < npass; ++pass) {
+            if (!fixed_weights) {
+                T d2 = T(0);
+                if (pass > 0) {
+                    float df = pass_delta(o, pass);
+                    d2 = (sizeof(T) == 4) ? (T)(df * df) : (T)((double)df * (double)df);
+                }
+                launch_weights<T>(g.desc, g.blk, g.nblk, pl, pass > 0 ? 1 : 0, d2, rpw, st);
+                launch_assemble<T>(g.desc, g.blk, g.nblk, pl, rpw, st);
+            }
+            launch_it<T>(g.desc, g.blk, g.nblk, pl, g.scal, g.part4, rpw, 0, 1, st);
+            launch_reduce_it(g.desc, g.ninst, g.part4, g.scal, RED_START, o.max_iter, tol2, st);
+            CUPHU_HIP_CHECK(hipMemcpyAsync(h.data(), g.scal, h.size() * sizeof(SolveScal), hipMemcpyDeviceToHost, st));
+            CUPHU_HIP_CHECK(hipStreamSynchronize(st));
+            int nch = 0;
+            auto any_active = [&]() {
+                for (auto &s : h) if (s.active) return true;
+                return false;
+            };
+            while (any_active()) {
+                double active_px = 0.0;
+                if (prof) {
+                    for (int i = 0; i < g.ninst; ++i)
+                        if (h[(size_t)i].active) active_px += (double)g.px[(size_t)i];
+                    step(0);
+                    CUPHU_HIP_CHECK(hipEventRecord(g_prof.ev[0], st));
+                    launch_it<T>(g.desc, g.blk, g.nblk, pl, g.scal, g.part4, rpw, 0, 0, st);
+                    CUPHU_HIP_CHECK(hipEventRecord(g_prof.ev[1], st));
+                    launch_reduce_it(g.desc, g.ninst, g.part4, g.scal, RED_IT, o.max_iter, tol2, st);
+                }
+                if (gexec) CUPHU_HIP_CHECK(hipGraphLaunch(gexec, st));
+                else chunk();
+                CUPHU_HIP_CHECK(hipMemcpyAsync(h.data(), g.scal, h.size() * sizeof(SolveScal), hipMemcpyDeviceToHost, st));
+                CUPHU_HIP_CHECK(hipStreamSynchronize(st));
+                if (prof) {
+                    float ms = 0.0f;
+                    CUPHU_HIP_CHECK(hipEventElapsedTime(&ms, g_prof.ev[0], g_prof.ev[1]));
+                    CuPhuKernelProfile &a = g_prof.acc;
+                    a.samples += 1;
+                    a.px_sum += active_px;
+                    a.it_ms_sum += ms;
+                    a.it_bytes_sum += active_px * kItValuesPerPx * sizeof(T);
+                    a.iterations += CH;
+                }
+                ++nch;
+                if ((long long)nch * CH > (long long)o.max_iter + CH) break;   // cannot happen: reduce caps iterations
+            }
+            if (o.verbose)
+                for (int i = 0; i < g.ninst; ++i)
+                    fprintf(stderr, "[cuphu-pcg] inst %d pass %d iters %d relres %.3e status %d\n", i, pass, h[i].iters,
+                            h[i].relres, h[i].status);
+            for (int i = 0; i < g.ninst; ++i) record_pass(stats[(size_t)i], pass, h[(size_t)i]);
+        }
+    } catch (...) {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        throw;
+    }
+    if (gexec) CUPHU_HIP_CHECK(hipGraphExecDestroy(gexec));
+}
+
+template <class T>
+static void solve_group(const Planes<T> &pl, const GroupDev &g, const CuPhuLaplaceOptions &o, int rpw,
+                        bool fixed_weights, hipStream_t st, const std::vector<CuPhuSolveStats *> &stats,
+                        std::vector<SolveScal> &h) {
+    if (o.solver == CUPHU_SOLVER_FUSED) solve_group_fused<T>(pl, g, o, rpw, fixed_weights, st, stats, h);
+    else solve_group_batched<T>(pl, g, o, rpw, fixed_weights, st, stats, h);
 }
 
 // ── plain solve of one instance (reference launch sequence) ─────────────────
@@ -459,6 +585,7 @@ static void run_batch(const std::vector<std::pair<int, int>> &dims, const CuPhuL
     g.blk = ws<int>(res, "grpblk", maxblk);
     g.scal = ws<SolveScal>(res, "grpscal", maxinst);
     g.part = ws<double2>(res, "grppart", maxblk);
+    g.part4 = ws<double4>(res, "grppart4", maxblk);
     std::vector<SolveScal> hs;
     std::vector<CuPhuSolveStats> dummy((size_t)ninst);
     auto sp = [&](int i) { return stats ? stats + i : &dummy[(size_t)i]; };
@@ -491,7 +618,7 @@ static void run_batch(const std::vector<std::pair<int, int>> &dims, const CuPhuL
             upload_group(gr.first, gr.second);
             std::vector<CuPhuSolveStats *> gs;
             for (int i = gr.first; i < gr.second; ++i) gs.push_back(sp(i));
-            solve_group_batched<T>(P.pl, g, o, rpw, false, st, gs, hs);
+            solve_group<T>(P.pl, g, o, rpw, false, st, gs, hs);
         }
     }
     launch_final<T>(d_gdesc, d_rowinst, L.total_rows, P.pl, P.out, L.max_pitch, st);
@@ -752,7 +879,7 @@ static void wls_impl(const T *d_w, const float *d_phase, const T *d_u, const T *
     o.n_irls = 0;
     o.max_iter = max_iter;
     o.tol = tol;
-    o.solver = CUPHU_SOLVER_BATCHED;
+    o.solver = CUPHU_SOLVER_FUSED;
     const int rpw = rows_per_wave(), th = 4 * rpw;
     Layout L = make_layout({{nrow, ncol}}, th);
     PoolSet<T> P = alloc_pools<T>(res, L.total_el);
@@ -766,6 +893,7 @@ static void wls_impl(const T *d_w, const float *d_phase, const T *d_u, const T *
     g.blk = ws<int>(res, "grpblk", (size_t)nb);
     g.scal = ws<SolveScal>(res, "grpscal", 1);
     g.part = ws<double2>(res, "grppart", (size_t)nb);
+    g.part4 = ws<double4>(res, "grppart4", (size_t)nb);
     g.ninst = 1;
     g.nblk = nb;
     g.px = {(long long)nrow * ncol};
@@ -783,7 +911,7 @@ static void wls_impl(const T *d_w, const float *d_phase, const T *d_u, const T *
     std::vector<SolveScal> h;
     CuPhuSolveStats stt{};
     std::vector<CuPhuSolveStats *> sp = {&stt};
-    solve_group_batched<T>(P.pl, g, o, rpw, true, st, sp, h);
+    solve_group<T>(P.pl, g, o, rpw, true, st, sp, h);
     if (iters_out) *iters_out = stt.iters[0];
     if (relres_out) *relres_out = stt.relres[0];
     double mx = device_mean<T>(P.pl.x + d.off, nrow, ncol, d.pitch, res, st);
@@ -1460,6 +1588,7 @@ void hook_setup(HookInst &h, DevRes &res, int nrow, int ncol, int rpw, hipStream
     h.g.blk = ws<int>(res, "grpblk", (size_t)nb);
     h.g.scal = ws<SolveScal>(res, "grpscal", 1);
     h.g.part = ws<double2>(res, "grppart", (size_t)nb);
+    h.g.part4 = ws<double4>(res, "grppart4", (size_t)nb);
     h.g.ninst = 1;
     h.g.nblk = nb;
     std::vector<int> blk((size_t)nb, 0);
@@ -1469,7 +1598,7 @@ void hook_setup(HookInst &h, DevRes &res, int nrow, int ncol, int rpw, hipStream
     unsigned g = (unsigned)ceil_div(2 * nel, 256);
     hipLaunchKernelGGL(k_zero<float>, dim3(g), dim3(256), 0, st, h.P.pl.w2, (long long)(2 * nel));
     for (float *p : {h.P.pl.b, h.P.pl.dinv, h.P.pl.x, h.P.pl.r, h.P.pl.p, h.P.pl.s, h.P.pl.wv, h.P.pl.phi,
-                     h.P.pl.p2})
+                     h.P.pl.p2, h.P.pl.r2, h.P.pl.q2})
         hipLaunchKernelGGL(k_zero<float>, dim3(g), dim3(256), 0, st, p, (long long)nel);
     CUPHU_HIP_CHECK(hipGetLastError());
     CUPHU_HIP_CHECK(hipStreamSynchronize(st));
@@ -1627,6 +1756,53 @@ int cuphu_test_pcg_step(const float *w, const float *dinv, int nrow, int ncol, d
         *pq = sc.delta;
         *alpha = sc.alpha;
         *rz = sc.gamma;
+        return 0;
+    });
+}
+
+int cuphu_test_pcg_fused_step(const float *w, const float *dinv, int nrow, int ncol, double alpha, double beta,
+                              int start, const float *rt, int rpw, float *x, float *r, float *p, float *q,
+                              double *sums) {
+    return guarded([&]() {
+        std::lock_guard<std::recursive_mutex> lk(g_mu);
+        CUPHU_REQUIRE(w && dinv && x && r && p && q && sums && nrow >= 1 && ncol >= 1, "null argument");
+        CUPHU_REQUIRE(rpw == 2 || rpw == 4 || rpw == 8, "rpw must be 2, 4 or 8");
+        DevRes &res = devres();
+        hipStream_t st = res.stream;
+        HookInst h;
+        hook_setup(h, res, nrow, ncol, rpw, st);
+        const InstDesc &d = h.d;
+        size_t na = (size_t)(nrow - 1) * ncol + (size_t)nrow * (ncol - 1);
+        DevBuf<float> dw(na + 1);
+        if (na) CUPHU_HIP_CHECK(hipMemcpyAsync(dw.get(), w, na * 4, hipMemcpyHostToDevice, st));
+        unsigned gp = (unsigned)ceil_div((size_t)nrow * d.pitch, 256);
+        hipLaunchKernelGGL(k_flat_w_to_planes<float>, dim3(gp), dim3(256), 0, st, (const float *)dw.get(), d,
+                           h.P.pl.w2);
+        CUPHU_HIP_CHECK(hipGetLastError());
+        to_plane(dinv, h.P.pl.dinv, d, st);
+        to_plane(x, h.P.pl.x, d, st);
+        to_plane(r, h.P.pl.r, d, st);
+        to_plane(p, h.P.pl.p, d, st);
+        to_plane(q, h.P.pl.s, d, st);
+        if (rt) to_plane(rt, h.P.pl.wv, d, st);
+        SolveScal sc{};
+        sc.alpha = alpha;
+        sc.beta = beta;
+        sc.replace = rt ? 1 : 0;
+        sc.active = 1;
+        CUPHU_HIP_CHECK(hipMemcpyAsync(h.g.scal, &sc, sizeof(sc), hipMemcpyHostToDevice, st));
+        // parity 0: reads set 0 (r, p, s), writes set 1 (r2, p2, q2)
+        launch_it<float>(h.g.desc, h.g.blk, h.g.nblk, h.P.pl, h.g.scal, h.g.part4, rpw, 0, start ? 1 : 0, st);
+        std::vector<double4> parts((size_t)h.g.nblk);
+        CUPHU_HIP_CHECK(hipMemcpyAsync(parts.data(), h.g.part4, parts.size() * sizeof(double4), hipMemcpyDeviceToHost, st));
+        from_plane(h.P.pl.x, x, d, st);
+        from_plane(h.P.pl.r2, r, d, st);
+        from_plane(h.P.pl.p2, p, d, st);
+        from_plane(h.P.pl.q2, q, d, st);
+        CUPHU_HIP_CHECK(hipStreamSynchronize(st));
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        for (auto &v : parts) { a0 += v.x; a1 += v.y; a2 += v.z; a3 += v.w; }
+        sums[0] = a0; sums[1] = a1; sums[2] = a2; sums[3] = a3;
         return 0;
     });
 }

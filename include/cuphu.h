@@ -274,9 +274,14 @@ int cuphu_residues_gpu(const float *d_phase, int nrow, int ncol,
 enum {
     CUPHU_SOLVER_PLAIN   = 0, /* reference launch sequence: 7 launches and 2
                                  blocking scalar reads per PCG iteration     */
-    CUPHU_SOLVER_BATCHED = 1  /* fused single-reduction PCG over all
-                                 instances, device-resident scalars, HIP
-                                 graph chunks                                */
+    CUPHU_SOLVER_BATCHED = 1, /* all instances at once, device-resident
+                                 scalars, HIP graph chunks; the reference's
+                                 recurrence in two fused stencil/update
+                                 kernels and two reductions per iteration    */
+    CUPHU_SOLVER_FUSED   = 2  /* as BATCHED with one stencil kernel and one
+                                 reduction per iteration: beta from the
+                                 expansion of r.z (equal to the reference's
+                                 sum up to FP rounding); default             */
 };
 #define CUPHU_MAX_IRLS 32
 typedef struct CuPhuLaplaceOptions {
@@ -285,7 +290,7 @@ typedef struct CuPhuLaplaceOptions {
     float  delta_min;
     int    max_iter;      /* PCG iteration cap per solve                     */
     double tol;           /* stop when r.z < tol^2 * (r.z)_0                 */
-    int    solver;        /* CUPHU_SOLVER_PLAIN / CUPHU_SOLVER_BATCHED       */
+    int    solver;        /* CUPHU_SOLVER_PLAIN / _BATCHED / _FUSED          */
     int    precision;     /* 32: FP32 vectors, FP64 reductions; 64: all FP64 */
     int    rr_interval;   /* batched: true-residual replacement period (50)  */
     int    use_graph;     /* batched: capture iteration chunks in hipGraphs  */
@@ -418,15 +423,18 @@ int cuphu_wls_adjoint_gpu(const void *d_w, const void *d_phase, const void *d_u,
                           int *iters_out, double *relres_out,
                           cuphu_stream_t stream);
 
-/* Live timing of the two PCG kernels.  While enabled, the first PCG
- * iteration of every iteration chunk is launched outside the chunk's HIP
- * graph with HIP events around each kernel on the solver's stream; the
- * durations and algorithmic bytes are accumulated per kernel:
- *   pq: p = z + beta p, q = L p, p.q  -- reads r, 1/diag, p_old and two arc
- *       weights, writes p_new and q (28 B per active pixel in FP32, 56 B in
- *       FP64);
- *   xr: x += alpha p, r -= alpha q, r.z -- reads x, p, r, q, 1/diag, writes
- *       x and r (28 B / 56 B per active pixel).
+/* Live timing of the PCG kernels.  While enabled, the first two PCG
+ * iterations of every iteration chunk are launched outside the chunk's HIP
+ * graph, the second with HIP events around its kernels on the solver's
+ * stream (it follows the first back to back, as the graph's launches do);
+ * the durations and algorithmic bytes are accumulated per kernel:
+ *   pq: (BATCHED) p = z + beta p, q = L p, p.q -- reads r, 1/diag, p_old and
+ *       two arc weights, writes p_new and q (28 B per active pixel in FP32,
+ *       56 B in FP64);
+ *   xr: (BATCHED) x += alpha p, r -= alpha q, r.z -- reads x, p, r, q,
+ *       1/diag, writes x and r (28 B / 56 B per active pixel);
+ *   it: (FUSED) one whole iteration -- reads x, r, p, q, 1/diag and two arc
+ *       weights, writes x, r, p, q (44 B / 88 B per active pixel).
  * cuphu_set_profiling resets the counters. */
 typedef struct CuPhuKernelProfile {
     long long samples;        /* timed iterations                          */
@@ -435,6 +443,8 @@ typedef struct CuPhuKernelProfile {
     double    pq_bytes_sum;   /* its summed algorithmic bytes              */
     double    xr_ms_sum;      /* summed duration of the xr kernel (ms)     */
     double    xr_bytes_sum;   /* its summed algorithmic bytes              */
+    double    it_ms_sum;      /* summed duration of the fused kernel (ms)  */
+    double    it_bytes_sum;   /* its summed algorithmic bytes              */
     long long iterations;     /* all PCG iterations launched while enabled */
 } CuPhuKernelProfile;
 int cuphu_set_profiling(int enabled);
@@ -468,6 +478,15 @@ int cuphu_test_pcg_step(const float *w, const float *dinv, int nrow, int ncol,
                         double gamma, double beta, int rpw,
                         float *x, float *r, float *p, float *q,
                         double *pq, double *alpha, double *rz);
+
+/* One fused-solver PCG iteration (FP32) through its stencil kernel:
+ * x += alpha p; r -= alpha q (r = rt when rt != NULL, the residual
+ * replacement); z = r*dinv; p = z + beta p (p = z when beta == 0); q = L p;
+ * start != 0 skips the x, r updates and sets p = z.  Updates x, r, p, q in
+ * place; sums[4] = p.q, r.z, q.(dinv r), q.(dinv q). */
+int cuphu_test_pcg_fused_step(const float *w, const float *dinv, int nrow, int ncol,
+                              double alpha, double beta, int start, const float *rt, int rpw,
+                              float *x, float *r, float *p, float *q, double *sums);
 
 #ifdef __cplusplus
 } /* extern "C" */

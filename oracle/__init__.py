@@ -211,6 +211,18 @@ def pcg_step_f32(w, dinv, gamma, beta, x, r, p, alpha=None):
     return x, r, p, q, pq.value, al.value, rz.value, st
 
 
+def pcg_fused_step_f32(w, dinv, alpha, beta, x, r, p, q, start=False, rt=None):
+    """One fused PCG iteration.  Returns (x, r, p, q, sums[4])."""
+    w, dinv = _f32(w), _f32(dinv)
+    x, r, p, q = (_f32(a).copy() for a in (x, r, p, q))
+    rta = None if rt is None else _f32(rt)
+    sums = np.zeros(4, np.float64)
+    _lib.o_pcg_fused_step_f32(_p(w), _p(dinv), C.c_int(x.shape[0]), C.c_int(x.shape[1]), C.c_double(alpha),
+                              C.c_double(beta), C.c_int(1 if start else 0), _p(rta), _p(x), _p(r), _p(p), _p(q),
+                              _p(sums))
+    return x, r, p, q, sums
+
+
 def conncomp(unw, corr, mask=None, minfrac=0.01, maxncomps=32) -> np.ndarray:
     unw, corr = _f32(unw), _f32(corr)
     m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
@@ -297,10 +309,10 @@ def _sig(fn, res, args):
 
 for _n in ("o_set_threads", "o_wrapped_phase", "o_phase_diffs", "o_boxcar2d", "o_costs", "o_residues",
            "o_arc_weights", "o_assemble", "o_matvec", "o_reweight", "o_congruence", "o_pcg_irls_f32",
-           "o_pcg_irls_f64", "o_pcg_step_f32", "o_conncomp", "o_integrate_phase", "o_tile_axis", "o_stitch",
+           "o_pcg_irls_f64", "o_pcg_step_f32", "o_pcg_fused_step_f32", "o_conncomp", "o_integrate_phase", "o_tile_axis", "o_stitch",
            "o_wls_solve_f64", "o_wls_adjoint_f64"):
     getattr(_lib, _n).restype = C.c_int if _n not in ("o_set_threads", "o_wrapped_phase", "o_phase_diffs",
                                                        "o_costs", "o_residues", "o_arc_weights",
                                                        "o_assemble", "o_matvec", "o_reweight",
-                                                       "o_congruence",
+                                                       "o_congruence", "o_pcg_fused_step_f32",
                                                        "o_integrate_phase") else None

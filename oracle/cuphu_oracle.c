@@ -564,6 +564,37 @@ int o_pcg_step_f32(const float *w, const float *dinv, int nrow, int ncol, double
     return 0;
 }
 
+/* ── one fused PCG iteration, FP32 vectors (the batched solver's one-kernel
+ * form of the loop body of o_pcg_irls_f32, cuphu_laplace.cu:452-498):
+ * with (alpha, beta) of the previous iteration,
+ *   x = fmaf(alpha, p, x); r = fmaf(-alpha, q, r)  (r = rt when rt != NULL);
+ *   z = r*dinv; p = fmaf(beta, p, z) (p = z when beta == 0); q = A p;
+ * start != 0 skips the x and r updates and sets p = z.  sums[0..3] =
+ * p.q, r.z, q.(dinv r), q.(dinv q) in FP64, sequential. */
+void o_pcg_fused_step_f32(const float *w, const float *dinv, int nrow, int ncol, double alpha, double beta,
+                          int start, const float *rt, float *x, float *r, float *p, float *q, double *sums) {
+    size_t n = (size_t)nrow * ncol;
+    float af = (float)alpha, bf = (float)beta;
+    for (size_t i = 0; i < n; ++i) {
+        if (!start) {
+            x[i] = fmaf(af, p[i], x[i]);
+            r[i] = rt ? rt[i] : fmaf(-af, q[i], r[i]);
+        }
+        float z = r[i] * dinv[i];
+        p[i] = (!start && beta != 0.0) ? fmaf(bf, p[i], z) : z;
+    }
+    o_matvec(w, p, nrow, ncol, q);
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    for (size_t i = 0; i < n; ++i) {
+        double qd = (double)q[i], dd = (double)dinv[i], rd = (double)r[i];
+        s0 += (double)p[i] * qd;
+        s1 += rd * (double)(r[i] * dinv[i]);
+        s2 += qd * dd * rd;
+        s3 += qd * dd * qd;
+    }
+    sums[0] = s0; sums[1] = s1; sums[2] = s2; sums[3] = s3;
+}
+
 /* ── a14: connected components (cuphu_conncomp_gpu cuphu_conncomp.cu:289-428)
  * Valid: !(corr < 0.05f) and mask != 0 (MaskLabelsKernel :102-114, floor at
  * :324-325).  Edge between 4-neighbours when both valid and

@@ -46,6 +46,9 @@ typedef struct OLapStats {
 int o_pcg_step_f32(const float *w, const float *dinv, int nrow, int ncol, double gamma, double beta,
                    double alpha_in, float *x, float *r, float *p, float *q, double *pq, double *alpha,
                    double *rz);
+/* one fused PCG iteration (the batched solver's one-kernel form), FP32 vectors */
+void o_pcg_fused_step_f32(const float *w, const float *dinv, int nrow, int ncol, double alpha, double beta,
+                          int start, const float *rt, float *x, float *r, float *p, float *q, double *sums);
 /* a14 */ int o_conncomp(const float *unw, const float *corr, const unsigned char *mask, int nrow,
                          int ncol, double minfrac, long maxncomps, uint32_t *labels);
 /* a13 */ void o_integrate_phase(const float *phi, const short *hflows, const short *vflows,

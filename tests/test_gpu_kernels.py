@@ -139,6 +139,34 @@ def test_pcg_step_kernels_bitexact(gpu_ready, shape, rpw, beta):
     assert rz.value == pytest.approx(ref[6], rel=1e-12)
 
 
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("rpw", [2, 4, 8])
+@pytest.mark.parametrize("mode", ["start", "step", "step_beta0", "replace"])
+def test_pcg_fused_step_kernel_bitexact(gpu_ready, shape, rpw, mode):
+    """Fused iteration kernel against the oracle: x, r, p, q bit-exact; the
+    four sums to FP64 summation-order tolerance."""
+    rng = np.random.default_rng(shape[0] + 5 * shape[1] + rpw)
+    nrow, ncol = shape
+    na = G.narcs(nrow, ncol)
+    w = rng.uniform(0.001, 0.3, na + 1).astype(np.float32)
+    if na:
+        w[::7] = 0
+    dinv = rng.uniform(0.5, 50, shape).astype(np.float32)
+    x, r, p, q, rt = (rng.standard_normal(shape).astype(np.float32) for _ in range(5))
+    alpha, beta = 0.37, (0.0 if mode == "step_beta0" else 0.81)
+    start = mode == "start"
+    rta = rt if mode == "replace" else None
+    ref = O.pcg_fused_step_f32(w[:na], dinv, alpha, beta, x, r, p, q, start=start, rt=rta)
+    got = [v.copy() for v in (x, r, p, q)]
+    sums = np.zeros(4, np.float64)
+    _capi.check(_capi.lib.cuphu_test_pcg_fused_step(w.ctypes.data, dinv.ctypes.data, nrow, ncol, alpha, beta,
+                                                    int(start), None if rta is None else rta.ctypes.data, rpw,
+                                                    *[v.ctypes.data for v in got], sums.ctypes.data), "fused_step")
+    for name, a, e in zip("xrpq", got, ref[:4]):
+        np.testing.assert_array_equal(a, e, err_msg=name)
+    np.testing.assert_allclose(sums, ref[4], rtol=1e-12, atol=1e-300)
+
+
 @pytest.mark.parametrize("shape", [(2, 2), (40, 70), (257, 300)])
 def test_wrap_diffs_boxcar_residues(gpu_ready, shape):
     torch = G.torch_mod()

@@ -1,12 +1,12 @@
 """Solver-level parity (GPU): PCG/IRLS against the oracle, the reference's
 unwrap() semantics with init='laplace', batching and tiling properties.
 
-Tolerances: the plain solver runs the oracle's FP32 operation sequence and
-differs only in FP64 reduction order; the batched solver is the
-single-reduction recurrence of the same Krylov method.  Both are compared
-to the FP64 oracle by the 2-pi ambiguity count k = rint((u - phi)/2pi):
->= 99.9 % of pixels must agree, and the FP64 modes must agree with the FP64
-oracle to 1e-6 rad on converged solves.
+Tolerances: the plain and batched solvers run the oracle's FP32 operation
+sequence and differ only in FP64 reduction order; the fused solver takes
+beta from the expansion of r.z, equal to the oracle's up to FP rounding.
+All are compared to the FP64 oracle by the 2-pi ambiguity count
+k = rint((u - phi)/2pi): >= 99.9 % of pixels must agree, and the FP64 modes
+must agree with the FP64 oracle to 1e-5 rad on converged solves.
 """
 
 import ctypes as C
@@ -52,7 +52,7 @@ def test_plain_solver_matches_fp32_oracle(gpu_ready, k):
     assert np.abs(u[0] - uo).max() < 1e-2
 
 
-@pytest.mark.parametrize("solver", ["plain", "batched"])
+@pytest.mark.parametrize("solver", ["plain", "batched", "fused"])
 def test_fp64_modes_match_fp64_oracle(gpu_ready, solver):
     s = SMALL[0]
     opts = dict(n_irls=2, max_iter=4000, tol=1e-12)
@@ -63,38 +63,41 @@ def test_fp64_modes_match_fp64_oracle(gpu_ready, solver):
     np.testing.assert_allclose(u[0], uo, atol=1e-5)
 
 
+@pytest.mark.parametrize("solver", ["batched", "fused"])
 @pytest.mark.parametrize("k", [0, 1])
-def test_batched_fp32_agrees_with_fp64_oracle(gpu_ready, k):
+def test_batched_fp32_agrees_with_fp64_oracle(gpu_ready, k, solver):
     s = SMALL[k]
     opts = dict(n_irls=5, max_iter=1000, tol=1e-6)
-    u, cc, st = G.batch_solve([s], _capi.default_options(solver="batched", **opts))
+    u, cc, st = G.batch_solve([s], _capi.default_options(solver=solver, **opts))
     uo, sto, ph = _oracle_solve(s, f64=True, **opts)
     agree = (amb(u[0], ph) == amb(uo, ph)).mean()
     print("batched vs oracle64", agree, st[0]["iters"], sto["iters"], np.abs(u[0] - uo).max())
     assert agree >= 0.999
 
 
-def test_batch_equals_solo_bitwise(gpu_ready):
+@pytest.mark.parametrize("solver", ["batched", "fused"])
+def test_batch_equals_solo_bitwise(gpu_ready, solver):
     scenes = [synth.make_scene(40 + 13 * i, 50 + 29 * i, nbumps=2, npatches=0, nlooks=[1, 5, 10, 20][i % 4],
                                seed=100 + i) for i in range(5)]
-    o = _capi.default_options(n_irls=3, max_iter=200, tol=1e-5)
+    o = _capi.default_options(n_irls=3, max_iter=200, tol=1e-5, solver=solver)
     ub, cb, sb = G.batch_solve(scenes, o)
     for i, s in enumerate(scenes):
         us, cs, ss = G.batch_solve([s], o)
         np.testing.assert_array_equal(ub[i], us[0])
         np.testing.assert_array_equal(cb[i], cs[0])
         assert sb[i]["iters"] == ss[0]["iters"]
-    og = _capi.default_options(n_irls=3, max_iter=200, tol=1e-5, group_bytes_mb=1)
+    og = _capi.default_options(n_irls=3, max_iter=200, tol=1e-5, group_bytes_mb=1, solver=solver)
     ug, cg, sg = G.batch_solve(scenes, og)
     for i in range(len(scenes)):
         np.testing.assert_array_equal(ug[i], ub[i])
 
 
-def test_graph_and_eager_launches_identical(gpu_ready):
+@pytest.mark.parametrize("solver", ["batched", "fused"])
+def test_graph_and_eager_launches_identical(gpu_ready, solver):
     s = SMALL[1]
-    u1, c1, s1 = G.batch_solve([s], _capi.default_options(n_irls=2, max_iter=150, use_graph=1))
-    u2, c2, s2 = G.batch_solve([s], _capi.default_options(n_irls=2, max_iter=150, use_graph=0))
-    u3, c3, s3 = G.batch_solve([s], _capi.default_options(n_irls=2, max_iter=150, use_graph=1))
+    u1, c1, s1 = G.batch_solve([s], _capi.default_options(n_irls=2, max_iter=150, use_graph=1, solver=solver))
+    u2, c2, s2 = G.batch_solve([s], _capi.default_options(n_irls=2, max_iter=150, use_graph=0, solver=solver))
+    u3, c3, s3 = G.batch_solve([s], _capi.default_options(n_irls=2, max_iter=150, use_graph=1, solver=solver))
     np.testing.assert_array_equal(u1[0], u2[0])
     np.testing.assert_array_equal(u1[0], u3[0])
     assert s1[0]["iters"] == s2[0]["iters"] == s3[0]["iters"]

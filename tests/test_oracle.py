@@ -370,6 +370,54 @@ def test_pcg_step_reproduces_dense_pcg_iterates():
     assert gam == pytest.approx(float((r.astype(np.float64) * (r * dinv)).sum()), rel=1e-12)
 
 
+def test_pcg_fused_step_reproduces_dense_pcg_iterates():
+    """o_pcg_fused_step_f32 driven with the fused solver's scalar recurrences
+    (alpha = rz / p.q, beta from the rz expansion) against textbook
+    Jacobi-PCG in dense float64; the expansion must match the r.z of the
+    vectors actually formed to FP32 rounding, and q must be A p exactly."""
+    rng = np.random.default_rng(19)
+    nrow, ncol = 11, 13
+    na = O.narcs(nrow, ncol)
+    w = rng.uniform(0.02, 0.1, na).astype(np.float32)
+    phi = rng.uniform(0, TWOPI, (nrow, ncol)).astype(np.float32)
+    b, diag = O.assemble(w, phi)
+    dinv = (1.0 / diag).astype(np.float32)
+    z0 = np.zeros_like(b)
+    x, r, p, q, sums = O.pcg_fused_step_f32(w, dinv, 0.0, 0.0, z0, b, z0, z0, start=True)
+    np.testing.assert_array_equal(r, b)
+    np.testing.assert_array_equal(q, O.matvec(w, p))
+    for _ in range(6):
+        pq, rz, qr, qq = sums
+        alpha = rz / pq
+        rz_next = rz - 2 * alpha * qr + alpha * alpha * qq
+        x, r, p, q, sums = O.pcg_fused_step_f32(w, dinv, alpha, rz_next / rz, x, r, p, q)
+        np.testing.assert_array_equal(q, O.matvec(w, p))
+        assert sums[1] == pytest.approx(rz_next, rel=1e-5)
+    L = dense_laplacian(w.astype(np.float64), nrow, ncol)
+    bb = b.ravel().astype(np.float64)
+    D = dinv.ravel().astype(np.float64)
+    xx = np.zeros_like(bb)
+    rr = bb.copy()
+    zz = rr * D
+    pp = zz.copy()
+    rz = rr @ zz
+    for _ in range(6):
+        qv = L @ pp
+        a = rz / (pp @ qv)
+        xx += a * pp
+        rr -= a * qv
+        zz = rr * D
+        rzn = rr @ zz
+        pp = zz + (rzn / rz) * pp
+        rz = rzn
+    np.testing.assert_allclose(x.ravel(), xx, rtol=2e-3, atol=2e-4 * np.abs(xx).max())
+    # a replacement step takes r from rt and leaves everything else as the update
+    rt = (r + 0.25).astype(np.float32)
+    x2, r2, p2, q2, _ = O.pcg_fused_step_f32(w, dinv, 0.5, 0.0, x, r, p, q, rt=rt)
+    np.testing.assert_array_equal(r2, rt)
+    np.testing.assert_array_equal(p2, rt * dinv)
+
+
 def test_pcg_step_reports_breakdown():
     nrow, ncol = 3, 4
     w = np.zeros(O.narcs(nrow, ncol), np.float32)
