@@ -11,7 +11,7 @@ are resident in HBM before the timed region; a step is one full unwrap.
 
 Reported beside the headline Mpixel/s:
 * roofline: the fused PCG iteration kernel k_it (x, r, p, q updated and
-  q = L p formed in one pass; 44 B per active pixel) -- algorithmic bytes
+  q = L p formed in one pass; 40 B per active pixel) -- algorithmic bytes
   over its duration, timed live with HIP events on the solver's stream
   during the timed steps (with --solver batched: the direction+stencil
   kernel k_pq, 28 B/px, with the update kernel k_xr beside it);
@@ -42,7 +42,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Mpixel/s unwrapped (Laplace PCG/IRLS) + stencil HBM GB/s; batched vs plain"
 HBM_PEAK_GBPS = 8000.0
-BYTES_PER_PX_IT = 44.0
+BYTES_PER_PX_IT = 40.0
 BYTES_PER_PX_PQ = 28.0
 BYTES_PER_PX_XR = 28.0
 # HBM bytes per active pixel actually moved per launch at this workload:
@@ -51,7 +51,7 @@ BYTES_PER_PX_XR = 28.0
 # (profiles/k_it_scene8192_hbm_traffic.md, profiles/k_pq_k_xr_scene8192_hbm_traffic.md).
 # Re-measure after kernel changes.
 TRAFFIC_PER_PX_PQ = 2770.6e6 / 84.93e6
-TRAFFIC_PER_PX_IT = 47.28
+TRAFFIC_PER_PX_IT = None   # re-measured after the 1/diag recompute
 
 
 def log(msg):

@@ -703,7 +703,9 @@ __global__ __launch_bounds__(256) void k_rr_commit(const InstDesc *descs, const 
 // use the sums of the vectors actually formed, as in the reference.
 //
 // Vector operations are the reference's, in the oracle's FP32 order: x =
-// fma(a, p, x); r = fma(-a, q, r); z = r * dinv; p = fma(b, p, z).
+// fma(a, p, x); r = fma(-a, q, r); z = r * dinv; p = fma(b, p, z).  dinv of
+// the own pixels is recomputed from their arcs (bit-identical to the stored
+// plane); halo pixels read the plane.
 // r, p and q are read at halo pixels, so they alternate between two plane
 // sets (parity 0: (r, p, s) -> (r2, p2, q2); parity 1: the reverse); x is
 // only touched at its own pixel.  start = 1 forms p_0 = z_0 from r in set 0
@@ -776,10 +778,11 @@ __global__ __launch_bounds__(256) void k_it(const InstDesc *descs, const int *bl
         T z = rn * di;
         pn = keep ? Prec<T>::fmaT(be, p, z) : z;
     };
-    T x4[RPW][4], r4[RPW][4], p4[RPW][4], q4[RPW][4], di[RPW][4], wd[RPW][4], wr[RPW][4], wu[4], wlo[RPW];
+    T x4[RPW][4], r4[RPW][4], p4[RPW][4], q4[RPW][4], di[RPW][4], wd[RPW][4], wr[RPW][4], wl[RPW][4], wu[4],
+        wlo[RPW];
 #pragma unroll
     for (int k = 0; k < RPW; ++k) {
-        zero4(x4[k]); zero4(r4[k]); zero4(p4[k]); zero4(q4[k]); zero4(di[k]); zero4(wd[k]); zero4(wr[k]);
+        zero4(x4[k]); zero4(r4[k]); zero4(p4[k]); zero4(q4[k]); zero4(wd[k]); zero4(wr[k]);
         wlo[k] = T(0);
         const int row = row0 + k;
         if (row < d.nrow) {
@@ -789,7 +792,6 @@ __global__ __launch_bounds__(256) void k_it(const InstDesc *descs, const int *bl
                 ld4(r_in + e, r4[k]);
                 if (need_p) ld4(p_in + e, p4[k]);
                 if (need_q) ld4(q_in + e, q4[k]);
-                ld4(dg + e, di[k]);
                 ld_arcs(w2, e, wd[k], wr[k]);
             }
             if (lane == 0 && t.tx0 > 0) wlo[k] = w2[2 * (e - 1) + 1];
@@ -821,6 +823,23 @@ __global__ __launch_bounds__(256) void k_it(const InstDesc *descs, const int *bl
             }
             lp[lr * kLdsStride + lc] = v;
         });
+    // Own pixels' Jacobi inverse from their four arcs, summed in the
+    // assemble kernel's order (down, up, right, left; a zero arc adds
+    // nothing), so it equals the stored 1/diag bit for bit without reading
+    // it.  Left arcs: the right arc of the previous column (neighbouring
+    // lane, or the load for lane 0).
+#pragma unroll
+    for (int k = 0; k < RPW; ++k) {
+        T from_left = __shfl_up(wr[k][3], 1, 64);
+        wl[k][0] = (lane == 0) ? wlo[k] : from_left;
+        wl[k][1] = wr[k][0]; wl[k][2] = wr[k][1]; wl[k][3] = wr[k][2];
+        const T *wuk = (k == 0) ? wu : wd[k > 0 ? k - 1 : 0];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            T ws = ((wd[k][j] + wuk[j]) + wr[k][j]) + wl[k][j];
+            di[k][j] = T(1) / (ws + Prec<T>::eps);
+        }
+    }
     // own pixels: updates, stores, LDS
 #pragma unroll
     for (int k = 0; k < RPW; ++k) {
@@ -848,11 +867,8 @@ __global__ __launch_bounds__(256) void k_it(const InstDesc *descs, const int *bl
     for (int k = 0; k < RPW; ++k) {
         const int lr = wave * RPW + k, row = row0 + k;
         if (row >= d.nrow) break;
-        T wl[4], y[4];
-        T from_left = __shfl_up(wr[k][3], 1, 64);
-        wl[0] = (lane == 0) ? wlo[k] : from_left;
-        wl[1] = wr[k][0]; wl[2] = wr[k][1]; wl[3] = wr[k][2];
-        stencil4(lp, lr, lane, wd[k], (k == 0) ? wu : wd[k > 0 ? k - 1 : 0], wr[k], wl, y);
+        T y[4];
+        stencil4(lp, lr, lane, wd[k], (k == 0) ? wu : wd[k > 0 ? k - 1 : 0], wr[k], wl[k], y);
         if (cok) {
             st4(S.q_out + d.off + (long long)row * d.pitch + c, y);
             const T *pc = lp + (lr + 1) * kLdsStride + kLdsC0 + 4 * lane;

@@ -352,8 +352,9 @@ static void solve_group_batched(const Planes<T> &pl, const GroupDev &g, const Cu
 }
 
 // Algorithmic HBM bytes per active pixel of the fused iteration kernel:
-// reads x, r, p, q, 1/diag and the {down, right} arc pair, writes x, r, p, q.
-constexpr double kItValuesPerPx = 11.0;
+// reads x, r, p, q and the {down, right} arc pair (1/diag is recomputed from
+// the arcs), writes x, r, p, q.
+constexpr double kItValuesPerPx = 10.0;
 
 // Fused solver: one stencil kernel and one reduction per PCG iteration.
 template <class T>

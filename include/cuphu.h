@@ -433,8 +433,9 @@ int cuphu_wls_adjoint_gpu(const void *d_w, const void *d_phase, const void *d_u,
  *       56 B in FP64);
  *   xr: (BATCHED) x += alpha p, r -= alpha q, r.z -- reads x, p, r, q,
  *       1/diag, writes x and r (28 B / 56 B per active pixel);
- *   it: (FUSED) one whole iteration -- reads x, r, p, q, 1/diag and two arc
- *       weights, writes x, r, p, q (44 B / 88 B per active pixel).
+ *   it: (FUSED) one whole iteration -- reads x, r, p, q and two arc
+ *       weights (1/diag is recomputed from the arcs), writes x, r, p, q
+ *       (40 B / 80 B per active pixel).
  * cuphu_set_profiling resets the counters. */
 typedef struct CuPhuKernelProfile {
     long long samples;        /* timed iterations                          */

@@ -151,7 +151,10 @@ def test_pcg_fused_step_kernel_bitexact(gpu_ready, shape, rpw, mode):
     w = rng.uniform(0.001, 0.3, na + 1).astype(np.float32)
     if na:
         w[::7] = 0
-    dinv = rng.uniform(0.5, 50, shape).astype(np.float32)
+    # the kernel recomputes its own pixels' 1/diag from the arcs, so dinv
+    # must be the assemble kernel's (the oracle's) for these weights
+    _, diag = O.assemble(w[:na], np.zeros(shape, np.float32))
+    dinv = (np.float32(1) / diag).astype(np.float32)
     x, r, p, q, rt = (rng.standard_normal(shape).astype(np.float32) for _ in range(5))
     alpha, beta = 0.37, (0.0 if mode == "step_beta0" else 0.81)
     start = mode == "start"
