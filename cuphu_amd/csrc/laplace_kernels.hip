@@ -44,6 +44,30 @@ __device__ __forceinline__ void st4(double *p, const double (&v)[4]) {
     *reinterpret_cast<double2 *>(p + 2) = make_double2(v[2], v[3]);
 }
 template <class T> __device__ __forceinline__ void zero4(T (&v)[4]) { v[0] = v[1] = v[2] = v[3] = T(0); }
+// Streaming (non-temporal) loads and stores: the fused PCG kernel's outputs
+// are read again only by the next launch, and x only by its own pixel, so
+// they bypass the caches the halo reads use.  Measured on scene8192:
+// 0.663 -> 0.598 ms per launch.
+typedef float nt_f4 __attribute__((ext_vector_type(4)));
+typedef double nt_d2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void st4_nt(float *p, const float (&v)[4]) {
+    nt_f4 t = {v[0], v[1], v[2], v[3]};
+    __builtin_nontemporal_store(t, reinterpret_cast<nt_f4 *>(p));
+}
+__device__ __forceinline__ void ld4_nt(const float *p, float (&o)[4]) {
+    nt_f4 t = __builtin_nontemporal_load(reinterpret_cast<const nt_f4 *>(p));
+    o[0] = t.x; o[1] = t.y; o[2] = t.z; o[3] = t.w;
+}
+__device__ __forceinline__ void ld4_nt(const double *p, double (&o)[4]) {
+    nt_d2 a = __builtin_nontemporal_load(reinterpret_cast<const nt_d2 *>(p));
+    nt_d2 b = __builtin_nontemporal_load(reinterpret_cast<const nt_d2 *>(p + 2));
+    o[0] = a.x; o[1] = a.y; o[2] = b.x; o[3] = b.y;
+}
+__device__ __forceinline__ void st4_nt(double *p, const double (&v)[4]) {
+    nt_d2 a = {v[0], v[1]}, b = {v[2], v[3]};
+    __builtin_nontemporal_store(a, reinterpret_cast<nt_d2 *>(p));
+    __builtin_nontemporal_store(b, reinterpret_cast<nt_d2 *>(p + 2));
+}
 
 // Arcs of 4 pixels starting at element e: w2 holds {down, right} pairs.
 __device__ __forceinline__ void ld_arcs(const float *w2, long long e, float (&wd)[4], float (&wr)[4]) {
@@ -788,7 +812,9 @@ __global__ __launch_bounds__(256) void k_it(const InstDesc *descs, const int *bl
         if (row < d.nrow) {
             long long e = d.off + (long long)row * d.pitch + c;
             if (cok) {
-                if (upd) ld4(xg + e, x4[k]);
+                // x is touched by this pixel only: stream it past the caches;
+                // r, p and q stay cached for the neighbours' halo reads
+                if (upd) ld4_nt(xg + e, x4[k]);
                 ld4(r_in + e, r4[k]);
                 if (need_p) ld4(p_in + e, p4[k]);
                 if (need_q) ld4(q_in + e, q4[k]);
@@ -853,9 +879,9 @@ __global__ __launch_bounds__(256) void k_it(const InstDesc *descs, const int *bl
         }
         if (row < d.nrow && cok) {
             long long e = d.off + (long long)row * d.pitch + c;
-            if (upd) st4(xg + e, x4[k]);
-            st4(S.r_out + e, rn);
-            st4(S.p_out + e, pn);
+            if (upd) st4_nt(xg + e, x4[k]);
+            st4_nt(S.r_out + e, rn);
+            st4_nt(S.p_out + e, pn);
         }
         T *lq = lp + (lr + 1) * kLdsStride + kLdsC0 + 4 * lane;
 #pragma unroll
@@ -870,7 +896,7 @@ __global__ __launch_bounds__(256) void k_it(const InstDesc *descs, const int *bl
         T y[4];
         stencil4(lp, lr, lane, wd[k], (k == 0) ? wu : wd[k > 0 ? k - 1 : 0], wr[k], wl[k], y);
         if (cok) {
-            st4(S.q_out + d.off + (long long)row * d.pitch + c, y);
+            st4_nt(S.q_out + d.off + (long long)row * d.pitch + c, y);
             const T *pc = lp + (lr + 1) * kLdsStride + kLdsC0 + 4 * lane;
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
