@@ -51,7 +51,7 @@ BYTES_PER_PX_XR = 28.0
 # (profiles/k_it_scene8192_hbm_traffic.md, profiles/k_pq_k_xr_scene8192_hbm_traffic.md).
 # Re-measure after kernel changes.
 TRAFFIC_PER_PX_PQ = 2770.6e6 / 84.93e6
-TRAFFIC_PER_PX_IT = None   # re-measured after the 1/diag recompute
+TRAFFIC_PER_PX_IT = 41.73
 
 
 def log(msg):
