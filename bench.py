@@ -10,8 +10,9 @@ tiles in one batched launch sequence, tile stitching and conncomp.  Inputs
 are resident in HBM before the timed region; a step is one full unwrap.
 
 Reported beside the headline Mpixel/s:
-* roofline: the fused PCG iteration kernel k_it (x, r, p, q updated and
-  q = L p formed in one pass; 40 B per active pixel) -- algorithmic bytes
+* roofline: the fused PCG iteration kernel k_iq (x, r, p updated, q = L p
+  recomputed from p and the next q = L p formed in one pass; 32 B per active
+  pixel) -- algorithmic bytes
   over its duration, timed live with HIP events on the solver's stream
   during the timed steps (with --solver batched: the direction+stencil
   kernel k_pq, 28 B/px, with the update kernel k_xr beside it);
@@ -42,7 +43,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Mpixel/s unwrapped (Laplace PCG/IRLS) + stencil HBM GB/s; batched vs plain"
 HBM_PEAK_GBPS = 8000.0
-BYTES_PER_PX_IT = 40.0
+BYTES_PER_PX_IT = 32.0
 BYTES_PER_PX_PQ = 28.0
 BYTES_PER_PX_XR = 28.0
 # HBM bytes per active pixel actually moved per launch at this workload:
@@ -51,7 +52,7 @@ BYTES_PER_PX_XR = 28.0
 # (profiles/k_it_scene8192_hbm_traffic.md, profiles/k_pq_k_xr_scene8192_hbm_traffic.md).
 # Re-measure after kernel changes.
 TRAFFIC_PER_PX_PQ = 2770.6e6 / 84.93e6
-TRAFFIC_PER_PX_IT = 41.73
+TRAFFIC_PER_PX_IT = None   # re-measure for the q-free kernel
 
 
 def log(msg):
@@ -75,6 +76,9 @@ def main() -> int:
     ap.add_argument("--n-irls", type=int, default=5)
     ap.add_argument("--calib-copy", action="store_true")
     ap.add_argument("--solver", default="fused", choices=["fused", "batched"])
+    ap.add_argument("--no-extra", action="store_true", help="skip the 2048^2 and 64-interferogram legs")
+    ap.add_argument("--plain-sample", type=int, default=8,
+                    help="instances of the 64-interferogram batch timed with the plain loop")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (tiles would be sharded across "
@@ -157,7 +161,7 @@ def main() -> int:
     ns = max(prof.samples, 1)
     gbs = lambda b, ms: (b / 1e9) / (ms / 1e3) if ms > 0 else None
     if args.solver == "fused":
-        kname, bpp, tpp = "k_it (fused PCG iteration: x, r, p, q updates + weighted 5-point stencil + 4 sums)", \
+        kname, bpp, tpp = "k_iq (fused PCG iteration: x, r, p updates + two weighted 5-point stencils + 4 sums)", \
             BYTES_PER_PX_IT, TRAFFIC_PER_PX_IT
         k_ms, k_bytes = prof.it_ms_sum / ns, prof.it_bytes_sum / ns
         achieved = gbs(prof.it_bytes_sum, prof.it_ms_sum)
@@ -274,8 +278,109 @@ def main() -> int:
         }
         log(f"cpu: {n * n / tc / 1e6:.4f} Mpx/s")
 
+    if not args.no_extra:
+        out["config2_scene2048"] = leg_single(torch, lib, _capi, synth, base_opts, log)
+        out["config4_batch64"] = leg_batch64(torch, lib, _capi, synth, base_opts, args.plain_sample, log)
+
     print(json.dumps(out), flush=True)
     return 0
+
+
+def _upload(torch, scenes):
+    keep, insts = [], []
+    for s in scenes:
+        nr, nc = s.corr.shape
+        dig = torch.from_numpy(s.igram.view(np.float32).reshape(nr, 2 * nc)).to("cuda")
+        dc = torch.from_numpy(s.corr).to("cuda")
+        du = torch.empty((nr, nc), dtype=torch.float32, device="cuda")
+        dl = torch.empty((nr, nc), dtype=torch.int32, device="cuda")
+        keep += [dig, dc, du, dl]
+        insts.append((nr, nc, dig, dc, du, dl, s.nlooks))
+    return keep, insts
+
+
+def _solve(torch, lib, _capi, insts, opts):
+    import ctypes as C
+
+    n = len(insts)
+    arr = (_capi.CuPhuInstance * n)()
+    for i, (nr, nc, dig, dc, du, dl, nl) in enumerate(insts):
+        arr[i] = _capi.CuPhuInstance(nr, nc, dig.data_ptr(), None, dc.data_ptr(), None, None, float(nl),
+                                     du.data_ptr(), dl.data_ptr())
+    prm = _capi.default_params()
+    stats = (_capi.CuPhuSolveStats * n)()
+    stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _capi.check(lib.cuphu_laplace_batch_gpu(arr, n, _capi.COST["defo"], C.byref(prm), C.byref(opts), stats, stream),
+                "cuphu_laplace_batch_gpu")
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, [stats[i].as_dict() for i in range(n)]
+
+
+def _amb(torch, a, b, igram):
+    phi = np.mod(np.angle(igram.astype(np.complex128)), 2 * np.pi)
+    ka = np.rint((a.cpu().numpy() - phi) / (2 * np.pi))
+    kb = np.rint((b.cpu().numpy() - phi) / (2 * np.pi))
+    return float((ka == kb).mean())
+
+
+def leg_single(torch, lib, _capi, synth, base_opts, log):
+    """2048x2048 interferogram (ramp + 5 bumps + 1 patch, nlooks 10, defo,
+    5 IRLS re-solves, tol 1e-6): one instance, fused vs the plain loop."""
+    s = synth.config_scene("scene2048")
+    npx = s.corr.size
+    keep, insts = _upload(torch, [s])
+    opts = _capi.default_options(**base_opts)
+    _solve(torch, lib, _capi, insts, opts)
+    t, st = _solve(torch, lib, _capi, insts, opts)
+    u_f = insts[0][4].clone()
+    _, insts_p = _upload(torch, [s])
+    tp, stp = _solve(torch, lib, _capi, insts_p, _capi.default_options(**{**base_opts, "solver": "plain"}))
+    r = {"workload": "2048x2048 single interferogram, defo, nlooks 10, 5 IRLS re-solves, tol 1e-6, max_iter 1000",
+         "mpix_s": round(npx / t / 1e6, 3), "seconds": round(t, 3),
+         "plain_mpix_s": round(npx / tp / 1e6, 3), "plain_seconds": round(tp, 3),
+         "fused_vs_plain": round(tp / t, 3), "ambiguity_agreement_with_plain": _amb(torch, u_f, insts_p[0][4], s.igram),
+         "pcg_iters": st[0]["iters"]}
+    log(f"config2: fused {r['mpix_s']} Mpx/s, plain {r['plain_mpix_s']} Mpx/s")
+    del keep
+    return r
+
+
+def leg_batch64(torch, lib, _capi, synth, base_opts, nsample, log):
+    """64 uneven interferograms (nrow U{512..3000}, ncol U{512..1800}, nlooks
+    U{1,5,10,20}) in one launch sequence vs the plain per-instance loop (timed
+    on the first nsample instances)."""
+    t0 = time.perf_counter()
+    scenes = synth.batch_instances(64)
+    log(f"config4: generated 64 instances in {time.perf_counter() - t0:.1f}s")
+    npx = sum(s.corr.size for s in scenes)
+    keep, insts = _upload(torch, scenes)
+    opts = _capi.default_options(**base_opts)
+    t, st = _solve(torch, lib, _capi, insts, opts)
+    sample = scenes[:nsample]
+    spx = sum(s.corr.size for s in sample)
+    _, ins_b = _upload(torch, sample)
+    tb, _ = _solve(torch, lib, _capi, ins_b, opts)
+    _, ins_p = _upload(torch, sample)
+    tp, _ = _solve(torch, lib, _capi, ins_p, _capi.default_options(**{**base_opts, "solver": "plain"}))
+    _, ins_64 = _upload(torch, sample)
+    _solve(torch, lib, _capi, ins_64, _capi.default_options(**{**base_opts, "precision": 64}))
+    agree = float(np.mean([_amb(torch, insts[i][4], ins_p[i][4], sample[i].igram) for i in range(len(sample))]))
+    ag64 = float(np.mean([_amb(torch, insts[i][4], ins_64[i][4], sample[i].igram) for i in range(len(sample))]))
+    ag64p = float(np.mean([_amb(torch, ins_p[i][4], ins_64[i][4], sample[i].igram) for i in range(len(sample))]))
+    r = {"workload": "64 interferograms, nrow U{512..3000} x ncol U{512..1800}, nlooks U{1,5,10,20}, defo, "
+                     "5 IRLS re-solves, tol 1e-6, max_iter 1000, one launch sequence",
+         "mpix": round(npx / 1e6, 2), "mpix_s": round(npx / t / 1e6, 3), "seconds": round(t, 2),
+         "plain_sample": nsample, "plain_sample_mpix_s": round(spx / tp / 1e6, 3),
+         "batched_sample_mpix_s": round(spx / tb / 1e6, 3),
+         "batched_vs_plain": round((npx / t) / (spx / tp), 3),
+         "ambiguity_agreement_with_plain": agree,
+         "sample_ambiguity_agreement_with_fp64": {"batched": ag64, "plain": ag64p},
+         "pcg_iters_total": int(sum(sum(x["iters"]) for x in st))}
+    log(f"config4: batched {r['mpix_s']} Mpx/s, plain sample {r['plain_sample_mpix_s']} Mpx/s")
+    del keep
+    return r
 
 
 if __name__ == "__main__":

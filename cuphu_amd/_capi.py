@@ -150,7 +150,7 @@ _SIGS = {
     "cuphu_test_laplace_ops": (_i, [_P, _P, _P, _i, _i, C.c_float, _P, _P, _P, _P, _P, _P, _P]),
     "cuphu_test_pcg_step": (_i, [_P, _P, _i, _i, _d, _d, _i, _P, _P, _P, _P, C.POINTER(_d), C.POINTER(_d),
                                  C.POINTER(_d)]),
-    "cuphu_test_pcg_fused_step": (_i, [_P, _P, _i, _i, _d, _d, _i, _P, _i, _P, _P, _P, _P, _P]),
+    "cuphu_test_pcg_fused_step": (_i, [_P, _P, _i, _i, _d, _d, _i, _P, _i, _i, _P, _P, _P, _P, _P]),
 }
 
 EXPORTED = tuple(_SIGS)

@@ -100,6 +100,14 @@ void launch_rr_it(const InstDesc *d_desc, const int *d_blk, int nblk, const Plan
                   double4 *partials, int rpw, hipStream_t st);
 void launch_reduce_it(const InstDesc *d_desc, int ninst, const double4 *partials, SolveScal *scal, int mode,
                       int max_iter, double tol2, hipStream_t st);
+// The same iteration without a stored q (q_j = A p_j recomputed from p_j):
+// plane sets (r, p) and (r2, p2); and its true-residual kernel.
+template <class T>
+void launch_iq(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, const SolveScal *scal,
+               double4 *partials, int rpw, int parity, int start, hipStream_t st);
+template <class T>
+void launch_rr_iq(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, const SolveScal *scal,
+                  double4 *partials, int rpw, hipStream_t st);
 template <class T>
 void launch_final(const InstDesc *d_desc, const int *d_rowinst, int nrows_total, const Planes<T> &pl,
                   float *out_pool, int max_pitch, hipStream_t st);

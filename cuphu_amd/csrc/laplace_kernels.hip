@@ -987,6 +987,341 @@ __global__ __launch_bounds__(256) void k_rr_it(const InstDesc *descs, const int 
     block_sum4(s_t, s_r, 0.0, 0.0, partials + t.blk);
 }
 
+// ═══════════════════════════════════════════════════════════════════════════
+// Fused PCG iteration without a stored q (the default fused kernel).
+//
+// Same iteration as k_it, but q_j = A p_j is recomputed from p_j instead of
+// being written by the previous launch and read back: the stencil of the
+// previous launch and this recomputation see the same p_j and arcs and use
+// the same operation order, so q_j is bit-identical.  Per pixel this moves
+// 32 B (reads x, r, p and the arc pair; writes x, r, p) instead of 40.
+//
+// LDS: lp0 holds p_j with a two-pixel halo (rows -2..TH+1, columns -2..257),
+// lp1 holds p_{j+1} with a one-pixel halo.  Phase 1 loads the own rows and
+// stages p_j; wave 0 loads the ring row above the tile (and p_j two rows
+// up), wave 3 the ring row below, waves 1 and 2 the ring columns left and
+// right.  Phase 2 forms q_j, r_{j+1}, z, p_{j+1} at the own pixels and on
+// the ring (whose arcs come from the loads of phase 1, the ring's 1/diag
+// recomputed from its arcs like the own pixels') and stores x, r, p.
+// Phase 3 applies the stencil to p_{j+1} for the sums.  Plane sets: (r, p)
+// and (r2, p2); start and replacement modes as in k_it.
+// ═══════════════════════════════════════════════════════════════════════════
+template <class T>
+__device__ __forceinline__ T stencil1(T a, T dn, T up, T rt, T lf, T wd, T wu, T wr, T wl) {
+    T y = Prec<T>::eps * a;
+    y = Prec<T>::fmaT(wd, a - dn, y);
+    y = Prec<T>::fmaT(wu, a - up, y);
+    y = Prec<T>::fmaT(wr, a - rt, y);
+    y = Prec<T>::fmaT(wl, a - lf, y);
+    return y;
+}
+template <class T>
+__device__ __forceinline__ T diag_inv(T wd, T wu, T wr, T wl) {
+    return T(1) / ((((wd + wu) + wr) + wl) + Prec<T>::eps);
+}
+
+template <class T, int RPW>
+__global__ __launch_bounds__(256) void k_iq(const InstDesc *descs, const int *blk_inst, Planes<T> pl,
+                                            const SolveScal *scal, double4 *partials, int parity, int start) {
+    constexpr int TH = 4 * RPW;
+    constexpr int S = kLdsStride;
+    __shared__ T lp0[(TH + 4) * S];   // p_j: LDS row = row + 2, col = col + kLdsC0
+    __shared__ T lp1[(TH + 2) * S];   // p_{j+1}: LDS row = row + 1
+    TileInfo t = get_tile(descs, blk_inst, TH);
+    const SolveScal *sc = scal + t.inst;
+    if (!start && !sc->active) return;
+    const bool upd = !start;                          // x += a p
+    const bool rep = upd && sc->replace != 0;         // r from the true-residual plane
+    const bool useq = upd && !rep;                    // r -= a q_j, q_j = A p_j
+    const bool keep = upd && sc->beta != 0.0;         // p = z + b p (else p = z)
+    const T al = upd ? (T)sc->alpha : T(0);
+    const T be = (T)sc->beta;
+    const T *__restrict__ r_src = rep ? pl.wv : (parity ? pl.r2 : pl.r);
+    const T *__restrict__ p_in = parity ? pl.p2 : pl.p;
+    T *__restrict__ r_out = parity ? pl.r : pl.r2;
+    T *__restrict__ p_out = parity ? pl.p : pl.p2;
+    const T *__restrict__ w2 = pl.w2;
+    T *__restrict__ xg = pl.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const InstDesc &d = t.d;
+    const int c = t.tx0 + 4 * lane;
+    const bool cok = c < d.pitch;
+    const int lrow0 = wave * RPW;            // tile-local first own row of this wave
+    auto grow = [&](int lr) { return t.ty0 + lr; };
+    auto elem = [&](int lr, int col) { return d.off + (long long)grow(lr) * d.pitch + col; };
+    auto row_ok = [&](int lr) { return grow(lr) >= 0 && grow(lr) < d.nrow; };
+    auto upd_px = [&](T r, T q, T p, T di, T &rn, T &pn) {
+        rn = useq ? Prec<T>::fmaT(-al, q, r) : r;
+        T z = rn * di;
+        pn = keep ? Prec<T>::fmaT(be, p, z) : z;
+    };
+
+    // ── phase 1: loads; stage p_j ──
+    T x4[RPW][4], r4[RPW][4], p4[RPW][4], wd[RPW][4], wr[RPW][4], wu[4], wlo[RPW];
+#pragma unroll
+    for (int k = 0; k < RPW; ++k) {
+        zero4(x4[k]); zero4(r4[k]); zero4(p4[k]); zero4(wd[k]); zero4(wr[k]);
+        wlo[k] = T(0);
+        const int lr = lrow0 + k;
+        if (row_ok(lr)) {
+            const long long e = elem(lr, c);
+            if (cok) {
+                if (upd) ld4_nt(xg + e, x4[k]);
+                ld4(r_src + e, r4[k]);
+                if (upd) ld4(p_in + e, p4[k]);
+                ld_arcs(w2, e, wd[k], wr[k]);
+            }
+            if (lane == 0 && t.tx0 > 0) wlo[k] = w2[2 * (e - 1) + 1];
+        }
+    }
+    up_arcs(w2, t, grow(lrow0), cok && row_ok(lrow0), lane, wu);
+    // ring rows (wave 0: row -1 and p_j of row -2; wave 3: row TH and p_j of
+    // row TH+1): r, p, arcs; ring row arcs {down, right} + up + left
+    T rr4[4], rp4[4], rwd[4], rwr[4], rwu[4], rwlo = T(0);
+    zero4(rr4); zero4(rp4); zero4(rwd); zero4(rwr); zero4(rwu);
+    if (wave == 0 || wave == 3) {
+        const int lr = (wave == 0) ? -1 : TH;
+        const int lr2 = (wave == 0) ? -2 : TH + 1;
+        T p2v[4];
+        zero4(p2v);
+        if (row_ok(lr)) {
+            const long long e = elem(lr, c);
+            if (cok) {
+                ld4(r_src + e, rr4);
+                if (upd) ld4(p_in + e, rp4);
+                ld_arcs(w2, e, rwd, rwr);
+            }
+            if (lane == 0 && t.tx0 > 0) rwlo = w2[2 * (e - 1) + 1];
+        }
+        if (wave == 0) {
+            if (row_ok(lr2) && cok) {
+                T dummy[4];
+                ld_arcs(w2, elem(lr2, c), rwu, dummy);   // down arcs of row -2 = up arcs of row -1
+                if (useq) ld4(p_in + elem(lr2, c), p2v);
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) rwu[j] = wd[RPW - 1][j];   // up arcs of row TH
+            if (useq && row_ok(lr2) && cok) ld4(p_in + elem(lr2, c), p2v);
+        }
+        if (upd) {
+            T *a = lp0 + (lr + 2) * S + kLdsC0 + 4 * lane;
+            T *b = lp0 + (lr2 + 2) * S + kLdsC0 + 4 * lane;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) { a[j] = rp4[j]; b[j] = p2v[j]; }
+        }
+    }
+    // ring columns (wave 1: column -1, wave 2: column 256): lane l < TH owns
+    // pixel (l, col); lanes l < TH + 2 stage p_j at (l - 1, col) and lanes
+    // l < TH p_j at (l, col -/+ 1) beyond it
+    T cr = T(0), cp = T(0), cwd = T(0), cwr = T(0), cwu = T(0), cwl = T(0);
+    if (wave == 1 || wave == 2) {
+        const int col = (wave == 1) ? t.tx0 - 1 : t.tx0 + kTileW;
+        const int col2 = (wave == 1) ? col - 1 : col + 1;
+        const bool colok = col >= 0 && col < d.pitch, col2ok = col2 >= 0 && col2 < d.pitch;
+        const int lcol = (wave == 1) ? -1 : kTileW, lcol2 = (wave == 1) ? -2 : kTileW + 1;
+        if (lane < TH + 2) {
+            const int lr = lane - 1;
+            T v = T(0);
+            if (upd && colok && row_ok(lr)) v = p_in[elem(lr, col)];
+            if (upd) lp0[(lr + 2) * S + kLdsC0 + lcol] = v;
+        }
+        if (lane < TH) {
+            const int lr = lane;
+            T v2 = T(0);
+            if (colok && row_ok(lr)) {
+                const long long e = elem(lr, col);
+                cr = r_src[e];
+                if (upd) cp = p_in[e];
+                cwd = w2[2 * e];
+                cwr = w2[2 * e + 1];
+                if (grow(lr) > 0) cwu = w2[2 * (e - d.pitch)];
+                // left arc: the right arc of the pixel before (column -2
+                // for the left ring, column 255 for the right ring)
+                if (wave == 2 || col2ok) cwl = w2[2 * (e - 1) + 1];
+                if (useq && col2ok) v2 = p_in[elem(lr, col2)];
+            }
+            if (useq) lp0[(lr + 2) * S + kLdsC0 + lcol2] = v2;
+        }
+    }
+    // own p_j
+    if (upd) {
+#pragma unroll
+        for (int k = 0; k < RPW; ++k) {
+            T *a = lp0 + (lrow0 + k + 2) * S + kLdsC0 + 4 * lane;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) a[j] = p4[k][j];
+        }
+    }
+    __syncthreads();
+
+    // ── phase 2: q_j, r_{j+1}, p_{j+1} on own pixels and the ring ──
+    T wl[RPW][4], di[RPW][4];
+#pragma unroll
+    for (int k = 0; k < RPW; ++k) {
+        T from_left = __shfl_up(wr[k][3], 1, 64);
+        wl[k][0] = (lane == 0) ? wlo[k] : from_left;
+        wl[k][1] = wr[k][0]; wl[k][2] = wr[k][1]; wl[k][3] = wr[k][2];
+        const T *wuk = (k == 0) ? wu : wd[k > 0 ? k - 1 : 0];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) di[k][j] = diag_inv(wd[k][j], wuk[j], wr[k][j], wl[k][j]);
+    }
+#pragma unroll
+    for (int k = 0; k < RPW; ++k) {
+        const int lr = lrow0 + k;
+        T q[4], rn[4], pn[4];
+        zero4(q);
+        if (useq) stencil4(lp0 + S, lr, lane, wd[k], (k == 0) ? wu : wd[k > 0 ? k - 1 : 0], wr[k], wl[k], q);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (upd) x4[k][j] = Prec<T>::fmaT(al, p4[k][j], x4[k][j]);
+            upd_px(r4[k][j], q[j], p4[k][j], di[k][j], rn[j], pn[j]);
+            r4[k][j] = rn[j];
+        }
+        if (row_ok(lr) && cok) {
+            const long long e = elem(lr, c);
+            if (upd) st4_nt(xg + e, x4[k]);
+            st4_nt(r_out + e, rn);
+            st4_nt(p_out + e, pn);
+        }
+        T *a = lp1 + (lr + 1) * S + kLdsC0 + 4 * lane;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) a[j] = pn[j];
+    }
+    if (wave == 0 || wave == 3) {
+        const int lr = (wave == 0) ? -1 : TH;
+        T rwl[4], q[4], pn[4];
+        T from_left = __shfl_up(rwr[3], 1, 64);
+        rwl[0] = (lane == 0) ? rwlo : from_left;
+        rwl[1] = rwr[0]; rwl[2] = rwr[1]; rwl[3] = rwr[2];
+        zero4(q);
+        if (useq) stencil4(lp0 + S, lr, lane, rwd, rwu, rwr, rwl, q);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            T rn;
+            upd_px(rr4[j], q[j], rp4[j], diag_inv(rwd[j], rwu[j], rwr[j], rwl[j]), rn, pn[j]);
+        }
+        T *a = lp1 + (lr + 1) * S + kLdsC0 + 4 * lane;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) a[j] = pn[j];
+    }
+    if ((wave == 1 || wave == 2) && lane < TH) {
+        const int lr = lane;
+        const int lcol = (wave == 1) ? -1 : kTileW;
+        const T *m = lp0 + (lr + 2) * S + kLdsC0 + lcol;
+        T q = T(0);
+        if (useq) q = stencil1(m[0], m[S], m[-S], m[1], m[-1], cwd, cwu, cwr, cwl);
+        T rn, pn;
+        upd_px(cr, q, cp, diag_inv(cwd, cwu, cwr, cwl), rn, pn);
+        lp1[(lr + 1) * S + kLdsC0 + lcol] = pn;
+    }
+    __syncthreads();
+
+    // ── phase 3: q_{j+1} = A p_{j+1} on own pixels, sums ──
+    double s_pq = 0.0, s_rz = 0.0, s_qr = 0.0, s_qq = 0.0;
+#pragma unroll
+    for (int k = 0; k < RPW; ++k) {
+        const int lr = lrow0 + k;
+        if (!row_ok(lr)) break;
+        T y[4];
+        stencil4(lp1, lr, lane, wd[k], (k == 0) ? wu : wd[k > 0 ? k - 1 : 0], wr[k], wl[k], y);
+        if (cok) {
+            const T *pc = lp1 + (lr + 1) * S + kLdsC0 + 4 * lane;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                double rd = (double)r4[k][j], dd = (double)di[k][j], qd = (double)y[j];
+                s_pq += (double)pc[j] * qd;
+                s_rz += rd * (double)(r4[k][j] * di[k][j]);
+                double qdi = qd * dd;
+                s_qr += qdi * rd;
+                s_qq += qdi * qd;
+            }
+        }
+    }
+    block_sum4(s_pq, s_rz, s_qr, s_qq, partials + t.blk);
+}
+
+// True residual for the q-free fused solver at the end of an even chunk
+// (state in set 1 = (r2, p2) with alpha_j pending): x' = x + alpha_j p_j and
+// q_j = A p_j from LDS tiles with a one-pixel halo; r_t = b - A x' into wv;
+// partial sums of r_t.(dinv r_t) and of r'.(dinv r') for r' = r_j - alpha_j q_j.
+template <class T, int RPW>
+__global__ __launch_bounds__(256) void k_rr_iq(const InstDesc *descs, const int *blk_inst, Planes<T> pl,
+                                               const SolveScal *scal, double4 *partials) {
+    constexpr int TH = 4 * RPW;
+    __shared__ T lx[(TH + 2) * kLdsStride];
+    __shared__ T lq[(TH + 2) * kLdsStride];
+    TileInfo t = get_tile(descs, blk_inst, TH);
+    const SolveScal *sc = scal + t.inst;
+    if (!sc->active) return;
+    const T al = (T)sc->alpha;
+    const T *__restrict__ pg = pl.p2;
+    const T *__restrict__ xg = pl.x;
+    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const InstDesc &d = t.d;
+    const int c = t.tx0 + 4 * lane;
+    const bool cok = c < d.pitch;
+    for (int k = 0; k < RPW; ++k) {
+        int lr = wave * RPW + k, row = t.ty0 + lr;
+        T xv[4], pv[4];
+        zero4(xv); zero4(pv);
+        if (row < d.nrow && cok) {
+            long long e = d.off + (long long)row * d.pitch + c;
+            ld4(xg + e, xv);
+            ld4(pg + e, pv);
+        }
+        const int o = (lr + 1) * kLdsStride + kLdsC0 + 4 * lane;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) { lx[o + j] = Prec<T>::fmaT(al, pv[j], xv[j]); lq[o + j] = pv[j]; }
+    }
+    fill_halo<TH>(
+        t, wave, lane,
+        [&](int lr, int lc, long long e, bool ok) {
+            T xv[4], pv[4];
+            zero4(xv); zero4(pv);
+            if (ok) { ld4(xg + e, xv); ld4(pg + e, pv); }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                lx[lr * kLdsStride + lc + j] = Prec<T>::fmaT(al, pv[j], xv[j]);
+                lq[lr * kLdsStride + lc + j] = pv[j];
+            }
+        },
+        [&](int lr, int lc, long long e, bool ok) {
+            lx[lr * kLdsStride + lc] = ok ? Prec<T>::fmaT(al, pg[e], xg[e]) : T(0);
+            lq[lr * kLdsStride + lc] = ok ? pg[e] : T(0);
+        });
+    __syncthreads();
+    double s_t = 0.0, s_r = 0.0;
+    T wu[4];
+    for (int k = 0; k < RPW; ++k) {
+        int lr = wave * RPW + k, row = t.ty0 + lr;
+        if (row >= d.nrow) break;
+        if (k == 0) up_arcs(pl.w2, t, row, cok, lane, wu);
+        T wd[4], wr[4], wl[4], y[4], q[4];
+        row_arcs(pl.w2, t, row, cok, lane, wd, wr, wl);
+        stencil4(lx, lr, lane, wd, wu, wr, wl, y);
+        stencil4(lq, lr, lane, wd, wu, wr, wl, q);
+        if (cok) {
+            long long e = d.off + (long long)row * d.pitch + c;
+            T b4[4], rt[4], r4[4];
+            ld4(pl.b + e, b4);
+            ld4(pl.r2 + e, r4);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                T di = diag_inv(wd[j], wu[j], wr[j], wl[j]);
+                rt[j] = b4[j] - y[j];
+                T rr = Prec<T>::fmaT(-al, q[j], r4[j]);
+                s_t += (double)rt[j] * (double)(rt[j] * di);
+                s_r += (double)rr * (double)(rr * di);
+            }
+            st4(pl.wv + e, rt);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) wu[j] = wd[j];
+    }
+    block_sum4(s_t, s_r, 0.0, 0.0, partials + t.blk);
+}
+
 // Reduction and scalar recurrences of the fused solver (one workgroup per
 // instance).
 //   start: rz_0; stop if not positive (:446-450) or if max_iter <= 0; else
@@ -1325,6 +1660,18 @@ void launch_rr_it(const InstDesc *d_desc, const int *d_blk, int nblk, const Plan
     CUPHU_RPW_DISPATCH(rpw, k_rr_it, dim3(nblk), dim3(256), 0, st, d_desc, d_blk, pl, scal, partials);
     CUPHU_LAUNCH_CHECK();
 }
+template <class T>
+void launch_iq(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, const SolveScal *scal,
+               double4 *partials, int rpw, int parity, int start, hipStream_t st) {
+    CUPHU_RPW_DISPATCH(rpw, k_iq, dim3(nblk), dim3(256), 0, st, d_desc, d_blk, pl, scal, partials, parity, start);
+    CUPHU_LAUNCH_CHECK();
+}
+template <class T>
+void launch_rr_iq(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, const SolveScal *scal,
+                  double4 *partials, int rpw, hipStream_t st) {
+    CUPHU_RPW_DISPATCH(rpw, k_rr_iq, dim3(nblk), dim3(256), 0, st, d_desc, d_blk, pl, scal, partials);
+    CUPHU_LAUNCH_CHECK();
+}
 void launch_reduce_it(const InstDesc *d_desc, int ninst, const double4 *partials, SolveScal *scal, int mode,
                       int max_iter, double tol2, hipStream_t st) {
     hipLaunchKernelGGL(k_reduce_it, dim3(ninst), dim3(256), 0, st, d_desc, partials, scal, mode, max_iter, tol2);
@@ -1394,6 +1741,10 @@ void launch_plain_dot(const T *a, const T *b, size_t n, double *partials, double
     template void launch_it<T>(const InstDesc *, const int *, int, const Planes<T> &, const SolveScal *,         \
                                double4 *, int, int, int, hipStream_t);                                           \
     template void launch_rr_it<T>(const InstDesc *, const int *, int, const Planes<T> &, const SolveScal *,      \
+                                  double4 *, int, hipStream_t);                                                  \
+    template void launch_iq<T>(const InstDesc *, const int *, int, const Planes<T> &, const SolveScal *,         \
+                               double4 *, int, int, int, hipStream_t);                                           \
+    template void launch_rr_iq<T>(const InstDesc *, const int *, int, const Planes<T> &, const SolveScal *,      \
                                   double4 *, int, hipStream_t);                                                  \
     template void launch_final<T>(const InstDesc *, const int *, int, const Planes<T> &, float *, int,           \
                                   hipStream_t);                                                                  \

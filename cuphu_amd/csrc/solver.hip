@@ -351,10 +351,19 @@ static void solve_group_batched(const Planes<T> &pl, const GroupDev &g, const Cu
     if (gexec) CUPHU_HIP_CHECK(hipGraphExecDestroy(gexec));
 }
 
-// Algorithmic HBM bytes per active pixel of the fused iteration kernel:
-// reads x, r, p, q and the {down, right} arc pair (1/diag is recomputed from
-// the arcs), writes x, r, p, q.
-constexpr double kItValuesPerPx = 10.0;
+// Algorithmic HBM bytes per active pixel of the fused iteration kernels
+// (1/diag is recomputed from the arcs in both):
+//   k_iq (default): reads x, r, p and the {down, right} arc pair, writes
+//   x, r, p -- q = A p is recomputed, never stored (8 values);
+//   k_it (CUPHU_FUSED_STORED_Q=1): reads x, r, p, q and the arc pair,
+//   writes x, r, p, q (10 values).
+static bool fused_stored_q() {
+    static const bool v = [] {
+        const char *e = std::getenv("CUPHU_FUSED_STORED_Q");
+        return e && std::atoi(e) != 0;
+    }();
+    return v;
+}
 
 // Fused solver: one stencil kernel and one reduction per PCG iteration.
 template <class T>
@@ -370,8 +379,14 @@ static void solve_group_fused(const Planes<T> &pl, const GroupDev &g, const CuPh
     const bool prof = g_prof.on;
     if (prof && !g_prof.ev[0])
         for (auto &e : g_prof.ev) CUPHU_HIP_CHECK(hipEventCreate(&e));
+    const bool sq = fused_stored_q();
+    const double values_per_px = sq ? 10.0 : 8.0;
+    auto it = [&](int parity, int start) {
+        if (sq) launch_it<T>(g.desc, g.blk, g.nblk, pl, g.scal, g.part4, rpw, parity, start, st);
+        else launch_iq<T>(g.desc, g.blk, g.nblk, pl, g.scal, g.part4, rpw, parity, start, st);
+    };
     auto step = [&](int i) {
-        launch_it<T>(g.desc, g.blk, g.nblk, pl, g.scal, g.part4, rpw, 1 - (i & 1), 0, st);
+        it(1 - (i & 1), 0);
         launch_reduce_it(g.desc, g.ninst, g.part4, g.scal, RED_IT, o.max_iter, tol2, st);
     };
     // With profiling on, the first two steps of a chunk run outside the
@@ -381,7 +396,8 @@ static void solve_group_fused(const Planes<T> &pl, const GroupDev &g, const CuPh
     auto chunk = [&]() {
         for (int i = prof ? 2 : 0; i < CH; ++i) step(i);
         if (o.rr_interval > 0) {
-            launch_rr_it<T>(g.desc, g.blk, g.nblk, pl, g.scal, g.part4, rpw, st);
+            if (sq) launch_rr_it<T>(g.desc, g.blk, g.nblk, pl, g.scal, g.part4, rpw, st);
+            else launch_rr_iq<T>(g.desc, g.blk, g.nblk, pl, g.scal, g.part4, rpw, st);
             launch_reduce_it(g.desc, g.ninst, g.part4, g.scal, RED_RR, o.max_iter, tol2, st);
         }
     };
@@ -413,7 +429,7 @@ static void solve_group_fused(const Planes<T> &pl, const GroupDev &g, const CuPh
                 launch_weights<T>(g.desc, g.blk, g.nblk, pl, pass > 0 ? 1 : 0, d2, rpw, st);
                 launch_assemble<T>(g.desc, g.blk, g.nblk, pl, rpw, st);
             }
-            launch_it<T>(g.desc, g.blk, g.nblk, pl, g.scal, g.part4, rpw, 0, 1, st);
+            it(0, 1);
             launch_reduce_it(g.desc, g.ninst, g.part4, g.scal, RED_START, o.max_iter, tol2, st);
             CUPHU_HIP_CHECK(hipMemcpyAsync(h.data(), g.scal, h.size() * sizeof(SolveScal), hipMemcpyDeviceToHost, st));
             CUPHU_HIP_CHECK(hipStreamSynchronize(st));
@@ -429,7 +445,7 @@ static void solve_group_fused(const Planes<T> &pl, const GroupDev &g, const CuPh
                         if (h[(size_t)i].active) active_px += (double)g.px[(size_t)i];
                     step(0);
                     CUPHU_HIP_CHECK(hipEventRecord(g_prof.ev[0], st));
-                    launch_it<T>(g.desc, g.blk, g.nblk, pl, g.scal, g.part4, rpw, 0, 0, st);
+                    it(0, 0);
                     CUPHU_HIP_CHECK(hipEventRecord(g_prof.ev[1], st));
                     launch_reduce_it(g.desc, g.ninst, g.part4, g.scal, RED_IT, o.max_iter, tol2, st);
                 }
@@ -444,7 +460,7 @@ static void solve_group_fused(const Planes<T> &pl, const GroupDev &g, const CuPh
                     a.samples += 1;
                     a.px_sum += active_px;
                     a.it_ms_sum += ms;
-                    a.it_bytes_sum += active_px * kItValuesPerPx * sizeof(T);
+                    a.it_bytes_sum += active_px * values_per_px * sizeof(T);
                     a.iterations += CH;
                 }
                 ++nch;
@@ -1762,8 +1778,8 @@ int cuphu_test_pcg_step(const float *w, const float *dinv, int nrow, int ncol, d
 }
 
 int cuphu_test_pcg_fused_step(const float *w, const float *dinv, int nrow, int ncol, double alpha, double beta,
-                              int start, const float *rt, int rpw, float *x, float *r, float *p, float *q,
-                              double *sums) {
+                              int start, const float *rt, int rpw, int stored_q, float *x, float *r, float *p,
+                              float *q, double *sums) {
     return guarded([&]() {
         std::lock_guard<std::recursive_mutex> lk(g_mu);
         CUPHU_REQUIRE(w && dinv && x && r && p && q && sums && nrow >= 1 && ncol >= 1, "null argument");
@@ -1793,13 +1809,14 @@ int cuphu_test_pcg_fused_step(const float *w, const float *dinv, int nrow, int n
         sc.active = 1;
         CUPHU_HIP_CHECK(hipMemcpyAsync(h.g.scal, &sc, sizeof(sc), hipMemcpyHostToDevice, st));
         // parity 0: reads set 0 (r, p, s), writes set 1 (r2, p2, q2)
-        launch_it<float>(h.g.desc, h.g.blk, h.g.nblk, h.P.pl, h.g.scal, h.g.part4, rpw, 0, start ? 1 : 0, st);
+        if (stored_q) launch_it<float>(h.g.desc, h.g.blk, h.g.nblk, h.P.pl, h.g.scal, h.g.part4, rpw, 0, start ? 1 : 0, st);
+        else launch_iq<float>(h.g.desc, h.g.blk, h.g.nblk, h.P.pl, h.g.scal, h.g.part4, rpw, 0, start ? 1 : 0, st);
         std::vector<double4> parts((size_t)h.g.nblk);
         CUPHU_HIP_CHECK(hipMemcpyAsync(parts.data(), h.g.part4, parts.size() * sizeof(double4), hipMemcpyDeviceToHost, st));
         from_plane(h.P.pl.x, x, d, st);
         from_plane(h.P.pl.r2, r, d, st);
         from_plane(h.P.pl.p2, p, d, st);
-        from_plane(h.P.pl.q2, q, d, st);
+        if (stored_q) from_plane(h.P.pl.q2, q, d, st);
         CUPHU_HIP_CHECK(hipStreamSynchronize(st));
         double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
         for (auto &v : parts) { a0 += v.x; a1 += v.y; a2 += v.z; a3 += v.w; }

@@ -433,9 +433,10 @@ int cuphu_wls_adjoint_gpu(const void *d_w, const void *d_phase, const void *d_u,
  *       56 B in FP64);
  *   xr: (BATCHED) x += alpha p, r -= alpha q, r.z -- reads x, p, r, q,
  *       1/diag, writes x and r (28 B / 56 B per active pixel);
- *   it: (FUSED) one whole iteration -- reads x, r, p, q and two arc
- *       weights (1/diag is recomputed from the arcs), writes x, r, p, q
- *       (40 B / 80 B per active pixel).
+ *   it: (FUSED) one whole iteration -- reads x, r, p and two arc weights,
+ *       writes x, r, p (32 B / 64 B per active pixel; q = L p and 1/diag are
+ *       recomputed); with CUPHU_FUSED_STORED_Q=1 the kernel that also reads
+ *       and writes q (40 B / 80 B).
  * cuphu_set_profiling resets the counters. */
 typedef struct CuPhuKernelProfile {
     long long samples;        /* timed iterations                          */
@@ -484,10 +485,13 @@ int cuphu_test_pcg_step(const float *w, const float *dinv, int nrow, int ncol,
  * x += alpha p; r -= alpha q (r = rt when rt != NULL, the residual
  * replacement); z = r*dinv; p = z + beta p (p = z when beta == 0); q = L p;
  * start != 0 skips the x, r updates and sets p = z.  Updates x, r, p, q in
- * place; sums[4] = p.q, r.z, q.(dinv r), q.(dinv q). */
+ * place; sums[4] = p.q, r.z, q.(dinv r), q.(dinv q).  stored_q != 0 runs
+ * the kernel that reads q from memory; 0 runs the default kernel, which
+ * recomputes q = L p from p (the q passed in must then be L p) and leaves
+ * q unchanged.  dinv must be the assemble kernel's 1/diag of w. */
 int cuphu_test_pcg_fused_step(const float *w, const float *dinv, int nrow, int ncol,
                               double alpha, double beta, int start, const float *rt, int rpw,
-                              float *x, float *r, float *p, float *q, double *sums);
+                              int stored_q, float *x, float *r, float *p, float *q, double *sums);
 
 #ifdef __cplusplus
 } /* extern "C" */

@@ -142,9 +142,11 @@ def test_pcg_step_kernels_bitexact(gpu_ready, shape, rpw, beta):
 @pytest.mark.parametrize("shape", SHAPES)
 @pytest.mark.parametrize("rpw", [2, 4, 8])
 @pytest.mark.parametrize("mode", ["start", "step", "step_beta0", "replace"])
-def test_pcg_fused_step_kernel_bitexact(gpu_ready, shape, rpw, mode):
-    """Fused iteration kernel against the oracle: x, r, p, q bit-exact; the
-    four sums to FP64 summation-order tolerance."""
+@pytest.mark.parametrize("stored_q", [0, 1])
+def test_pcg_fused_step_kernel_bitexact(gpu_ready, shape, rpw, mode, stored_q):
+    """Fused iteration kernels against the oracle: x, r, p (and q when stored)
+    bit-exact; the four sums to FP64 summation-order tolerance.  The q-free
+    kernel recomputes q = L p, so q is passed as the oracle's L p."""
     rng = np.random.default_rng(shape[0] + 5 * shape[1] + rpw)
     nrow, ncol = shape
     na = G.narcs(nrow, ncol)
@@ -156,6 +158,8 @@ def test_pcg_fused_step_kernel_bitexact(gpu_ready, shape, rpw, mode):
     _, diag = O.assemble(w[:na], np.zeros(shape, np.float32))
     dinv = (np.float32(1) / diag).astype(np.float32)
     x, r, p, q, rt = (rng.standard_normal(shape).astype(np.float32) for _ in range(5))
+    if not stored_q:
+        q = O.matvec(w[:na], p)
     alpha, beta = 0.37, (0.0 if mode == "step_beta0" else 0.81)
     start = mode == "start"
     rta = rt if mode == "replace" else None
@@ -164,8 +168,9 @@ def test_pcg_fused_step_kernel_bitexact(gpu_ready, shape, rpw, mode):
     sums = np.zeros(4, np.float64)
     _capi.check(_capi.lib.cuphu_test_pcg_fused_step(w.ctypes.data, dinv.ctypes.data, nrow, ncol, alpha, beta,
                                                     int(start), None if rta is None else rta.ctypes.data, rpw,
-                                                    *[v.ctypes.data for v in got], sums.ctypes.data), "fused_step")
-    for name, a, e in zip("xrpq", got, ref[:4]):
+                                                    stored_q, *[v.ctypes.data for v in got], sums.ctypes.data),
+                "fused_step")
+    for name, a, e in zip("xrpq" if stored_q else "xrp", got, ref[:4]):
         np.testing.assert_array_equal(a, e, err_msg=name)
     np.testing.assert_allclose(sums, ref[4], rtol=1e-12, atol=1e-300)
 
