@@ -163,6 +163,32 @@ __device__ __forceinline__ void stencil4(const T *lds, int lr, int lane, const T
     }
 }
 
+// The same stencil accumulated in FP64 (the true residual b - L x of a
+// residual replacement: b and L x nearly cancel, so the difference is formed
+// from exact products and rounded once).
+template <class T>
+__device__ __forceinline__ void stencil4_wide(const T *lds, int lr, int lane, const T (&wd)[4], const T (&wu)[4],
+                                              const T (&wr)[4], const T (&wl)[4], double (&y)[4]) {
+    const T *rc = lds + (lr + 1) * kLdsStride + kLdsC0 + 4 * lane;
+    T c4[4], u4[4], d4[4];
+    ld4(rc, c4);
+    ld4(rc - kLdsStride, u4);
+    ld4(rc + kLdsStride, d4);
+    T left = rc[-1], right = rc[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        double a = (double)c4[j];
+        double xl = (double)((j == 0) ? left : c4[j - 1]);
+        double xr = (double)((j == 3) ? right : c4[j + 1]);
+        double yv = (double)Prec<T>::eps * a;
+        yv = fma((double)wd[j], a - (double)d4[j], yv);
+        yv = fma((double)wu[j], a - (double)u4[j], yv);
+        yv = fma((double)wr[j], a - xr, yv);
+        yv = fma((double)wl[j], a - xl, yv);
+        y[j] = yv;
+    }
+}
+
 // Arcs of a lane's 4 pixels in one row: own down/right arcs and the left
 // arc (the right arc of the previous column: the neighbouring lane's, or a
 // load for lane 0).  Executed by every lane (contains a shuffle).
@@ -665,9 +691,10 @@ __global__ __launch_bounds__(256) void k_rr(const InstDesc *descs, const int *bl
         int lr = wave * RPW + k, row = t.ty0 + lr;
         if (row >= d.nrow) break;
         if (k == 0) up_arcs(pl.w2, t, row, cok, lane, wu);
-        T wd[4], wr[4], wl[4], y[4];
+        T wd[4], wr[4], wl[4];
+        double y[4];
         row_arcs(pl.w2, t, row, cok, lane, wd, wr, wl);
-        stencil4(lx, lr, lane, wd, wu, wr, wl, y);
+        stencil4_wide(lx, lr, lane, wd, wu, wr, wl, y);
         if (cok) {
             long long e = d.off + (long long)row * d.pitch + c;
             T b4[4], di[4], r4[4];
@@ -675,7 +702,7 @@ __global__ __launch_bounds__(256) void k_rr(const InstDesc *descs, const int *bl
             ld4(pl.dinv + e, di);
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                r4[j] = b4[j] - y[j];
+                r4[j] = (T)((double)b4[j] - y[j]);
                 g += (double)r4[j] * (double)(r4[j] * di[j]);
             }
             st4(pl.p2 + e, r4);
@@ -962,9 +989,10 @@ __global__ __launch_bounds__(256) void k_rr_it(const InstDesc *descs, const int 
         int lr = wave * RPW + k, row = t.ty0 + lr;
         if (row >= d.nrow) break;
         if (k == 0) up_arcs(pl.w2, t, row, cok, lane, wu);
-        T wd[4], wr[4], wl[4], y[4];
+        T wd[4], wr[4], wl[4];
+        double y[4];
         row_arcs(pl.w2, t, row, cok, lane, wd, wr, wl);
-        stencil4(lx, lr, lane, wd, wu, wr, wl, y);
+        stencil4_wide(lx, lr, lane, wd, wu, wr, wl, y);
         if (cok) {
             long long e = d.off + (long long)row * d.pitch + c;
             T b4[4], di[4], rt[4], r4[4], q4[4];
@@ -974,7 +1002,7 @@ __global__ __launch_bounds__(256) void k_rr_it(const InstDesc *descs, const int 
             ld4(pl.q2 + e, q4);
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                rt[j] = b4[j] - y[j];
+                rt[j] = (T)((double)b4[j] - y[j]);
                 T rr = Prec<T>::fmaT(-al, q4[j], r4[j]);
                 s_t += (double)rt[j] * (double)(rt[j] * di[j]);
                 s_r += (double)rr * (double)(rr * di[j]);
@@ -1297,9 +1325,10 @@ __global__ __launch_bounds__(256) void k_rr_iq(const InstDesc *descs, const int 
         int lr = wave * RPW + k, row = t.ty0 + lr;
         if (row >= d.nrow) break;
         if (k == 0) up_arcs(pl.w2, t, row, cok, lane, wu);
-        T wd[4], wr[4], wl[4], y[4], q[4];
+        T wd[4], wr[4], wl[4], q[4];
+        double y[4];
         row_arcs(pl.w2, t, row, cok, lane, wd, wr, wl);
-        stencil4(lx, lr, lane, wd, wu, wr, wl, y);
+        stencil4_wide(lx, lr, lane, wd, wu, wr, wl, y);
         stencil4(lq, lr, lane, wd, wu, wr, wl, q);
         if (cok) {
             long long e = d.off + (long long)row * d.pitch + c;
@@ -1309,7 +1338,7 @@ __global__ __launch_bounds__(256) void k_rr_iq(const InstDesc *descs, const int 
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 T di = diag_inv(wd[j], wu[j], wr[j], wl[j]);
-                rt[j] = b4[j] - y[j];
+                rt[j] = (T)((double)b4[j] - y[j]);
                 T rr = Prec<T>::fmaT(-al, q[j], r4[j]);
                 s_t += (double)rt[j] * (double)(rt[j] * di);
                 s_r += (double)rr * (double)(rr * di);
@@ -1329,9 +1358,15 @@ __global__ __launch_bounds__(256) void k_rr_iq(const InstDesc *descs, const int 
 //   it:    one iteration done (iters++); converged if rz < tol^2 rz_0
 //          (:478); max_iter; p.q <= 0 breakdown (:460); alpha = rz / p.q
 //          (:462); beta from the rz expansion (see k_it).
-//   rr:    replace r by the true residual unless it exceeds the recursive
-//          one by more than 4x (the recursion is then below the accuracy the
-//          arithmetic attains); beta = (chosen rz) / rz_j exactly.
+//   rr:    a monitor: the true residual (FP64-evaluated) replaces r only
+//          when its r.z already meets the tolerance, so the next iteration
+//          stops on an honest residual; otherwise the recursion continues
+//          with beta = (recursive r.z) / rz_j computed from the vectors
+//          themselves.  Replacing an unconverged FP32 residual perturbs the
+//          short recurrence: on nlooks-5 interferograms stopped by the
+//          1000-iteration cap it left up to 5 % of pixels a cycle away from
+//          the FP64 solve, whose iterates the unreplaced FP32 recursion
+//          tracks to 99.98 %.
 __global__ __launch_bounds__(256) void k_reduce_it(const InstDesc *descs, const double4 *partials, SolveScal *scal,
                                                    int mode, int max_iter, double tol2) {
     int inst = blockIdx.x;
@@ -1353,8 +1388,13 @@ __global__ __launch_bounds__(256) void k_reduce_it(const InstDesc *descs, const 
     s.replace = 0;
     if (mode == RED_RR) {
         const double rt = v.x, rr = v.y;
-        if (isfinite(rt) && rt <= 4.0 * rr) { s.replace = 1; s.beta = rt / s.gamma; }
-        else if (isfinite(rr)) s.beta = rr / s.gamma;
+        s.delta = rt / rr;   // true over recursive r.z, kept until the next reduction
+        if (isfinite(rt) && rt < tol2 * s.gamma0) {
+            s.replace = 1;
+            s.beta = rt / s.gamma;
+        } else if (isfinite(rr)) {
+            s.beta = rr / s.gamma;
+        }
         *sp = s;
         return;
     }
@@ -1397,10 +1437,9 @@ __global__ __launch_bounds__(256) void k_reduce_it(const InstDesc *descs, const 
 //   pq:    stop on p.q <= 0 (:460); alpha = rz / p.q (:462)
 //   rz:    iteration done; converged if rz' < tol^2 rz0 (:478);
 //          beta = rz' / rz (:493)
-//   rr:    true residual r_t.z_t: converged if below tol^2 rz0; otherwise
-//          r := r_t (beta = r_t.z_t / rz_prev) unless r_t.z_t exceeds the
-//          recursive rz by more than 4x, i.e. the recursion has gone below
-//          the accuracy the arithmetic attains and continues unreplaced.
+//   rr:    true residual r_t.z_t (FP64-evaluated): converged if below
+//          tol^2 rz0; otherwise the recursion continues unreplaced (a
+//          monitor, as in the fused solver's reduction).
 // ═══════════════════════════════════════════════════════════════════════════
 __global__ __launch_bounds__(256) void k_reduce(const InstDesc *descs, const double2 *partials, SolveScal *scal,
                                                 int mode, int max_iter, double tol2) {
@@ -1441,11 +1480,6 @@ __global__ __launch_bounds__(256) void k_reduce(const InstDesc *descs, const dou
         else if (v < tol2 * s.gamma0) {
             s.status = CUPHU_SOLVE_CONVERGED; s.active = 0; s.gamma = v;
             s.relres = (s.gamma0 > 0.0) ? sqrt(v / s.gamma0) : 0.0;
-        } else if (v <= 4.0 * s.gamma) {
-            s.replace = 1;
-            s.beta = v / s.gamma_prev;
-            s.gamma = v;
-            s.relres = sqrt(v / s.gamma0);
         }
     }
     *sp = s;

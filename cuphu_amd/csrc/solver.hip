@@ -105,7 +105,7 @@ void default_options(CuPhuLaplaceOptions *o) {
     env_float("CUPHU_IRLS_DELTA_MIN", &o->delta_min);
     env_int("CUPHU_LAPLACE_MAX_ITER", &o->max_iter);
     if (const char *e = std::getenv("CUPHU_LAPLACE_TOL")) o->tol = std::atof(e);
-    if (std::getenv("CUPHU_LAPLACE_VERBOSE")) o->verbose = 1;
+    if (const char *e = std::getenv("CUPHU_LAPLACE_VERBOSE")) o->verbose = std::max(1, std::atoi(e));
     if (const char *e = std::getenv("CUPHU_LAPLACE_SOLVER"))
         o->solver = (std::strcmp(e, "plain") == 0)     ? CUPHU_SOLVER_PLAIN
                     : (std::strcmp(e, "batched") == 0) ? CUPHU_SOLVER_BATCHED
@@ -453,6 +453,12 @@ static void solve_group_fused(const Planes<T> &pl, const GroupDev &g, const CuPh
                 else chunk();
                 CUPHU_HIP_CHECK(hipMemcpyAsync(h.data(), g.scal, h.size() * sizeof(SolveScal), hipMemcpyDeviceToHost, st));
                 CUPHU_HIP_CHECK(hipStreamSynchronize(st));
+                if (o.verbose > 1 && o.rr_interval > 0)
+                    for (int i = 0; i < g.ninst; ++i)
+                        if (h[(size_t)i].active)
+                            fprintf(stderr, "[cuphu-rr] inst %d pass %d iters %d relres %.3e true/recursive r.z %.4g %s\n",
+                                    i, pass, h[i].iters, h[i].relres, h[i].delta,
+                                    h[i].replace ? "replaced" : "kept");
                 if (prof) {
                     float ms = 0.0f;
                     CUPHU_HIP_CHECK(hipEventElapsedTime(&ms, g_prof.ev[0], g_prof.ev[1]));
