@@ -49,10 +49,10 @@ BYTES_PER_PX_XR = 28.0
 # HBM bytes per active pixel actually moved per launch at this workload:
 # rocprofv3 FETCH_SIZE (x2.000, calibrated on a copy of known size) +
 # WRITE_SIZE (x1.000) per launch / active pixels per launch
-# (profiles/k_it_scene8192_hbm_traffic.md, profiles/k_pq_k_xr_scene8192_hbm_traffic.md).
+# (profiles/k_iq_scene8192_hbm_traffic.md, profiles/k_pq_k_xr_scene8192_hbm_traffic.md).
 # Re-measure after kernel changes.
 TRAFFIC_PER_PX_PQ = 2770.6e6 / 84.93e6
-TRAFFIC_PER_PX_IT = None   # re-measure for the q-free kernel
+TRAFFIC_PER_PX_IT = 32.21
 
 
 def log(msg):
