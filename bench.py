@@ -43,7 +43,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Mpixel/s unwrapped (Laplace PCG/IRLS) + stencil HBM GB/s; batched vs plain"
 HBM_PEAK_GBPS = 8000.0
-BYTES_PER_PX_IT = 32.0
+BYTES_PER_PX_IT = 30.0
 BYTES_PER_PX_PQ = 28.0
 BYTES_PER_PX_XR = 28.0
 # HBM bytes per active pixel actually moved per launch at this workload:
@@ -52,7 +52,7 @@ BYTES_PER_PX_XR = 28.0
 # (profiles/k_iq_scene8192_hbm_traffic.md, profiles/k_pq_k_xr_scene8192_hbm_traffic.md).
 # Re-measure after kernel changes.
 TRAFFIC_PER_PX_PQ = 2770.6e6 / 84.93e6
-TRAFFIC_PER_PX_IT = 32.21
+TRAFFIC_PER_PX_IT = None   # re-measure after the x deferral
 
 
 def log(msg):

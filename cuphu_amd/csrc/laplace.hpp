@@ -22,6 +22,10 @@ struct InstDesc {
 struct SolveScal {
     double gamma, delta, gamma_prev, alpha, alpha_prev, beta, gamma0, relres;
     int iters, active, status, replace;   // replace: the last residual replacement was taken
+    // x update of the next fused q-free launch: 0 = defer alpha_j p_j,
+    // 1 = apply the deferred alpha_{j-1} p_{j-1} (alpha_prev, p from the
+    // output plane set) then alpha_j p_j, 2 = apply alpha_j p_j (no deferral)
+    int xmode, pad2;
 };
 
 enum RedMode { RED_START = 0, RED_PQ = 1, RED_RZ = 2, RED_RR = 3, RED_IT = 4 };
@@ -99,7 +103,7 @@ template <class T>
 void launch_rr_it(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, const SolveScal *scal,
                   double4 *partials, int rpw, hipStream_t st);
 void launch_reduce_it(const InstDesc *d_desc, int ninst, const double4 *partials, SolveScal *scal, int mode,
-                      int max_iter, double tol2, hipStream_t st);
+                      int max_iter, double tol2, hipStream_t st, int defer_x = 0);
 // The same iteration without a stored q (q_j = A p_j recomputed from p_j):
 // plane sets (r, p) and (r2, p2); and its true-residual kernel.
 template <class T>
@@ -108,6 +112,10 @@ void launch_iq(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<
 template <class T>
 void launch_rr_iq(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, const SolveScal *scal,
                   double4 *partials, int rpw, hipStream_t st);
+// After a solve: apply an x update still deferred (xmode 1) when it stopped.
+template <class T>
+void launch_x_finalize(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, const SolveScal *scal,
+                       int rpw, hipStream_t st);
 template <class T>
 void launch_final(const InstDesc *d_desc, const int *d_rowinst, int nrows_total, const Planes<T> &pl,
                   float *out_pool, int max_pitch, hipStream_t st);

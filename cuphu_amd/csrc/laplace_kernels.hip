@@ -1064,6 +1064,12 @@ __global__ __launch_bounds__(256) void k_iq(const InstDesc *descs, const int *bl
     const bool keep = upd && sc->beta != 0.0;         // p = z + b p (else p = z)
     const T al = upd ? (T)sc->alpha : T(0);
     const T be = (T)sc->beta;
+    // x update of this launch (see SolveScal::xmode): ax = apply alpha_j p_j,
+    // ax2 = first apply the deferred alpha_{j-1} p_{j-1} (own pixel of the
+    // output set, read before this launch overwrites it)
+    const int xm = upd ? sc->xmode : 0;
+    const bool ax = upd && xm != 0, ax2 = upd && xm == 1;
+    const T alp = ax2 ? (T)sc->alpha_prev : T(0);
     const T *__restrict__ r_src = rep ? pl.wv : (parity ? pl.r2 : pl.r);
     const T *__restrict__ p_in = parity ? pl.p2 : pl.p;
     T *__restrict__ r_out = parity ? pl.r : pl.r2;
@@ -1085,16 +1091,17 @@ __global__ __launch_bounds__(256) void k_iq(const InstDesc *descs, const int *bl
     };
 
     // ── phase 1: loads; stage p_j ──
-    T x4[RPW][4], r4[RPW][4], p4[RPW][4], wd[RPW][4], wr[RPW][4], wu[4], wlo[RPW];
+    T x4[RPW][4], r4[RPW][4], p4[RPW][4], wd[RPW][4], wr[RPW][4], wu[4], wlo[RPW], pp4[RPW][4];
 #pragma unroll
     for (int k = 0; k < RPW; ++k) {
-        zero4(x4[k]); zero4(r4[k]); zero4(p4[k]); zero4(wd[k]); zero4(wr[k]);
+        zero4(x4[k]); zero4(r4[k]); zero4(p4[k]); zero4(wd[k]); zero4(wr[k]); zero4(pp4[k]);
         wlo[k] = T(0);
         const int lr = lrow0 + k;
         if (row_ok(lr)) {
             const long long e = elem(lr, c);
             if (cok) {
-                if (upd) ld4_nt(xg + e, x4[k]);
+                if (ax) ld4_nt(xg + e, x4[k]);
+                if (ax2) ld4_nt(p_out + e, pp4[k]);
                 ld4(r_src + e, r4[k]);
                 if (upd) ld4(p_in + e, p4[k]);
                 ld_arcs(w2, e, wd[k], wr[k]);
@@ -1202,13 +1209,14 @@ __global__ __launch_bounds__(256) void k_iq(const InstDesc *descs, const int *bl
         if (useq) stencil4(lp0 + S, lr, lane, wd[k], (k == 0) ? wu : wd[k > 0 ? k - 1 : 0], wr[k], wl[k], q);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            if (upd) x4[k][j] = Prec<T>::fmaT(al, p4[k][j], x4[k][j]);
+            if (ax2) x4[k][j] = Prec<T>::fmaT(alp, pp4[k][j], x4[k][j]);
+            if (ax) x4[k][j] = Prec<T>::fmaT(al, p4[k][j], x4[k][j]);
             upd_px(r4[k][j], q[j], p4[k][j], di[k][j], rn[j], pn[j]);
             r4[k][j] = rn[j];
         }
         if (row_ok(lr) && cok) {
             const long long e = elem(lr, c);
-            if (upd) st4_nt(xg + e, x4[k]);
+            if (ax) st4_nt(xg + e, x4[k]);
             st4_nt(r_out + e, rn);
             st4_nt(p_out + e, pn);
         }
@@ -1351,6 +1359,34 @@ __global__ __launch_bounds__(256) void k_rr_iq(const InstDesc *descs, const int 
     block_sum4(s_t, s_r, 0.0, 0.0, partials + t.blk);
 }
 
+// Deferred x update still pending when a q-free solve stopped (xmode 1 after
+// an odd number of launches): x += alpha_j p_j, p_j in plane set 1 (the
+// input set of the odd-numbered launches).
+template <class T, int RPW>
+__global__ __launch_bounds__(256) void k_x_finalize(const InstDesc *descs, const int *blk_inst, Planes<T> pl,
+                                                    const SolveScal *scal) {
+    constexpr int TH = 4 * RPW;
+    TileInfo t = get_tile(descs, blk_inst, TH);
+    const SolveScal *sc = scal + t.inst;
+    if (sc->xmode != 1) return;
+    const T al = (T)sc->alpha_prev;
+    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const InstDesc &d = t.d;
+    const int c = t.tx0 + 4 * lane;
+    if (c >= d.pitch) return;
+    for (int k = 0; k < RPW; ++k) {
+        const int row = t.ty0 + wave * RPW + k;
+        if (row >= d.nrow) break;
+        const long long e = d.off + (long long)row * d.pitch + c;
+        T x4[4], p4[4];
+        ld4(pl.x + e, x4);
+        ld4(pl.p2 + e, p4);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) x4[j] = Prec<T>::fmaT(al, p4[j], x4[j]);
+        st4(pl.x + e, x4);
+    }
+}
+
 // Reduction and scalar recurrences of the fused solver (one workgroup per
 // instance).
 //   start: rz_0; stop if not positive (:446-450) or if max_iter <= 0; else
@@ -1368,7 +1404,7 @@ __global__ __launch_bounds__(256) void k_rr_iq(const InstDesc *descs, const int 
 //          the FP64 solve, whose iterates the unreplaced FP32 recursion
 //          tracks to 99.98 %.
 __global__ __launch_bounds__(256) void k_reduce_it(const InstDesc *descs, const double4 *partials, SolveScal *scal,
-                                                   int mode, int max_iter, double tol2) {
+                                                   int mode, int max_iter, double tol2, int defer_x) {
     int inst = blockIdx.x;
     const InstDesc d = descs[inst];
     SolveScal *sp = scal + inst;
@@ -1401,11 +1437,16 @@ __global__ __launch_bounds__(256) void k_reduce_it(const InstDesc *descs, const 
     const double pq = v.x, rz = v.y, qr = v.z, qq = v.w;
     if (mode == RED_START) {
         s.iters = 0; s.gamma0 = rz; s.gamma_prev = rz; s.relres = 1.0; s.active = 0;
-        s.alpha = 0.0; s.beta = 0.0; s.delta = pq; s.gamma = rz;
+        s.alpha = 0.0; s.alpha_prev = 0.0; s.beta = 0.0; s.delta = pq; s.gamma = rz;
+        s.xmode = defer_x ? 0 : 2;
         if (!(rz > 0.0)) { s.status = (rz == 0.0) ? CUPHU_SOLVE_EMPTY : CUPHU_SOLVE_NONFINITE; s.relres = 0.0; *sp = s; return; }
         if (max_iter <= 0) { s.status = CUPHU_SOLVE_MAXITER; *sp = s; return; }
     } else {
         s.iters += 1;
+        // the launch just done applied xmode 0 (deferred alpha_j p_j) or 1
+        // (applied both); alpha_prev = alpha_j for the finalize / next launch
+        s.xmode = (s.xmode == 0) ? 1 : (s.xmode == 1 ? 0 : 2);
+        s.alpha_prev = s.alpha;
         s.relres = (s.gamma0 > 0.0) ? sqrt(fabs(rz) / s.gamma0) : 0.0;
         s.gamma_prev = s.gamma;
         s.gamma = rz;
@@ -1706,9 +1747,16 @@ void launch_rr_iq(const InstDesc *d_desc, const int *d_blk, int nblk, const Plan
     CUPHU_RPW_DISPATCH(rpw, k_rr_iq, dim3(nblk), dim3(256), 0, st, d_desc, d_blk, pl, scal, partials);
     CUPHU_LAUNCH_CHECK();
 }
+template <class T>
+void launch_x_finalize(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, const SolveScal *scal,
+                       int rpw, hipStream_t st) {
+    CUPHU_RPW_DISPATCH(rpw, k_x_finalize, dim3(nblk), dim3(256), 0, st, d_desc, d_blk, pl, scal);
+    CUPHU_LAUNCH_CHECK();
+}
 void launch_reduce_it(const InstDesc *d_desc, int ninst, const double4 *partials, SolveScal *scal, int mode,
-                      int max_iter, double tol2, hipStream_t st) {
-    hipLaunchKernelGGL(k_reduce_it, dim3(ninst), dim3(256), 0, st, d_desc, partials, scal, mode, max_iter, tol2);
+                      int max_iter, double tol2, hipStream_t st, int defer_x) {
+    hipLaunchKernelGGL(k_reduce_it, dim3(ninst), dim3(256), 0, st, d_desc, partials, scal, mode, max_iter, tol2,
+                       defer_x);
     CUPHU_LAUNCH_CHECK();
 }
 void launch_reduce(const InstDesc *d_desc, int ninst, const double2 *partials, SolveScal *scal, int mode,
@@ -1780,6 +1828,8 @@ void launch_plain_dot(const T *a, const T *b, size_t n, double *partials, double
                                double4 *, int, int, int, hipStream_t);                                           \
     template void launch_rr_iq<T>(const InstDesc *, const int *, int, const Planes<T> &, const SolveScal *,      \
                                   double4 *, int, hipStream_t);                                                  \
+    template void launch_x_finalize<T>(const InstDesc *, const int *, int, const Planes<T> &, const SolveScal *,  \
+                                       int, hipStream_t);                                                        \
     template void launch_final<T>(const InstDesc *, const int *, int, const Planes<T> &, float *, int,           \
                                   hipStream_t);                                                                  \
     template void launch_plain_matvec<T>(const InstDesc &, const T *, const T *, T *, hipStream_t);              \

@@ -380,7 +380,10 @@ static void solve_group_fused(const Planes<T> &pl, const GroupDev &g, const CuPh
     if (prof && !g_prof.ev[0])
         for (auto &e : g_prof.ev) CUPHU_HIP_CHECK(hipEventCreate(&e));
     const bool sq = fused_stored_q();
-    const double values_per_px = sq ? 10.0 : 8.0;
+    // values moved per pixel by two consecutive launches: k_it 10 + 10;
+    // k_iq 6 (x deferred: reads r, p, arc pair; writes r, p) + 9 (also reads
+    // x and the previous p, writes x)
+    const double pair_values_per_px = sq ? 20.0 : 15.0;
     auto it = [&](int parity, int start) {
         if (sq) launch_it<T>(g.desc, g.blk, g.nblk, pl, g.scal, g.part4, rpw, parity, start, st);
         else launch_iq<T>(g.desc, g.blk, g.nblk, pl, g.scal, g.part4, rpw, parity, start, st);
@@ -390,9 +393,8 @@ static void solve_group_fused(const Planes<T> &pl, const GroupDev &g, const CuPh
         launch_reduce_it(g.desc, g.ninst, g.part4, g.scal, RED_IT, o.max_iter, tol2, st);
     };
     // With profiling on, the first two steps of a chunk run outside the
-    // graph: step 0 keeps the GPU busy so that step 1, timed between two
-    // events, runs back to back like the graph's launches (a launch onto an
-    // idle GPU runs faster than the steady state); the graph holds CH-2.
+    // graph, each between two events (a deferral pair: together they move
+    // the average bytes of the steady state); the graph holds CH-2.
     auto chunk = [&]() {
         for (int i = prof ? 2 : 0; i < CH; ++i) step(i);
         if (o.rr_interval > 0) {
@@ -430,7 +432,7 @@ static void solve_group_fused(const Planes<T> &pl, const GroupDev &g, const CuPh
                 launch_assemble<T>(g.desc, g.blk, g.nblk, pl, rpw, st);
             }
             it(0, 1);
-            launch_reduce_it(g.desc, g.ninst, g.part4, g.scal, RED_START, o.max_iter, tol2, st);
+            launch_reduce_it(g.desc, g.ninst, g.part4, g.scal, RED_START, o.max_iter, tol2, st, sq ? 0 : 1);
             CUPHU_HIP_CHECK(hipMemcpyAsync(h.data(), g.scal, h.size() * sizeof(SolveScal), hipMemcpyDeviceToHost, st));
             CUPHU_HIP_CHECK(hipStreamSynchronize(st));
             int nch = 0;
@@ -443,10 +445,13 @@ static void solve_group_fused(const Planes<T> &pl, const GroupDev &g, const CuPh
                 if (prof) {
                     for (int i = 0; i < g.ninst; ++i)
                         if (h[(size_t)i].active) active_px += (double)g.px[(size_t)i];
-                    step(0);
                     CUPHU_HIP_CHECK(hipEventRecord(g_prof.ev[0], st));
-                    it(0, 0);
+                    it(1, 0);
                     CUPHU_HIP_CHECK(hipEventRecord(g_prof.ev[1], st));
+                    launch_reduce_it(g.desc, g.ninst, g.part4, g.scal, RED_IT, o.max_iter, tol2, st);
+                    CUPHU_HIP_CHECK(hipEventRecord(g_prof.ev[2], st));
+                    it(0, 0);
+                    CUPHU_HIP_CHECK(hipEventRecord(g_prof.ev[3], st));
                     launch_reduce_it(g.desc, g.ninst, g.part4, g.scal, RED_IT, o.max_iter, tol2, st);
                 }
                 if (gexec) CUPHU_HIP_CHECK(hipGraphLaunch(gexec, st));
@@ -460,18 +465,22 @@ static void solve_group_fused(const Planes<T> &pl, const GroupDev &g, const CuPh
                                     i, pass, h[i].iters, h[i].relres, h[i].delta,
                                     h[i].replace ? "replaced" : "kept");
                 if (prof) {
-                    float ms = 0.0f;
-                    CUPHU_HIP_CHECK(hipEventElapsedTime(&ms, g_prof.ev[0], g_prof.ev[1]));
+                    // both launches of a deferral pair (the first defers the
+                    // x update, the second applies two)
+                    float ms0 = 0.0f, ms1 = 0.0f;
+                    CUPHU_HIP_CHECK(hipEventElapsedTime(&ms0, g_prof.ev[0], g_prof.ev[1]));
+                    CUPHU_HIP_CHECK(hipEventElapsedTime(&ms1, g_prof.ev[2], g_prof.ev[3]));
                     CuPhuKernelProfile &a = g_prof.acc;
-                    a.samples += 1;
-                    a.px_sum += active_px;
-                    a.it_ms_sum += ms;
-                    a.it_bytes_sum += active_px * values_per_px * sizeof(T);
+                    a.samples += 2;
+                    a.px_sum += 2.0 * active_px;
+                    a.it_ms_sum += (double)ms0 + (double)ms1;
+                    a.it_bytes_sum += active_px * pair_values_per_px * sizeof(T);
                     a.iterations += CH;
                 }
                 ++nch;
                 if ((long long)nch * CH > (long long)o.max_iter + CH) break;   // cannot happen: reduce caps iterations
             }
+            if (!sq) launch_x_finalize<T>(g.desc, g.blk, g.nblk, pl, g.scal, rpw, st);
             if (o.verbose)
                 for (int i = 0; i < g.ninst; ++i)
                     fprintf(stderr, "[cuphu-pcg] inst %d pass %d iters %d relres %.3e status %d\n", i, pass, h[i].iters,
@@ -1813,6 +1822,7 @@ int cuphu_test_pcg_fused_step(const float *w, const float *dinv, int nrow, int n
         sc.beta = beta;
         sc.replace = rt ? 1 : 0;
         sc.active = 1;
+        sc.xmode = 2;   // apply this step's x update (no deferral)
         CUPHU_HIP_CHECK(hipMemcpyAsync(h.g.scal, &sc, sizeof(sc), hipMemcpyHostToDevice, st));
         // parity 0: reads set 0 (r, p, s), writes set 1 (r2, p2, q2)
         if (stored_q) launch_it<float>(h.g.desc, h.g.blk, h.g.nblk, h.P.pl, h.g.scal, h.g.part4, rpw, 0, start ? 1 : 0, st);

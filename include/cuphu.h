@@ -433,10 +433,13 @@ int cuphu_wls_adjoint_gpu(const void *d_w, const void *d_phase, const void *d_u,
  *       56 B in FP64);
  *   xr: (BATCHED) x += alpha p, r -= alpha q, r.z -- reads x, p, r, q,
  *       1/diag, writes x and r (28 B / 56 B per active pixel);
- *   it: (FUSED) one whole iteration -- reads x, r, p and two arc weights,
- *       writes x, r, p (32 B / 64 B per active pixel; q = L p and 1/diag are
- *       recomputed); with CUPHU_FUSED_STORED_Q=1 the kernel that also reads
- *       and writes q (40 B / 80 B).
+ *   it: (FUSED) one whole iteration -- q = L p and 1/diag are recomputed;
+ *       every other launch defers its x update to the next, so a pair of
+ *       launches reads r, p and two arc weights twice, x and the previous p
+ *       once, and writes r, p twice and x once (60 B / 120 B per active pixel
+ *       per pair, 30 B / 60 B per launch on average); both launches of the
+ *       pair are timed.  With CUPHU_FUSED_STORED_Q=1 the kernel that also
+ *       reads and writes q, without deferral (40 B / 80 B per launch).
  * cuphu_set_profiling resets the counters. */
 typedef struct CuPhuKernelProfile {
     long long samples;        /* timed iterations                          */
