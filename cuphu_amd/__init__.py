@@ -11,15 +11,17 @@ Quick start::
 The public names mirror the reference package ``cuphu``
 (src/cuphu/__init__.py): unwrap, regrow_conncomp, gpu_count, gpu_info,
 gpu_name, get_effective_looks.  Additions: ``unwrap_batch`` (many
-interferograms in one launch sequence, per-instance nlooks) and
-``WLSSolve`` (differentiable fixed-weight solve with an adjoint backward;
-needs torch).
+interferograms in one launch sequence, per-instance nlooks), ``WLSSolve``
+(differentiable fixed-weight solve with an adjoint backward; needs torch)
+and ``set_flow_solver`` (a host network-flow solver for init='mcf'/'mst',
+around which the MCF path's GPU stages run).
 """
 
 from cuphu_amd._unwrap import unwrap
 from cuphu_amd._conncomp import regrow_conncomp
 from cuphu_amd._gpu import gpu_count, gpu_info, gpu_name
 from cuphu_amd._looks import get_effective_looks
+from cuphu_amd.mcf import set_flow_solver
 
 __version__ = "0.1.0"
 
@@ -30,6 +32,7 @@ __all__ = [
     "gpu_info",
     "gpu_name",
     "regrow_conncomp",
+    "set_flow_solver",
     "unwrap",
     "unwrap_batch",
     "WLSSolve",

@@ -151,6 +151,8 @@ _SIGS = {
     "cuphu_test_pcg_step": (_i, [_P, _P, _i, _i, _d, _d, _i, _P, _P, _P, _P, C.POINTER(_d), C.POINTER(_d),
                                  C.POINTER(_d)]),
     "cuphu_test_pcg_fused_step": (_i, [_P, _P, _i, _i, _d, _d, _i, _P, _i, _i, _P, _P, _P, _P, _P]),
+    "cuphu_incrcost_gpu": (_i, [_P, _P, _i, _l, _i, _P, _P, C.POINTER(_i), _P]),
+    "cuphu_set_flow_solver": (_i, [_P, _P]),
 }
 
 EXPORTED = tuple(_SIGS)
@@ -182,6 +184,11 @@ class CuPhuError(RuntimeError):
         self.code = code
         msg = lib.cuphu_last_error().decode(errors="replace")
         super().__init__(f"{where} failed with status {code}: {msg}")
+
+
+# int (*)(const void *costs, int cost_mode, const float *phase, int nrow, int ncol, int init_method,
+#        short *flows_out, void *user)
+FLOW_SOLVER_FN = C.CFUNCTYPE(_i, C.c_void_p, _i, C.c_void_p, _i, _i, _i, C.c_void_p, C.c_void_p)
 
 
 def check(code: int, where: str) -> int:

@@ -5,7 +5,9 @@ NaN scrub, auto-tiling toward ``target_tile_size``, mask dilation for
 ``mask_pad_distance``, the reference-pixel integer-cycle shift and writing
 into caller-provided outputs.  The solve itself goes through the pybind11
 module to the gfx950 library.  Only ``init='laplace'`` is implemented;
-``'mcf'``/``'mst'`` need SNAPHU's CPU network-flow solver and raise.
+``'mcf'``/``'mst'`` need a network-flow solver registered with
+``cuphu_amd.set_flow_solver`` (SNAPHU's CPU solver is not part of this
+library) and run on one tile; without one they raise.
 
 One keyword is added: ``laplace_options`` (dict; keys n_irls, delta_max,
 delta_min, max_iter, tol, solver ('plain'|'batched'|'fused'), precision (32|64),

@@ -2,8 +2,9 @@
 // phase (reference: src/cxx/cuphu/cuphu_conncomp.cu, Laplace criterion).
 //
 // A pixel is valid when !(corr < 0.05) and mask != 0 (:102-114, :324-325).
-// Two valid 4-neighbours join when fabsf(unw_a - unw_b) < pi (:178-218; with
-// no MCF solve every arc is confident, :84-93).  The label of a component is
+// Two valid 4-neighbours join when fabsf(unw_a - unw_b) < pi (:178-218) and,
+// after an MCF solve, their arc is confident: min(poscost, negcost) >=
+// conncompthresh at the solved flows (:84-93; without costs every arc is).  The label of a component is
 // decided by its minimum raster index (the invariant of :16-20), components
 // smaller than max(1, (int)(minfrac*npix)) get 0, and the rest are numbered
 // 1..maxncomps in increasing root order (build_label_map :272-286).
@@ -33,6 +34,22 @@ __device__ __forceinline__ bool cc_valid(const float *corr, const unsigned char 
 __device__ __forceinline__ bool cc_join(const float *unw, long long a, long long b) {
     return fabsf(unw[a] - unw[b]) < kPiF;
 }
+// MCF path arc gate (arc_confident, cuphu_conncomp.cu:84-93): confident when
+// min(poscost, negcost) >= thresh; no costs = confident.  Arc indices are the
+// flat row-arc-then-column-arc layout (arc_index, :67-78).
+struct CcGate {
+    const short *pos, *neg;
+    long thresh;
+    long long nra;   // (nrow - 1) * ncol
+    int ncol;
+    __device__ __forceinline__ bool ok(long long a) const {
+        if (!pos) return true;
+        short m = pos[a] < neg[a] ? pos[a] : neg[a];
+        return (long)m >= thresh;
+    }
+    __device__ __forceinline__ bool down(int r, int c) const { return ok((long long)r * ncol + c); }
+    __device__ __forceinline__ bool right(int r, int c) const { return ok(nra + (long long)r * (ncol - 1) + c); }
+};
 
 __device__ __forceinline__ int lds_find(const int *par, int x) {
     int p = par[x];
@@ -52,7 +69,7 @@ __device__ void lds_union(int *par, int a, int b) {
 }
 
 __global__ __launch_bounds__(256) void k_cc_local(const float *unw, const float *corr, const unsigned char *mask,
-                                                  int nrow, int ncol, int stride, float th, uint32_t *lab,
+                                                  int nrow, int ncol, int stride, float th, CcGate gate, uint32_t *lab,
                                                   uint32_t *rootflag) {
     __shared__ int par[kCcN];
     int tx = blockIdx.x, ty = blockIdx.y;
@@ -67,8 +84,10 @@ __global__ __launch_bounds__(256) void k_cc_local(const float *unw, const float 
         if (par[l] < 0) continue;
         int lr = l / kCcTW, lc = l % kCcTW, r = r0 + lr, c = c0 + lc;
         long long g = (long long)r * stride + c;
-        if (lc + 1 < kCcTW && par[l + 1] >= 0 && cc_join(unw, g, g + 1)) lds_union(par, l, l + 1);
-        if (lr + 1 < kCcTH && par[l + kCcTW] >= 0 && cc_join(unw, g, g + stride)) lds_union(par, l, l + kCcTW);
+        if (lc + 1 < kCcTW && par[l + 1] >= 0 && cc_join(unw, g, g + 1) && gate.right(r, c))
+            lds_union(par, l, l + 1);
+        if (lr + 1 < kCcTH && par[l + kCcTW] >= 0 && cc_join(unw, g, g + stride) && gate.down(r, c))
+            lds_union(par, l, l + kCcTW);
     }
     __syncthreads();
     for (int l = threadIdx.x; l < kCcN; l += 256) {
@@ -108,16 +127,18 @@ __device__ void g_union(uint32_t *lab, uint32_t a, uint32_t b) {
 
 // Unions across tile borders: the right edge of a tile's last column and
 // the down edge of its last row.
-__global__ __launch_bounds__(256) void k_cc_border(const float *unw, int nrow, int ncol, int stride, uint32_t *lab) {
+__global__ __launch_bounds__(256) void k_cc_border(const float *unw, int nrow, int ncol, int stride, CcGate gate,
+                                                   uint32_t *lab) {
     long long i = (long long)blockIdx.x * 256 + threadIdx.x;
     if (i >= (long long)nrow * ncol) return;
     int r = (int)(i / ncol), c = (int)(i % ncol);
     long long g = (long long)r * stride + c;
     if (g_load(lab + g) == kInvalidLabel) return;
-    if ((c % kCcTW) == kCcTW - 1 && c + 1 < ncol && g_load(lab + g + 1) != kInvalidLabel && cc_join(unw, g, g + 1))
+    if ((c % kCcTW) == kCcTW - 1 && c + 1 < ncol && g_load(lab + g + 1) != kInvalidLabel && cc_join(unw, g, g + 1) &&
+        gate.right(r, c))
         g_union(lab, (uint32_t)g, (uint32_t)(g + 1));
     if ((r % kCcTH) == kCcTH - 1 && r + 1 < nrow && g_load(lab + g + stride) != kInvalidLabel &&
-        cc_join(unw, g, g + stride))
+        cc_join(unw, g, g + stride) && gate.down(r, c))
         g_union(lab, (uint32_t)g, (uint32_t)(g + stride));
 }
 
@@ -200,7 +221,8 @@ __global__ void k_cc_count(const uint32_t *rank, const uint32_t *flags, size_t n
 }
 
 int conncomp_run(const float *unw, const float *corr, const unsigned char *mask, int nrow, int ncol, int stride,
-                 double minfrac, long maxncomps, uint32_t *labels, hipStream_t st, float corr_thresh) {
+                 double minfrac, long maxncomps, uint32_t *labels, hipStream_t st, float corr_thresh,
+                 const short *poscost, const short *negcost, long conncompthresh) {
     if (nrow <= 0 || ncol <= 0) return 0;
     size_t n = (size_t)nrow * stride;
     size_t npx = (size_t)nrow * ncol;
@@ -211,9 +233,10 @@ int conncomp_run(const float *unw, const float *corr, const unsigned char *mask,
     dim3 tiles((unsigned)ceil_div((size_t)ncol, kCcTW), (unsigned)ceil_div((size_t)nrow, kCcTH));
     hipLaunchKernelGGL(k_cc_fill, dim3(g), dim3(256), 0, st, cnt.get(), n, 0u);
     if (stride != ncol) hipLaunchKernelGGL(k_cc_fill, dim3(g), dim3(256), 0, st, labels, n, kInvalidLabel);
-    hipLaunchKernelGGL(k_cc_local, tiles, dim3(256), 0, st, unw, corr, mask, nrow, ncol, stride, corr_thresh, labels,
-                       flags.get());
-    hipLaunchKernelGGL(k_cc_border, dim3(gp), dim3(256), 0, st, unw, nrow, ncol, stride, labels);
+    CcGate gate{poscost, negcost, conncompthresh, (long long)(nrow - 1) * ncol, ncol};
+    hipLaunchKernelGGL(k_cc_local, tiles, dim3(256), 0, st, unw, corr, mask, nrow, ncol, stride, corr_thresh, gate,
+                       labels, flags.get());
+    hipLaunchKernelGGL(k_cc_border, dim3(gp), dim3(256), 0, st, unw, nrow, ncol, stride, gate, labels);
     hipLaunchKernelGGL(k_cc_flatten, dim3(gp), dim3(256), 0, st, nrow, ncol, stride, flags.get(), labels);
     hipLaunchKernelGGL(k_cc_compress, dim3(gp), dim3(256), 0, st, nrow, ncol, stride, labels, cnt.get());
     int min_size = (int)(minfrac * (double)(int)npx);

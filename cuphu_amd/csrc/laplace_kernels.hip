@@ -1092,18 +1092,29 @@ __global__ __launch_bounds__(256) void k_iq(const InstDesc *descs, const int *bl
 
     // ── phase 1: loads; stage p_j ──
     T x4[RPW][4], r4[RPW][4], p4[RPW][4], wd[RPW][4], wr[RPW][4], wu[4], wlo[RPW], pp4[RPW][4];
+    // p_j and the x-update operands first: the x update (own pixels only)
+    // then waits for these loads alone and retires x and the previous p
+    // before the stencil phases
 #pragma unroll
     for (int k = 0; k < RPW; ++k) {
-        zero4(x4[k]); zero4(r4[k]); zero4(p4[k]); zero4(wd[k]); zero4(wr[k]); zero4(pp4[k]);
+        zero4(x4[k]); zero4(p4[k]); zero4(pp4[k]);
+        const int lr = lrow0 + k;
+        if (row_ok(lr) && cok) {
+            const long long e = elem(lr, c);
+            if (upd) ld4(p_in + e, p4[k]);
+            if (ax) ld4_nt(xg + e, x4[k]);
+            if (ax2) ld4_nt(p_out + e, pp4[k]);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < RPW; ++k) {
+        zero4(r4[k]); zero4(wd[k]); zero4(wr[k]);
         wlo[k] = T(0);
         const int lr = lrow0 + k;
         if (row_ok(lr)) {
             const long long e = elem(lr, c);
             if (cok) {
-                if (ax) ld4_nt(xg + e, x4[k]);
-                if (ax2) ld4_nt(p_out + e, pp4[k]);
                 ld4(r_src + e, r4[k]);
-                if (upd) ld4(p_in + e, p4[k]);
                 ld_arcs(w2, e, wd[k], wr[k]);
             }
             if (lane == 0 && t.tx0 > 0) wlo[k] = w2[2 * (e - 1) + 1];
@@ -1179,6 +1190,19 @@ __global__ __launch_bounds__(256) void k_iq(const InstDesc *descs, const int *bl
             if (useq) lp0[(lr + 2) * S + kLdsC0 + lcol2] = v2;
         }
     }
+    // x update of the own pixels
+    if (ax) {
+#pragma unroll
+        for (int k = 0; k < RPW; ++k) {
+            const int lr = lrow0 + k;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if (ax2) x4[k][j] = Prec<T>::fmaT(alp, pp4[k][j], x4[k][j]);
+                x4[k][j] = Prec<T>::fmaT(al, p4[k][j], x4[k][j]);
+            }
+            if (row_ok(lr) && cok) st4_nt(xg + elem(lr, c), x4[k]);
+        }
+    }
     // own p_j
     if (upd) {
 #pragma unroll
@@ -1209,14 +1233,11 @@ __global__ __launch_bounds__(256) void k_iq(const InstDesc *descs, const int *bl
         if (useq) stencil4(lp0 + S, lr, lane, wd[k], (k == 0) ? wu : wd[k > 0 ? k - 1 : 0], wr[k], wl[k], q);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            if (ax2) x4[k][j] = Prec<T>::fmaT(alp, pp4[k][j], x4[k][j]);
-            if (ax) x4[k][j] = Prec<T>::fmaT(al, p4[k][j], x4[k][j]);
             upd_px(r4[k][j], q[j], p4[k][j], di[k][j], rn[j], pn[j]);
             r4[k][j] = rn[j];
         }
         if (row_ok(lr) && cok) {
             const long long e = elem(lr, c);
-            if (ax) st4_nt(xg + e, x4[k]);
             st4_nt(r_out + e, rn);
             st4_nt(p_out + e, pn);
         }

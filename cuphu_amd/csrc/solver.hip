@@ -36,6 +36,10 @@ void set_last_error(const std::string &msg) { g_last_error = msg; }
 
 static std::recursive_mutex g_mu;   // serialises calls (shared workspace and stream)
 
+// Host network-flow solver for init 'mcf'/'mst' (cuphu_set_flow_solver).
+static cuphu_flow_solver_fn g_flow_solver = nullptr;
+static void *g_flow_solver_user = nullptr;
+
 // ── per-device cached resources ─────────────────────────────────────────────
 struct DevRes {
     hipStream_t stream = nullptr;
@@ -1139,6 +1143,13 @@ void cuphu_default_bridge_params(CuPhuBridgeParams *bp) {
     bp->ramp_max_num_sample = 1000000;
 }
 
+int cuphu_set_flow_solver(cuphu_flow_solver_fn fn, void *user) {
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    g_flow_solver = fn;
+    g_flow_solver_user = fn ? user : nullptr;
+    return 0;
+}
+
 int cuphu_set_profiling(int enabled) {
     std::lock_guard<std::recursive_mutex> lk(g_mu);
     g_prof.on = enabled != 0;
@@ -1270,6 +1281,81 @@ int cuphu_unwrap_scene_gpu(const float *d_igram, const float *d_corr, const unsi
     });
 }
 
+int cuphu_incrcost_gpu(const CuPhuSmoothCost *d_costs, const short *d_flows, int narcs, long nshortcycle, int nflow,
+                       short *d_poscost, short *d_negcost, int *no_improvement, cuphu_stream_t stream) {
+    return guarded([&]() {
+        std::lock_guard<std::recursive_mutex> lk(g_mu);
+        CUPHU_REQUIRE(narcs >= 0 && nflow >= 1 && nshortcycle >= 1, "invalid argument");
+        CUPHU_REQUIRE(narcs == 0 || (d_costs && d_flows && d_poscost && d_negcost), "null argument");
+        DevRes &res = devres();
+        StreamJoin join(res, (hipStream_t)stream);
+        int *mins = ws<int>(res, "incrmins", 2);
+        int flag = util_incrcost(reinterpret_cast<const short *>(d_costs), d_flows, (size_t)narcs, nshortcycle, nflow,
+                                 d_poscost, d_negcost, mins, res.stream);
+        if (no_improvement) *no_improvement = flag;
+        return 0;
+    });
+}
+
+// ── MCF / MST initialisation through a registered host solver ──────────────
+
+__global__ void k_mask_arc_weights(const unsigned char *mask, int nrow, int ncol, short *w) {
+    long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    long long nra = (long long)(nrow - 1) * ncol, na = nra + (long long)nrow * (ncol - 1);
+    if (i >= na) return;
+    long long t, h;
+    if (i < nra) { t = i; h = i + ncol; }
+    else { long long j = i - nra, r = j / (ncol - 1), c = j % (ncol - 1); t = r * ncol + c; h = t + 1; }
+    w[i] = (mask[t] != 0 && mask[h] != 0) ? 1 : 0;
+}
+
+// One tile on the MCF path (cuphu_solver.cpp:370-510 costs, :512-880 the
+// CPU network-flow solve, :900-993 the GPU tail): wrapped phase and SNAPHU
+// costs on the GPU, the registered host solver for the integer flows, then
+// on the GPU the incremental costs at the solved flows (smooth costs), the
+// flow integration and the connected components gated by the incremental
+// costs.  Inputs and outputs are device arrays.
+static void mcf_tile(const float2 *d_ig, const float *d_corr, const unsigned char *d_mask, int nrow, int ncol,
+                     CuPhuCostMode cost_mode, CuPhuInitMethod init_meth, const CuPhuParams &prm, float *d_unw,
+                     uint32_t *d_cc, DevRes &res, hipStream_t st) {
+    size_t n = (size_t)nrow * ncol;
+    size_t nra = (size_t)(nrow - 1) * ncol, na = nra + (size_t)nrow * (ncol - 1);
+    int defo = cost_mode != CUPHU_COST_SMOOTH;
+    size_t esz = defo ? 8 : 4;
+    DevBuf<float> d_ph(n);
+    DevBuf<short> d_w(d_mask ? na + 1 : 0), d_costs(na * esz / 2 + 2), d_flows(na + 1), d_pos(na + 1), d_neg(na + 1);
+    phase_from_igram(d_ig, d_ph.get(), n, st);
+    if (d_mask && na) {
+        hipLaunchKernelGGL(k_mask_arc_weights, dim3((unsigned)ceil_div(na, 256)), dim3(256), 0, st, d_mask, nrow, ncol,
+                           d_w.get());
+        CUPHU_HIP_CHECK(hipGetLastError());
+    }
+    CostConsts k = make_cost_consts(prm, prm.ncorrlooks);
+    util_costs(d_ph.get(), d_corr, d_mask ? d_w.get() : nullptr, nrow, ncol, k, prm.kperpdpsi, prm.kpardpsi, defo,
+               d_costs.get(), st);
+    std::vector<unsigned char> h_costs(na * esz + 8);
+    std::vector<float> h_ph(n);
+    std::vector<short> h_flows(na + 1, 0);
+    if (na) CUPHU_HIP_CHECK(hipMemcpyAsync(h_costs.data(), d_costs.get(), na * esz, hipMemcpyDeviceToHost, st));
+    CUPHU_HIP_CHECK(hipMemcpyAsync(h_ph.data(), d_ph.get(), n * 4, hipMemcpyDeviceToHost, st));
+    CUPHU_HIP_CHECK(hipStreamSynchronize(st));
+    int rc = g_flow_solver(h_costs.data(), (int)cost_mode, h_ph.data(), nrow, ncol, (int)init_meth, h_flows.data(),
+                           g_flow_solver_user);
+    if (rc != 0) throw Error(CUPHU_ERR_INTERNAL, "the registered network-flow solver failed (" + std::to_string(rc) + ")");
+    if (na) CUPHU_HIP_CHECK(hipMemcpyAsync(d_flows.get(), h_flows.data(), na * 2, hipMemcpyHostToDevice, st));
+    const short *pos = nullptr, *neg = nullptr;
+    if (!defo) {
+        int *mins = ws<int>(res, "incrmins", 2);
+        util_incrcost(d_costs.get(), d_flows.get(), na, (long)prm.nshortcycle, 1, d_pos.get(), d_neg.get(), mins, st);
+        pos = d_pos.get();
+        neg = d_neg.get();
+    }
+    util_integrate(d_ph.get(), d_flows.get(), d_flows.get() + nra, nrow, ncol, d_unw, st);
+    conncomp_run(d_unw, d_corr, d_mask, nrow, ncol, ncol, prm.minconncompfrac, prm.maxncomps, d_cc, st, 0.05f, pos, neg,
+                 prm.conncompthresh);
+    CUPHU_HIP_CHECK(hipStreamSynchronize(st));
+}
+
 int cuphu_unwrap_ex(const float *igram_r, const float *igram_i, const float *igram_c64, const float *corr,
                     const unsigned char *mask, int nrow, int ncol, CuPhuCostMode cost_mode, CuPhuInitMethod init_meth,
                     const CuPhuParams *params, const CuPhuTileParams *tile, const CuPhuBridgeParams *bridge,
@@ -1284,9 +1370,13 @@ int cuphu_unwrap_ex(const float *igram_r, const float *igram_i, const float *igr
         result->conncomp = nullptr;
         result->nrow = nrow;
         result->ncol = ncol;
-        if (init_meth != CUPHU_INIT_LAPLACE)
+        const bool mcf = init_meth != CUPHU_INIT_LAPLACE;
+        if (mcf && !g_flow_solver)
             throw Error(CUPHU_ERR_UNSUPPORTED,
-                        "init 'mcf'/'mst' needs SNAPHU's CPU network-flow solver, which is not part of this library");
+                        "init 'mcf'/'mst' needs a network-flow solver (SNAPHU's CPU solver is not part of this "
+                        "library); register one with cuphu_set_flow_solver");
+        if (mcf && tile->ntilerow * tile->ntilecol > 1)
+            throw Error(CUPHU_ERR_UNSUPPORTED, "init 'mcf'/'mst' runs on one tile here (ntiles = (1, 1))");
         CUPHU_REQUIRE(cost_mode >= CUPHU_COST_SMOOTH && cost_mode <= CUPHU_COST_TOPO, "invalid cost mode");
         if (bridge && bridge->enabled) throw Error(CUPHU_ERR_UNSUPPORTED, "phase bridging is not implemented");
         if (tile->single_tile_reoptimize && tile->ntilerow * tile->ntilecol > 1)
@@ -1332,7 +1422,10 @@ int cuphu_unwrap_ex(const float *igram_r, const float *igram_i, const float *igr
             const unsigned char *conf = orig_mask ? d_orig.get() : nullptr;
             int single = (tile->ntilerow == 1 && tile->ntilecol == 1);
             int rov = single ? 0 : tile->rowovrlp, cov = single ? 0 : tile->colovrlp;
-            if (o.precision == 64)
+            if (mcf)
+                mcf_tile(d_ig.get(), d_corr.get(), mask ? d_mask.get() : nullptr, nrow, ncol, cost_mode, init_meth,
+                         *params, d_unw.get(), d_cc.get(), res, st);
+            else if (o.precision == 64)
                 scene_impl<double>(d_ig.get(), d_corr.get(), mask ? d_mask.get() : nullptr, conf, nrow, ncol,
                                    params->ncorrlooks, *params, 0, rov, cov, tile->ntilerow, tile->ntilecol, o,
                                    d_unw.get(), d_cc.get(), stats, nullptr, res, st);
@@ -1518,11 +1611,10 @@ int cuphu_conncomp_gpu(const float *d_unw, const float *d_corr, const unsigned c
                        uint32_t *d_labels_out) {
     return guarded([&]() {
         CUPHU_REQUIRE(d_unw && d_corr && params && d_labels_out && nrow >= 1 && ncol >= 1, "invalid argument");
-        if (d_poscost || d_negcost)
-            throw Error(CUPHU_ERR_UNSUPPORTED, "incremental-cost gating needs an MCF solve (not in this library)");
+        CUPHU_REQUIRE((d_poscost == nullptr) == (d_negcost == nullptr), "poscost and negcost go together");
         DeviceGuard dg(gpu_id);
         conncomp_run(d_unw, d_corr, d_mask, nrow, ncol, ncol, params->minconncompfrac, params->maxncomps, d_labels_out,
-                     nullptr);
+                     nullptr, 0.05f, d_poscost, d_negcost, params->conncompthresh);
         return 0;
     });
 }

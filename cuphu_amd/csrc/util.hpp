@@ -19,7 +19,17 @@ void util_integrate(const float *ph, const short *hflows, const short *vflows, i
 // Connected components of one image (stride = elements per row of the unw,
 // corr, mask and label arrays).  Returns the number of labels assigned
 // (labels are 1..K).  Scratch is allocated internally.
+// poscost/negcost (flat arc layout, both or neither): the MCF path's arc gate
+// min(poscost, negcost) >= conncompthresh.
 int conncomp_run(const float *unw, const float *corr, const unsigned char *mask, int nrow, int ncol, int stride,
-                 double minfrac, long maxncomps, uint32_t *labels, hipStream_t st, float corr_thresh = 0.05f);
+                 double minfrac, long maxncomps, uint32_t *labels, hipStream_t st, float corr_thresh = 0.05f,
+                 const short *poscost = nullptr, const short *negcost = nullptr, long conncompthresh = 0);
+
+// Incremental costs of smooth-cost arcs at the given flows (SNAPHU
+// CalcCostSmooth for a +-nflow change; costs are {offset, sigsq} pairs).
+// mins: 2 device ints of scratch.  Returns 1 when min(poscost) >= 0 and
+// min(negcost) >= 0 (no improving unit augmentation), else 0.  Synchronous.
+int util_incrcost(const short *costs, const short *flows, size_t narcs, long nshortcycle, int nflow, short *poscost,
+                  short *negcost, int *mins, hipStream_t st);
 
 }  // namespace cuphu

@@ -272,4 +272,65 @@ void util_integrate(const float *ph, const short *hflows, const short *vflows, i
     CUPHU_HIP_CHECK(hipStreamSynchronize(st));
 }
 
+// ── incremental costs (cuphu_incrcost.cu:55-191, SNAPHU CalcCostSmooth) ────
+__device__ __forceinline__ long long ic_ceil_div(long long a, long long b) { return (a > 0) ? (a + b - 1) / b : a / b; }
+__device__ __forceinline__ long long ic_floor_div(long long a, long long b) { return (a >= 0) ? a / b : (a - b + 1) / b; }
+
+__global__ void k_fill_i32(int *p, int n, int v) {
+    int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) p[i] = v;
+}
+
+__global__ __launch_bounds__(256) void k_incrcost(const short *costs, const short *flows, size_t narcs,
+                                                  long long nsc, int nflow, short *pos, short *neg, int *mins) {
+    int mp = 32000, mn = 32000;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < narcs; i += (size_t)gridDim.x * 256) {
+        const long long off = costs[2 * i], sig = costs[2 * i + 1], f = flows[i];
+        long long pc = 0, nc = 0;
+        if (sig != 32000) {
+            const long long nfl = nflow;
+            long long idz1 = f * nsc + off, idp = (f + nfl) * nsc + off, idn = (f - nfl) * nsc + off;
+            idz1 = idz1 < 0 ? -idz1 : idz1;
+            idp = idp < 0 ? -idp : idp;
+            idn = idn < 0 ? -idn : idn;
+            const long long c1 = (idz1 * idz1) / sig;
+            const long long pr = (idp * idp) / sig - c1, nr = (idn * idn) / sig - c1;
+            const long long nsq = nfl * nfl;
+            pc = pr > 0 ? ic_ceil_div(pr, nsq) : ic_floor_div(pr, nsq);
+            nc = nr > 0 ? ic_ceil_div(nr, nsq) : ic_floor_div(nr, nsq);
+            pc = pc > 32000 ? 32000 : (pc < -32000 ? -32000 : pc);
+            nc = nc > 32000 ? 32000 : (nc < -32000 ? -32000 : nc);
+        }
+        pos[i] = (short)pc;
+        neg[i] = (short)nc;
+        mp = min(mp, (int)pc);
+        mn = min(mn, (int)nc);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        mp = min(mp, __shfl_xor(mp, o, 64));
+        mn = min(mn, __shfl_xor(mn, o, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMin(mins, mp);
+        atomicMin(mins + 1, mn);
+    }
+}
+
+int util_incrcost(const short *costs, const short *flows, size_t narcs, long nshortcycle, int nflow, short *poscost,
+                  short *negcost, int *mins, hipStream_t st) {
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, mins, 2, 32000);
+    CUPHU_HIP_CHECK(hipGetLastError());
+    if (narcs) {
+        unsigned g = (unsigned)std::min<size_t>(ceil_div(narcs, 256), 65535);
+        hipLaunchKernelGGL(k_incrcost, dim3(g), dim3(256), 0, st, costs, flows, narcs, (long long)nshortcycle, nflow,
+                           poscost, negcost, mins);
+        CUPHU_HIP_CHECK(hipGetLastError());
+    }
+    int h[2] = {0, 0};
+    CUPHU_HIP_CHECK(hipMemcpyAsync(h, mins, sizeof(h), hipMemcpyDeviceToHost, st));
+    CUPHU_HIP_CHECK(hipStreamSynchronize(st));
+    return (h[0] >= 0 && h[1] >= 0) ? 1 : 0;
+}
+
 }  // namespace cuphu

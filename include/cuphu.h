@@ -209,13 +209,46 @@ int cuphu_integrate_phase_gpu(const float *d_phase, const short *d_hflows,
                               int gpu_id, float *d_unw);
 
 /* Connected components (ref cuphu.h:275-289, cuphu_conncomp.cu:289).
- * Device pointers.  d_poscost/d_negcost must be NULL (Laplace criterion:
- * every arc confident); a non-NULL pair returns CUPHU_ERR_UNSUPPORTED. */
+ * Device pointers.  d_poscost/d_negcost: NULL for the Laplace criterion
+ * (every arc confident), or both set (flat arc layout, from
+ * cuphu_incrcost_gpu at the solved flows) to join only arcs with
+ * min(poscost, negcost) >= params->conncompthresh. */
 int cuphu_conncomp_gpu(const float *d_unw, const float *d_corr,
                        const unsigned char *d_mask, int nrow, int ncol,
                        const short *d_poscost, const short *d_negcost,
                        const CuPhuParams *params, int gpu_id,
                        uint32_t *d_labels_out);
+
+/* Incremental arc costs at the given flows (ref cuphu_incrcost.cu:146-191,
+ * cuphu_incrcost_early_exit): SNAPHU CalcCostSmooth of every arc for a
+ * +-nflow flow change, into caller-allocated d_poscost/d_negcost [narcs]
+ * (flat row-arc-then-column-arc layout, like d_costs and d_flows).
+ * *no_improvement (nullable) = 1 when min(poscost) >= 0 and
+ * min(negcost) >= 0: no unit augmentation lowers the cost, so the caller may
+ * skip its network-flow solve.  Smooth costs only, as in the reference. */
+int cuphu_incrcost_gpu(const CuPhuSmoothCost *d_costs, const short *d_flows,
+                       int narcs, long nshortcycle, int nflow,
+                       short *d_poscost, short *d_negcost, int *no_improvement,
+                       cuphu_stream_t stream);
+
+/* Host network-flow solver for init 'mcf'/'mst'.  The reference links
+ * SNAPHU's CPU MCFInitFlows/MSTInitFlows and TreeSolve here
+ * (cuphu_solver.cpp:512-880); that code is not part of this library, so a
+ * caller registers a solver.  costs: host array of narcs CuPhuSmoothCost
+ * (cost_mode smooth) or CuPhuCost (defo/topo), flat row-arc-then-column-arc;
+ * phase: wrapped phase [nrow*ncol]; init_method: CUPHU_INIT_MCF or
+ * CUPHU_INIT_MST; flows_out: narcs shorts, same arc layout.  Return 0 on
+ * success.  With a solver registered, cuphu_unwrap/cuphu_unwrap_ex with
+ * init 'mcf'/'mst' (one tile) run the reference's GPU sequence around it:
+ * costs -> solver -> incremental costs at the solved flows (smooth) ->
+ * flow integration -> connected components gated by the incremental costs
+ * (min(poscost, negcost) >= params->conncompthresh).  Without one they
+ * return CUPHU_ERR_UNSUPPORTED. */
+typedef int (*cuphu_flow_solver_fn)(const void *costs, int cost_mode, const float *phase,
+                                   int nrow, int ncol, int init_method, short *flows_out,
+                                   void *user);
+/* Register (fn != NULL) or clear (fn == NULL) the process-wide solver. */
+int cuphu_set_flow_solver(cuphu_flow_solver_fn fn, void *user);
 
 /* Device-pointer building blocks (ref cuphu_util.cu:187-287). */
 int cuphu_wrap_phase(float *d_phase, int n, cuphu_stream_t stream);

@@ -223,13 +223,32 @@ def pcg_fused_step_f32(w, dinv, alpha, beta, x, r, p, q, start=False, rt=None):
     return x, r, p, q, sums
 
 
-def conncomp(unw, corr, mask=None, minfrac=0.01, maxncomps=32) -> np.ndarray:
+def conncomp(unw, corr, mask=None, minfrac=0.01, maxncomps=32, poscost=None, negcost=None,
+             conncompthresh=300) -> np.ndarray:
+    """Connected components; with poscost/negcost (flat arc arrays), the MCF
+    path's incremental-cost gate min(pos, neg) >= conncompthresh."""
     unw, corr = _f32(unw), _f32(corr)
     m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
+    pc = None if poscost is None else np.ascontiguousarray(poscost, dtype=np.int16)
+    nc = None if negcost is None else np.ascontiguousarray(negcost, dtype=np.int16)
     out = np.empty(unw.shape, np.uint32)
-    _lib.o_conncomp(_p(unw), _p(corr), _p(m), C.c_int(unw.shape[0]), C.c_int(unw.shape[1]),
-                    C.c_double(minfrac), C.c_long(maxncomps), _p(out))
+    _lib.o_conncomp_gated(_p(unw), _p(corr), _p(m), C.c_int(unw.shape[0]), C.c_int(unw.shape[1]),
+                          C.c_double(minfrac), C.c_long(maxncomps), _p(pc), _p(nc), C.c_long(conncompthresh),
+                          _p(out))
     return out
+
+
+def incrcost_smooth(costs, flows, nshortcycle=200, nflow=1):
+    """Incremental costs of smooth-cost arcs at the given flows.  costs:
+    int16 [narcs, 2] (offset, sigsq); flows int16 [narcs].  Returns
+    (poscost, negcost, early_exit)."""
+    cs = np.ascontiguousarray(costs, dtype=np.int16)
+    fl = np.ascontiguousarray(flows, dtype=np.int16)
+    n = fl.size
+    pc = np.empty(n, np.int16)
+    nc = np.empty(n, np.int16)
+    ee = _lib.o_incrcost_smooth(_p(cs), _p(fl), C.c_size_t(n), C.c_long(nshortcycle), C.c_int(nflow), _p(pc), _p(nc))
+    return pc, nc, bool(ee)
 
 
 def integrate_phase(phi, hflows, vflows) -> np.ndarray:
@@ -309,7 +328,7 @@ def _sig(fn, res, args):
 
 for _n in ("o_set_threads", "o_wrapped_phase", "o_phase_diffs", "o_boxcar2d", "o_costs", "o_residues",
            "o_arc_weights", "o_assemble", "o_matvec", "o_reweight", "o_congruence", "o_pcg_irls_f32",
-           "o_pcg_irls_f64", "o_pcg_step_f32", "o_pcg_fused_step_f32", "o_conncomp", "o_integrate_phase", "o_tile_axis", "o_stitch",
+           "o_pcg_irls_f64", "o_pcg_step_f32", "o_pcg_fused_step_f32", "o_conncomp", "o_conncomp_gated", "o_incrcost_smooth", "o_integrate_phase", "o_tile_axis", "o_stitch",
            "o_wls_solve_f64", "o_wls_adjoint_f64"):
     getattr(_lib, _n).restype = C.c_int if _n not in ("o_set_threads", "o_wrapped_phase", "o_phase_diffs",
                                                        "o_costs", "o_residues", "o_arc_weights",

@@ -612,8 +612,26 @@ static void uf_union(uint32_t *par, uint32_t a, uint32_t b) {
     if (a == b) return;
     if (a < b) par[b] = a; else par[a] = b;
 }
+/* Incremental-cost gate of an arc (arc_confident, cuphu_conncomp.cu:84-93):
+ * confident when min(poscost, negcost) >= thresh; no costs = confident. */
+static int o_arc_ok(const short *pos, const short *neg, size_t a, long thresh) {
+    if (!pos) return 1;
+    short m = pos[a] < neg[a] ? pos[a] : neg[a];
+    return (long)m >= thresh;
+}
+int o_conncomp_gated(const float *unw, const float *corr, const unsigned char *mask, int nrow, int ncol,
+                     double minfrac, long maxncomps, const short *poscost, const short *negcost,
+                     long conncompthresh, uint32_t *labels);
 int o_conncomp(const float *unw, const float *corr, const unsigned char *mask, int nrow, int ncol,
                double minfrac, long maxncomps, uint32_t *labels) {
+    return o_conncomp_gated(unw, corr, mask, nrow, ncol, minfrac, maxncomps, NULL, NULL, 0, labels);
+}
+/* The same with the MCF path's arc gate: row arc (r,c)-(r+1,c) at index
+ * r*ncol + c, column arc (r,c)-(r,c+1) at (nrow-1)*ncol + r*(ncol-1) + c
+ * (arc_index, cuphu_conncomp.cu:67-78). */
+int o_conncomp_gated(const float *unw, const float *corr, const unsigned char *mask, int nrow, int ncol,
+                     double minfrac, long maxncomps, const short *poscost, const short *negcost,
+                     long conncompthresh, uint32_t *labels) {
     size_t n = (size_t)nrow * ncol;
     uint32_t *par = (uint32_t *)malloc(n * sizeof(uint32_t));
     uint32_t *cnt = (uint32_t *)calloc(n, sizeof(uint32_t));
@@ -630,8 +648,13 @@ int o_conncomp(const float *unw, const float *corr, const unsigned char *mask, i
         for (int c = 0; c < ncol; ++c) {
             size_t i = (size_t)r * ncol + c;
             if (!valid[i]) continue;
-            if (c + 1 < ncol && valid[i + 1] && fabsf(unw[i] - unw[i + 1]) < pth) uf_union(par, (uint32_t)i, (uint32_t)(i + 1));
-            if (r + 1 < nrow && valid[i + ncol] && fabsf(unw[i] - unw[i + ncol]) < pth) uf_union(par, (uint32_t)i, (uint32_t)(i + ncol));
+            size_t ra = (size_t)r * ncol + c, ca = (size_t)(nrow - 1) * ncol + (size_t)r * (ncol - 1) + c;
+            if (c + 1 < ncol && valid[i + 1] && fabsf(unw[i] - unw[i + 1]) < pth &&
+                o_arc_ok(poscost, negcost, ca, conncompthresh))
+                uf_union(par, (uint32_t)i, (uint32_t)(i + 1));
+            if (r + 1 < nrow && valid[i + ncol] && fabsf(unw[i] - unw[i + ncol]) < pth &&
+                o_arc_ok(poscost, negcost, ra, conncompthresh))
+                uf_union(par, (uint32_t)i, (uint32_t)(i + ncol));
         }
     for (size_t i = 0; i < n; ++i) if (valid[i]) cnt[uf_find(par, (uint32_t)i)]++;
     int min_size = (int)(minfrac * (double)(int)n);
@@ -643,6 +666,46 @@ int o_conncomp(const float *unw, const float *corr, const unsigned char *mask, i
     for (size_t i = 0; i < n; ++i) labels[i] = valid[i] ? map[uf_find(par, (uint32_t)i)] : 0u;
     free(par); free(cnt); free(valid); free(map);
     return 0;
+}
+
+/* ── MCF path: incremental arc costs (ComputeIncrCostsKernel,
+ * cuphu_incrcost.cu:55-98, SNAPHU CalcCostSmooth) ───────────────────────────
+ * costs: smoothcostT pairs {offset, sigsq}.  sigsq == 32000 (LARGESHORT): 0.
+ * idz1 = |f*nsc + off|, idz2pos/neg = |(f +- nflow)*nsc + off|;
+ * cost1 = idz1^2 / sigsq (integer division), raw = idz2^2 / sigsq - cost1;
+ * result = ceil(raw / nflow^2) if raw > 0 else floor(raw / nflow^2), clamped
+ * to [-32000, 32000].  Returns 1 when min(poscost) >= 0 and min(negcost) >= 0
+ * over all arcs (the early-exit test of cuphu_incrcost_early_exit, :146-191;
+ * narcs == 0 gives 1). */
+static long o_ceil_div(long a, long b) { return (a > 0L) ? (a + b - 1L) / b : a / b; }
+static long o_floor_div(long a, long b) { return (a >= 0L) ? a / b : (a - b + 1L) / b; }
+int o_incrcost_smooth(const short *costs, const short *flows, size_t narcs, long nshortcycle, int nflow,
+                      short *poscost, short *negcost) {
+    int minp = 32000, minn = 32000;
+    for (size_t i = 0; i < narcs; ++i) {
+        long off = costs[2 * i], sig = costs[2 * i + 1], f = flows[i];
+        long pc = 0, nc = 0;
+        if (sig != 32000) {
+            long nfl = nflow;
+            long idz1 = labs(f * nshortcycle + off);
+            long idp = labs((f + nfl) * nshortcycle + off);
+            long idn = labs((f - nfl) * nshortcycle + off);
+            long c1 = (idz1 * idz1) / sig;
+            long pr = (idp * idp) / sig - c1, nr = (idn * idn) / sig - c1;
+            long nsq = nfl * nfl;
+            pc = pr > 0L ? o_ceil_div(pr, nsq) : o_floor_div(pr, nsq);
+            nc = nr > 0L ? o_ceil_div(nr, nsq) : o_floor_div(nr, nsq);
+            if (pc > 32000L) pc = 32000L;
+            if (pc < -32000L) pc = -32000L;
+            if (nc > 32000L) nc = 32000L;
+            if (nc < -32000L) nc = -32000L;
+        }
+        poscost[i] = (short)pc;
+        negcost[i] = (short)nc;
+        if ((int)pc < minp) minp = (int)pc;
+        if ((int)nc < minn) minn = (int)nc;
+    }
+    return (minp >= 0 && minn >= 0) ? 1 : 0;
 }
 
 /* ── a13: integer flows -> phase (cuphu_integrate_phase_gpu cuphu_phase.cu) ──

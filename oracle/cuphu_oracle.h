@@ -49,6 +49,13 @@ int o_pcg_step_f32(const float *w, const float *dinv, int nrow, int ncol, double
 /* one fused PCG iteration (the batched solver's one-kernel form), FP32 vectors */
 void o_pcg_fused_step_f32(const float *w, const float *dinv, int nrow, int ncol, double alpha, double beta,
                           int start, const float *rt, float *x, float *r, float *p, float *q, double *sums);
+/* MCF path: incremental arc costs at the given flows; returns the early-exit flag */
+int o_incrcost_smooth(const short *costs, const short *flows, size_t narcs, long nshortcycle, int nflow,
+                      short *poscost, short *negcost);
+/* a14 with the MCF path's incremental-cost arc gate (costs NULL = no gate) */
+int o_conncomp_gated(const float *unw, const float *corr, const unsigned char *mask, int nrow, int ncol,
+                     double minfrac, long maxncomps, const short *poscost, const short *negcost,
+                     long conncompthresh, uint32_t *labels);
 /* a14 */ int o_conncomp(const float *unw, const float *corr, const unsigned char *mask, int nrow,
                          int ncol, double minfrac, long maxncomps, uint32_t *labels);
 /* a13 */ void o_integrate_phase(const float *phi, const short *hflows, const short *vflows,

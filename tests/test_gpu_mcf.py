@@ -1,0 +1,148 @@
+"""MCF path's GPU stages (incremental costs, cost-gated connected components,
+flow integration) and the host network-flow solver hook, against the oracle.
+The network-flow solve itself (SNAPHU's CPU code in the reference) is not
+part of this library; the tests register stand-in solvers whose flows are
+known: zero flows on a residue-free ramp (the optimum there) and a fixed
+integer flow field."""
+
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import cuphu_amd as cuphu
+import oracle as O
+from cuphu_amd import _capi, synth
+from cuphu_amd.mcf import incremental_costs
+from tests import gpu_util as G
+
+pytestmark = pytest.mark.gpu
+TWOPI = 2 * np.pi
+
+
+@pytest.fixture
+def solver_slot():
+    yield
+    cuphu.set_flow_solver(None)
+
+
+@pytest.mark.parametrize("nflow", [1, 2])
+def test_incremental_costs_bitexact(gpu_ready, nflow):
+    torch = G.torch_mod()
+    rng = np.random.default_rng(nflow)
+    n = 200_003
+    costs = np.stack([rng.integers(-3000, 3000, n), rng.integers(1, 32000, n)], 1).astype(np.int16)
+    costs[::101, 1] = 32000
+    flows = rng.integers(-20, 20, n).astype(np.int16)
+    dc, df = G.dev(costs), G.dev(flows)
+    pos, neg, flag = incremental_costs(dc, df, nshortcycle=200, nflow=nflow)
+    torch.cuda.synchronize()
+    rp, rn, rflag = O.incrcost_smooth(costs, flows, nshortcycle=200, nflow=nflow)
+    np.testing.assert_array_equal(G.host(pos), rp)
+    np.testing.assert_array_equal(G.host(neg), rn)
+    assert flag == rflag
+    # the early-exit flag on an all-non-negative case
+    z = np.stack([np.zeros(1000, np.int16), np.full(1000, 300, np.int16)], 1)
+    _, _, f2 = incremental_costs(G.dev(z), G.dev(np.zeros(1000, np.int16)))
+    assert f2
+
+
+@pytest.mark.parametrize("shape", [(23, 31), (130, 300)])
+def test_conncomp_gated_bitexact(gpu_ready, shape):
+    torch = G.torch_mod()
+    rng = np.random.default_rng(shape[0])
+    unw = np.cumsum(rng.normal(0, 0.9, shape), axis=1).astype(np.float32)
+    corr = rng.uniform(0, 1, shape).astype(np.float32)
+    na = O.narcs(*shape)
+    pos = rng.integers(0, 900, na).astype(np.int16)
+    neg = rng.integers(0, 900, na).astype(np.int16)
+    prm = _capi.default_params()
+    prm.minconncompfrac = 0.0
+    prm.maxncomps = 100000
+    du, dc, dp, dn = G.dev(unw), G.dev(corr), G.dev(pos), G.dev(neg)
+    lab = G.empty(shape, torch.int32)
+    _capi.check(_capi.lib.cuphu_conncomp_gpu(G.p(du), G.p(dc), None, shape[0], shape[1], G.p(dp), G.p(dn),
+                                             C.byref(prm), 0, G.p(lab)), "conncomp")
+    ref = O.conncomp(unw, corr, None, 0.0, 100000, poscost=pos, negcost=neg, conncompthresh=prm.conncompthresh)
+    np.testing.assert_array_equal(G.host(lab).view(np.uint32), ref)
+
+
+def test_mcf_zero_flows_on_ramp_is_closed_form(gpu_ready, solver_slot):
+    """Config 1's 'mcf' leg: the 256x256 ramp has no residues, so zero flows
+    are the network-flow optimum; the GPU stages must then give the closed
+    form unw = pi(x+y) + 6pi with one component."""
+    s = synth.ramp_256()
+    seen = {}
+
+    def zero(costs, phase, init):
+        seen.update(costs=costs.shape, phase=phase.shape, init=init)
+        return np.zeros(costs.shape[0], np.int16)
+
+    cuphu.set_flow_solver(zero)
+    unw, cc = cuphu.unwrap(s.igram, s.corr, 1.0, init="mcf")
+    assert seen == {"costs": (O.narcs(256, 256), 2), "phase": (256, 256), "init": "mcf"}
+    assert np.abs(unw - s.truth).max() < 1e-3
+    assert (cc == 1).all()
+
+
+def test_mcf_flows_integrated_and_conncomp_gated(gpu_ready, solver_slot):
+    """A fixed integer flow field: the unwrapped phase is the oracle's
+    integration of those flows, and the components are the oracle's with
+    the incremental-cost gate at those flows."""
+    s = synth.make_scene(90, 120, nbumps=2, npatches=1, nlooks=5.0, seed=8, patch=25)
+    nrow, ncol = s.corr.shape
+    na = O.narcs(nrow, ncol)
+    rng = np.random.default_rng(5)
+    flows = np.where(rng.uniform(size=na) < 0.02, rng.integers(-2, 3, na), 0).astype(np.int16)
+    got = {}
+
+    def fixed(costs, phase, init):
+        got["costs"] = costs
+        got["phase"] = phase
+        return flows
+
+    cuphu.set_flow_solver(fixed)
+    unw, cc = cuphu.unwrap(s.igram, s.corr, 5.0, cost="smooth", init="mcf", reference_pixel=None,
+                           min_conncomp_frac=0.0)
+    ph = O.wrapped_phase(s.igram, correctly_rounded=True)
+    np.testing.assert_array_equal(got["phase"], ph)
+    d = np.abs(got["costs"].astype(np.int32) - O.costs(ph, s.corr, 5.0))
+    assert d.max() <= 1 and (d > 0).sum() <= 2
+    nra = (nrow - 1) * ncol
+    ref_unw = O.integrate_phase(ph, flows[:nra].reshape(nrow - 1, ncol), flows[nra:].reshape(nrow, ncol - 1))
+    np.testing.assert_array_equal(unw, ref_unw)
+    prm = _capi.default_params()
+    pc, nc, _ = O.incrcost_smooth(got["costs"], flows, nshortcycle=int(prm.nshortcycle))
+    ref_cc = O.conncomp(ref_unw, s.corr, None, 0.0, prm.maxncomps, poscost=pc, negcost=nc,
+                        conncompthresh=prm.conncompthresh)
+    np.testing.assert_array_equal(cc, ref_cc)
+    assert cc.max() >= 1
+
+
+def test_mcf_defo_cost_and_mst_reach_the_solver(gpu_ready, solver_slot):
+    s = synth.make_scene(40, 50, nbumps=1, npatches=0, nlooks=10.0, seed=9)
+    seen = []
+    cuphu.set_flow_solver(lambda c, p, i: (seen.append((c.shape[1], i)), np.zeros(c.shape[0], np.int16))[1])
+    cuphu.unwrap(s.igram, s.corr, 10.0, cost="defo", init="mst")
+    assert seen == [(4, "mst")]
+
+
+def test_mcf_without_solver_or_with_tiles_raises(gpu_ready, solver_slot):
+    s = synth.make_scene(40, 50, nbumps=1, npatches=0, nlooks=10.0, seed=9)
+    cuphu.set_flow_solver(None)
+    with pytest.raises(ValueError, match="network-flow solver"):
+        cuphu.unwrap(s.igram, s.corr, 10.0, init="mcf")
+    cuphu.set_flow_solver(lambda c, p, i: np.zeros(c.shape[0], np.int16))
+    with pytest.raises(ValueError, match="one tile"):
+        cuphu.unwrap(s.igram, s.corr, 10.0, init="mcf", ntiles=(2, 2))
+
+
+def test_mcf_failing_solver_reports_error(gpu_ready, solver_slot):
+    s = synth.make_scene(30, 30, nbumps=1, npatches=0, nlooks=10.0, seed=9)
+
+    def bad(costs, phase, init):
+        raise RuntimeError("solver failure on purpose")
+
+    cuphu.set_flow_solver(bad)
+    with pytest.raises(RuntimeError, match="network-flow solver failed"):
+        cuphu.unwrap(s.igram, s.corr, 10.0, init="mcf")

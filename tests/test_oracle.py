@@ -623,3 +623,91 @@ def test_synth_generator_frozen():
 
 
 GOLDEN_SYNTH_SHA = "062084a27175bc59f7dc1561622450e223c4eca21a96c51ca6d308ee015ac156"
+
+
+# ── MCF path: incremental costs and the cost-gated conncomp ────────────────
+def test_incrcost_smooth_known_answers():
+    # rows: (offset, sigsq, flow, nflow) -> (poscost, negcost), nshortcycle 200
+    cases = [
+        ((0, 32000, 5, 1), (0, 0)),            # LARGESHORT: no preference
+        ((0, 100, 0, 1), (400, 400)),          # 200^2/100 both ways
+        ((-200, 100, 1, 1), (400, 400)),       # a unit flow that fits exactly
+        ((150, 100, 0, 1), (1000, -200)),      # 1225-225, 25-225
+        ((150, 100, 0, 2), (700, 100)),        # ceil(2800/4), ceil(400/4)
+        ((0, 1, 100, 1), (32000, -32000)),     # clamped both ways
+        ((-7, 4, 0, 2), (9650, 10350)),        # (393^2//4 - 12)/4, (407^2//4 - 12)/4
+        ((-7, 3, 0, 2), (12867, 13800)),       # ceil(51467/4) = 12867, 55200/4
+        ((390, 3, 0, 2), (32000, -12667)),     # ceil(157333/4) clamped; floor(-50667/4)
+    ]
+    for (off, sig, f, nfl), exp in cases:
+        pc, nc, _ = O.incrcost_smooth([[off, sig]], [f], nshortcycle=200, nflow=nfl)
+        assert (int(pc[0]), int(nc[0])) == exp, (off, sig, f, nfl)
+
+
+def test_incrcost_smooth_exact_integer_arithmetic_and_early_exit():
+    rng = np.random.default_rng(3)
+    n = 4000
+    off = rng.integers(-3000, 3000, n)
+    sig = rng.integers(1, 32000, n)
+    sig[::97] = 32000
+    flo = rng.integers(-30, 30, n)
+    for nfl in (1, 2, 3):
+        pc, nc, ee = O.incrcost_smooth(np.stack([off, sig], 1), flo, nshortcycle=200, nflow=nfl)
+        for i in range(0, n, 7):
+            if sig[i] == 32000:
+                assert pc[i] == 0 and nc[i] == 0
+                continue
+            c1 = abs(int(flo[i]) * 200 + int(off[i])) ** 2 // int(sig[i])
+            for k, got in ((nfl, pc[i]), (-nfl, nc[i])):
+                raw = abs((int(flo[i]) + k) * 200 + int(off[i])) ** 2 // int(sig[i]) - c1
+                q = -(-raw // (nfl * nfl)) if raw > 0 else raw // (nfl * nfl)
+                assert int(got) == max(-32000, min(32000, q))
+        assert ee == bool(pc.min() >= 0 and nc.min() >= 0)
+    # zero flows on zero offsets: every incremental cost is >= 0 -> early exit
+    pc, nc, ee = O.incrcost_smooth(np.stack([np.zeros(50, int), np.full(50, 300)], 1), np.zeros(50))
+    assert ee and pc.min() > 0
+
+
+def test_conncomp_gate_splits_on_low_incremental_cost():
+    nrow, ncol = 3, 4
+    unw = np.zeros((nrow, ncol), np.float32)
+    corr = np.ones((nrow, ncol), np.float32)
+    na = O.narcs(nrow, ncol)
+    pos = np.full(na, 1000, np.int16)
+    neg = np.full(na, 1000, np.int16)
+    nra = (nrow - 1) * ncol
+    # cut every arc between column 1 and column 2
+    for r in range(nrow):
+        neg[nra + r * (ncol - 1) + 1] = 10
+    lab = O.conncomp(unw, corr, minfrac=0.0, maxncomps=10, poscost=pos, negcost=neg, conncompthresh=300)
+    assert (lab[:, :2] == 1).all() and (lab[:, 2:] == 2).all()
+    # gate off (costs above threshold): one component; no costs: one component
+    assert (O.conncomp(unw, corr, minfrac=0.0, poscost=pos, negcost=pos, conncompthresh=300) == 1).all()
+    assert (O.conncomp(unw, corr, minfrac=0.0) == 1).all()
+
+
+def test_conncomp_gated_matches_scipy_partition():
+    rng = np.random.default_rng(12)
+    nrow, ncol = 40, 55
+    unw = np.cumsum(rng.normal(0, 1.0, (nrow, ncol)), axis=1).astype(np.float32)
+    corr = rng.uniform(0, 1, (nrow, ncol)).astype(np.float32)
+    na = O.narcs(nrow, ncol)
+    pos = rng.integers(0, 800, na).astype(np.int16)
+    neg = rng.integers(0, 800, na).astype(np.int16)
+    lab = O.conncomp(unw, corr, minfrac=0.0, maxncomps=100000, poscost=pos, negcost=neg, conncompthresh=300)
+    valid = ~(corr < 0.05)
+    nra = (nrow - 1) * ncol
+    ok = np.minimum(pos, neg) >= 300
+    v_ok = valid[:-1, :] & valid[1:, :] & (np.abs(unw[:-1, :] - unw[1:, :]) < np.float32(np.pi)) & \
+        ok[:nra].reshape(nrow - 1, ncol)
+    h_ok = valid[:, :-1] & valid[:, 1:] & (np.abs(unw[:, :-1] - unw[:, 1:]) < np.float32(np.pi)) & \
+        ok[nra:].reshape(nrow, ncol - 1)
+    idx = np.arange(nrow * ncol).reshape(nrow, ncol)
+    ri = np.concatenate([idx[:, :-1][h_ok], idx[:-1, :][v_ok]])
+    ci = np.concatenate([idx[:, 1:][h_ok], idx[1:, :][v_ok]])
+    g = coo_matrix((np.ones(ri.size), (ri, ci)), shape=(nrow * ncol, nrow * ncol))
+    _, comp = connected_components(g, directed=False)
+    comp = comp.reshape(nrow, ncol)
+    # same partition of the valid pixels (labels are a relabelling)
+    pairs = set(zip(comp[valid].tolist(), lab[valid].tolist()))
+    assert len(pairs) == len(set(comp[valid].tolist())) == len(set(lab[valid].tolist()))
