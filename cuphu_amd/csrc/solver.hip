@@ -117,6 +117,7 @@ void default_options(CuPhuLaplaceOptions *o) {
     env_int("CUPHU_LAPLACE_PRECISION", &o->precision);
     env_int("CUPHU_LAPLACE_GROUP_MB", &o->group_bytes_mb);
     env_int("CUPHU_LAPLACE_GRAPH", &o->use_graph);
+    env_int("CUPHU_LAPLACE_AUTOTUNE", &o->autotune);
 }
 
 static void check_options(const CuPhuLaplaceOptions &o) {
@@ -573,13 +574,104 @@ struct BatchOut {
     unsigned char *mask = nullptr;
 };
 
+// ── tile-height autotuner ───────────────────────────────────────────────────
+// The stencil kernels' tile height (4 * rows-per-wave: 8, 16 or 32 rows)
+// trades halo re-reads against registers and occupancy, and the best value
+// depends on the instance shapes.  On the first batched/fused solve of a
+// shape class (precision, widest pitch, instance count and pixel count
+// classes) each candidate is timed on the real pooled planes with the fused
+// kernel's steady-state step form before the solve starts (scratch scalars;
+// reads plane set 1, writes set 0, which the first assemble resets; x is not
+// touched), 1 warm-up + 3 timed launches, minimum taken; the fastest is
+// cached for the process.  Opt-in (options.autotune
+// or CUPHU_LAPLACE_AUTOTUNE=1): the tiling fixes the order of the FP64
+// partial sums, so a timing-dependent choice can change results in the last
+// bits between processes.  CUPHU_LAPLACE_RPW forces a value.
+static void retile(Layout &L, int th) {
+    for (auto &d : L.desc) d.tiles_y = (int)ceil_div((size_t)d.nrow, (size_t)th);
+}
+
+static int ilog2_ceil(long long v) {
+    int k = 0;
+    while ((1LL << k) < v && k < 62) ++k;
+    return k;
+}
+
+template <class T>
+static int autotune_rpw(Layout L, const Planes<T> &pl, int verbose, DevRes &res, hipStream_t st) {
+    if (std::getenv("CUPHU_LAPLACE_RPW")) return rows_per_wave();
+    static std::map<std::string, int> cache;
+    const int ninst = (int)L.desc.size();
+    const std::string key = std::to_string(sizeof(T)) + "/" + std::to_string(L.max_pitch) + "/" +
+                            std::to_string(ilog2_ceil(ninst)) + "/" + std::to_string(ilog2_ceil(L.total_el));
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    const int cand[3] = {2, 4, 8};
+    size_t maxblk = 0;
+    for (int rpw : cand) {
+        retile(L, 4 * rpw);
+        size_t nb = 0;
+        for (auto &d : L.desc) nb += (size_t)d.tiles_x * d.tiles_y;
+        maxblk = std::max(maxblk, nb);
+    }
+    InstDesc *d_desc = ws<InstDesc>(res, "tunedesc", (size_t)ninst);
+    int *d_blk = ws<int>(res, "tuneblk", maxblk);
+    double4 *d_part = ws<double4>(res, "tunepart", maxblk);
+    SolveScal *d_scal = ws<SolveScal>(res, "tunescal", (size_t)ninst);
+    // scratch scalars for the steady-state step form: q recomputed, r and p
+    // updated, x untouched (xmode 0)
+    std::vector<SolveScal> zs((size_t)ninst);
+    std::memset(zs.data(), 0, zs.size() * sizeof(SolveScal));
+    for (auto &z : zs) { z.active = 1; z.alpha = 0.1; z.beta = 0.5; z.xmode = 0; }
+    CUPHU_HIP_CHECK(hipMemcpyAsync(d_scal, zs.data(), zs.size() * sizeof(SolveScal), hipMemcpyHostToDevice, st));
+    hipEvent_t e0, e1;
+    CUPHU_HIP_CHECK(hipEventCreate(&e0));
+    CUPHU_HIP_CHECK(hipEventCreate(&e1));
+    int best = 4;
+    float best_ms = 1e30f;
+    try {
+        for (int rpw : cand) {
+            retile(L, 4 * rpw);
+            std::vector<InstDesc> gd = L.desc;
+            std::vector<int> blk;
+            for (int i = 0; i < ninst; ++i) {
+                gd[(size_t)i].block_begin = (int)blk.size();
+                for (int k = 0; k < gd[(size_t)i].tiles_x * gd[(size_t)i].tiles_y; ++k) blk.push_back(i);
+            }
+            CUPHU_HIP_CHECK(hipMemcpyAsync(d_desc, gd.data(), gd.size() * sizeof(InstDesc), hipMemcpyHostToDevice, st));
+            CUPHU_HIP_CHECK(hipMemcpyAsync(d_blk, blk.data(), blk.size() * sizeof(int), hipMemcpyHostToDevice, st));
+            const int nblk = (int)blk.size();
+            launch_iq<T>(d_desc, d_blk, nblk, pl, d_scal, d_part, rpw, 1, 0, st);
+            float ms_min = 1e30f;
+            for (int rep = 0; rep < 3; ++rep) {
+                CUPHU_HIP_CHECK(hipEventRecord(e0, st));
+                launch_iq<T>(d_desc, d_blk, nblk, pl, d_scal, d_part, rpw, 1, 0, st);
+                CUPHU_HIP_CHECK(hipEventRecord(e1, st));
+                CUPHU_HIP_CHECK(hipEventSynchronize(e1));
+                float ms = 0.0f;
+                CUPHU_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+                ms_min = std::min(ms_min, ms);
+            }
+            if (verbose) fprintf(stderr, "[cuphu-autotune] %s rows/wave %d: %.4f ms\n", key.c_str(), rpw, ms_min);
+            if (ms_min < best_ms) { best_ms = ms_min; best = rpw; }
+        }
+    } catch (...) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
+    CUPHU_HIP_CHECK(hipEventDestroy(e0));
+    CUPHU_HIP_CHECK(hipEventDestroy(e1));
+    cache[key] = best;
+    return best;
+}
+
 // ingest(planes, layout, d_desc, d_rowinst) fills phi/corr/mask/sigsq and x.
 template <class T, class Ingest>
 static void run_batch(const std::vector<std::pair<int, int>> &dims, const CuPhuLaplaceOptions &o, double minfrac,
                       long maxncomps, bool do_conncomp, DevRes &res, hipStream_t st, CuPhuSolveStats *stats,
                       BatchOut &out, Ingest &&ingest) {
-    const int rpw = rows_per_wave(), th = 4 * rpw;
-    Layout L = make_layout(dims, th);
+    Layout L = make_layout(dims, 4 * rows_per_wave());
     int ninst = (int)dims.size();
     PoolSet<T> P = alloc_pools<T>(res, L.total_el);
     InstDesc *d_gdesc = ws<InstDesc>(res, "gdesc", (size_t)ninst);
@@ -590,6 +682,11 @@ static void run_batch(const std::vector<std::pair<int, int>> &dims, const CuPhuL
     CUPHU_HIP_CHECK(hipMemcpyAsync(d_gdesc, L.desc.data(), (size_t)ninst * sizeof(InstDesc), hipMemcpyHostToDevice, st));
     CUPHU_HIP_CHECK(hipMemcpyAsync(d_rowinst, rowinst.data(), rowinst.size() * sizeof(int), hipMemcpyHostToDevice, st));
     ingest(P.pl, L, d_gdesc, d_rowinst);
+    // tile height: tuned for the batched solvers (the element layout does
+    // not depend on it, only the workgroup tiling)
+    const int rpw = (o.solver == CUPHU_SOLVER_PLAIN || !o.autotune) ? rows_per_wave()
+                                                                     : autotune_rpw<T>(L, P.pl, o.verbose, res, st);
+    retile(L, 4 * rpw);
 
     // groups of consecutive instances bounded by the working-set budget
     // bytes per pixel one PCG iteration touches: w2 pair + x, r, p, p2, q, dinv

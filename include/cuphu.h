@@ -330,6 +330,12 @@ typedef struct CuPhuLaplaceOptions {
     int    group_bytes_mb;/* batched: solve instances in groups whose working
                              set fits this many MB (0 = all at once)         */
     int    verbose;
+    int    autotune;      /* batched/fused: time the stencil tile heights (8,
+                             16, 32 rows) on the first solve of a shape class
+                             and use the fastest (0 = fixed 16 rows).  The
+                             tiling orders the FP64 partial sums, so results
+                             may then differ in the last bits between
+                             processes; the default keeps them reproducible */
 } CuPhuLaplaceOptions;
 void cuphu_default_laplace_options(CuPhuLaplaceOptions *o);
 

@@ -140,7 +140,7 @@ int main(void) {
   P(CuPhuSolveStats) P(CuPhuInstance) P(CuPhuSmoothCost) P(CuPhuCost)
   F(CuPhuParams, sigsqshortmin) F(CuPhuParams, transmitmode) F(CuPhuParams, shadow)
   F(CuPhuParams, kperpdpsi) F(CuPhuParams, minconncompfrac) F(CuPhuParams, maxncomps)
-  F(CuPhuLaplaceOptions, tol) F(CuPhuLaplaceOptions, verbose) F(CuPhuSolveStats, relres)
+  F(CuPhuLaplaceOptions, tol) F(CuPhuLaplaceOptions, verbose) F(CuPhuLaplaceOptions, autotune) F(CuPhuSolveStats, relres)
   F(CuPhuSolveStats, pass_status) F(CuPhuInstance, nlooks) F(CuPhuInstance, d_conncomp)
   return 0;
 }

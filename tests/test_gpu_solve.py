@@ -118,6 +118,23 @@ def test_graph_and_eager_launches_identical(gpu_ready, solver):
     assert s1[0]["iters"] == s2[0]["iters"] == s3[0]["iters"]
 
 
+def test_autotuned_tile_height_agrees_with_default(gpu_ready, capfd):
+    scenes = [synth.make_scene(300, 700, nbumps=2, npatches=1, nlooks=10.0, seed=31, patch=40),
+              synth.make_scene(90, 200, nbumps=1, npatches=0, nlooks=5.0, seed=32)]
+    opts = dict(n_irls=2, max_iter=200, tol=1e-6)
+    u0, c0, s0 = G.batch_solve(scenes, _capi.default_options(**opts))
+    u1, c1, s1 = G.batch_solve(scenes, _capi.default_options(autotune=1, verbose=1, **opts))
+    err = capfd.readouterr().err
+    tuned = [ln for ln in err.splitlines() if ln.startswith("[cuphu-autotune]")]
+    assert len(tuned) == 3, err[-2000:]          # rows/wave 2, 4, 8 timed once
+    u2, c2, s2 = G.batch_solve(scenes, _capi.default_options(autotune=1, verbose=1, **opts))
+    assert "[cuphu-autotune]" not in capfd.readouterr().err   # cached for the shape class
+    for i, s in enumerate(scenes):
+        ph = O.wrapped_phase(s.igram, correctly_rounded=True)
+        assert (amb(u1[i], ph) == amb(u0[i], ph)).mean() >= 0.999
+        np.testing.assert_array_equal(u2[i], u1[i])
+
+
 def test_conncomp_of_batched_output_matches_oracle(gpu_ready):
     s = SMALL[0]
     mask = np.ones(s.corr.shape, np.uint8)
