@@ -73,7 +73,7 @@ class CuPhuLaplaceOptions(C.Structure):
     _fields_ = [("n_irls", _i), ("delta_max", C.c_float), ("delta_min", C.c_float), ("max_iter", _i),
                 ("tol", _d), ("solver", _i), ("precision", _i), ("rr_interval", _i),
                 ("use_graph", _i), ("group_bytes_mb", _i), ("verbose", _i),
-                ("autotune", _i)]
+                ("autotune", _i), ("stitch_integer", _i), ("merge_labels", _i)]
 
 
 class CuPhuSolveStats(C.Structure):

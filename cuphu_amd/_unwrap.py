@@ -11,7 +11,9 @@ library) and run on one tile; without one they raise.
 
 One keyword is added: ``laplace_options`` (dict; keys n_irls, delta_max,
 delta_min, max_iter, tol, solver ('plain'|'batched'|'fused'), precision (32|64),
-rr_interval, use_graph, group_bytes_mb, verbose).  Unset keys follow the
+rr_interval, use_graph, group_bytes_mb, verbose, autotune, and for tiled runs
+stitch_integer (whole-cycle tile offsets) and merge_labels (connected
+components labelled on the stitched scene)).  Unset keys follow the
 reference defaults and its CUPHU_* environment overrides.
 """
 

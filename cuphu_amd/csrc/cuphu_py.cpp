@@ -160,6 +160,8 @@ py::tuple py_unwrap_arrays(py::array_t<std::complex<float>, py::array::c_style> 
             else if (k == "group_bytes_mb") opt.group_bytes_mb = v.cast<int>();
             else if (k == "verbose") opt.verbose = v.cast<bool>() ? 1 : 0;
             else if (k == "autotune") opt.autotune = v.cast<bool>() ? 1 : 0;
+            else if (k == "stitch_integer") opt.stitch_integer = v.cast<bool>() ? 1 : 0;
+            else if (k == "merge_labels") opt.merge_labels = v.cast<bool>() ? 1 : 0;
             else throw py::value_error("unknown laplace option '" + k + "'");
         }
     }

@@ -832,7 +832,9 @@ static void scene_impl(const float2 *d_igram, const float *d_corr, const unsigne
             dims[(size_t)(tr * ntc + tc)] = {rs[(size_t)tr], cs[(size_t)tc]};
         }
     BatchOut bo;
-    run_batch<T>(dims, o, prm.minconncompfrac, prm.maxncomps, d_cc != nullptr, res, st, stats, bo,
+    // merge_labels: components are labelled once on the stitched scene
+    const bool scene_cc = d_cc != nullptr && o.merge_labels != 0 && nt > 1;
+    run_batch<T>(dims, o, prm.minconncompfrac, prm.maxncomps, d_cc != nullptr && !scene_cc, res, st, stats, bo,
                  src_ingest<T>(srcs, false, res, st));
     if (nt == 1) {
         const InstDesc &d = bo.lay.desc[0];
@@ -844,41 +846,75 @@ static void scene_impl(const float2 *d_igram, const float *d_corr, const unsigne
         CUPHU_HIP_CHECK(hipStreamSynchronize(st));
         return;
     }
-    // stitch on the host from the solved tiles (registration + BFS)
-    long long nel = bo.lay.total_el;
-    float *h_unw = reinterpret_cast<float *>(res.pin("h_unw", (size_t)nel * 4));
-    float *h_corr = reinterpret_cast<float *>(res.pin("h_corr", (size_t)nel * 4));
-    unsigned char *h_conf = nullptr;
-    CUPHU_HIP_CHECK(hipMemcpyAsync(h_unw, bo.unw, (size_t)nel * 4, hipMemcpyDeviceToHost, st));
-    CUPHU_HIP_CHECK(hipMemcpyAsync(h_corr, bo.corr, (size_t)nel * 4, hipMemcpyDeviceToHost, st));
-    bool conf_scene = false;
-    if (d_conf) {
-        h_conf = reinterpret_cast<unsigned char *>(res.pin("h_conf", (size_t)nrow * ncol));
-        CUPHU_HIP_CHECK(hipMemcpyAsync(h_conf, d_conf, (size_t)nrow * ncol, hipMemcpyDeviceToHost, st));
-        conf_scene = true;
-    } else if (d_mask) {
-        h_conf = reinterpret_cast<unsigned char *>(res.pin("h_conf", (size_t)nel));
-        CUPHU_HIP_CHECK(hipMemcpyAsync(h_conf, bo.mask, (size_t)nel, hipMemcpyDeviceToHost, st));
-    }
-    CUPHU_HIP_CHECK(hipStreamSynchronize(st));
-    std::vector<TileView> tv((size_t)nt);
+    // pair registrations on the GPU (weighted medians of the overlap
+    // differences), tile offsets by the BFS over the tile grid on the host
+    const unsigned char *conf = d_conf ? d_conf : (d_mask ? bo.mask : nullptr);
+    std::vector<PairDesc> pairs;
+    std::vector<std::pair<int, bool>> pair_of;
+    int max_chunks = 0;
     for (int t = 0; t < nt; ++t) {
-        const InstDesc &d = bo.lay.desc[(size_t)t];
-        TileView &v = tv[(size_t)t];
-        v.nrow = d.nrow; v.ncol = d.ncol; v.stride = d.pitch;
-        v.unw = h_unw + d.off; v.corr = h_corr + d.off;
-        v.conf = nullptr;
-        v.conf_stride = d.pitch;
-        if (h_conf) {
-            if (conf_scene) {
-                v.conf = h_conf + (size_t)rf[(size_t)(t / ntc)] * ncol + cf[(size_t)(t % ntc)];
-                v.conf_stride = ncol;
-            } else {
-                v.conf = h_conf + d.off;
+        const int br = t / ntc, bc = t % ntc;
+        const InstDesc &A = bo.lay.desc[(size_t)t];
+        for (int h = 1; h >= 0; --h) {
+            if (h ? (bc + 1 >= ntc) : (br + 1 >= ntr)) continue;
+            const int u = h ? t + 1 : t + ntc;
+            const InstDesc &B = bo.lay.desc[(size_t)u];
+            PairDesc pd{};
+            pd.a_pitch = A.pitch;
+            pd.b_pitch = B.pitch;
+            // window origin in A (B's is its (0, 0)); empty windows as the
+            // host registration (:1884-1890)
+            int ar = 0, ac = 0;
+            if (h) {
+                if (covl > 0 && covl <= A.ncol && covl <= B.ncol) {
+                    pd.nr = std::min(A.nrow, B.nrow);
+                    pd.nc = covl;
+                    ac = A.ncol - covl;
+                }
+            } else if (rovl > 0 && rovl <= A.nrow && rovl <= B.nrow) {
+                pd.nr = rovl;
+                pd.nc = std::min(A.ncol, B.ncol);
+                ar = A.nrow - rovl;
             }
+            pd.a_off = A.off + (long long)ar * A.pitch + ac;
+            pd.b_off = B.off;
+            if (d_conf) {   // scene-level mask: tile origins in the scene
+                pd.ca_off = (long long)(rf[(size_t)br] + ar) * ncol + cf[(size_t)bc] + ac;
+                pd.cb_off = (long long)rf[(size_t)(u / ntc)] * ncol + cf[(size_t)(u % ntc)];
+                pd.ca_stride = pd.cb_stride = ncol;
+            } else {        // pooled solve mask: same elements as unw
+                pd.ca_off = pd.a_off;
+                pd.cb_off = pd.b_off;
+                pd.ca_stride = A.pitch;
+                pd.cb_stride = B.pitch;
+            }
+            max_chunks = std::max(max_chunks, pair_chunks((long long)pd.nr * pd.nc));
+            pairs.push_back(pd);
+            pair_of.push_back({t, h != 0});
         }
     }
-    std::vector<double> tk = stitch_offsets(ntr, ntc, tv, rovl, covl, true);
+    const int npairs = (int)pairs.size();
+    CUPHU_REQUIRE(npairs <= 65535, "too many tile pairs for the registration grid");
+    std::vector<double> hk((size_t)nt, 0.0), vk((size_t)nt, 0.0);
+    if (npairs > 0) {
+        PairDesc *d_pairs = ws<PairDesc>(res, "s_pairs", (size_t)npairs);
+        PairSel *d_sel = ws<PairSel>(res, "s_sel", (size_t)npairs);
+        double *d_hist = ws<double>(res, "s_hist", (size_t)npairs * 256);
+        CUPHU_HIP_CHECK(hipMemcpyAsync(d_pairs, pairs.data(), (size_t)npairs * sizeof(PairDesc),
+                                       hipMemcpyHostToDevice, st));
+        pair_medians(d_pairs, npairs, max_chunks, bo.unw, bo.corr, conf, d_sel, d_hist, st);
+        std::vector<PairSel> sel((size_t)npairs);
+        CUPHU_HIP_CHECK(hipMemcpyAsync(sel.data(), d_sel, (size_t)npairs * sizeof(PairSel), hipMemcpyDeviceToHost, st));
+        CUPHU_HIP_CHECK(hipStreamSynchronize(st));
+        for (int i = 0; i < npairs; ++i) {
+            // cycles: fractional for the Laplace path (:1928), whole cycles
+            // when the congruent stitch is asked for
+            double cyc = (double)sel[(size_t)i].value / (2.0 * M_PI);
+            if (o.stitch_integer) cyc = std::round(cyc);
+            (pair_of[(size_t)i].second ? hk : vk)[(size_t)pair_of[(size_t)i].first] = cyc;
+        }
+    }
+    std::vector<double> tk = bfs_offsets(ntr, ntc, hk, vk);
     // merge tables (later tiles win where odd overlaps make trims meet)
     std::vector<int> rtab((size_t)nrow, 0), rloc((size_t)nrow, 0), ctab((size_t)ncol, 0), cloc((size_t)ncol, 0);
     for (int tr = 0; tr < ntr; ++tr) {
@@ -919,8 +955,11 @@ static void scene_impl(const float2 *d_igram, const float *d_corr, const unsigne
     CUPHU_HIP_CHECK(hipMemcpyAsync(d_coff, coff.data(), (size_t)nt * 4, hipMemcpyHostToDevice, st));
     long long npx = (long long)nrow * ncol;
     hipLaunchKernelGGL(k_merge, dim3((unsigned)ceil_div((size_t)npx, 256)), dim3(256), 0, st, bo.unw, bo.lab, d_toff,
-                       d_tpitch, d_rtab, d_rloc, d_ctab, d_cloc, ntc, d_add, d_coff, nrow, ncol, d_unw, d_cc);
+                       d_tpitch, d_rtab, d_rloc, d_ctab, d_cloc, ntc, d_add, d_coff, nrow, ncol, d_unw,
+                       scene_cc ? nullptr : d_cc);
     CUPHU_HIP_CHECK(hipGetLastError());
+    if (scene_cc)
+        conncomp_run(d_unw, d_corr, d_mask, nrow, ncol, ncol, prm.minconncompfrac, prm.maxncomps, d_cc, st);
     CUPHU_HIP_CHECK(hipStreamSynchronize(st));
 }
 

@@ -134,6 +134,12 @@ std::vector<double> stitch_offsets(int ntr, int ntc, const std::vector<TileView>
             }
         });
     for (auto &th : pool) th.join();
+    return bfs_offsets(ntr, ntc, hk, vk);
+}
+
+std::vector<double> bfs_offsets(int ntr, int ntc, const std::vector<double> &hk, const std::vector<double> &vk) {
+    int nt = ntr * ntc;
+    std::vector<double> tk((size_t)nt, 0.0);
     // BFS from tile 0: right neighbour first, then bottom (:1969-1999).
     std::vector<char> vis((size_t)nt, 0);
     std::queue<int> q;

@@ -21,6 +21,10 @@ int tile_axis(int n, int tile_size, int overlap, int ntiles, int *first, int *si
 // registrations and the BFS over the tile grid.
 std::vector<double> stitch_offsets(int ntr, int ntc, const std::vector<TileView> &tiles, int rovl, int covl,
                                    bool laplace);
+// Tile offsets from the pair registrations (hk[t]: tile t -> t+1,
+// vk[t]: t -> t+ntc) by the reference's BFS from tile 0, right neighbour
+// first, then bottom (cuphu_solver.cpp:1969-1999).
+std::vector<double> bfs_offsets(int ntr, int ntc, const std::vector<double> &hk, const std::vector<double> &vk);
 // Rows/cols [a, b) of a tile kept by the merge (interior sides trimmed by ovl/2).
 void trim_range(int t, int nt, int size, int ovl, int *a, int *b);
 

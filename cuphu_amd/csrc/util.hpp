@@ -32,4 +32,29 @@ int conncomp_run(const float *unw, const float *corr, const unsigned char *mask,
 int util_incrcost(const short *costs, const short *flows, size_t narcs, long nshortcycle, int nflow, short *poscost,
                   short *negcost, int *mins, hipStream_t st);
 
+// Tile-pair registration (stitch_kernels.hip).  A pair's overlap window is
+// nr x nc samples; sample (r, c) compares pool element a_off + r*a_pitch + c
+// (tile A) with b_off + r*b_pitch + c (tile B) and, when a confidence mask
+// is given, its bytes at ca_off + r*ca_stride + c and cb_off + r*cb_stride + c.
+struct PairDesc {
+    long long a_off, b_off, ca_off, cb_off;
+    int a_pitch, b_pitch, ca_stride, cb_stride;
+    int nr, nc;
+};
+// Selection state per pair; value is the weighted median of A - B (0 for a
+// pair without samples) once done.
+struct PairSel {
+    double half, below;
+    uint32_t prefix;
+    int done;
+    float value;
+    int pad_;
+};
+// Samples per histogram block = 256 * 16; chunks covering nsamples.
+int pair_chunks(long long nsamples);
+// Weighted medians of all pairs (4 passes of 8 bits).  d_hist holds
+// npairs * 256 doubles; max_chunks = max over pairs of pair_chunks().
+void pair_medians(const PairDesc *d_pairs, int npairs, int max_chunks, const float *unw, const float *corr,
+                  const unsigned char *conf, PairSel *d_sel, double *d_hist, hipStream_t st);
+
 }  // namespace cuphu

@@ -336,6 +336,16 @@ typedef struct CuPhuLaplaceOptions {
                              tiling orders the FP64 partial sums, so results
                              may then differ in the last bits between
                              processes; the default keeps them reproducible */
+    int    stitch_integer;/* tiled scenes: round each tile's stitch offset to
+                             whole cycles, so the scene stays congruent with
+                             the wrapped phase (0 = the reference's
+                             fractional Laplace offsets, :1928)             */
+    int    merge_labels;  /* tiled scenes: label connected components on the
+                             stitched scene, so a component crossing tile
+                             seams gets one label and the size threshold is
+                             minconncompfrac of the scene (0 = the
+                             reference's per-tile labels offset by running
+                             count, cuphu_solver.cpp:2025-2029)             */
 } CuPhuLaplaceOptions;
 void cuphu_default_laplace_options(CuPhuLaplaceOptions *o);
 

@@ -285,6 +285,80 @@ def test_scene_api_equals_batch_plus_oracle_stitch(gpu_ready):
     np.testing.assert_array_equal(G.host(dl).view(np.uint32), ec)
 
 
+def _scene_tiles(s, ts, ov, mask=None):
+    nrow, ncol = s.corr.shape
+    rf, rs = O.tile_axis(nrow, ts, ov)
+    cf, cs = O.tile_axis(ncol, ts, ov)
+    tiles, tmask = [], []
+    for a in range(len(rf)):
+        for b in range(len(cf)):
+            sl = (slice(rf[a], rf[a] + rs[a]), slice(cf[b], cf[b] + cs[b]))
+            tiles.append(synth.Scene(np.ascontiguousarray(s.igram[sl]), np.ascontiguousarray(s.corr[sl]),
+                                     s.truth[sl], s.nlooks, 0))
+            tmask.append(np.ascontiguousarray(mask[sl]) if mask is not None else None)
+    return (rf, rs, cf, cs), tiles, tmask
+
+
+def _scene_gpu(s, ts, ov, o, mask=None, orig=None):
+    torch = G.torch_mod()
+    nrow, ncol = s.corr.shape
+    dig = G.dev(s.igram.view(np.float32).reshape(nrow, 2 * ncol))
+    dc = G.dev(s.corr)
+    dm = G.dev(mask, np.uint8) if mask is not None else None
+    do = G.dev(orig, np.uint8) if orig is not None else None
+    du = G.empty((nrow, ncol), torch.float32)
+    dl = G.empty((nrow, ncol), torch.int32)
+    nt = C.c_int()
+    prm = _capi.default_params()
+    _capi.check(_capi.lib.cuphu_unwrap_scene_gpu(G.p(dig), G.p(dc), G.p(dm), G.p(do), nrow, ncol, s.nlooks, 0,
+                                                 C.byref(prm), ts, ov, 0, 0, C.byref(o), G.p(du), G.p(dl), None,
+                                                 C.byref(nt), G.stream()), "scene")
+    return G.host(du), G.host(dl).view(np.uint32), nt.value
+
+
+@pytest.mark.parametrize("mode", ["mask", "orig_mask"])
+@pytest.mark.parametrize("integer", [0, 1])
+def test_scene_gpu_registration_matches_oracle_stitch(gpu_ready, mode, integer):
+    """The GPU pair registration (weighted medians of the overlap
+    differences) gated by the solve mask or by orig_mask, with fractional or
+    whole-cycle tile offsets, against the oracle's host stitch."""
+    s = synth.make_scene(300, 410, nbumps=3, npatches=1, nlooks=10.0, seed=78, patch=40)
+    nrow, ncol = s.corr.shape
+    rng = np.random.default_rng(5)
+    m = (rng.random((nrow, ncol)) > 0.1).astype(np.uint8)
+    m[100:140, 120:260] = 0
+    mask, orig = (m, None) if mode == "mask" else (None, m)
+    o = _capi.default_options(n_irls=2, max_iter=150, stitch_integer=integer)
+    ts, ov = 160, 32
+    (rf, rs, cf, cs), tiles, tmask = _scene_tiles(s, ts, ov, mask)
+    tconf = _scene_tiles(s, ts, ov, m)[2]
+    tu, tc, _ = G.batch_solve(tiles, o, masks=tmask if mask is not None else None)
+    eu, ec, ek = O.stitch(nrow, ncol, rf, rs, cf, cs, ov, ov, tu, tc, [t.corr for t in tiles], tconf,
+                          laplace=not integer)
+    if integer:
+        assert np.array_equal(ek, np.rint(ek))
+    gu, gl, nt = _scene_gpu(s, ts, ov, o, mask, orig)
+    assert nt == len(rf) * len(cf)
+    np.testing.assert_array_equal(gu, eu)
+    np.testing.assert_array_equal(gl, ec)
+
+
+def test_scene_merge_labels_is_scene_conncomp(gpu_ready):
+    """merge_labels: the stitched scene is labelled as a whole (a component
+    crossing tile seams keeps one label; the size threshold is taken over
+    the scene), equal to the oracle's conncomp of the stitched phase; the
+    phase itself is unchanged."""
+    s = synth.make_scene(300, 410, nbumps=3, npatches=1, nlooks=10.0, seed=79, patch=40)
+    ts, ov = 160, 32
+    u0, l0, _ = _scene_gpu(s, ts, ov, _capi.default_options(n_irls=2, max_iter=150))
+    u1, l1, _ = _scene_gpu(s, ts, ov, _capi.default_options(n_irls=2, max_iter=150, merge_labels=1))
+    np.testing.assert_array_equal(u0, u1)
+    prm = _capi.default_params()
+    ref = O.conncomp(u1, s.corr, None, prm.minconncompfrac, prm.maxncomps)
+    np.testing.assert_array_equal(l1, ref)
+    assert l1.max() < l0.max()   # per-tile labels split the seam-crossing components
+
+
 def test_repeat_runs_are_deterministic(gpu_ready):
     s = SMALL[0]
     o = _capi.default_options(n_irls=2, max_iter=300)
