@@ -152,6 +152,10 @@ _SIGS = {
     "cuphu_test_pcg_step": (_i, [_P, _P, _i, _i, _d, _d, _i, _P, _P, _P, _P, C.POINTER(_d), C.POINTER(_d),
                                  C.POINTER(_d)]),
     "cuphu_test_pcg_fused_step": (_i, [_P, _P, _i, _i, _d, _d, _i, _P, _i, _i, _P, _P, _P, _P, _P]),
+    "cuphu_fix_cycle_spikes_gpu": (_i, [_P, _P, _i, _i, _P]),
+    "cuphu_neighbor_feedback_gpu": (_i, [_P, _P, _i, _i, _i, _i, _i, _i, _i, _P]),
+    "cuphu_fix_cycle_spikes_test": (_i, [_P, _P, _i, _i]),
+    "cuphu_laplace_neighbor_feedback_test": (_i, [_P, _P, _i, _i, _i, _i, _i, _i, _i]),
     "cuphu_incrcost_gpu": (_i, [_P, _P, _i, _l, _i, _P, _P, C.POINTER(_i), _P]),
     "cuphu_set_flow_solver": (_i, [_P, _P]),
 }

@@ -3,8 +3,10 @@
 Same signature, defaults, validation and post-processing as the reference:
 NaN scrub, auto-tiling toward ``target_tile_size``, mask dilation for
 ``mask_pad_distance``, the reference-pixel integer-cycle shift and writing
-into caller-provided outputs.  The solve itself goes through the pybind11
-module to the gfx950 library.  Only ``init='laplace'`` is implemented;
+into caller-provided outputs, and the optional post-passes
+``laplace_neighbor_feedback`` and ``fix_cycle_spikes``.  The solve itself goes
+through the pybind11 module to the gfx950 library.  ``init='laplace'`` is
+the GPU path;
 ``'mcf'``/``'mst'`` need a network-flow solver registered with
 ``cuphu_amd.set_flow_solver`` (SNAPHU's CPU solver is not part of this
 library) and run on one tile; without one they raise.
@@ -167,12 +169,14 @@ def unwrap(  # noqa: C901 - mirrors the reference's single entry point
     if mask_u8 is not None and mask_pad_distance > 0:
         solve_mask_u8 = _dilate_mask(mask_u8 != 0, int(mask_pad_distance)).astype(np.uint8)
 
+    # Neighbour feedback samples the seam pixels; with a padded solve mask it
+    # runs after the solve on the ORIGINAL mask instead, so padded-through
+    # pixels never count as boundary data.  single_tile_reoptimize supersedes
+    # it (_unwrap.py:529-557).
     effective_neighbor_feedback = laplace_neighbor_feedback and not single_tile_reoptimize
     feedback_needs_orig_mask = (
         mask_u8 is not None and mask_pad_distance > 0 and effective_neighbor_feedback)
     internal_feedback = effective_neighbor_feedback and not feedback_needs_orig_mask
-    if feedback_needs_orig_mask and ntilerow * ntilecol > 1:
-        raise ValueError("laplace_neighbor_feedback is not implemented in this library")
 
     kperpdpsi, kpardpsi = int(phase_grad_window[0]), int(phase_grad_window[1])
 
@@ -207,6 +211,11 @@ def unwrap(  # noqa: C901 - mirrors the reference's single entry point
         gpu_id=gpu_id,
         laplace_options=laplace_options,
     )
+
+    if feedback_needs_orig_mask:
+        unw_out = _cuphu_ext._laplace_neighbor_feedback_test(
+            np.ascontiguousarray(unw_out, dtype=np.float32), mask_u8, ntilerow, ntilecol,
+            row_ovrlp, col_ovrlp, int(laplace_neighbor_feedback_feather))
 
     if mask_u8 is not None and mask_pad_distance > 0:
         # padded-through pixels are never reported valid (_unwrap.py:600-603)

@@ -7,6 +7,7 @@
 #include <pybind11/pybind11.h>
 
 #include <complex>
+#include <functional>
 #include <cstdlib>
 #include <cstring>
 #include <stdexcept>
@@ -263,6 +264,46 @@ py::dict py_gpu_info(int device_id) {
     return d;
 }
 
+// Test-only bindings with the reference's names and arguments
+// (cuphu_py.cu:373-431): run a post-pass on a copy of a caller-supplied
+// unw array and return it.
+py::array_t<float> post_pass_copy(const py::array_t<float, py::array::c_style> &unw, const py::object &mask_obj,
+                                  const std::function<int(float *, const unsigned char *, int, int)> &fn,
+                                  const char *what) {
+    require_2d(unw, "unw");
+    int nrow = (int)unw.shape(0), ncol = (int)unw.shape(1);
+    py::array_t<uint8_t, py::array::c_style> m;
+    const unsigned char *mp = nullptr;
+    if (!mask_obj.is_none()) {
+        m = mask_obj.cast<py::array_t<uint8_t, py::array::c_style>>();
+        require_2d(m, "mask");
+        if (m.shape(0) != nrow || m.shape(1) != ncol) throw py::value_error("mask must have the same shape as unw");
+        mp = m.data();
+    }
+    py::array_t<float> out({(py::ssize_t)nrow, (py::ssize_t)ncol});
+    std::memcpy(out.mutable_data(), unw.data(), (size_t)nrow * ncol * sizeof(float));
+    int rc;
+    {
+        py::gil_scoped_release nogil;
+        rc = fn(out.mutable_data(), mp, nrow, ncol);
+    }
+    raise_status(rc, what);
+    return out;
+}
+
+py::array_t<float> py_neighbor_feedback_test(py::array_t<float, py::array::c_style> unw, py::object mask, int ntilerow,
+                                             int ntilecol, int row_ovrlp, int col_ovrlp, int feather_px) {
+    return post_pass_copy(unw, mask, [&](float *u, const unsigned char *mp, int nr, int nc) {
+        return cuphu_laplace_neighbor_feedback_test(u, mp, nr, nc, ntilerow, ntilecol, row_ovrlp, col_ovrlp, feather_px);
+    }, "cuphu_laplace_neighbor_feedback_test");
+}
+
+py::array_t<float> py_fix_cycle_spikes_test(py::array_t<float, py::array::c_style> unw, py::object mask) {
+    return post_pass_copy(unw, mask, [](float *u, const unsigned char *mp, int nr, int nc) {
+        return cuphu_fix_cycle_spikes_test(u, mp, nr, nc);
+    }, "cuphu_fix_cycle_spikes_test");
+}
+
 }  // namespace
 
 PYBIND11_MODULE(_cuphu_ext, m) {
@@ -284,6 +325,11 @@ PYBIND11_MODULE(_cuphu_ext, m) {
     m.def("regrow_conncomp_arrays", &py_regrow_conncomp, "unw"_a, "corr"_a, "mask"_a = py::none(),
           "cost_thresh"_a = 0.1, "min_conncomp_frac"_a = 0.01, "max_ncomps"_a = 32L, "gpu_id"_a = 0,
           "Connected components of an unwrapped phase on the GPU.");
+    m.def("_laplace_neighbor_feedback_test", &py_neighbor_feedback_test, "unw"_a, "mask"_a = py::none(), "ntilerow"_a,
+          "ntilecol"_a, "row_ovrlp"_a, "col_ovrlp"_a, "feather_px"_a = 200,
+          "Test-only: the Laplace tile-seam neighbour feedback on a copy of an already-stitched unw array.");
+    m.def("_fix_cycle_spikes_test", &py_fix_cycle_spikes_test, "unw"_a, "mask"_a = py::none(),
+          "Test-only: the isolated row/column whole-cycle spike correction on a copy of an unw array.");
     m.def("gpu_count", &py_gpu_count, "Return the number of GPUs available.");
     m.def("gpu_name", &py_gpu_name, "device_id"_a = 0, "Return the name of the device.");
     m.def("gpu_info", &py_gpu_info, "device_id"_a = 0, "Return a dict of device properties.");

@@ -1492,6 +1492,18 @@ static void mcf_tile(const float2 *d_ig, const float *d_corr, const unsigned cha
     CUPHU_HIP_CHECK(hipStreamSynchronize(st));
 }
 
+// Internal tile seams of the reference tile formula (cuphu_solver.cpp:1475-
+// 1476, :1586, :1623): each tile's first row/column past the first tile,
+// plus half the overlap.
+static std::vector<int> seams(int n, int ovl, int ntiles) {
+    std::vector<int> out;
+    if (ntiles < 2) return out;
+    std::vector<int> f((size_t)ntiles), sz((size_t)ntiles);
+    if (cuphu::tile_axis(n, 0, ovl, ntiles, f.data(), sz.data(), ntiles) != ntiles) return out;
+    for (int t = 1; t < ntiles; ++t) out.push_back(f[(size_t)t] + ovl / 2);
+    return out;
+}
+
 int cuphu_unwrap_ex(const float *igram_r, const float *igram_i, const float *igram_c64, const float *corr,
                     const unsigned char *mask, int nrow, int ncol, CuPhuCostMode cost_mode, CuPhuInitMethod init_meth,
                     const CuPhuParams *params, const CuPhuTileParams *tile, const CuPhuBridgeParams *bridge,
@@ -1517,9 +1529,6 @@ int cuphu_unwrap_ex(const float *igram_r, const float *igram_i, const float *igr
         if (bridge && bridge->enabled) throw Error(CUPHU_ERR_UNSUPPORTED, "phase bridging is not implemented");
         if (tile->single_tile_reoptimize && tile->ntilerow * tile->ntilecol > 1)
             throw Error(CUPHU_ERR_UNSUPPORTED, "single_tile_reoptimize needs the CPU network-flow solver");
-        if (tile->laplace_neighbor_feedback && tile->ntilerow * tile->ntilecol > 1)
-            throw Error(CUPHU_ERR_UNSUPPORTED, "laplace_neighbor_feedback is not implemented");
-        if (tile->fix_cycle_spikes) throw Error(CUPHU_ERR_UNSUPPORTED, "fix_cycle_spikes is not implemented");
         CUPHU_REQUIRE(tile->ntilerow >= 1 && tile->ntilecol >= 1, "ntilerow/ntilecol must be >= 1");
         CUPHU_REQUIRE(tile->rowovrlp >= 0 && tile->colovrlp >= 0, "tile overlaps must be >= 0");
         CUPHU_REQUIRE(params->ncorrlooks >= 1.0, "nlooks must be >= 1");
@@ -1569,6 +1578,12 @@ int cuphu_unwrap_ex(const float *igram_r, const float *igram_i, const float *igr
                 scene_impl<float>(d_ig.get(), d_corr.get(), mask ? d_mask.get() : nullptr, conf, nrow, ncol,
                                   params->ncorrlooks, *params, 0, rov, cov, tile->ntilerow, tile->ntilecol, o,
                                   d_unw.get(), d_cc.get(), stats, nullptr, res, st);
+            // optional post-passes in the reference's order (cuphu_solver.cpp:2037-2095)
+            if (!mcf && tile->laplace_neighbor_feedback && !single)
+                neighbor_feedback_dev(d_unw.get(), mask ? d_mask.get() : nullptr, nrow, ncol,
+                                      seams(ncol, cov, tile->ntilecol), seams(nrow, rov, tile->ntilerow),
+                                      tile->laplace_neighbor_feedback_feather, st);
+            if (tile->fix_cycle_spikes) fix_cycle_spikes_dev(d_unw.get(), mask ? d_mask.get() : nullptr, nrow, ncol, st);
             CUPHU_HIP_CHECK(hipMemcpyAsync(result->unw, d_unw.get(), n * 4, hipMemcpyDeviceToHost, st));
             CUPHU_HIP_CHECK(hipMemcpyAsync(result->conncomp, d_cc.get(), n * 4, hipMemcpyDeviceToHost, st));
             CUPHU_HIP_CHECK(hipStreamSynchronize(st));
@@ -2065,6 +2080,73 @@ int cuphu_test_pcg_fused_step(const float *w, const float *dinv, int nrow, int n
         double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
         for (auto &v : parts) { a0 += v.x; a1 += v.y; a2 += v.z; a3 += v.w; }
         sums[0] = a0; sums[1] = a1; sums[2] = a2; sums[3] = a3;
+        return 0;
+    });
+}
+
+
+int cuphu_fix_cycle_spikes_gpu(float *d_unw, const unsigned char *d_mask, int nrow, int ncol, cuphu_stream_t stream) {
+    return guarded([&]() {
+        std::lock_guard<std::recursive_mutex> lk(g_mu);
+        CUPHU_REQUIRE(d_unw && nrow >= 1 && ncol >= 1, "invalid argument");
+        DevRes &res = devres();
+        StreamJoin join(res, (hipStream_t)stream);
+        fix_cycle_spikes_dev(d_unw, d_mask, nrow, ncol, res.stream);
+        return 0;
+    });
+}
+
+int cuphu_neighbor_feedback_gpu(float *d_unw, const unsigned char *d_mask, int nrow, int ncol, int ntr, int ntc,
+                                int row_ovrlp, int col_ovrlp, int feather_px, cuphu_stream_t stream) {
+    return guarded([&]() {
+        std::lock_guard<std::recursive_mutex> lk(g_mu);
+        CUPHU_REQUIRE(d_unw && nrow >= 1 && ncol >= 1 && ntr >= 1 && ntc >= 1, "invalid argument");
+        CUPHU_REQUIRE(row_ovrlp >= 0 && col_ovrlp >= 0, "overlaps must be >= 0");
+        DevRes &res = devres();
+        StreamJoin join(res, (hipStream_t)stream);
+        neighbor_feedback_dev(d_unw, d_mask, nrow, ncol, seams(ncol, col_ovrlp, ntc), seams(nrow, row_ovrlp, ntr),
+                              feather_px, res.stream);
+        return 0;
+    });
+}
+
+// Host-array forms with the reference's test-hook names and arguments
+// (cuphu.h:397-419), run on the current GPU.
+int cuphu_fix_cycle_spikes_test(float *unw, const unsigned char *mask, int nrow, int ncol) {
+    return guarded([&]() {
+        std::lock_guard<std::recursive_mutex> lk(g_mu);
+        CUPHU_REQUIRE(unw && nrow >= 1 && ncol >= 1, "invalid argument");
+        DevRes &res = devres();
+        hipStream_t st = res.stream;
+        size_t n = (size_t)nrow * ncol;
+        DevBuf<float> du(n);
+        DevBuf<unsigned char> dm(mask ? n : 0);
+        CUPHU_HIP_CHECK(hipMemcpyAsync(du.get(), unw, n * 4, hipMemcpyHostToDevice, st));
+        if (mask) CUPHU_HIP_CHECK(hipMemcpyAsync(dm.get(), mask, n, hipMemcpyHostToDevice, st));
+        fix_cycle_spikes_dev(du.get(), mask ? dm.get() : nullptr, nrow, ncol, st);
+        CUPHU_HIP_CHECK(hipMemcpyAsync(unw, du.get(), n * 4, hipMemcpyDeviceToHost, st));
+        CUPHU_HIP_CHECK(hipStreamSynchronize(st));
+        return 0;
+    });
+}
+
+int cuphu_laplace_neighbor_feedback_test(float *unw, const unsigned char *mask, int nrow, int ncol, int ntr, int ntc,
+                                         int row_ovrlp, int col_ovrlp, int feather_px) {
+    return guarded([&]() {
+        std::lock_guard<std::recursive_mutex> lk(g_mu);
+        CUPHU_REQUIRE(unw && nrow >= 1 && ncol >= 1 && ntr >= 1 && ntc >= 1, "invalid argument");
+        CUPHU_REQUIRE(row_ovrlp >= 0 && col_ovrlp >= 0, "overlaps must be >= 0");
+        DevRes &res = devres();
+        hipStream_t st = res.stream;
+        size_t n = (size_t)nrow * ncol;
+        DevBuf<float> du(n);
+        DevBuf<unsigned char> dm(mask ? n : 0);
+        CUPHU_HIP_CHECK(hipMemcpyAsync(du.get(), unw, n * 4, hipMemcpyHostToDevice, st));
+        if (mask) CUPHU_HIP_CHECK(hipMemcpyAsync(dm.get(), mask, n, hipMemcpyHostToDevice, st));
+        neighbor_feedback_dev(du.get(), mask ? dm.get() : nullptr, nrow, ncol, seams(ncol, col_ovrlp, ntc),
+                              seams(nrow, row_ovrlp, ntr), feather_px, st);
+        CUPHU_HIP_CHECK(hipMemcpyAsync(unw, du.get(), n * 4, hipMemcpyDeviceToHost, st));
+        CUPHU_HIP_CHECK(hipStreamSynchronize(st));
         return 0;
     });
 }

@@ -50,6 +50,14 @@ struct PairSel {
     float value;
     int pad_;
 };
+// Post-passes on a stitched scene (postpass.hip), in place on the device.
+// Isolated whole-cycle row/column spike correction.
+void fix_cycle_spikes_dev(float *d_unw, const unsigned char *d_mask, int nrow, int ncol, hipStream_t st);
+// Tile-seam neighbour feedback: column seams (first column of the right
+// tile's kept half) in order, then row seams; feather <= 0 does nothing.
+void neighbor_feedback_dev(float *d_unw, const unsigned char *d_mask, int nrow, int ncol, const std::vector<int> &col_seams,
+                           const std::vector<int> &row_seams, int feather, hipStream_t st);
+
 // Samples per histogram block = 256 * 16; chunks covering nsamples.
 int pair_chunks(long long nsamples);
 // Weighted medians of all pairs (4 passes of 8 bits).  d_hist holds

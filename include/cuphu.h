@@ -545,6 +545,28 @@ int cuphu_test_pcg_fused_step(const float *w, const float *dinv, int nrow, int n
                               double alpha, double beta, int start, const float *rt, int rpw,
                               int stored_q, float *x, float *r, float *p, float *q, double *sums);
 
+
+/* ── Optional post-passes on a stitched scene ─────────────────────────────
+ * Isolated whole-cycle row/column spike correction (reference
+ * cuphu_solver.cpp:1385-1441): the median of each row (then each column) of
+ * valid pixels is compared with its neighbours' medians, and a line offset
+ * from both by the same nonzero whole number of cycles (within 0.5 rad) is
+ * shifted back on its valid pixels.  d_mask may be NULL.  In place. */
+int cuphu_fix_cycle_spikes_gpu(float *d_unw, const unsigned char *d_mask, int nrow, int ncol,
+                               cuphu_stream_t stream);
+/* Laplace tile-seam neighbour feedback (reference cuphu_solver.cpp:1469-1657):
+ * seams of the reference tile formula for ntr x ntc tiles with the given
+ * overlaps; each seam's per-row (per-column) mismatch profile, smoothed and
+ * gap-filled, is feathered into the next feather_px pixels.  In place. */
+int cuphu_neighbor_feedback_gpu(float *d_unw, const unsigned char *d_mask, int nrow, int ncol, int ntr, int ntc,
+                                int row_ovrlp, int col_ovrlp, int feather_px, cuphu_stream_t stream);
+/* The same two passes on host arrays, with the reference's test-hook names
+ * and arguments (cuphu.h:397-419 there, which return void; these return a
+ * status). */
+int cuphu_laplace_neighbor_feedback_test(float *unw, const unsigned char *mask, int nrow, int ncol, int ntr,
+                                         int ntc, int row_ovrlp, int col_ovrlp, int feather_px);
+int cuphu_fix_cycle_spikes_test(float *unw, const unsigned char *mask, int nrow, int ncol);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

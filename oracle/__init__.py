@@ -298,6 +298,24 @@ def stitch(nrow, ncol, rf, rs, cf, cs, rovl, covl, tiles_unw, tiles_cc, tiles_co
     return unw, cc, np.array(tk[:])
 
 
+def fix_cycle_spikes(unw, mask=None) -> np.ndarray:
+    """Isolated whole-cycle row/column spike correction (a new array)."""
+    out = np.array(unw, dtype=np.float32, order="C", copy=True)
+    m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
+    _lib.o_fix_cycle_spikes(_p(out), _p(m), C.c_int(out.shape[0]), C.c_int(out.shape[1]))
+    return out
+
+
+def neighbor_feedback(unw, mask, ntilerow, ntilecol, row_ovrlp, col_ovrlp, feather_px) -> np.ndarray:
+    """Laplace tile-seam neighbour feedback (a new array); seams at the
+    reference tile formula's tile starts + overlap/2."""
+    out = np.array(unw, dtype=np.float32, order="C", copy=True)
+    m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
+    _lib.o_neighbor_feedback(_p(out), _p(m), C.c_int(out.shape[0]), C.c_int(out.shape[1]), C.c_int(ntilerow),
+                             C.c_int(ntilecol), C.c_int(row_ovrlp), C.c_int(col_ovrlp), C.c_int(feather_px))
+    return out
+
+
 def wls_solve_f64(w, phi, max_iter=10000, tol=1e-12):
     w = np.ascontiguousarray(w, dtype=np.float64)
     phi = np.ascontiguousarray(phi, dtype=np.float64)
@@ -329,9 +347,10 @@ def _sig(fn, res, args):
 for _n in ("o_set_threads", "o_wrapped_phase", "o_phase_diffs", "o_boxcar2d", "o_costs", "o_residues",
            "o_arc_weights", "o_assemble", "o_matvec", "o_reweight", "o_congruence", "o_pcg_irls_f32",
            "o_pcg_irls_f64", "o_pcg_step_f32", "o_pcg_fused_step_f32", "o_conncomp", "o_conncomp_gated", "o_incrcost_smooth", "o_integrate_phase", "o_tile_axis", "o_stitch",
-           "o_wls_solve_f64", "o_wls_adjoint_f64"):
+           "o_wls_solve_f64", "o_wls_adjoint_f64", "o_fix_cycle_spikes", "o_neighbor_feedback"):
     getattr(_lib, _n).restype = C.c_int if _n not in ("o_set_threads", "o_wrapped_phase", "o_phase_diffs",
                                                        "o_costs", "o_residues", "o_arc_weights",
                                                        "o_assemble", "o_matvec", "o_reweight",
                                                        "o_congruence", "o_pcg_fused_step_f32",
-                                                       "o_integrate_phase") else None
+                                                       "o_integrate_phase", "o_fix_cycle_spikes",
+                                                       "o_neighbor_feedback") else None

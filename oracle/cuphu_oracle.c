@@ -27,6 +27,13 @@
  *   o_tile_axis/stitch reference test_auto_tiling.py / geometry formula KATs;
  *                     hand-built two-tile cases
  *   o_wls_*           dense numpy solve + central finite differences
+ *   o_fix_cycle_spikes reference test_fix_cycle_spikes.py cases (isolated
+ *                     row / column, smooth ramp, sustained band, mask);
+ *                     brute-force sorted medians
+ *   o_neighbor_feedback reference test_laplace_neighbor_feedback.py cases
+ *                     (constant offset, local gradient, sparse boundary,
+ *                     far-from-data fade, single tile); hand-evaluated
+ *                     profile (moving median, gap fill, feather)
  */
 #include "cuphu_oracle.h"
 
@@ -950,4 +957,163 @@ int o_wls_adjoint_f64(const double *w, const double *phi, const double *u, const
     if (lambda) memcpy(lambda, lam, n * sizeof(double));
     free(b); free(dinv); free(lam);
     return 0;
+}
+
+/* ── post-passes (reference cuphu_solver.cpp:1385-1657) ─────────────────── */
+
+static int cmp_float(const void *a, const void *b) {
+    float x = *(const float *)a, y = *(const float *)b;
+    return (x < y) ? -1 : (x > y) ? 1 : 0;
+}
+static int cmp_double(const void *a, const void *b) {
+    double x = *(const double *)a, y = *(const double *)b;
+    return (x < y) ? -1 : (x > y) ? 1 : 0;
+}
+
+/* Isolated whole-cycle row (by_row) or column spikes (:1385-1434): the
+ * median of a line's valid pixels (the element of rank count/2, only for
+ * count > 20) is compared with its two neighbours' medians, lines in order;
+ * a line whose offsets from both round to the same nonzero cycle count,
+ * each within 0.5 rad of it, is shifted back on its valid pixels and its
+ * median updated before the next line is examined. */
+static void fix_spikes_1d(float *unw, const unsigned char *mask, int nrow, int ncol, int by_row) {
+    const double two_pi = 2.0 * M_PI, tol = 0.5;
+    int nl = by_row ? nrow : ncol, len = by_row ? ncol : nrow;
+    double *med = (double *)malloc(sizeof(double) * (size_t)(nl > 0 ? nl : 1));
+    unsigned char *have = (unsigned char *)calloc((size_t)(nl > 0 ? nl : 1), 1);
+    float *buf = (float *)malloc(sizeof(float) * (size_t)(len > 0 ? len : 1));
+    for (int i = 0; i < nl; ++i) {
+        int n = 0;
+        for (int j = 0; j < len; ++j) {
+            size_t g = by_row ? (size_t)i * ncol + j : (size_t)j * ncol + i;
+            if (!mask || mask[g]) buf[n++] = unw[g];
+        }
+        med[i] = 0.0;
+        if (n > 20) {
+            qsort(buf, (size_t)n, sizeof(float), cmp_float);
+            med[i] = (double)buf[n / 2];
+            have[i] = 1;
+        }
+    }
+    for (int i = 1; i < nl - 1; ++i) {
+        if (!have[i] || !have[i - 1] || !have[i + 1]) continue;
+        double up = med[i] - med[i - 1], down = med[i] - med[i + 1];
+        double nu = round(up / two_pi), nd = round(down / two_pi);
+        if (nu == 0.0 || nu != nd) continue;
+        if (fabs(up - nu * two_pi) > tol || fabs(down - nd * two_pi) > tol) continue;
+        float c = (float)(nu * two_pi);
+        for (int j = 0; j < len; ++j) {
+            size_t g = by_row ? (size_t)i * ncol + j : (size_t)j * ncol + i;
+            if (!mask || mask[g]) unw[g] -= c;
+        }
+        med[i] -= c;
+    }
+    free(med); free(have); free(buf);
+}
+
+/* :1436-1441: rows, then columns */
+void o_fix_cycle_spikes(float *unw, const unsigned char *mask, int nrow, int ncol) {
+    fix_spikes_1d(unw, mask, nrow, ncol, 1);
+    fix_spikes_1d(unw, mask, nrow, ncol, 0);
+}
+
+/* Moving median over a boundary profile (:1492-1507): for every measured
+ * entry, the element of rank m/2 among the m measured entries within
+ * +-w/2; unmeasured entries become 0. */
+static void nf_moving_median(double *x, const unsigned char *have, int n, int w) {
+    double *out = (double *)calloc((size_t)(n > 0 ? n : 1), sizeof(double));
+    double *seg = (double *)malloc(sizeof(double) * (size_t)(w + 1));
+    int half = w / 2;
+    for (int i = 0; i < n; ++i) {
+        if (!have[i]) continue;
+        int lo = i - half < 0 ? 0 : i - half, hi = i + half + 1 > n ? n : i + half + 1, m = 0;
+        for (int j = lo; j < hi; ++j)
+            if (have[j]) seg[m++] = x[j];
+        qsort(seg, (size_t)m, sizeof(double), cmp_double);
+        out[i] = seg[m / 2];
+    }
+    memcpy(x, out, sizeof(double) * (size_t)n);
+    free(out); free(seg);
+}
+
+/* Gap fill with bounded reach (:1541-1583): linear interpolation between
+ * measured entries at most 2*max_fill apart; otherwise, and before the first
+ * / after the last measurement, a linear fade to 0 over max_fill entries. */
+static void nf_fill_gaps(double *x, unsigned char *have, int n, int max_fill) {
+    int *idx = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+    int ni = 0;
+    for (int i = 0; i < n; ++i)
+        if (have[i]) idx[ni++] = i;
+    if (ni == 0) { free(idx); return; }
+    double *out = (double *)calloc((size_t)n, sizeof(double));
+    unsigned char *oh = (unsigned char *)calloc((size_t)n, 1);
+#define NF_FADE(src, i) do { int d_ = abs((i) - (src)); double w_ = 1.0 - (double)d_ / (double)max_fill; \
+                             out[i] = x[src] * w_; oh[i] = 1; } while (0)
+    int f = idx[0], l = idx[ni - 1];
+    for (int i = (f - max_fill > 0 ? f - max_fill : 0); i < f; ++i) NF_FADE(f, i);
+    for (int i = l + 1; i < (l + 1 + max_fill < n ? l + 1 + max_fill : n); ++i) NF_FADE(l, i);
+    for (int k = 0; k + 1 < ni; ++k) {
+        int a = idx[k], b = idx[k + 1], gap = b - a;
+        double va = x[a], vb = x[b];
+        if (gap <= 2 * max_fill) {
+            for (int i = a + 1; i < b; ++i) {
+                double t = (double)(i - a) / (double)gap;
+                out[i] = va + t * (vb - va);
+                oh[i] = 1;
+            }
+        } else {
+            for (int i = a + 1; i <= a + max_fill; ++i) NF_FADE(a, i);
+            for (int i = b - max_fill; i < b; ++i) NF_FADE(b, i);
+        }
+    }
+#undef NF_FADE
+    for (int k = 0; k < ni; ++k) { out[idx[k]] = x[idx[k]]; oh[idx[k]] = 1; }
+    memcpy(x, out, sizeof(double) * (size_t)n);
+    memcpy(have, oh, (size_t)n);
+    free(idx); free(out); free(oh);
+}
+
+/* One internal boundary at column bc (vertical seam; by_col = 1) or row bc:
+ * profile diff = unw[just before] - unw[at] where both are valid, moving
+ * median (21), bounded gap fill (42), then unw += diff * (1 - d/feather) on
+ * valid pixels d = 0..feather-1 past the seam (:1585-1649). */
+static void nf_boundary(float *unw, const unsigned char *mask, int nrow, int ncol, int bc, int by_col,
+                        int feather) {
+    const int smooth = 21, max_fill = 2 * smooth;
+    int n = by_col ? nrow : ncol, lim = by_col ? ncol : nrow;
+    if (bc - 1 < 0 || bc + 1 >= lim) return;
+    double *diff = (double *)calloc((size_t)n, sizeof(double));
+    unsigned char *have = (unsigned char *)calloc((size_t)n, 1);
+    for (int i = 0; i < n; ++i) {
+        size_t lp = by_col ? (size_t)i * ncol + (bc - 1) : (size_t)(bc - 1) * ncol + i;
+        size_t rp = by_col ? (size_t)i * ncol + bc : (size_t)bc * ncol + i;
+        if ((mask && !mask[lp]) || (mask && !mask[rp])) continue;
+        diff[i] = (double)unw[lp] - (double)unw[rp];
+        have[i] = 1;
+    }
+    nf_moving_median(diff, have, n, smooth);
+    nf_fill_gaps(diff, have, n, max_fill);
+    int e1 = bc + feather < lim ? bc + feather : lim;
+    for (int i = 0; i < n; ++i) {
+        if (!have[i]) continue;
+        for (int k = bc; k < e1; ++k) {
+            size_t g = by_col ? (size_t)i * ncol + k : (size_t)k * ncol + i;
+            if (mask && !mask[g]) continue;
+            float w = 1.0f - (float)(k - bc) / (float)feather;
+            unw[g] += (float)(diff[i] * w);
+        }
+    }
+    free(diff); free(have);
+}
+
+/* Laplace neighbour-feedback refinement of internal tile seams (:1469-1657):
+ * column seams first (tile c's first column + col_ovrlp/2, reference tile
+ * formula :1475-1476), then row seams, each on the result of the previous. */
+void o_neighbor_feedback(float *unw, const unsigned char *mask, int nrow, int ncol, int ntr, int ntc,
+                         int row_ovrlp, int col_ovrlp, int feather) {
+    if (feather <= 0) return;
+    for (int tc = 1; tc < ntc; ++tc)
+        nf_boundary(unw, mask, nrow, ncol, (int)((long)tc * (ncol - col_ovrlp) / ntc) + col_ovrlp / 2, 1, feather);
+    for (int tr = 1; tr < ntr; ++tr)
+        nf_boundary(unw, mask, nrow, ncol, (int)((long)tr * (nrow - row_ovrlp) / ntr) + row_ovrlp / 2, 0, feather);
 }

@@ -70,5 +70,9 @@ int o_wls_solve_f64(const double *w, const double *phi, int nrow, int ncol, int 
                     double *u, int *iters);
 int o_wls_adjoint_f64(const double *w, const double *phi, const double *u, const double *gu,
                       int nrow, int ncol, int max_iter, double tol, double *gw, double *lambda);
+/* post-passes: isolated whole-cycle row/column spikes; tile-seam neighbour feedback */
+void o_fix_cycle_spikes(float *unw, const unsigned char *mask, int nrow, int ncol);
+void o_neighbor_feedback(float *unw, const unsigned char *mask, int nrow, int ncol, int ntr, int ntc,
+                         int row_ovrlp, int col_ovrlp, int feather);
 void o_set_threads(int n);
 #endif

@@ -711,3 +711,115 @@ def test_conncomp_gated_matches_scipy_partition():
     # same partition of the valid pixels (labels are a relabelling)
     pairs = set(zip(comp[valid].tolist(), lab[valid].tolist()))
     assert len(pairs) == len(set(comp[valid].tolist())) == len(set(lab[valid].tolist()))
+
+
+# ── post-passes: the reference's own hook tests (test_fix_cycle_spikes.py,
+#    test_laplace_neighbor_feedback.py) run on the oracle ──────────────────────
+
+def _ramp(nrow, ncol, a, b):
+    r, c = np.mgrid[0:nrow, 0:ncol]
+    return a * r + b * c
+
+
+def test_oracle_fix_cycle_spikes_reference_cases():
+    two_pi = 2.0 * np.pi
+    ramp = _ramp(60, 40, 0.02, 0.01)
+    u = ramp.astype(np.float32).copy()
+    u[27, :] -= np.float32(two_pi)   # isolated row
+    np.testing.assert_allclose(O.fix_cycle_spikes(u), ramp, atol=1e-3)
+    ramp = _ramp(40, 60, 0.015, 0.02)
+    u = ramp.astype(np.float32).copy()
+    u[:, 33] += np.float32(2.0 * two_pi)   # isolated column, two cycles
+    np.testing.assert_allclose(O.fix_cycle_spikes(u), ramp, atol=1e-3)
+    u = _ramp(50, 50, 0.05, 0.03).astype(np.float32)   # smooth ramp: untouched
+    np.testing.assert_array_equal(O.fix_cycle_spikes(u), u)
+    ramp = _ramp(60, 40, 0.02, 0.01)
+    u = ramp.astype(np.float32).copy()
+    u[20:40, :] -= np.float32(two_pi)   # sustained band: untouched
+    np.testing.assert_array_equal(O.fix_cycle_spikes(u), u)
+    u = ramp.astype(np.float32).copy()
+    u[27, :] -= np.float32(two_pi)
+    m = np.ones((60, 40), np.uint8)
+    m[27, :10] = 0   # masked pixels neither sampled nor corrected
+    out = O.fix_cycle_spikes(u, m)
+    np.testing.assert_array_equal(out[27, :10], u[27, :10])
+    np.testing.assert_allclose(out[27, 10:], ramp[27, 10:], atol=1e-3)
+
+
+def test_oracle_fix_cycle_spikes_short_lines_and_edges():
+    """Lines with <= 20 valid pixels have no median and are never judged;
+    the first and last lines are never shifted."""
+    u = _ramp(30, 20, 0.02, 0.01).astype(np.float32)   # rows of 20: no row medians
+    u[10, :] -= np.float32(2 * np.pi)
+    out = O.fix_cycle_spikes(u)
+    np.testing.assert_array_equal(out[10, :], u[10, :])   # rows skipped (20 valid)
+    u = _ramp(40, 40, 0.02, 0.01).astype(np.float32)
+    u[0, :] -= np.float32(2 * np.pi)
+    np.testing.assert_array_equal(O.fix_cycle_spikes(u)[0], u[0])
+
+
+def test_oracle_neighbor_feedback_reference_cases():
+    unw = np.zeros((50, 200), np.float32)
+    unw[:, :100], unw[:, 100:] = 10.0, 15.0
+    m = np.ones((50, 200), np.uint8)
+    fixed = O.neighbor_feedback(unw, m, 1, 2, 0, 64, 50)
+    assert abs(fixed[0, 100] - fixed[0, 99]) < 0.5
+    np.testing.assert_allclose(fixed[:, :90], unw[:, :90])
+    np.testing.assert_allclose(fixed[:, 160:], unw[:, 160:])
+    # closed form: seam at 1*(200-64)/2 + 32 = 100, diff = -5 on every row,
+    # unw[:, c] += -5 * (1 - (c - 100)/50) for c in [100, 150)
+    c = np.arange(100, 150)
+    w = (np.float32(1) - (c - 100).astype(np.float32) / np.float32(50)).astype(np.float32)
+    expect = (np.float32(15) + (-5.0 * w.astype(np.float64)).astype(np.float32)).astype(np.float32)
+    np.testing.assert_array_equal(fixed[:, 100:150], np.broadcast_to(expect, (50, 50)))
+    # local gradient: the seam mismatch must not grow
+    r, cc = np.mgrid[0:500, 0:200]
+    unw = (0.15 * cc).astype(np.float32)
+    before = np.abs(unw[:, 100] - unw[:, 99]).mean()
+    fixed = O.neighbor_feedback(unw, np.ones((500, 200), np.uint8), 1, 2, 0, 64, 200)
+    assert np.abs(fixed[:, 100] - fixed[:, 99]).mean() <= before * 1.1
+    # sparse boundary data is interpolated, not skipped
+    unw = np.zeros((60, 200), np.float32)
+    unw[:, :100], unw[:, 100:] = 10.0, 15.0
+    m = np.ones((60, 200), np.uint8)
+    m[:, 99:101] = 0
+    m[::10, 99:101] = 1
+    fixed = O.neighbor_feedback(unw, m, 1, 2, 0, 64, 50)
+    corr = fixed[:, 110] - unw[:, 110]
+    sparse = np.delete(corr, np.arange(0, 60, 10))
+    assert np.abs(sparse).min() > 1.0
+    np.testing.assert_allclose(sparse, np.median(corr[::10]), atol=1.0)
+    assert np.abs(np.diff(corr)).max() < 2.0
+    # far from any measurement the correction fades to zero
+    unw = np.zeros((400, 200), np.float32)
+    unw[:, :100], unw[:, 100:] = 10.0, 15.0
+    m = np.zeros((400, 200), np.uint8)
+    m[:, 150] = 1
+    m[190:200, 99:101] = 1
+    corr = O.neighbor_feedback(unw, m, 1, 2, 0, 64, 50)[:, 150] - unw[:, 150]
+    for i in (0, -1, 50, 350):
+        assert abs(corr[i]) < 0.5
+    # single tile: no seam
+    unw = np.full((40, 40), 3.5, np.float32)
+    np.testing.assert_array_equal(O.neighbor_feedback(unw, None, 1, 1, 0, 0, 200), unw)
+
+
+def test_oracle_neighbor_feedback_fill_closed_forms():
+    """Two measured rows 10 apart (gap <= 2*42): the moving median of the two
+    is the upper one (rank 1 of 2) at both, and rows between are linear in
+    between (equal here); beyond the last measurement the profile fades
+    linearly to 0 over 42 rows."""
+    nrow, ncol = 200, 64
+    unw = np.zeros((nrow, ncol), np.float32)
+    unw[:, 32:] = 4.0
+    m = np.zeros((nrow, ncol), np.uint8)
+    m[:, 40] = 1
+    m[[50, 60], 31:33] = 1
+    out = O.neighbor_feedback(unw, m, 1, 2, 0, 0, 16)   # seam at 64/2 = 32
+    d = (out[:, 40] - unw[:, 40]).astype(np.float64)
+    w40 = np.float32(1) - np.float32(8) / np.float32(16)
+    np.testing.assert_allclose(d[50:61], -4.0 * w40, rtol=0, atol=1e-6)
+    for k in (1, 20, 41):   # fade after row 60 and before row 50
+        np.testing.assert_allclose(d[60 + k], -4.0 * (1 - k / 42) * w40, atol=1e-5)
+        np.testing.assert_allclose(d[50 - k], -4.0 * (1 - k / 42) * w40, atol=1e-5)
+    assert d[60 + 42] == 0.0 and d[50 - 42] == 0.0
