@@ -1578,12 +1578,14 @@ int cuphu_unwrap_ex(const float *igram_r, const float *igram_i, const float *igr
                 scene_impl<float>(d_ig.get(), d_corr.get(), mask ? d_mask.get() : nullptr, conf, nrow, ncol,
                                   params->ncorrlooks, *params, 0, rov, cov, tile->ntilerow, tile->ntilecol, o,
                                   d_unw.get(), d_cc.get(), stats, nullptr, res, st);
-            // optional post-passes in the reference's order (cuphu_solver.cpp:2037-2095)
+            // optional post-passes in the reference's order (cuphu_solver.cpp:2037-2095); like the
+            // reference, the single-tile path skips both (it returns before them, :1714-1723)
             if (!mcf && tile->laplace_neighbor_feedback && !single)
                 neighbor_feedback_dev(d_unw.get(), mask ? d_mask.get() : nullptr, nrow, ncol,
                                       seams(ncol, cov, tile->ntilecol), seams(nrow, rov, tile->ntilerow),
                                       tile->laplace_neighbor_feedback_feather, st);
-            if (tile->fix_cycle_spikes) fix_cycle_spikes_dev(d_unw.get(), mask ? d_mask.get() : nullptr, nrow, ncol, st);
+            if (tile->fix_cycle_spikes && !single)
+                fix_cycle_spikes_dev(d_unw.get(), mask ? d_mask.get() : nullptr, nrow, ncol, st);
             CUPHU_HIP_CHECK(hipMemcpyAsync(result->unw, d_unw.get(), n * 4, hipMemcpyDeviceToHost, st));
             CUPHU_HIP_CHECK(hipMemcpyAsync(result->conncomp, d_cc.get(), n * 4, hipMemcpyDeviceToHost, st));
             CUPHU_HIP_CHECK(hipStreamSynchronize(st));

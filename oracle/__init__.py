@@ -316,6 +316,54 @@ def neighbor_feedback(unw, mask, ntilerow, ntilecol, row_ovrlp, col_ovrlp, feath
     return out
 
 
+RAMP_TYPES = {None: 0, "linear": 1, "quadratic": 2, "linear_range": 3, "linear_azimuth": 4,
+              "quadratic_range": 5, "quadratic_azimuth": 6}
+
+
+def bridge(unw, mask=None, radius=500, min_num_pixel=14, erosion_size=2, max_boundary_samples=4096,
+           ramp_type=None, ramp_max_num_sample=1000000):
+    """Phase bridging; returns (unw, cc, nregions) -- copies; cc is all 0
+    and unw unchanged when fewer than two regions remain."""
+    out = np.array(unw, dtype=np.float32, order="C", copy=True)
+    cc = np.zeros(out.shape, np.uint32)
+    m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
+    nreg = _lib.o_bridge(_p(out), _p(cc), _p(m), C.c_int(out.shape[0]), C.c_int(out.shape[1]), C.c_int(radius),
+                         C.c_int(min_num_pixel), C.c_int(erosion_size), C.c_int(max_boundary_samples),
+                         C.c_int(RAMP_TYPES[ramp_type]), C.c_int(ramp_max_num_sample))
+    return out, cc, nreg
+
+
+def bridge_label(topo):
+    topo = _f32(topo)
+    lab = np.zeros(topo.shape, np.uint32)
+    n = _lib.o_bridge_label(_p(topo), C.c_int(topo.shape[0]), C.c_int(topo.shape[1]), _p(lab))
+    return lab, n
+
+
+def bridge_erode_prune(labels, n, e, circular):
+    lab = np.array(labels, dtype=np.uint32, order="C", copy=True)
+    k = _lib.o_bridge_erode_prune(_p(lab), C.c_int(lab.shape[0]), C.c_int(lab.shape[1]), C.c_int(n), C.c_int(e),
+                                  C.c_int(int(circular)))
+    return lab, k
+
+
+def bridge_boundary(labels):
+    lab = np.ascontiguousarray(labels, dtype=np.uint32)
+    b = np.zeros(lab.shape, np.uint8)
+    _lib.o_bridge_boundary(_p(lab), C.c_int(lab.shape[0]), C.c_int(lab.shape[1]), _p(b))
+    return b
+
+
+def bridge_nn(labels, boundary, n, max_samples=4096):
+    lab = np.ascontiguousarray(labels, dtype=np.uint32)
+    b = np.ascontiguousarray(boundary, dtype=np.uint8)
+    dm = np.empty((n + 1) * (n + 1), np.float32)
+    ep = np.empty((n + 1) * (n + 1) * 4, np.int32)
+    _lib.o_bridge_nn(_p(lab), _p(b), C.c_int(lab.shape[0]), C.c_int(lab.shape[1]), C.c_int(n),
+                     C.c_int(max_samples), _p(dm), _p(ep))
+    return dm.reshape(n + 1, n + 1), ep.reshape(n + 1, n + 1, 4)
+
+
 def wls_solve_f64(w, phi, max_iter=10000, tol=1e-12):
     w = np.ascontiguousarray(w, dtype=np.float64)
     phi = np.ascontiguousarray(phi, dtype=np.float64)
@@ -347,10 +395,12 @@ def _sig(fn, res, args):
 for _n in ("o_set_threads", "o_wrapped_phase", "o_phase_diffs", "o_boxcar2d", "o_costs", "o_residues",
            "o_arc_weights", "o_assemble", "o_matvec", "o_reweight", "o_congruence", "o_pcg_irls_f32",
            "o_pcg_irls_f64", "o_pcg_step_f32", "o_pcg_fused_step_f32", "o_conncomp", "o_conncomp_gated", "o_incrcost_smooth", "o_integrate_phase", "o_tile_axis", "o_stitch",
-           "o_wls_solve_f64", "o_wls_adjoint_f64", "o_fix_cycle_spikes", "o_neighbor_feedback"):
+           "o_wls_solve_f64", "o_wls_adjoint_f64", "o_fix_cycle_spikes", "o_neighbor_feedback", "o_bridge",
+           "o_bridge_label", "o_bridge_prune_small", "o_bridge_erode_prune", "o_bridge_boundary", "o_bridge_nn"):
     getattr(_lib, _n).restype = C.c_int if _n not in ("o_set_threads", "o_wrapped_phase", "o_phase_diffs",
                                                        "o_costs", "o_residues", "o_arc_weights",
                                                        "o_assemble", "o_matvec", "o_reweight",
                                                        "o_congruence", "o_pcg_fused_step_f32",
                                                        "o_integrate_phase", "o_fix_cycle_spikes",
-                                                       "o_neighbor_feedback") else None
+                                                       "o_neighbor_feedback", "o_bridge_boundary",
+                                                       "o_bridge_nn") else None

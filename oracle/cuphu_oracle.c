@@ -1117,3 +1117,387 @@ void o_neighbor_feedback(float *unw, const unsigned char *mask, int nrow, int nc
     for (int tr = 1; tr < ntr; ++tr)
         nf_boundary(unw, mask, nrow, ncol, (int)((long)tr * (nrow - row_ovrlp) / ntr) + row_ovrlp / 2, 0, feather);
 }
+
+/* ── phase bridging (reference cuphu_solver.cpp:1000-1366 and
+ *    cuphu_bridge_gpu.cu) ──────────────────────────────────────────────── */
+
+static uint32_t br_find(uint32_t *p, uint32_t x) {
+    while (p[x] != x) x = p[x];
+    return x;
+}
+static void br_union(uint32_t *p, uint32_t a, uint32_t b) {
+    a = br_find(p, a);
+    b = br_find(p, b);
+    if (a == b) return;
+    if (a < b) p[b] = a; else p[a] = b;   /* the smaller index stays root */
+}
+
+/* 8-connected regions of topo != 0 (cuphu_bridge_gpu.cu:1-20 and the
+ * label entry point, ~:620-672): labels 1..N numbered by each region's
+ * minimum raster index, 0 elsewhere.  Returns N. */
+int o_bridge_label(const float *topo, int nrow, int ncol, uint32_t *labels) {
+    size_t n = (size_t)nrow * ncol;
+    uint32_t *p = (uint32_t *)malloc(sizeof(uint32_t) * (n ? n : 1));
+    for (size_t i = 0; i < n; ++i) p[i] = (uint32_t)i;
+    for (int r = 0; r < nrow; ++r)
+        for (int c = 0; c < ncol; ++c) {
+            size_t i = (size_t)r * ncol + c;
+            if (topo[i] == 0.0f) continue;
+            if (c + 1 < ncol && topo[i + 1] != 0.0f) br_union(p, (uint32_t)i, (uint32_t)(i + 1));
+            if (r + 1 < nrow) {
+                for (int dc = -1; dc <= 1; ++dc) {
+                    int cc = c + dc;
+                    if (cc < 0 || cc >= ncol) continue;
+                    size_t j = (size_t)(r + 1) * ncol + cc;
+                    if (topo[j] != 0.0f) br_union(p, (uint32_t)i, (uint32_t)j);
+                }
+            }
+        }
+    uint32_t *map = (uint32_t *)calloc(n ? n : 1, sizeof(uint32_t));
+    uint32_t next = 1;
+    for (size_t i = 0; i < n; ++i)
+        if (topo[i] != 0.0f && br_find(p, (uint32_t)i) == i) map[i] = next++;
+    for (size_t i = 0; i < n; ++i) labels[i] = (topo[i] != 0.0f) ? map[br_find(p, (uint32_t)i)] : 0u;
+    free(p); free(map);
+    return (int)next - 1;
+}
+
+static int br_remap(uint32_t *labels, size_t n, int N, const unsigned char *keep) {
+    uint32_t *rm = (uint32_t *)calloc((size_t)N + 1, sizeof(uint32_t));
+    uint32_t next = 1;
+    for (int l = 1; l <= N; ++l)
+        if (keep[l]) rm[l] = next++;
+    for (size_t i = 0; i < n; ++i) labels[i] = rm[labels[i]];
+    free(rm);
+    return (int)next - 1;
+}
+
+/* regions smaller than min_num_pixel dropped, compact relabel (:1021-1032) */
+int o_bridge_prune_small(uint32_t *labels, int nrow, int ncol, int N, int min_num_pixel) {
+    size_t n = (size_t)nrow * ncol;
+    long *cnt = (long *)calloc((size_t)N + 1, sizeof(long));
+    unsigned char *keep = (unsigned char *)calloc((size_t)N + 1, 1);
+    for (size_t i = 0; i < n; ++i) cnt[labels[i]]++;
+    for (int l = 1; l <= N; ++l) keep[l] = cnt[l] >= min_num_pixel;
+    int out = br_remap(labels, n, N, keep);
+    free(cnt); free(keep);
+    return out;
+}
+
+/* A region is dropped when none of its pixels survives a binary erosion by
+ * the (2e+1)^2 square (circular: offsets with dr^2+dc^2 <= e^2) with the
+ * outside counted as background; kept regions keep their full shape
+ * (cuphu_bridge_gpu.cu:143-262). */
+int o_bridge_erode_prune(uint32_t *labels, int nrow, int ncol, int N, int e, int circular) {
+    if (e <= 0 || N == 0) return N;
+    size_t n = (size_t)nrow * ncol;
+    unsigned char *keep = (unsigned char *)calloc((size_t)N + 1, 1);
+    for (int r = 0; r < nrow; ++r)
+        for (int c = 0; c < ncol; ++c) {
+            uint32_t l = labels[(size_t)r * ncol + c];
+            if (!l || keep[l]) continue;
+            int ok = 1;
+            for (int dr = -e; dr <= e && ok; ++dr)
+                for (int dc = -e; dc <= e; ++dc) {
+                    if (circular && dr * dr + dc * dc > e * e) continue;
+                    int rr = r + dr, cc = c + dc;
+                    if (rr < 0 || rr >= nrow || cc < 0 || cc >= ncol || !labels[(size_t)rr * ncol + cc]) { ok = 0; break; }
+                }
+            if (ok) keep[l] = 1;
+        }
+    int out = br_remap(labels, n, N, keep);
+    free(keep);
+    return out;
+}
+
+/* boundary = labelled pixel whose 3x3 neighbourhood (outside = itself)
+ * holds more than one label value (cuphu_bridge_gpu.cu:264-306) */
+void o_bridge_boundary(const uint32_t *labels, int nrow, int ncol, unsigned char *b) {
+    for (int r = 0; r < nrow; ++r)
+        for (int c = 0; c < ncol; ++c) {
+            uint32_t ctr = labels[(size_t)r * ncol + c], mx = ctr, mn = ctr;
+            for (int dr = -1; dr <= 1; ++dr)
+                for (int dc = -1; dc <= 1; ++dc) {
+                    int rr = r + dr, cc = c + dc;
+                    uint32_t v = (rr < 0 || rr >= nrow || cc < 0 || cc >= ncol) ? ctr : labels[(size_t)rr * ncol + cc];
+                    if (v > mx) mx = v;
+                    if (v < mn) mn = v;
+                }
+            b[(size_t)r * ncol + c] = (mx != mn) && ctr > 0;
+        }
+}
+
+/* packed search key: distance quantised to 1/64 px above the two points'
+ * per-region indices, 12 bits each (cuphu_bridge_gpu.cu:338-360) */
+static uint64_t br_key(float d, int lp, int lq) {
+    uint64_t qd = (uint64_t)(d * 64.0f);
+    return (qd << 24) | ((uint64_t)lp << 12) | (uint64_t)lq;
+}
+
+/* Nearest boundary points between every two regions.  Each region's
+ * boundary pixels in raster order, every max(1, m/max_samples)-th one up to
+ * max_samples (<= 4096); for every ordered region pair the minimum packed
+ * key over all point pairs (the reference's grid search covers the whole
+ * image, so it is this minimum, :363-600).  distmat [(N+1)^2] gets the
+ * quantised distance or -1, endpoints [(N+1)^2 * 4] y0, x0, y1, x1 or -1. */
+void o_bridge_nn(const uint32_t *labels, const unsigned char *bnd, int nrow, int ncol, int N, int max_samples,
+                 float *distmat, int *endpoints) {
+    int n1 = N + 1;
+    if (max_samples > 4096) max_samples = 4096;
+    for (long i = 0; i < (long)n1 * n1; ++i) {
+        distmat[i] = -1.0f;
+        endpoints[4 * i] = endpoints[4 * i + 1] = endpoints[4 * i + 2] = endpoints[4 * i + 3] = -1;
+    }
+    if (N <= 1) return;
+    size_t n = (size_t)nrow * ncol;
+    long *cnt = (long *)calloc((size_t)n1, sizeof(long));
+    for (size_t i = 0; i < n; ++i)
+        if (bnd[i] && labels[i] && (int)labels[i] <= N) cnt[labels[i]]++;
+    long *start = (long *)calloc((size_t)n1 + 1, sizeof(long));
+    for (int l = 1; l <= N; ++l) start[l + 1] = start[l] + cnt[l];
+    int *all = (int *)malloc(sizeof(int) * (size_t)(start[n1] ? start[n1] : 1));
+    long *fill = (long *)calloc((size_t)n1, sizeof(long));
+    for (size_t i = 0; i < n; ++i)
+        if (bnd[i] && labels[i] && (int)labels[i] <= N) { uint32_t l = labels[i]; all[start[l] + fill[l]++] = (int)i; }
+    /* subsampled points per region */
+    int *px = (int *)malloc(sizeof(int) * (size_t)n1 * max_samples), *py = (int *)malloc(sizeof(int) * (size_t)n1 * max_samples);
+    int *np = (int *)calloc((size_t)n1, sizeof(int));
+    for (int l = 1; l <= N; ++l) {
+        long m = cnt[l], stride = m / max_samples > 1 ? m / max_samples : 1;
+        for (long k = 0; k < m && np[l] < max_samples; k += stride) {
+            int idx = all[start[l] + k];
+            px[(long)l * max_samples + np[l]] = idx % ncol;
+            py[(long)l * max_samples + np[l]] = idx / ncol;
+            np[l]++;
+        }
+    }
+    for (int li = 1; li <= N; ++li)
+        for (int lj = 1; lj <= N; ++lj) {
+            if (li == lj) continue;
+            uint64_t best = ~0ull;
+            for (int a = 0; a < np[li]; ++a)
+                for (int b = 0; b < np[lj]; ++b) {
+                    float dx = (float)(px[(long)lj * max_samples + b] - px[(long)li * max_samples + a]);
+                    float dy = (float)(py[(long)lj * max_samples + b] - py[(long)li * max_samples + a]);
+                    float dxx = dx * dx, dyy = dy * dy;
+                    float d = sqrtf(dxx + dyy);
+                    uint64_t k = br_key(d, a, b);
+                    if (k < best) best = k;
+                }
+            if (best == ~0ull) continue;
+            long t = (long)li * n1 + lj;
+            int a = (int)((best >> 12) & 4095u), b = (int)(best & 4095u);
+            distmat[t] = (float)(best >> 24) / 64.0f;
+            endpoints[4 * t + 0] = py[(long)li * max_samples + a];
+            endpoints[4 * t + 1] = px[(long)li * max_samples + a];
+            endpoints[4 * t + 2] = py[(long)lj * max_samples + b];
+            endpoints[4 * t + 3] = px[(long)lj * max_samples + b];
+        }
+    free(cnt); free(start); free(all); free(fill); free(px); free(py); free(np);
+}
+
+/* ramp basis (:1049-1067) */
+static int br_basis(int rt, double row, double col, double g[6]) {
+    switch (rt) {
+    case 1: g[0] = row; g[1] = col; g[2] = 1; return 3;
+    case 2: g[0] = row * row; g[1] = col * col; g[2] = row * col; g[3] = row; g[4] = col; g[5] = 1; return 6;
+    case 3: g[0] = col; g[1] = 1; return 2;
+    case 4: g[0] = row; g[1] = 1; return 2;
+    case 5: g[0] = col * col; g[1] = col; g[2] = 1; return 3;
+    case 6: g[0] = row * row; g[1] = row; g[2] = 1; return 3;
+    default: return 0;
+    }
+}
+
+/* normal equations over the reference region's pixels (every step-th row
+ * and column at offset step/2 once the count exceeds max_samples, raster
+ * order), Gaussian elimination with partial pivoting (:1069-1141) */
+static int br_fit_ramp(const float *unw, const uint32_t *labels, int nrow, int ncol, uint32_t ref, int rt,
+                       int max_samples, double *coef, int *nterm) {
+    double dummy[6];
+    *nterm = br_basis(rt, 0, 0, dummy);
+    if (*nterm == 0) return 0;
+    long count = 0;
+    for (size_t i = 0; i < (size_t)nrow * ncol; ++i)
+        if (labels[i] == ref && unw[i] != 0.0f) ++count;
+    if (count == 0) return 0;
+    int step = 1;
+    if (max_samples > 0 && count > max_samples) step = (int)ceil(sqrt((double)count / max_samples));
+    int half = step / 2, m = *nterm;
+    double A[6][6] = {{0}}, b[6] = {0};
+    for (int r = 0; r < nrow; ++r) {
+        if (step > 1 && r % step != half) continue;
+        for (int c = 0; c < ncol; ++c) {
+            if (step > 1 && c % step != half) continue;
+            size_t i = (size_t)r * ncol + c;
+            if (labels[i] != ref || unw[i] == 0.0f) continue;
+            double g[6];
+            br_basis(rt, (double)r, (double)c, g);
+            for (int a = 0; a < m; ++a) {
+                b[a] += g[a] * (double)unw[i];
+                for (int k = 0; k < m; ++k) A[a][k] += g[a] * g[k];
+            }
+        }
+    }
+    for (int p = 0; p < m; ++p) {
+        int piv = p;
+        for (int r = p + 1; r < m; ++r)
+            if (fabs(A[r][p]) > fabs(A[piv][p])) piv = r;
+        if (fabs(A[piv][p]) < 1e-12) return 0;
+        if (piv != p) {
+            for (int k = 0; k < 6; ++k) { double t = A[piv][k]; A[piv][k] = A[p][k]; A[p][k] = t; }
+            double t = b[piv]; b[piv] = b[p]; b[p] = t;
+        }
+        for (int r = p + 1; r < m; ++r) {
+            double f = A[r][p] / A[p][p];
+            for (int k = p; k < m; ++k) A[r][k] -= f * A[p][k];
+            b[r] -= f * b[p];
+        }
+    }
+    for (int r = m - 1; r >= 0; --r) {
+        double s = b[r];
+        for (int k = r + 1; k < m; ++k) s -= A[r][k] * coef[k];
+        coef[r] = s / A[r][r];
+    }
+    return 1;
+}
+
+static void br_apply_ramp(float *unw, int nrow, int ncol, int rt, const double *coef, int m, double sign,
+                          const unsigned char *nz) {
+    for (int r = 0; r < nrow; ++r)
+        for (int c = 0; c < ncol; ++c) {
+            size_t i = (size_t)r * ncol + c;
+            if (!nz[i]) continue;
+            double g[6], ramp = 0.0;
+            br_basis(rt, (double)r, (double)c, g);
+            for (int a = 0; a < m; ++a) ramp += coef[a] * g[a];
+            unw[i] += (float)(sign * ramp);
+        }
+}
+
+static int br_median(const float *unw, const uint32_t *labels, int nrow, int ncol, int cy, int cx, int rad,
+                     uint32_t region, double *out) {
+    int r0 = cy - rad > 0 ? cy - rad : 0, r1 = cy + rad + 1 < nrow ? cy + rad + 1 : nrow;
+    int c0 = cx - rad > 0 ? cx - rad : 0, c1 = cx + rad + 1 < ncol ? cx + rad + 1 : ncol;
+    if (r1 <= r0 || c1 <= c0) return 0;
+    float *v = (float *)malloc(sizeof(float) * (size_t)(r1 - r0) * (c1 - c0));
+    int m = 0;
+    for (int r = r0; r < r1; ++r)
+        for (int c = c0; c < c1; ++c)
+            if (labels[(size_t)r * ncol + c] == region) v[m++] = unw[(size_t)r * ncol + c];
+    if (m == 0) { free(v); return 0; }
+    qsort(v, (size_t)m, sizeof(float), cmp_float);
+    *out = (double)v[m / 2];
+    free(v);
+    return 1;
+}
+
+/* The whole pass (:1270-1353).  Regions are computed on unw with masked
+ * pixels zeroed; the reference region is the largest (lowest label on a
+ * tie); an optional ramp fitted on it is removed first and restored last;
+ * regions are joined along a Prim minimum spanning tree of the boundary
+ * distances, visited breadth-first from the reference region; each bridge
+ * takes the whole-cycle jump between the two regions' window medians
+ * around its endpoints.  cc receives the region labels.  Returns the
+ * number of regions bridged (0 or 1: unw and cc untouched). */
+int o_bridge(float *unw, uint32_t *cc, const unsigned char *mask, int nrow, int ncol, int radius,
+             int min_num_pixel, int erosion_size, int max_samples, int ramp_type, int ramp_max_samples) {
+    size_t n = (size_t)nrow * ncol;
+    float *topo = (float *)malloc(sizeof(float) * (n ? n : 1));
+    for (size_t i = 0; i < n; ++i) topo[i] = (mask && !mask[i]) ? 0.0f : unw[i];
+    uint32_t *lab = (uint32_t *)malloc(sizeof(uint32_t) * (n ? n : 1));
+    int N = o_bridge_label(topo, nrow, ncol, lab);
+    free(topo);
+    if (N > 1) N = o_bridge_prune_small(lab, nrow, ncol, N, min_num_pixel);
+    if (N > 1) N = o_bridge_erode_prune(lab, nrow, ncol, N, erosion_size, 0);
+    if (N > 1) N = o_bridge_erode_prune(lab, nrow, ncol, N, erosion_size, 1);
+    if (N <= 1) { free(lab); return N; }
+    int n1 = N + 1;
+    long *cnt = (long *)calloc((size_t)n1, sizeof(long));
+    for (size_t i = 0; i < n; ++i) cnt[lab[i]]++;
+    int ref = 1;
+    for (int l = 2; l <= N; ++l)
+        if (cnt[l] > cnt[ref]) ref = l;
+    double coef[6] = {0};
+    int nterm = 0, ramp = 0;
+    unsigned char *nz = NULL;
+    if (ramp_type != 0 && br_fit_ramp(unw, lab, nrow, ncol, (uint32_t)ref, ramp_type, ramp_max_samples, coef, &nterm)) {
+        ramp = 1;
+        nz = (unsigned char *)malloc(n ? n : 1);
+        for (size_t i = 0; i < n; ++i) nz[i] = unw[i] != 0.0f;
+        br_apply_ramp(unw, nrow, ncol, ramp_type, coef, nterm, -1.0, nz);
+    }
+    unsigned char *bnd = (unsigned char *)malloc(n ? n : 1);
+    o_bridge_boundary(lab, nrow, ncol, bnd);
+    float *dm = (float *)malloc(sizeof(float) * (size_t)n1 * n1);
+    int *ep = (int *)malloc(sizeof(int) * 4 * (size_t)n1 * n1);
+    o_bridge_nn(lab, bnd, nrow, ncol, N, max_samples, dm, ep);
+    /* Prim from ref (strict comparisons: the lowest label wins ties), then
+     * BFS over the tree in edge-insertion order (:1169-1229) */
+    unsigned char *in = (unsigned char *)calloc((size_t)n1, 1);
+    float *bd = (float *)malloc(sizeof(float) * (size_t)n1);
+    int *bp = (int *)malloc(sizeof(int) * (size_t)n1);
+    int *ea = (int *)malloc(sizeof(int) * (size_t)n1), *eb = (int *)malloc(sizeof(int) * (size_t)n1);
+    int ne = 0;
+#define BR_D(i, j) (dm[(long)(i) * n1 + (j)] < 0 ? INFINITY : dm[(long)(i) * n1 + (j)])
+    in[ref] = 1;
+    for (int j = 1; j <= N; ++j) { bd[j] = (j == ref) ? INFINITY : BR_D(ref, j); bp[j] = ref; }
+    for (int it = 1; it < N; ++it) {
+        int u = -1;
+        float ud = INFINITY;
+        for (int j = 1; j <= N; ++j)
+            if (!in[j] && bd[j] < ud) { ud = bd[j]; u = j; }
+        if (u < 0) break;
+        in[u] = 1;
+        ea[ne] = bp[u]; eb[ne] = u; ne++;
+        for (int j = 1; j <= N; ++j) {
+            if (in[j]) continue;
+            float d = BR_D(u, j);
+            if (d < bd[j]) { bd[j] = d; bp[j] = u; }
+        }
+    }
+#undef BR_D
+    /* adjacency in edge order */
+    int *deg = (int *)calloc((size_t)n1 + 1, sizeof(int)), *adj = (int *)malloc(sizeof(int) * (size_t)(2 * ne + 1));
+    for (int k = 0; k < ne; ++k) { deg[ea[k] + 1]++; deg[eb[k] + 1]++; }
+    for (int l = 0; l < n1; ++l) deg[l + 1] += deg[l];
+    int *pos = (int *)calloc((size_t)n1, sizeof(int));
+    for (int k = 0; k < ne; ++k) {
+        adj[deg[ea[k]] + pos[ea[k]]++] = eb[k];
+        adj[deg[eb[k]] + pos[eb[k]]++] = ea[k];
+    }
+    int rad = radius < (nrow < ncol ? nrow : ncol) / 2 ? radius : (nrow < ncol ? nrow : ncol) / 2;
+    double *corr = (double *)calloc((size_t)n1, sizeof(double));
+    unsigned char *vis = (unsigned char *)calloc((size_t)n1, 1);
+    int *q = (int *)malloc(sizeof(int) * (size_t)n1);
+    int qh = 0, qt = 0;
+    q[qt++] = ref;
+    vis[ref] = 1;
+    while (qh < qt) {
+        int u = q[qh++];
+        for (int k = deg[u]; k < deg[u + 1]; ++k) {
+            int v = adj[k];
+            if (vis[v]) continue;
+            vis[v] = 1;
+            q[qt++] = v;
+            long t = (long)u * n1 + v;
+            double m0, m1;
+            int ok0 = br_median(unw, lab, nrow, ncol, ep[4 * t], ep[4 * t + 1], rad, (uint32_t)u, &m0);
+            int ok1 = br_median(unw, lab, nrow, ncol, ep[4 * t + 2], ep[4 * t + 3], rad, (uint32_t)v, &m1);
+            if (!ok0 || !ok1) continue;
+            double diff = m1 - (m0 + corr[u]);
+            double nj = floor((fabs(diff) + M_PI) / (2.0 * M_PI));
+            if (diff > 0) nj *= -1.0;
+            corr[v] = nj * (2.0 * M_PI);
+        }
+    }
+    for (size_t i = 0; i < n; ++i) {
+        uint32_t l = lab[i];
+        cc[i] = l;
+        if (l) unw[i] += (float)corr[l];
+    }
+    if (ramp) br_apply_ramp(unw, nrow, ncol, ramp_type, coef, nterm, 1.0, nz);
+    free(lab); free(cnt); free(nz); free(bnd); free(dm); free(ep); free(in); free(bd); free(bp); free(ea); free(eb);
+    free(deg); free(adj); free(pos); free(corr); free(vis); free(q);
+    return N;
+}

@@ -74,5 +74,14 @@ int o_wls_adjoint_f64(const double *w, const double *phi, const double *u, const
 void o_fix_cycle_spikes(float *unw, const unsigned char *mask, int nrow, int ncol);
 void o_neighbor_feedback(float *unw, const unsigned char *mask, int nrow, int ncol, int ntr, int ntc,
                          int row_ovrlp, int col_ovrlp, int feather);
+/* phase bridging and its stages */
+int o_bridge_label(const float *topo, int nrow, int ncol, uint32_t *labels);
+int o_bridge_prune_small(uint32_t *labels, int nrow, int ncol, int N, int min_num_pixel);
+int o_bridge_erode_prune(uint32_t *labels, int nrow, int ncol, int N, int e, int circular);
+void o_bridge_boundary(const uint32_t *labels, int nrow, int ncol, unsigned char *b);
+void o_bridge_nn(const uint32_t *labels, const unsigned char *bnd, int nrow, int ncol, int N, int max_samples,
+                 float *distmat, int *endpoints);
+int o_bridge(float *unw, uint32_t *cc, const unsigned char *mask, int nrow, int ncol, int radius,
+             int min_num_pixel, int erosion_size, int max_samples, int ramp_type, int ramp_max_samples);
 void o_set_threads(int n);
 #endif

@@ -823,3 +823,113 @@ def test_oracle_neighbor_feedback_fill_closed_forms():
         np.testing.assert_allclose(d[60 + k], -4.0 * (1 - k / 42) * w40, atol=1e-5)
         np.testing.assert_allclose(d[50 - k], -4.0 * (1 - k / 42) * w40, atol=1e-5)
     assert d[60 + 42] == 0.0 and d[50 - 42] == 0.0
+
+
+# ── phase bridging: the reference's hook tests (test_bridge.py) and its stages
+#    against scipy.ndimage ────────────────────────────────────────────────────
+
+def _bridge_fixture_three_regions():
+    nrow, ncol = 60, 90
+    rng = np.random.default_rng(0)
+    r, c = np.mgrid[0:nrow, 0:ncol]
+    ramp = 0.02 * r + 0.015 * c
+    off = np.zeros((nrow, ncol))
+    off[(r >= 20) & (r < 39)] = 3.0 * 2 * np.pi
+    off[r >= 40] = -2.0 * 2 * np.pi
+    unw = (ramp + off).astype(np.float32) + rng.normal(0.0, 0.05, (nrow, ncol)).astype(np.float32)
+    m = np.ones((nrow, ncol), np.uint8)
+    m[19, :] = 0
+    m[39, :] = 0
+    return unw, m, ramp
+
+
+def test_oracle_bridge_reference_cases():
+    two_pi = 2 * np.pi
+    unw, m, ramp = _bridge_fixture_three_regions()
+    u, cc, n = O.bridge(unw, m, min_num_pixel=1, erosion_size=0)
+    v = m == 1
+    assert n == 3 and (u[v] - ramp[v]).std() < 0.3 and not np.allclose(u[v], unw[v])
+    u, cc, n = O.bridge(np.full((30, 30), 1.2345, np.float32))
+    assert n == 1 and np.array_equal(u, np.full((30, 30), 1.2345, np.float32)) and not cc.any()
+    r, c = np.mgrid[0:40, 0:60]
+    ramp = 1.0 + 0.02 * r + 0.01 * c
+    unw = (ramp + np.where(c >= 30, 4.0 * two_pi, 0.0)).astype(np.float32)
+    m = np.ones((40, 60), np.uint8)
+    m[:, 28:32] = 0
+    unw[:, 28:32] = 99.0
+    u, cc, n = O.bridge(unw, m, min_num_pixel=1, erosion_size=0)
+    v = m == 1
+    assert (u[v] - ramp[v]).std() < 0.3
+    np.testing.assert_array_equal(u[:, 28:32], unw[:, 28:32])
+    r, _ = np.mgrid[0:200, 0:10]
+    ramp = 5.0 + 0.2 * r.astype(np.float64)
+    unw = ramp.astype(np.float32)
+    m = np.ones((200, 10), np.uint8)
+    m[90:110, :] = 0
+    u0, _, _ = O.bridge(unw, m, radius=15, min_num_pixel=1, erosion_size=0)
+    u1, _, _ = O.bridge(unw, m, radius=15, min_num_pixel=1, erosion_size=0, ramp_type="linear_azimuth")
+    v = m == 1
+    assert (u0[v] - ramp[v]).std() > 1.0
+    np.testing.assert_allclose(u1[v], ramp[v], atol=1e-3)
+
+
+def test_oracle_bridge_stages_match_scipy():
+    from scipy import ndimage
+
+    rng = np.random.default_rng(11)
+    topo = ndimage.gaussian_filter(rng.normal(size=(120, 140)), 3)
+    topo = np.where(topo > 0.05, 1.0 + rng.random((120, 140)), 0.0).astype(np.float32)
+    lab, n = O.bridge_label(topo)
+    ref, nref = ndimage.label(topo != 0, structure=np.ones((3, 3)))
+    assert n == nref > 3
+    np.testing.assert_array_equal(lab, ref.astype(np.uint32))
+    for e in (1, 2, 3):
+        for circ in (0, 1):
+            se = np.ones((2 * e + 1, 2 * e + 1), bool)
+            if circ:
+                dr, dc = np.mgrid[-e:e + 1, -e:e + 1]
+                se = dr * dr + dc * dc <= e * e
+            er = ndimage.binary_erosion(lab > 0, structure=se, border_value=0)
+            keep = np.unique(lab[er])
+            keep = keep[keep > 0]
+            remap = np.zeros(n + 1, np.uint32)
+            remap[keep] = np.arange(1, len(keep) + 1)
+            got, k = O.bridge_erode_prune(lab, n, e, circ)
+            assert k == len(keep)
+            np.testing.assert_array_equal(got, remap[lab])
+    b = O.bridge_boundary(lab)
+    mx = ndimage.maximum_filter(lab, size=3, mode="reflect")
+    mn = ndimage.minimum_filter(lab, size=3, mode="reflect")
+    np.testing.assert_array_equal(b.astype(bool), (mx != mn) & (lab > 0))
+
+
+def test_oracle_bridge_nearest_pairs_brute_force():
+    """Per ordered region pair: the minimum of (floor(64*dist), local index
+    of the first point, of the second) over the subsampled boundary points,
+    by a numpy brute force."""
+    rng = np.random.default_rng(12)
+    lab = np.zeros((50, 70), np.uint32)
+    lab[2:15, 3:30], lab[20:45, 5:20], lab[25:48, 40:66], lab[3:12, 45:60] = 1, 2, 3, 4
+    b = O.bridge_boundary(lab)
+    for ms in (4096, 7):
+        dm, ep = O.bridge_nn(lab, b, 4, ms)
+        pts = {}
+        for l in range(1, 5):
+            idx = np.flatnonzero((b.ravel() > 0) & (lab.ravel() == l))
+            stride = max(1, len(idx) // ms)
+            pts[l] = idx[::stride][:ms]
+        for i in range(1, 5):
+            for j in range(1, 5):
+                if i == j:
+                    assert dm[i, j] == -1
+                    continue
+                pi, pj = pts[i], pts[j]
+                dy = (pj[None, :] // 70 - pi[:, None] // 70).astype(np.float32)
+                dx = (pj[None, :] % 70 - pi[:, None] % 70).astype(np.float32)
+                d = np.sqrt(dx * dx + dy * dy).astype(np.float32)
+                key = (np.floor(d * np.float32(64)).astype(np.uint64) << np.uint64(24)) | \
+                      (np.arange(len(pi), dtype=np.uint64)[:, None] << np.uint64(12)) | \
+                      np.arange(len(pj), dtype=np.uint64)[None, :]
+                a, bb = np.unravel_index(np.argmin(key), key.shape)
+                assert dm[i, j] == np.float32(int(key[a, bb] >> np.uint64(24))) / np.float32(64)
+                assert tuple(ep[i, j]) == (pi[a] // 70, pi[a] % 70, pj[bb] // 70, pj[bb] % 70)
