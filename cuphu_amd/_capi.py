@@ -156,6 +156,8 @@ _SIGS = {
     "cuphu_neighbor_feedback_gpu": (_i, [_P, _P, _i, _i, _i, _i, _i, _i, _i, _P]),
     "cuphu_fix_cycle_spikes_test": (_i, [_P, _P, _i, _i]),
     "cuphu_laplace_neighbor_feedback_test": (_i, [_P, _P, _i, _i, _i, _i, _i, _i, _i]),
+    "cuphu_bridge_gpu": (_i, [_P, _P, _P, _i, _i, C.POINTER(CuPhuBridgeParams), _P]),
+    "cuphu_bridge_apply_test": (_i, [_P, _P, _i, _i, C.POINTER(CuPhuBridgeParams), _P, _i]),
     "cuphu_incrcost_gpu": (_i, [_P, _P, _i, _l, _i, _P, _P, C.POINTER(_i), _P]),
     "cuphu_set_flow_solver": (_i, [_P, _P]),
 }

@@ -304,6 +304,44 @@ py::array_t<float> py_fix_cycle_spikes_test(py::array_t<float, py::array::c_styl
     }, "cuphu_fix_cycle_spikes_test");
 }
 
+// Test-only binding with the reference's name and arguments
+// (cuphu_py.cu:323-366): phase bridging on a copy of an unw array; returns
+// (unw, conncomp) with conncomp starting at 0.
+py::tuple py_bridge_apply_test(py::array_t<float, py::array::c_style> unw, py::object mask_obj, int bridge_radius,
+                               int bridge_min_num_pixel, int bridge_erosion_size, int bridge_max_boundary_samples,
+                               py::object bridge_ramp_type_obj, int bridge_ramp_max_num_sample, int gpu_id) {
+    require_2d(unw, "unw");
+    int nrow = (int)unw.shape(0), ncol = (int)unw.shape(1);
+    py::array_t<uint8_t, py::array::c_style> m;
+    const unsigned char *mp = nullptr;
+    if (!mask_obj.is_none()) {
+        m = mask_obj.cast<py::array_t<uint8_t, py::array::c_style>>();
+        require_2d(m, "mask");
+        if (m.shape(0) != nrow || m.shape(1) != ncol) throw py::value_error("mask must have the same shape as unw");
+        mp = m.data();
+    }
+    CuPhuBridgeParams bp;
+    cuphu_default_bridge_params(&bp);
+    bp.enabled = 1;
+    bp.radius = bridge_radius;
+    bp.min_num_pixel = bridge_min_num_pixel;
+    bp.erosion_size = bridge_erosion_size;
+    bp.max_boundary_samples = bridge_max_boundary_samples;
+    bp.ramp_type = parse_ramp_type(bridge_ramp_type_obj);
+    bp.ramp_max_num_sample = bridge_ramp_max_num_sample;
+    py::array_t<float> out({(py::ssize_t)nrow, (py::ssize_t)ncol});
+    py::array_t<uint32_t> cc({(py::ssize_t)nrow, (py::ssize_t)ncol});
+    std::memcpy(out.mutable_data(), unw.data(), (size_t)nrow * ncol * sizeof(float));
+    std::memset(cc.mutable_data(), 0, (size_t)nrow * ncol * sizeof(uint32_t));
+    int rc;
+    {
+        py::gil_scoped_release nogil;
+        rc = cuphu_bridge_apply_test(out.mutable_data(), cc.mutable_data(), nrow, ncol, &bp, mp, gpu_id);
+    }
+    raise_status(rc, "cuphu_bridge_apply_test");
+    return py::make_tuple(out, cc);
+}
+
 }  // namespace
 
 PYBIND11_MODULE(_cuphu_ext, m) {
@@ -328,6 +366,10 @@ PYBIND11_MODULE(_cuphu_ext, m) {
     m.def("_laplace_neighbor_feedback_test", &py_neighbor_feedback_test, "unw"_a, "mask"_a = py::none(), "ntilerow"_a,
           "ntilecol"_a, "row_ovrlp"_a, "col_ovrlp"_a, "feather_px"_a = 200,
           "Test-only: the Laplace tile-seam neighbour feedback on a copy of an already-stitched unw array.");
+    m.def("_bridge_apply_test", &py_bridge_apply_test, "unw"_a, "mask"_a = py::none(), "bridge_radius"_a = 500,
+          "bridge_min_num_pixel"_a = 14, "bridge_erosion_size"_a = 2, "bridge_max_boundary_samples"_a = 4096,
+          "bridge_ramp_type"_a = py::none(), "bridge_ramp_max_num_sample"_a = 1000000, "gpu_id"_a = 0,
+          "Test-only: phase bridging on a copy of an unw array; returns (unw, conncomp).");
     m.def("_fix_cycle_spikes_test", &py_fix_cycle_spikes_test, "unw"_a, "mask"_a = py::none(),
           "Test-only: the isolated row/column whole-cycle spike correction on a copy of an unw array.");
     m.def("gpu_count", &py_gpu_count, "Return the number of GPUs available.");

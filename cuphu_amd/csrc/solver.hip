@@ -1526,7 +1526,6 @@ int cuphu_unwrap_ex(const float *igram_r, const float *igram_i, const float *igr
         if (mcf && tile->ntilerow * tile->ntilecol > 1)
             throw Error(CUPHU_ERR_UNSUPPORTED, "init 'mcf'/'mst' runs on one tile here (ntiles = (1, 1))");
         CUPHU_REQUIRE(cost_mode >= CUPHU_COST_SMOOTH && cost_mode <= CUPHU_COST_TOPO, "invalid cost mode");
-        if (bridge && bridge->enabled) throw Error(CUPHU_ERR_UNSUPPORTED, "phase bridging is not implemented");
         if (tile->single_tile_reoptimize && tile->ntilerow * tile->ntilecol > 1)
             throw Error(CUPHU_ERR_UNSUPPORTED, "single_tile_reoptimize needs the CPU network-flow solver");
         CUPHU_REQUIRE(tile->ntilerow >= 1 && tile->ntilecol >= 1, "ntilerow/ntilecol must be >= 1");
@@ -1586,6 +1585,9 @@ int cuphu_unwrap_ex(const float *igram_r, const float *igram_i, const float *igr
                                       tile->laplace_neighbor_feedback_feather, st);
             if (tile->fix_cycle_spikes && !single)
                 fix_cycle_spikes_dev(d_unw.get(), mask ? d_mask.get() : nullptr, nrow, ncol, st);
+            // phase bridging, on single tiles too (:1721-1722, :2097-2098)
+            if (bridge && bridge->enabled)
+                bridge_dev(d_unw.get(), d_cc.get(), mask ? d_mask.get() : nullptr, nrow, ncol, *bridge, st);
             CUPHU_HIP_CHECK(hipMemcpyAsync(result->unw, d_unw.get(), n * 4, hipMemcpyDeviceToHost, st));
             CUPHU_HIP_CHECK(hipMemcpyAsync(result->conncomp, d_cc.get(), n * 4, hipMemcpyDeviceToHost, st));
             CUPHU_HIP_CHECK(hipStreamSynchronize(st));
@@ -2148,6 +2150,44 @@ int cuphu_laplace_neighbor_feedback_test(float *unw, const unsigned char *mask, 
         neighbor_feedback_dev(du.get(), mask ? dm.get() : nullptr, nrow, ncol, seams(ncol, col_ovrlp, ntc),
                               seams(nrow, row_ovrlp, ntr), feather_px, st);
         CUPHU_HIP_CHECK(hipMemcpyAsync(unw, du.get(), n * 4, hipMemcpyDeviceToHost, st));
+        CUPHU_HIP_CHECK(hipStreamSynchronize(st));
+        return 0;
+    });
+}
+
+
+int cuphu_bridge_gpu(float *d_unw, uint32_t *d_conncomp, const unsigned char *d_mask, int nrow, int ncol,
+                     const CuPhuBridgeParams *bp, cuphu_stream_t stream) {
+    return guarded([&]() {
+        std::lock_guard<std::recursive_mutex> lk(g_mu);
+        CUPHU_REQUIRE(d_unw && d_conncomp && bp && nrow >= 1 && ncol >= 1, "invalid argument");
+        DevRes &res = devres();
+        StreamJoin join(res, (hipStream_t)stream);
+        bridge_dev(d_unw, d_conncomp, d_mask, nrow, ncol, *bp, res.stream);
+        return 0;
+    });
+}
+
+// Host-array form with the reference's test-hook name and arguments
+// (cuphu_solver.cpp:1360-1366; bp->enabled is not consulted, as there).
+int cuphu_bridge_apply_test(float *unw, uint32_t *conncomp, int nrow, int ncol, const CuPhuBridgeParams *bp,
+                            const unsigned char *mask, int gpu_id) {
+    return guarded([&]() {
+        std::lock_guard<std::recursive_mutex> lk(g_mu);
+        CUPHU_REQUIRE(unw && conncomp && bp && nrow >= 1 && ncol >= 1, "invalid argument");
+        DeviceGuard dg(gpu_id);
+        DevRes &res = devres();
+        hipStream_t st = res.stream;
+        size_t n = (size_t)nrow * ncol;
+        DevBuf<float> du(n);
+        DevBuf<uint32_t> dc(n);
+        DevBuf<unsigned char> dm(mask ? n : 0);
+        CUPHU_HIP_CHECK(hipMemcpyAsync(du.get(), unw, n * 4, hipMemcpyHostToDevice, st));
+        CUPHU_HIP_CHECK(hipMemcpyAsync(dc.get(), conncomp, n * 4, hipMemcpyHostToDevice, st));
+        if (mask) CUPHU_HIP_CHECK(hipMemcpyAsync(dm.get(), mask, n, hipMemcpyHostToDevice, st));
+        bridge_dev(du.get(), dc.get(), mask ? dm.get() : nullptr, nrow, ncol, *bp, st);
+        CUPHU_HIP_CHECK(hipMemcpyAsync(unw, du.get(), n * 4, hipMemcpyDeviceToHost, st));
+        CUPHU_HIP_CHECK(hipMemcpyAsync(conncomp, dc.get(), n * 4, hipMemcpyDeviceToHost, st));
         CUPHU_HIP_CHECK(hipStreamSynchronize(st));
         return 0;
     });

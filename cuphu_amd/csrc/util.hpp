@@ -58,6 +58,12 @@ void fix_cycle_spikes_dev(float *d_unw, const unsigned char *d_mask, int nrow, i
 void neighbor_feedback_dev(float *d_unw, const unsigned char *d_mask, int nrow, int ncol, const std::vector<int> &col_seams,
                            const std::vector<int> &row_seams, int feather, hipStream_t st);
 
+// Phase bridging of disconnected regions (bridge.hip): unw corrected in
+// place by whole cycles per region, cc set to the region labels; both left
+// untouched when fewer than two regions remain.  d_mask may be NULL.
+void bridge_dev(float *d_unw, uint32_t *d_cc, const unsigned char *d_mask, int nrow, int ncol,
+                const CuPhuBridgeParams &bp, hipStream_t st);
+
 // Samples per histogram block = 256 * 16; chunks covering nsamples.
 int pair_chunks(long long nsamples);
 // Weighted medians of all pairs (4 passes of 8 bits).  d_hist holds

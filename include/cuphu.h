@@ -567,6 +567,21 @@ int cuphu_laplace_neighbor_feedback_test(float *unw, const unsigned char *mask, 
                                          int ntc, int row_ovrlp, int col_ovrlp, int feather_px);
 int cuphu_fix_cycle_spikes_test(float *unw, const unsigned char *mask, int nrow, int ncol);
 
+
+/* Phase bridging (reference cuphu_solver.cpp:1000-1366, cuphu_bridge_gpu.cu;
+ * isce3's bridge_unwrapped_phase()): regions of nonzero, unmasked phase
+ * (8-connected, pruned by size and erosion) are joined along a minimum
+ * spanning tree of their boundary distances and each shifted by the whole
+ * cycles that match its neighbour's median across the bridge; d_conncomp
+ * receives the region labels.  With fewer than two regions both arrays are
+ * left as they are.  At most 4096 regions after pruning. */
+int cuphu_bridge_gpu(float *d_unw, uint32_t *d_conncomp, const unsigned char *d_mask, int nrow, int ncol,
+                     const CuPhuBridgeParams *bp, cuphu_stream_t stream);
+/* The same on host arrays, with the reference's test-hook name and
+ * arguments (there it returns void). */
+int cuphu_bridge_apply_test(float *unw, uint32_t *conncomp, int nrow, int ncol, const CuPhuBridgeParams *bp,
+                            const unsigned char *mask, int gpu_id);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

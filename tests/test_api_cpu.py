@@ -204,12 +204,11 @@ def test_unsupported_and_invalid_requests_return_status():
     rc = _capi.lib.cuphu_unwrap(None, None, corr.ctypes.data, None, None, n, n, 0, 2, C.byref(p), C.byref(t),
                                 None, None, 0, C.byref(res))
     assert rc == _capi.ERR_INVALID
-    bp = _capi.CuPhuBridgeParams()
-    _capi.lib.cuphu_default_bridge_params(C.byref(bp))
-    bp.enabled = 1
+    t.ntilerow, t.single_tile_reoptimize = 2, 1   # a full-scene CPU TreeSolve: needs SNAPHU
     rc = _capi.lib.cuphu_unwrap(re_.ctypes.data, im_.ctypes.data, corr.ctypes.data, None, None, n, n, 0, 2,
-                                C.byref(p), C.byref(t), C.byref(bp), None, 0, C.byref(res))
+                                C.byref(p), C.byref(t), None, None, 0, C.byref(res))
     assert rc == _capi.ERR_UNSUPPORTED
+    assert b"single_tile_reoptimize" in _capi.lib.cuphu_last_error()
 
 
 def test_mcf_init_raises_through_python(small_igram, small_corr):
