@@ -155,13 +155,11 @@ def unwrap(  # noqa: C901 - mirrors the reference's single entry point
     else:
         ntilerow, ntilecol = int(ntiles[0]), int(ntiles[1])
 
+    # Non-finite interferogram elements and coherences count as 0
+    # (_unwrap.py:515-523); the library applies that on the device after
+    # the upload, so the arrays are passed as they are.
     igram_c64 = np.ascontiguousarray(igram, dtype=np.complex64)
-    nan_mask = ~np.isfinite(igram_c64)
-    if nan_mask.any():
-        igram_c64 = igram_c64.copy()
-        igram_c64[nan_mask] = 0.0
-
-    corr_f32 = np.ascontiguousarray(np.where(np.isfinite(corr), corr, 0.0), dtype=np.float32)
+    corr_f32 = np.ascontiguousarray(corr, dtype=np.float32)
     mask_u8 = np.ascontiguousarray(mask, dtype=np.uint8) if mask is not None else None
     mag_f32 = np.ascontiguousarray(mag, dtype=np.float32) if mag is not None else None
 
@@ -225,7 +223,8 @@ def unwrap(  # noqa: C901 - mirrors the reference's single entry point
         # integer-cycle shift so the reference pixel equals its own wrapped
         # phase in [0, 2pi) (_unwrap.py:605-617)
         rp, rc = int(reference_pixel[0]), int(reference_pixel[1])
-        ref_phase = float(np.angle(igram_c64[rp, rc]))
+        z = igram_c64[rp, rc]
+        ref_phase = float(np.angle(z)) if np.isfinite(z) else 0.0   # a scrubbed element is 0 + 0i
         if ref_phase < 0.0:
             ref_phase += 2.0 * np.pi
         n = round((float(unw_out[rp, rc]) - ref_phase) / (2.0 * np.pi))

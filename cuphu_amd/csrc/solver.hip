@@ -23,6 +23,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <thread>
 #include <type_traits>
 
 #include "laplace.hpp"
@@ -44,6 +45,7 @@ static void *g_flow_solver_user = nullptr;
 struct DevRes {
     hipStream_t stream = nullptr;
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    hipEvent_t stage_ev[2] = {nullptr, nullptr};   // staged host copies
     std::map<std::string, DevBuf<unsigned char>> bufs;
     std::map<std::string, PinBuf<unsigned char>> pins;
     void *get(const std::string &name, size_t bytes) {
@@ -71,6 +73,71 @@ static DevRes &devres() {
 }
 template <class T> static T *ws(DevRes &r, const char *name, size_t count) {
     return reinterpret_cast<T *>(r.get(name, std::max<size_t>(count, 1) * sizeof(T) + 256));
+}
+
+// ── host <-> device copies of caller buffers through pinned staging ────────
+// Pageable device-to-host copies run at a few GB/s; staging through two
+// pinned chunks runs at the link rate.  The DMA of one chunk overlaps the
+// CPU copy (split over threads) of the previous one.
+static void par_memcpy(void *dst, const void *src, size_t n) {
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const size_t nth = n >= ((size_t)8 << 20) ? std::min<size_t>(8, hw) : 1;
+    if (nth <= 1) { std::memcpy(dst, src, n); return; }
+    const size_t part = ((n + nth - 1) / nth + 4095) & ~(size_t)4095;
+    std::vector<std::thread> th;
+    for (size_t t = 0; t < nth && t * part < n; ++t) {
+        const size_t off = t * part, len = std::min(part, n - off);
+        th.emplace_back([=]() { std::memcpy((char *)dst + off, (const char *)src + off, len); });
+    }
+    for (auto &x : th) x.join();
+}
+static constexpr size_t kStageChunk = (size_t)64 << 20;
+static void stage_events(DevRes &r) {
+    for (auto &e : r.stage_ev)
+        if (!e) CUPHU_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+}
+static void copy_d2h_staged(void *dst, const void *src, size_t n, DevRes &r, hipStream_t st) {
+    if (n < ((size_t)8 << 20)) {
+        CUPHU_HIP_CHECK(hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToHost, st));
+        CUPHU_HIP_CHECK(hipStreamSynchronize(st));
+        return;
+    }
+    stage_events(r);
+    unsigned char *pb[2] = {(unsigned char *)r.pin("stage0", kStageChunk), (unsigned char *)r.pin("stage1", kStageChunk)};
+    const size_t nch = (n + kStageChunk - 1) / kStageChunk;
+    auto len = [&](size_t k) { return std::min(kStageChunk, n - k * kStageChunk); };
+    for (size_t k = 0; k < nch; ++k) {
+        // buffer k&1: the CPU copy of chunk k-2 out of it is done, and an
+        // earlier host-to-device DMA from it precedes this one on the stream
+        CUPHU_HIP_CHECK(hipMemcpyAsync(pb[k & 1], (const char *)src + k * kStageChunk, len(k), hipMemcpyDeviceToHost, st));
+        CUPHU_HIP_CHECK(hipEventRecord(r.stage_ev[k & 1], st));
+        if (k >= 1) {
+            CUPHU_HIP_CHECK(hipEventSynchronize(r.stage_ev[(k - 1) & 1]));
+            par_memcpy((char *)dst + (k - 1) * kStageChunk, pb[(k - 1) & 1], len(k - 1));
+        }
+    }
+    CUPHU_HIP_CHECK(hipEventSynchronize(r.stage_ev[(nch - 1) & 1]));
+    par_memcpy((char *)dst + (nch - 1) * kStageChunk, pb[(nch - 1) & 1], len(nch - 1));
+}
+static void copy_h2d_staged(void *dst, const void *src, size_t n, DevRes &r, hipStream_t st) {
+    if (n < ((size_t)8 << 20)) {
+        CUPHU_HIP_CHECK(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, st));
+        CUPHU_HIP_CHECK(hipStreamSynchronize(st));   // the caller's buffer may change after return
+        return;
+    }
+    stage_events(r);
+    unsigned char *pb[2] = {(unsigned char *)r.pin("stage0", kStageChunk), (unsigned char *)r.pin("stage1", kStageChunk)};
+    const size_t nch = (n + kStageChunk - 1) / kStageChunk;
+    for (size_t k = 0; k < nch; ++k) {
+        const size_t len = std::min(kStageChunk, n - k * kStageChunk);
+        // buffer k&1 is free once the last DMA that used it (chunk k-2, or an
+        // earlier copy's) has finished; an event never recorded is complete
+        CUPHU_HIP_CHECK(hipEventSynchronize(r.stage_ev[k & 1]));
+        par_memcpy(pb[k & 1], (const char *)src + k * kStageChunk, len);
+        CUPHU_HIP_CHECK(hipMemcpyAsync((char *)dst + k * kStageChunk, pb[k & 1], len, hipMemcpyHostToDevice, st));
+        CUPHU_HIP_CHECK(hipEventRecord(r.stage_ev[k & 1], st));
+    }
+    // the staging buffers are reused by the next copy, which waits on these events
 }
 
 // Orders the library's work stream after the caller's stream on entry and
@@ -1163,6 +1230,17 @@ __global__ void k_interleave(const float *re, const float *im, float2 *out, size
     if (i < n) out[i] = make_float2(re[i], im[i]);
 }
 
+// Non-finite inputs as the reference's Python layer scrubs them before the
+// solve (_unwrap.py:515-523): an interferogram element with a non-finite
+// part becomes 0 + 0i, a non-finite coherence 0.
+__global__ void k_scrub_inputs(float2 *ig, float *corr, size_t n) {
+    size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float2 v = ig[i];
+    if (!(isfinite(v.x) && isfinite(v.y))) ig[i] = make_float2(0.0f, 0.0f);
+    if (!isfinite(corr[i])) corr[i] = 0.0f;
+}
+
 }  // namespace cuphu
 
 using namespace cuphu;
@@ -1549,19 +1627,22 @@ int cuphu_unwrap_ex(const float *igram_r, const float *igram_i, const float *igr
             DevBuf<uint32_t> d_cc(n);
             DevBuf<unsigned char> d_mask(mask ? n : 0), d_orig(orig_mask ? n : 0);
             if (igram_c64) {
-                CUPHU_HIP_CHECK(hipMemcpyAsync(d_ig.get(), igram_c64, n * 8, hipMemcpyHostToDevice, st));
+                copy_h2d_staged(d_ig.get(), igram_c64, n * 8, res, st);
             } else {
                 DevBuf<float> d_re(n), d_im(n);
-                CUPHU_HIP_CHECK(hipMemcpyAsync(d_re.get(), igram_r, n * 4, hipMemcpyHostToDevice, st));
-                CUPHU_HIP_CHECK(hipMemcpyAsync(d_im.get(), igram_i, n * 4, hipMemcpyHostToDevice, st));
+                copy_h2d_staged(d_re.get(), igram_r, n * 4, res, st);
+                copy_h2d_staged(d_im.get(), igram_i, n * 4, res, st);
                 hipLaunchKernelGGL(k_interleave, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st, d_re.get(),
                                    d_im.get(), d_ig.get(), n);
                 CUPHU_HIP_CHECK(hipGetLastError());
                 CUPHU_HIP_CHECK(hipStreamSynchronize(st));
             }
-            CUPHU_HIP_CHECK(hipMemcpyAsync(d_corr.get(), corr, n * 4, hipMemcpyHostToDevice, st));
-            if (mask) CUPHU_HIP_CHECK(hipMemcpyAsync(d_mask.get(), mask, n, hipMemcpyHostToDevice, st));
-            if (orig_mask) CUPHU_HIP_CHECK(hipMemcpyAsync(d_orig.get(), orig_mask, n, hipMemcpyHostToDevice, st));
+            copy_h2d_staged(d_corr.get(), corr, n * 4, res, st);
+            if (mask) copy_h2d_staged(d_mask.get(), mask, n, res, st);
+            if (orig_mask) copy_h2d_staged(d_orig.get(), orig_mask, n, res, st);
+            hipLaunchKernelGGL(k_scrub_inputs, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st, d_ig.get(),
+                               d_corr.get(), n);
+            CUPHU_HIP_CHECK(hipGetLastError());
             // tile-stitching confidence mask: orig_mask, else mask (cuphu_solver.cpp:1696)
             const unsigned char *conf = orig_mask ? d_orig.get() : nullptr;
             int single = (tile->ntilerow == 1 && tile->ntilecol == 1);
@@ -1588,9 +1669,8 @@ int cuphu_unwrap_ex(const float *igram_r, const float *igram_i, const float *igr
             // phase bridging, on single tiles too (:1721-1722, :2097-2098)
             if (bridge && bridge->enabled)
                 bridge_dev(d_unw.get(), d_cc.get(), mask ? d_mask.get() : nullptr, nrow, ncol, *bridge, st);
-            CUPHU_HIP_CHECK(hipMemcpyAsync(result->unw, d_unw.get(), n * 4, hipMemcpyDeviceToHost, st));
-            CUPHU_HIP_CHECK(hipMemcpyAsync(result->conncomp, d_cc.get(), n * 4, hipMemcpyDeviceToHost, st));
-            CUPHU_HIP_CHECK(hipStreamSynchronize(st));
+            copy_d2h_staged(result->unw, d_unw.get(), n * 4, res, st);
+            copy_d2h_staged(result->conncomp, d_cc.get(), n * 4, res, st);
         } catch (...) {
             std::free(result->unw); std::free(result->conncomp);
             result->unw = nullptr; result->conncomp = nullptr;

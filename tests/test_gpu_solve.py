@@ -359,6 +359,32 @@ def test_scene_merge_labels_is_scene_conncomp(gpu_ready):
     assert l1.max() < l0.max()   # per-tile labels split the seam-crossing components
 
 
+def test_nonfinite_inputs_scrubbed_like_the_reference(gpu_ready):
+    """Non-finite interferogram elements (either part) and coherences are
+    taken as 0 (_unwrap.py:515-523), now on the device: the result equals
+    unwrap() of arrays scrubbed beforehand, bit for bit, tiled and with the
+    reference pixel itself non-finite."""
+    s = synth.make_scene(130, 170, nbumps=2, npatches=1, nlooks=10.0, seed=61, patch=30)
+    rng = np.random.default_rng(4)
+    ig = s.igram.copy()
+    corr = s.corr.copy()
+    idx = rng.choice(ig.size, 300, replace=False)
+    ig.ravel()[idx[:100]] = np.complex64(complex(np.nan, 0.0))
+    ig.ravel()[idx[100:150]] = np.complex64(complex(1.0, np.inf))
+    corr.ravel()[idx[150:250]] = np.nan
+    corr.ravel()[idx[250:]] = -np.inf
+    ig[0, 0] = np.complex64(complex(np.nan, np.nan))
+    clean_ig = np.where(np.isfinite(ig), ig, 0).astype(np.complex64)
+    clean_corr = np.where(np.isfinite(corr), corr, 0).astype(np.float32)
+    for kw in (dict(ntiles=(1, 1)), dict(ntiles=(2, 2), tile_overlap=32)):
+        a, ca = cuphu.unwrap(ig, corr, 10.0, "defo", init="laplace", laplace_options=dict(n_irls=2, max_iter=200), **kw)
+        b, cb = cuphu.unwrap(clean_ig, clean_corr, 10.0, "defo", init="laplace",
+                             laplace_options=dict(n_irls=2, max_iter=200), **kw)
+        assert np.isfinite(a).all()
+        np.testing.assert_array_equal(a, b)
+        np.testing.assert_array_equal(ca, cb)
+
+
 def test_repeat_runs_are_deterministic(gpu_ready):
     s = SMALL[0]
     o = _capi.default_options(n_irls=2, max_iter=300)
