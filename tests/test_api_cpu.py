@@ -113,6 +113,17 @@ def test_effective_looks_known_answer():
     assert cuphu.get_effective_looks(13, 16, 2.33, 2.75, 7.5, 6.0) == 29.616888888888894
 
 
+def test_effective_looks_reference_properties():
+    """test_looks.py:17-45: below the naive product when oversampled, equal
+    to it at Nyquist spacing, linear in the number of looks."""
+    kw = dict(range_spacing=2.33, azimuth_spacing=2.75, range_resolution=7.5, azimuth_resolution=6.0)
+    assert cuphu.get_effective_looks(range_looks=13, azimuth_looks=16, **kw) < 13 * 16
+    assert cuphu.get_effective_looks(range_looks=13, azimuth_looks=16, range_spacing=3.0, azimuth_spacing=4.0,
+                                     range_resolution=3.0, azimuth_resolution=4.0) == pytest.approx(13 * 16)
+    n1 = cuphu.get_effective_looks(range_looks=1, azimuth_looks=1, **kw)
+    assert cuphu.get_effective_looks(range_looks=2, azimuth_looks=3, **kw) == pytest.approx(6 * n1)
+
+
 # ── the C ABI boundary ──────────────────────────────────────────────────────
 def _header_functions():
     txt = open(HEADER).read()
