@@ -97,3 +97,34 @@ def test_degenerate_shapes(gpu_ready, solver):
     for i, (ig, corr) in enumerate(scenes[:-1]):
         u1, c1 = cuphu.unwrap_batch([ig], [corr], 5.0, "defo", laplace_options=opts)
         np.testing.assert_array_equal(u1[0], unw[i])
+
+
+def test_many_small_instances(gpu_ready):
+    """3000 interferograms of 4..96 px per side in one launch sequence: every
+    instance finite and congruent, a sample bit-identical to its solo solve
+    and agreeing with the FP64 oracle."""
+    rng = np.random.default_rng(99)
+    n = 3000
+    igs, cos, shapes = [], [], []
+    for i in range(n):
+        nr, nc = int(rng.integers(4, 97)), int(rng.integers(4, 97))
+        r, c = np.mgrid[0:nr, 0:nc]
+        ph = 0.3 * r - 0.2 * c + rng.normal(0, 0.3, (nr, nc))
+        igs.append(np.exp(1j * ph).astype(np.complex64))
+        cos.append(rng.uniform(0.2, 0.95, (nr, nc)).astype(np.float32))
+        shapes.append((nr, nc))
+    opts = dict(n_irls=1, max_iter=200, tol=1e-6)
+    unw, cc, st = cuphu.unwrap_batch(igs, cos, 5.0, "defo", laplace_options=opts, return_stats=True)
+    assert len(unw) == n
+    for u, sh in zip(unw, shapes):
+        assert u.shape == sh and np.isfinite(u).all()
+    for i in rng.choice(n, 12, replace=False):
+        u1, c1 = cuphu.unwrap_batch([igs[i]], [cos[i]], 5.0, "defo", laplace_options=opts)
+        np.testing.assert_array_equal(unw[i], u1[0])
+        np.testing.assert_array_equal(cc[i], c1[0])
+        ph = O.wrapped_phase(igs[i], correctly_rounded=True)
+        k = (unw[i].astype(np.float64) - ph) / TWOPI
+        assert np.abs(k - np.rint(k)).max() < 1e-4
+        sig = O.costs(ph, cos[i], 5.0, defo=True)[:, 1]
+        uo, _ = O.pcg_irls_f64(sig, ph, **opts)
+        assert (np.rint((unw[i] - ph) / TWOPI) == np.rint((uo - ph) / TWOPI)).mean() >= 0.99
