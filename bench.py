@@ -282,6 +282,7 @@ def main() -> int:
         out["python_api_scene8192"] = leg_python_api(scene, base_opts, log)
         out["config2_scene2048"] = leg_single(torch, lib, _capi, synth, base_opts, log)
         out["config4_batch64"] = leg_batch64(torch, lib, _capi, synth, base_opts, args.plain_sample, log)
+        out["config5_autograd1024"] = leg_autograd(torch, synth, log)
 
     print(json.dumps(out), flush=True)
     return 0
@@ -374,6 +375,52 @@ def leg_python_api(scene, base_opts, log):
          "host_side_seconds": round(t - t_dev, 3),
          "ncomponents": int(cc.max())}
     log(f"python api: {r['mpix_s']} Mpx/s ({t:.2f} s; device-resident same tiling {t_dev:.2f} s)")
+    return r
+
+
+def leg_autograd(torch, synth, log):
+    """Differentiable solve at 1024x1024 (ramp + 5 bumps + 1 patch, nlooks
+    10): arc weights w = a * c^b / (1 - c^2 + eps) from the arc coherence with
+    learnable FP32 (a, b, eps), the fixed-weight WLS solve (WLSSolve, FP32
+    vectors, FP64 reductions, tol 1e-6), loss = MSE against the truth with
+    its mean removed, and the backward pass (adjoint PCG on the same
+    operator, then torch through the mapping).  One step = forward + loss +
+    backward."""
+    from cuphu_amd.autograd import WLSSolve, coherence_weights
+
+    s = synth.config_scene("scene1024")
+    nrow, ncol = s.corr.shape
+    ph = np.angle(s.igram.astype(np.complex128))
+    ph = np.where(ph < 0, ph + 2 * np.pi, ph).astype(np.float32)
+    phase = torch.from_numpy(ph).to("cuda")
+    corr = torch.from_numpy(s.corr).to("cuda")
+    truth = torch.from_numpy((s.truth - s.truth.mean()).astype(np.float32)).to("cuda")
+    a = torch.tensor(1.0, device="cuda", requires_grad=True)
+    b = torch.tensor(2.0, device="cuda", requires_grad=True)
+    eps = torch.tensor(0.05, device="cuda", requires_grad=True)
+
+    def step():
+        for t in (a, b, eps):
+            t.grad = None
+        w = coherence_weights(corr, a, b, eps)
+        u = WLSSolve.apply(w, phase, 4000, 1e-6)
+        loss = ((u - truth) ** 2).mean()
+        loss.backward()
+        return loss
+
+    step()
+    torch.cuda.synchronize()
+    n = 3
+    t0 = time.perf_counter()
+    for _ in range(n):
+        loss = step()
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / n
+    r = {"workload": "1024x1024, w = a*c^b/(1-c^2+eps) learnable (FP32), WLS forward + MSE loss + adjoint backward, "
+                     "tol 1e-6",
+         "seconds_per_step": round(t, 4), "mpix_s": round(nrow * ncol / t / 1e6, 3), "loss": float(loss.detach()),
+         "grads": [float(a.grad), float(b.grad), float(eps.grad)]}
+    log(f"config5: {r['seconds_per_step'] * 1e3:.1f} ms per forward+backward step ({r['mpix_s']} Mpx/s)")
     return r
 
 
