@@ -1048,7 +1048,26 @@ __device__ __forceinline__ T diag_inv(T wd, T wu, T wr, T wl) {
     return T(1) / ((((wd + wu) + wr) + wl) + Prec<T>::eps);
 }
 
-template <class T, int RPW>
+// Workgroups that start together on one CU run identical instruction streams
+// and, left alone, stay in step for the whole launch: all halo loads in flight
+// at once, then all ALU work, then all stores.  Holding back two thirds of the
+// first resident round (3 workgroups x 256 CUs) by 0.85 / 1.7 us puts them
+// out of phase, and the workgroups that replace them inherit the offsets:
+// 0.446 -> 0.433 ms per k_iq launch over 8192^2 pixels.  Any offset from 0.4
+// to 3.4 us gives the same; s_sleep(32) is 2048 clocks.  A grid of a few
+// rounds (a single 2048^2 or 1024^2 image) lasts too short a time to repay
+// the delay: the host launches the unstaggered instantiation for it.
+constexpr unsigned kStaggerBlocks = 768;
+constexpr int kStaggerMinGrid = 4 * (int)kStaggerBlocks;
+
+__device__ __forceinline__ void stagger_first_round() {
+    if (blockIdx.x < kStaggerBlocks) {
+        const unsigned ph = (blockIdx.x / 8u) % 3u;   // consecutive ids alternate over the 8 XCDs
+        for (unsigned i = 0; i < ph; ++i) __builtin_amdgcn_s_sleep(32);
+    }
+}
+
+template <class T, int RPW, bool STAGGER>
 __global__ __launch_bounds__(256) void k_iq(const InstDesc *descs, const int *blk_inst, Planes<T> pl,
                                             const SolveScal *scal, double4 *partials, int parity, int start) {
     constexpr int TH = 4 * RPW;
@@ -1058,6 +1077,7 @@ __global__ __launch_bounds__(256) void k_iq(const InstDesc *descs, const int *bl
     TileInfo t = get_tile(descs, blk_inst, TH);
     const SolveScal *sc = scal + t.inst;
     if (!start && !sc->active) return;
+    if (STAGGER) stagger_first_round();
     const bool upd = !start;                          // x += a p
     const bool rep = upd && sc->replace != 0;         // r from the true-residual plane
     const bool useq = upd && !rep;                    // r -= a q_j, q_j = A p_j
@@ -1702,6 +1722,12 @@ void launch_ingest(const IngestSrc *d_src, const InstDesc *d_desc, const int *d_
     case 8: hipLaunchKernelGGL((KERNEL<T, 8>), __VA_ARGS__); break;               \
     default: hipLaunchKernelGGL((KERNEL<T, 4>), __VA_ARGS__); break;              \
     }
+#define CUPHU_RPW_DISPATCH_X(rpw, KERNEL, X, ...)                                 \
+    switch (rpw) {                                                                \
+    case 2: hipLaunchKernelGGL((KERNEL<T, 2, X>), __VA_ARGS__); break;            \
+    case 8: hipLaunchKernelGGL((KERNEL<T, 8, X>), __VA_ARGS__); break;            \
+    default: hipLaunchKernelGGL((KERNEL<T, 4, X>), __VA_ARGS__); break;           \
+    }
 
 template <class T>
 void launch_weights(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, int reweight,
@@ -1759,7 +1785,13 @@ void launch_rr_it(const InstDesc *d_desc, const int *d_blk, int nblk, const Plan
 template <class T>
 void launch_iq(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, const SolveScal *scal,
                double4 *partials, int rpw, int parity, int start, hipStream_t st) {
-    CUPHU_RPW_DISPATCH(rpw, k_iq, dim3(nblk), dim3(256), 0, st, d_desc, d_blk, pl, scal, partials, parity, start);
+    if (nblk >= kStaggerMinGrid) {
+        CUPHU_RPW_DISPATCH_X(rpw, k_iq, true, dim3(nblk), dim3(256), 0, st, d_desc, d_blk, pl, scal, partials, parity,
+                             start);
+    } else {
+        CUPHU_RPW_DISPATCH_X(rpw, k_iq, false, dim3(nblk), dim3(256), 0, st, d_desc, d_blk, pl, scal, partials, parity,
+                             start);
+    }
     CUPHU_LAUNCH_CHECK();
 }
 template <class T>
