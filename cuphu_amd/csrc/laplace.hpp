@@ -5,7 +5,8 @@
 namespace cuphu {
 
 // Stencil tile geometry: a workgroup of 4 waves covers TW columns (64 lanes x
-// 4 pixels) by 4*RPW rows; each wave owns RPW consecutive rows.
+// 4 pixels) by 4*RPW rows; each wave owns RPW consecutive rows.  The fused
+// solver's kernels also run a narrow shape (TileShape below).
 constexpr int kTileW = 256;
 constexpr int kLdsStride = 264;   // LDS row stride (elements); tile col j -> lds col 4+j
 constexpr int kLdsC0 = 4;
@@ -59,8 +60,16 @@ template <class T> struct Planes {
     T *p2 = nullptr, *r2 = nullptr, *q2 = nullptr;
 };
 
-struct KernelCfg {
-    int rpw = 4;   // rows per wave (autotuned: 2, 4 or 8)
+// Workgroup tile of the fused solver's kernels (weights, assemble, the
+// iteration, the residual monitor, the x finalize): lpr lanes per tile row,
+// rpw rows per lane group.  lpr 64: 256 columns x 4*rpw rows (rpw 2, 4, 8);
+// lpr 32: 128 columns x 8*rpw rows (rpw 2, 4).  Every other kernel uses
+// lpr 64.
+struct TileShape {
+    int rpw = 4, lpr = 64;
+    int th() const { return 4 * rpw * (64 / lpr); }
+    int tw() const { return 4 * lpr; }
+    bool operator==(const TileShape &o) const { return rpw == o.rpw && lpr == o.lpr; }
 };
 
 // ── launchers (laplace_kernels.hip) ─────────────────────────────────────────
@@ -69,9 +78,10 @@ void launch_ingest(const IngestSrc *d_src, const InstDesc *d_desc, const int *d_
                    const Planes<T> &pl, int has_u0, int max_pitch, hipStream_t st);
 template <class T>
 void launch_weights(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, int reweight,
-                    T delta2, int rpw, hipStream_t st);
+                    T delta2, TileShape sh, hipStream_t st);
 template <class T>
-void launch_assemble(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, int rpw, hipStream_t st);
+void launch_assemble(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, TileShape sh,
+                     hipStream_t st);
 // start: partial r.z (force: also for inactive instances)
 template <class T>
 void launch_rz(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, const SolveScal *scal,
@@ -108,14 +118,14 @@ void launch_reduce_it(const InstDesc *d_desc, int ninst, const double4 *partials
 // plane sets (r, p) and (r2, p2); and its true-residual kernel.
 template <class T>
 void launch_iq(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, const SolveScal *scal,
-               double4 *partials, int rpw, int parity, int start, hipStream_t st);
+               double4 *partials, TileShape sh, int parity, int start, hipStream_t st);
 template <class T>
 void launch_rr_iq(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, const SolveScal *scal,
-                  double4 *partials, int rpw, hipStream_t st);
+                  double4 *partials, TileShape sh, hipStream_t st);
 // After a solve: apply an x update still deferred (xmode 1) when it stopped.
 template <class T>
 void launch_x_finalize(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, const SolveScal *scal,
-                       int rpw, hipStream_t st);
+                       TileShape sh, hipStream_t st);
 template <class T>
 void launch_final(const InstDesc *d_desc, const int *d_rowinst, int nrows_total, const Planes<T> &pl,
                   float *out_pool, int max_pitch, hipStream_t st);

@@ -11,7 +11,9 @@
 //
 // Tiles.  A 256-thread workgroup (4 waves of 64) owns a 256-column x
 // 4*RPW-row tile; each lane owns 4 consecutive columns of RPW rows and
-// moves them with 16-byte loads.  Stencil kernels stage the stencil operand
+// moves them with 16-byte loads.  The fused solver's kernels also run
+// 128-column x 8*RPW-row tiles (their default), where each half-wave owns
+// RPW rows (Shape<RPW, LPR> below).  Stencil kernels stage the stencil operand
 // of the tile plus a one-pixel halo in LDS, then apply the 5-point weighted
 // Laplacian from LDS.  Workgroups are renumbered so each XCD receives a
 // contiguous run of tiles (vertically adjacent tiles share halo rows in the
@@ -105,34 +107,54 @@ struct TileInfo {
     InstDesc d;
     int inst, blk, tx0, ty0;
 };
-__device__ __forceinline__ TileInfo get_tile(const InstDesc *descs, const int *blk_inst, int th) {
+__device__ __forceinline__ TileInfo get_tile(const InstDesc *descs, const int *blk_inst, int th, int tw = kTileW) {
     TileInfo t;
     t.blk = xcd_block((int)blockIdx.x, (int)gridDim.x);
     t.inst = blk_inst[t.blk];
     t.d = descs[t.inst];
     int local = t.blk - t.d.block_begin;
-    t.tx0 = (local % t.d.tiles_x) * kTileW;
+    t.tx0 = (local % t.d.tiles_x) * tw;
     t.ty0 = (local / t.d.tiles_x) * th;
     return t;
 }
 
-// Halo fill: tile rows -1 and TH (waves 0 and 3, 4 pixels per lane) and
-// tile columns -1 and TW for rows -1..TH (waves 1 and 2, one pixel per
-// lane).  put4(lr, lc, e, ok) / put1(lr, lc, e, ok) store the operand
-// values of pixel element e at LDS (row lr, col lc); ok = false stores 0.
-template <int TH, class P4, class P1>
+// Tile shapes.  LPR lanes cover one row of a tile, 4 pixels each, so a tile
+// is 4*LPR columns wide; a wave holds 64/LPR such lane groups, and lane group
+// v = wave * (64/LPR) + lane / LPR of the workgroup owns rows v*RPW ..
+// v*RPW + RPW - 1.  LPR 64 is the 256 x 4*RPW tile; LPR 32 the 128 x 8*RPW
+// tile, for instances whose pitch leaves most of a last 256-column tile idle
+// (same pixels per workgroup).
+template <int RPW, int LPR>
+struct Shape {
+    static_assert(LPR == 64 || LPR == 32, "lanes per tile row");
+    static constexpr int G = 64 / LPR;      // lane groups per wave
+    static constexpr int TW = 4 * LPR;      // tile width
+    static constexpr int TH = 4 * G * RPW;  // tile height
+    static constexpr int S = TW + 8;        // LDS row stride; tile col j -> LDS col kLdsC0 + j
+    static_assert(TH + 2 <= 64, "ring columns: one lane per row");
+};
+
+// Halo fill: tile rows -1 and TH (lane group 0 of waves 0 and 3, 4 pixels
+// per lane) and tile columns -1 and TW for rows -1..TH (waves 1 and 2, one
+// pixel per lane).  put4(lr, lc, e, ok) / put1(lr, lc, e, ok) store the
+// operand values of pixel element e at LDS (row lr, col lc); ok = false
+// stores 0.
+template <int TH, int LPR = 64, class P4, class P1>
 __device__ __forceinline__ void fill_halo(const TileInfo &t, int wave, int lane, P4 &&put4, P1 &&put1) {
+    constexpr int TW = 4 * LPR;
     const InstDesc &d = t.d;
     if (wave == 0 || wave == 3) {
-        int lr = (wave == 0) ? 0 : TH + 1;
-        int row = t.ty0 + lr - 1;
-        int c = t.tx0 + 4 * lane;
-        bool ok = row >= 0 && row < d.nrow && c < d.pitch;
-        put4(lr, kLdsC0 + 4 * lane, d.off + (long long)row * d.pitch + c, ok);
+        if (lane < LPR) {
+            int lr = (wave == 0) ? 0 : TH + 1;
+            int row = t.ty0 + lr - 1;
+            int c = t.tx0 + 4 * lane;
+            bool ok = row >= 0 && row < d.nrow && c < d.pitch;
+            put4(lr, kLdsC0 + 4 * lane, d.off + (long long)row * d.pitch + c, ok);
+        }
     } else if (lane < TH + 2) {
         int lr = lane, row = t.ty0 + lr - 1;
-        int c = (wave == 1) ? t.tx0 - 1 : t.tx0 + kTileW;
-        int lc = (wave == 1) ? kLdsC0 - 1 : kLdsC0 + kTileW;
+        int c = (wave == 1) ? t.tx0 - 1 : t.tx0 + TW;
+        int lc = (wave == 1) ? kLdsC0 - 1 : kLdsC0 + TW;
         bool ok = row >= 0 && row < d.nrow && c >= 0 && c < d.pitch;
         put1(lr, lc, d.off + (long long)row * d.pitch + c, ok);
     }
@@ -140,14 +162,16 @@ __device__ __forceinline__ void fill_halo(const TileInfo &t, int wave, int lane,
 
 // Weighted 5-point Laplacian of 4 pixels from an LDS tile (laplace_matvec
 // cuphu_laplace.cu:156-190): y = eps*a, then fma over down, up, right, left.
-template <class T>
-__device__ __forceinline__ void stencil4(const T *lds, int lr, int lane, const T (&wd)[4], const T (&wu)[4],
+// lr: tile row; cl: the lane's column group (lane mod LPR).
+template <class T, int LPR = 64>
+__device__ __forceinline__ void stencil4(const T *lds, int lr, int cl, const T (&wd)[4], const T (&wu)[4],
                                          const T (&wr)[4], const T (&wl)[4], T (&y)[4]) {
-    const T *rc = lds + (lr + 1) * kLdsStride + kLdsC0 + 4 * lane;
+    constexpr int S = 4 * LPR + 8;
+    const T *rc = lds + (lr + 1) * S + kLdsC0 + 4 * cl;
     T c4[4], u4[4], d4[4];
     ld4(rc, c4);
-    ld4(rc - kLdsStride, u4);
-    ld4(rc + kLdsStride, d4);
+    ld4(rc - S, u4);
+    ld4(rc + S, d4);
     T left = rc[-1], right = rc[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -166,14 +190,15 @@ __device__ __forceinline__ void stencil4(const T *lds, int lr, int lane, const T
 // The same stencil accumulated in FP64 (the true residual b - L x of a
 // residual replacement: b and L x nearly cancel, so the difference is formed
 // from exact products and rounded once).
-template <class T>
-__device__ __forceinline__ void stencil4_wide(const T *lds, int lr, int lane, const T (&wd)[4], const T (&wu)[4],
+template <class T, int LPR = 64>
+__device__ __forceinline__ void stencil4_wide(const T *lds, int lr, int cl, const T (&wd)[4], const T (&wu)[4],
                                               const T (&wr)[4], const T (&wl)[4], double (&y)[4]) {
-    const T *rc = lds + (lr + 1) * kLdsStride + kLdsC0 + 4 * lane;
+    constexpr int S = 4 * LPR + 8;
+    const T *rc = lds + (lr + 1) * S + kLdsC0 + 4 * cl;
     T c4[4], u4[4], d4[4];
     ld4(rc, c4);
-    ld4(rc - kLdsStride, u4);
-    ld4(rc + kLdsStride, d4);
+    ld4(rc - S, u4);
+    ld4(rc + S, d4);
     T left = rc[-1], right = rc[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -191,24 +216,25 @@ __device__ __forceinline__ void stencil4_wide(const T *lds, int lr, int lane, co
 
 // Arcs of a lane's 4 pixels in one row: own down/right arcs and the left
 // arc (the right arc of the previous column: the neighbouring lane's, or a
-// load for lane 0).  Executed by every lane (contains a shuffle).
-template <class T>
-__device__ __forceinline__ void row_arcs(const T *w2, const TileInfo &t, int row, bool cok, int lane,
+// load for the first lane of a row).  Executed by every lane (contains a
+// shuffle); cl = lane mod LPR.
+template <class T, int LPR = 64>
+__device__ __forceinline__ void row_arcs(const T *w2, const TileInfo &t, int row, bool cok, int cl,
                                          T (&wd)[4], T (&wr)[4], T (&wl)[4]) {
-    long long e = t.d.off + (long long)row * t.d.pitch + t.tx0 + 4 * lane;
+    long long e = t.d.off + (long long)row * t.d.pitch + t.tx0 + 4 * cl;
     if (cok) ld_arcs(w2, e, wd, wr);
     else { zero4(wd); zero4(wr); }
-    T from_left = __shfl_up(wr[3], 1, 64);
-    if (lane == 0) from_left = (t.tx0 > 0) ? w2[2 * (e - 1) + 1] : T(0);
+    T from_left = __shfl_up(wr[3], 1, LPR);
+    if (cl == 0) from_left = (t.tx0 > 0) ? w2[2 * (e - 1) + 1] : T(0);
     wl[0] = from_left; wl[1] = wr[0]; wl[2] = wr[1]; wl[3] = wr[2];
 }
 
 // Down arcs of the row above (the up arcs of this row); zero on row 0.
 template <class T>
-__device__ __forceinline__ void up_arcs(const T *w2, const TileInfo &t, int row, bool cok, int lane, T (&wu)[4]) {
+__device__ __forceinline__ void up_arcs(const T *w2, const TileInfo &t, int row, bool cok, int cl, T (&wu)[4]) {
     if (row > 0 && cok) {
         T dummy[4];
-        ld_arcs(w2, t.d.off + (long long)(row - 1) * t.d.pitch + t.tx0 + 4 * lane, wu, dummy);
+        ld_arcs(w2, t.d.off + (long long)(row - 1) * t.d.pitch + t.tx0 + 4 * cl, wu, dummy);
     } else {
         zero4(wu);
     }
@@ -322,17 +348,18 @@ __device__ __forceinline__ void ld_sigsq4(const short2 *p, short (&sd)[4], short
 }
 __device__ __forceinline__ void ld4f(const float *p, float (&o)[4]) { ld4(p, o); }
 
-template <class T, int RPW>
+template <class T, int RPW, int LPR>
 __global__ __launch_bounds__(256) void k_weights(const InstDesc *descs, const int *blk_inst, Planes<T> pl,
                                                  int reweight, T d2) {
-    constexpr int TH = 4 * RPW;
-    TileInfo t = get_tile(descs, blk_inst, TH);
-    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    using Sh = Shape<RPW, LPR>;
+    TileInfo t = get_tile(descs, blk_inst, Sh::TH, Sh::TW);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int cl = lane % LPR, v = wave * Sh::G + lane / LPR;
     const InstDesc &d = t.d;
-    int c = t.tx0 + 4 * lane;
+    int c = t.tx0 + 4 * cl;
     bool cok = c < d.pitch;
     for (int k = 0; k < RPW; ++k) {
-        int row = t.ty0 + wave * RPW + k;
+        int row = t.ty0 + v * RPW + k;
         if (row >= d.nrow) break;
         long long e = d.off + (long long)row * d.pitch + c;
         short sd[4] = {0, 0, 0, 0}, sr[4] = {0, 0, 0, 0};
@@ -351,9 +378,9 @@ __global__ __launch_bounds__(256) void k_weights(const InstDesc *descs, const in
                 ld4(pl.x + e, xa);
                 if (row + 1 < d.nrow) { ld4f(pl.phi + e + d.pitch, pd); ld4(pl.x + e + d.pitch, xd); }
             }
-            float pr = __shfl_down(pa[0], 1, 64);
-            T xr = __shfl_down(xa[0], 1, 64);
-            if (lane == 63) {
+            float pr = __shfl_down(pa[0], 1, LPR);
+            T xr = __shfl_down(xa[0], 1, LPR);
+            if (cl == LPR - 1) {
                 bool ok = cok && (c + 4 < d.pitch);
                 pr = ok ? pl.phi[e + 4] : 0.0f;
                 xr = ok ? pl.x[e + 4] : T(0);
@@ -374,18 +401,20 @@ __global__ __launch_bounds__(256) void k_weights(const InstDesc *descs, const in
 // Assemble (laplace_assemble :88-148) fused with the initial residual
 // r = b - L x (:431-434) and the Jacobi inverse 1/diag; zeroes p and s.
 // ═══════════════════════════════════════════════════════════════════════════
-template <class T, int RPW>
+template <class T, int RPW, int LPR>
 __global__ __launch_bounds__(256) void k_assemble(const InstDesc *descs, const int *blk_inst, Planes<T> pl) {
-    constexpr int TH = 4 * RPW;
-    __shared__ float lph[(TH + 2) * kLdsStride];
-    __shared__ T lx[(TH + 2) * kLdsStride];
-    TileInfo t = get_tile(descs, blk_inst, TH);
-    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    using Sh = Shape<RPW, LPR>;
+    constexpr int TH = Sh::TH, S = Sh::S;
+    __shared__ float lph[(TH + 2) * S];
+    __shared__ T lx[(TH + 2) * S];
+    TileInfo t = get_tile(descs, blk_inst, TH, Sh::TW);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int cl = lane % LPR, v = wave * Sh::G + lane / LPR;
     const InstDesc &d = t.d;
-    int c = t.tx0 + 4 * lane;
+    int c = t.tx0 + 4 * cl;
     bool cok = c < d.pitch;
     for (int k = 0; k < RPW; ++k) {
-        int lr = wave * RPW + k, row = t.ty0 + lr;
+        int lr = v * RPW + k, row = t.ty0 + lr;
         float ph[4] = {0, 0, 0, 0};
         T xv[4];
         zero4(xv);
@@ -394,12 +423,12 @@ __global__ __launch_bounds__(256) void k_assemble(const InstDesc *descs, const i
             ld4f(pl.phi + e, ph);
             ld4(pl.x + e, xv);
         }
-        float *lp = lph + (lr + 1) * kLdsStride + kLdsC0 + 4 * lane;
-        T *lq = lx + (lr + 1) * kLdsStride + kLdsC0 + 4 * lane;
+        float *lp = lph + (lr + 1) * S + kLdsC0 + 4 * cl;
+        T *lq = lx + (lr + 1) * S + kLdsC0 + 4 * cl;
 #pragma unroll
         for (int j = 0; j < 4; ++j) { lp[j] = ph[j]; lq[j] = xv[j]; }
     }
-    fill_halo<TH>(
+    fill_halo<TH, LPR>(
         t, wave, lane,
         [&](int lr, int lc, long long e, bool ok) {
             float ph[4] = {0, 0, 0, 0};
@@ -407,29 +436,29 @@ __global__ __launch_bounds__(256) void k_assemble(const InstDesc *descs, const i
             zero4(xv);
             if (ok) { ld4f(pl.phi + e, ph); ld4(pl.x + e, xv); }
 #pragma unroll
-            for (int j = 0; j < 4; ++j) { lph[lr * kLdsStride + lc + j] = ph[j]; lx[lr * kLdsStride + lc + j] = xv[j]; }
+            for (int j = 0; j < 4; ++j) { lph[lr * S + lc + j] = ph[j]; lx[lr * S + lc + j] = xv[j]; }
         },
         [&](int lr, int lc, long long e, bool ok) {
-            lph[lr * kLdsStride + lc] = ok ? pl.phi[e] : 0.0f;
-            lx[lr * kLdsStride + lc] = ok ? pl.x[e] : T(0);
+            lph[lr * S + lc] = ok ? pl.phi[e] : 0.0f;
+            lx[lr * S + lc] = ok ? pl.x[e] : T(0);
         });
     __syncthreads();
     T wu[4];
     for (int k = 0; k < RPW; ++k) {
-        int lr = wave * RPW + k, row = t.ty0 + lr;
+        int lr = v * RPW + k, row = t.ty0 + lr;
         if (row >= d.nrow) break;
-        if (k == 0) up_arcs(pl.w2, t, row, cok, lane, wu);
+        if (k == 0) up_arcs(pl.w2, t, row, cok, cl, wu);
         T wd[4], wr[4], wl[4];
-        row_arcs(pl.w2, t, row, cok, lane, wd, wr, wl);
-        const float *pc = lph + (lr + 1) * kLdsStride + kLdsC0 + 4 * lane;
+        row_arcs<T, LPR>(pl.w2, t, row, cok, cl, wd, wr, wl);
+        const float *pc = lph + (lr + 1) * S + kLdsC0 + 4 * cl;
         T b[4], di[4], y[4], rr[4], z4[4];
-        stencil4(lx, lr, lane, wd, wu, wr, wl, y);
+        stencil4<T, LPR>(lx, lr, cl, wd, wu, wr, wl, y);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             T pa = (T)pc[j];
             T bv = T(0), ws = T(0);
-            if (wd[j] > T(0)) { bv = Prec<T>::fmaT(-wd[j], Prec<T>::wrap((T)pc[j + kLdsStride] - pa), bv); ws += wd[j]; }
-            if (wu[j] > T(0)) { bv = Prec<T>::fmaT(wu[j], Prec<T>::wrap(pa - (T)pc[j - kLdsStride]), bv); ws += wu[j]; }
+            if (wd[j] > T(0)) { bv = Prec<T>::fmaT(-wd[j], Prec<T>::wrap((T)pc[j + S] - pa), bv); ws += wd[j]; }
+            if (wu[j] > T(0)) { bv = Prec<T>::fmaT(wu[j], Prec<T>::wrap(pa - (T)pc[j - S]), bv); ws += wu[j]; }
             if (wr[j] > T(0)) { bv = Prec<T>::fmaT(-wr[j], Prec<T>::wrap((T)pc[j + 1] - pa), bv); ws += wr[j]; }
             if (wl[j] > T(0)) { bv = Prec<T>::fmaT(wl[j], Prec<T>::wrap(pa - (T)pc[j - 1]), bv); ws += wl[j]; }
             b[j] = bv;
@@ -1067,14 +1096,14 @@ __device__ __forceinline__ void stagger_first_round() {
     }
 }
 
-template <class T, int RPW, bool STAGGER>
+template <class T, int RPW, int LPR, bool STAGGER>
 __global__ __launch_bounds__(256) void k_iq(const InstDesc *descs, const int *blk_inst, Planes<T> pl,
                                             const SolveScal *scal, double4 *partials, int parity, int start) {
-    constexpr int TH = 4 * RPW;
-    constexpr int S = kLdsStride;
+    using Sh = Shape<RPW, LPR>;
+    constexpr int TH = Sh::TH, S = Sh::S, TW = Sh::TW;
     __shared__ T lp0[(TH + 4) * S];   // p_j: LDS row = row + 2, col = col + kLdsC0
     __shared__ T lp1[(TH + 2) * S];   // p_{j+1}: LDS row = row + 1
-    TileInfo t = get_tile(descs, blk_inst, TH);
+    TileInfo t = get_tile(descs, blk_inst, TH, TW);
     const SolveScal *sc = scal + t.inst;
     if (!start && !sc->active) return;
     if (STAGGER) stagger_first_round();
@@ -1097,10 +1126,12 @@ __global__ __launch_bounds__(256) void k_iq(const InstDesc *descs, const int *bl
     const T *__restrict__ w2 = pl.w2;
     T *__restrict__ xg = pl.x;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int cl = lane % LPR, v = wave * Sh::G + lane / LPR;   // column group, lane group
     const InstDesc &d = t.d;
-    const int c = t.tx0 + 4 * lane;
+    const int c = t.tx0 + 4 * cl;
     const bool cok = c < d.pitch;
-    const int lrow0 = wave * RPW;            // tile-local first own row of this wave
+    const int lrow0 = v * RPW;               // tile-local first own row of this lane group
+    const bool top = v == 0, bot = v == 4 * Sh::G - 1;   // lane groups next to ring rows -1 and TH
     auto grow = [&](int lr) { return t.ty0 + lr; };
     auto elem = [&](int lr, int col) { return d.off + (long long)grow(lr) * d.pitch + col; };
     auto row_ok = [&](int lr) { return grow(lr) >= 0 && grow(lr) < d.nrow; };
@@ -1137,17 +1168,18 @@ __global__ __launch_bounds__(256) void k_iq(const InstDesc *descs, const int *bl
                 ld4(r_src + e, r4[k]);
                 ld_arcs(w2, e, wd[k], wr[k]);
             }
-            if (lane == 0 && t.tx0 > 0) wlo[k] = w2[2 * (e - 1) + 1];
+            if (cl == 0 && t.tx0 > 0) wlo[k] = w2[2 * (e - 1) + 1];
         }
     }
-    up_arcs(w2, t, grow(lrow0), cok && row_ok(lrow0), lane, wu);
-    // ring rows (wave 0: row -1 and p_j of row -2; wave 3: row TH and p_j of
-    // row TH+1): r, p, arcs; ring row arcs {down, right} + up + left
+    up_arcs(w2, t, grow(lrow0), cok && row_ok(lrow0), cl, wu);
+    // ring rows (top lane group: row -1 and p_j of row -2; bottom lane group:
+    // row TH and p_j of row TH+1): r, p, arcs; ring row arcs {down, right} +
+    // up + left
     T rr4[4], rp4[4], rwd[4], rwr[4], rwu[4], rwlo = T(0);
     zero4(rr4); zero4(rp4); zero4(rwd); zero4(rwr); zero4(rwu);
-    if (wave == 0 || wave == 3) {
-        const int lr = (wave == 0) ? -1 : TH;
-        const int lr2 = (wave == 0) ? -2 : TH + 1;
+    if (top || bot) {
+        const int lr = top ? -1 : TH;
+        const int lr2 = top ? -2 : TH + 1;
         T p2v[4];
         zero4(p2v);
         if (row_ok(lr)) {
@@ -1157,9 +1189,9 @@ __global__ __launch_bounds__(256) void k_iq(const InstDesc *descs, const int *bl
                 if (upd) ld4(p_in + e, rp4);
                 ld_arcs(w2, e, rwd, rwr);
             }
-            if (lane == 0 && t.tx0 > 0) rwlo = w2[2 * (e - 1) + 1];
+            if (cl == 0 && t.tx0 > 0) rwlo = w2[2 * (e - 1) + 1];
         }
-        if (wave == 0) {
+        if (top) {
             if (row_ok(lr2) && cok) {
                 T dummy[4];
                 ld_arcs(w2, elem(lr2, c), rwu, dummy);   // down arcs of row -2 = up arcs of row -1
@@ -1171,21 +1203,21 @@ __global__ __launch_bounds__(256) void k_iq(const InstDesc *descs, const int *bl
             if (useq && row_ok(lr2) && cok) ld4(p_in + elem(lr2, c), p2v);
         }
         if (upd) {
-            T *a = lp0 + (lr + 2) * S + kLdsC0 + 4 * lane;
-            T *b = lp0 + (lr2 + 2) * S + kLdsC0 + 4 * lane;
+            T *a = lp0 + (lr + 2) * S + kLdsC0 + 4 * cl;
+            T *b = lp0 + (lr2 + 2) * S + kLdsC0 + 4 * cl;
 #pragma unroll
             for (int j = 0; j < 4; ++j) { a[j] = rp4[j]; b[j] = p2v[j]; }
         }
     }
-    // ring columns (wave 1: column -1, wave 2: column 256): lane l < TH owns
+    // ring columns (wave 1: column -1, wave 2: column TW): lane l < TH owns
     // pixel (l, col); lanes l < TH + 2 stage p_j at (l - 1, col) and lanes
     // l < TH p_j at (l, col -/+ 1) beyond it
     T cr = T(0), cp = T(0), cwd = T(0), cwr = T(0), cwu = T(0), cwl = T(0);
     if (wave == 1 || wave == 2) {
-        const int col = (wave == 1) ? t.tx0 - 1 : t.tx0 + kTileW;
+        const int col = (wave == 1) ? t.tx0 - 1 : t.tx0 + TW;
         const int col2 = (wave == 1) ? col - 1 : col + 1;
         const bool colok = col >= 0 && col < d.pitch, col2ok = col2 >= 0 && col2 < d.pitch;
-        const int lcol = (wave == 1) ? -1 : kTileW, lcol2 = (wave == 1) ? -2 : kTileW + 1;
+        const int lcol = (wave == 1) ? -1 : TW, lcol2 = (wave == 1) ? -2 : TW + 1;
         if (lane < TH + 2) {
             const int lr = lane - 1;
             T v = T(0);
@@ -1227,7 +1259,7 @@ __global__ __launch_bounds__(256) void k_iq(const InstDesc *descs, const int *bl
     if (upd) {
 #pragma unroll
         for (int k = 0; k < RPW; ++k) {
-            T *a = lp0 + (lrow0 + k + 2) * S + kLdsC0 + 4 * lane;
+            T *a = lp0 + (lrow0 + k + 2) * S + kLdsC0 + 4 * cl;
 #pragma unroll
             for (int j = 0; j < 4; ++j) a[j] = p4[k][j];
         }
@@ -1238,8 +1270,8 @@ __global__ __launch_bounds__(256) void k_iq(const InstDesc *descs, const int *bl
     T wl[RPW][4], di[RPW][4];
 #pragma unroll
     for (int k = 0; k < RPW; ++k) {
-        T from_left = __shfl_up(wr[k][3], 1, 64);
-        wl[k][0] = (lane == 0) ? wlo[k] : from_left;
+        T from_left = __shfl_up(wr[k][3], 1, LPR);
+        wl[k][0] = (cl == 0) ? wlo[k] : from_left;
         wl[k][1] = wr[k][0]; wl[k][2] = wr[k][1]; wl[k][3] = wr[k][2];
         const T *wuk = (k == 0) ? wu : wd[k > 0 ? k - 1 : 0];
 #pragma unroll
@@ -1250,7 +1282,7 @@ __global__ __launch_bounds__(256) void k_iq(const InstDesc *descs, const int *bl
         const int lr = lrow0 + k;
         T q[4], rn[4], pn[4];
         zero4(q);
-        if (useq) stencil4(lp0 + S, lr, lane, wd[k], (k == 0) ? wu : wd[k > 0 ? k - 1 : 0], wr[k], wl[k], q);
+        if (useq) stencil4<T, LPR>(lp0 + S, lr, cl, wd[k], (k == 0) ? wu : wd[k > 0 ? k - 1 : 0], wr[k], wl[k], q);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             upd_px(r4[k][j], q[j], p4[k][j], di[k][j], rn[j], pn[j]);
@@ -1261,30 +1293,30 @@ __global__ __launch_bounds__(256) void k_iq(const InstDesc *descs, const int *bl
             st4_nt(r_out + e, rn);
             st4_nt(p_out + e, pn);
         }
-        T *a = lp1 + (lr + 1) * S + kLdsC0 + 4 * lane;
+        T *a = lp1 + (lr + 1) * S + kLdsC0 + 4 * cl;
 #pragma unroll
         for (int j = 0; j < 4; ++j) a[j] = pn[j];
     }
-    if (wave == 0 || wave == 3) {
-        const int lr = (wave == 0) ? -1 : TH;
+    if (top || bot) {
+        const int lr = top ? -1 : TH;
         T rwl[4], q[4], pn[4];
-        T from_left = __shfl_up(rwr[3], 1, 64);
-        rwl[0] = (lane == 0) ? rwlo : from_left;
+        T from_left = __shfl_up(rwr[3], 1, LPR);
+        rwl[0] = (cl == 0) ? rwlo : from_left;
         rwl[1] = rwr[0]; rwl[2] = rwr[1]; rwl[3] = rwr[2];
         zero4(q);
-        if (useq) stencil4(lp0 + S, lr, lane, rwd, rwu, rwr, rwl, q);
+        if (useq) stencil4<T, LPR>(lp0 + S, lr, cl, rwd, rwu, rwr, rwl, q);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             T rn;
             upd_px(rr4[j], q[j], rp4[j], diag_inv(rwd[j], rwu[j], rwr[j], rwl[j]), rn, pn[j]);
         }
-        T *a = lp1 + (lr + 1) * S + kLdsC0 + 4 * lane;
+        T *a = lp1 + (lr + 1) * S + kLdsC0 + 4 * cl;
 #pragma unroll
         for (int j = 0; j < 4; ++j) a[j] = pn[j];
     }
     if ((wave == 1 || wave == 2) && lane < TH) {
         const int lr = lane;
-        const int lcol = (wave == 1) ? -1 : kTileW;
+        const int lcol = (wave == 1) ? -1 : TW;
         const T *m = lp0 + (lr + 2) * S + kLdsC0 + lcol;
         T q = T(0);
         if (useq) q = stencil1(m[0], m[S], m[-S], m[1], m[-1], cwd, cwu, cwr, cwl);
@@ -1301,9 +1333,9 @@ __global__ __launch_bounds__(256) void k_iq(const InstDesc *descs, const int *bl
         const int lr = lrow0 + k;
         if (!row_ok(lr)) break;
         T y[4];
-        stencil4(lp1, lr, lane, wd[k], (k == 0) ? wu : wd[k > 0 ? k - 1 : 0], wr[k], wl[k], y);
+        stencil4<T, LPR>(lp1, lr, cl, wd[k], (k == 0) ? wu : wd[k > 0 ? k - 1 : 0], wr[k], wl[k], y);
         if (cok) {
-            const T *pc = lp1 + (lr + 1) * S + kLdsC0 + 4 * lane;
+            const T *pc = lp1 + (lr + 1) * S + kLdsC0 + 4 * cl;
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 double rd = (double)r4[k][j], dd = (double)di[k][j], qd = (double)y[j];
@@ -1322,24 +1354,26 @@ __global__ __launch_bounds__(256) void k_iq(const InstDesc *descs, const int *bl
 // (state in set 1 = (r2, p2) with alpha_j pending): x' = x + alpha_j p_j and
 // q_j = A p_j from LDS tiles with a one-pixel halo; r_t = b - A x' into wv;
 // partial sums of r_t.(dinv r_t) and of r'.(dinv r') for r' = r_j - alpha_j q_j.
-template <class T, int RPW>
+template <class T, int RPW, int LPR>
 __global__ __launch_bounds__(256) void k_rr_iq(const InstDesc *descs, const int *blk_inst, Planes<T> pl,
                                                const SolveScal *scal, double4 *partials) {
-    constexpr int TH = 4 * RPW;
-    __shared__ T lx[(TH + 2) * kLdsStride];
-    __shared__ T lq[(TH + 2) * kLdsStride];
-    TileInfo t = get_tile(descs, blk_inst, TH);
+    using Sh = Shape<RPW, LPR>;
+    constexpr int TH = Sh::TH, S = Sh::S;
+    __shared__ T lx[(TH + 2) * S];
+    __shared__ T lq[(TH + 2) * S];
+    TileInfo t = get_tile(descs, blk_inst, TH, Sh::TW);
     const SolveScal *sc = scal + t.inst;
     if (!sc->active) return;
     const T al = (T)sc->alpha;
     const T *__restrict__ pg = pl.p2;
     const T *__restrict__ xg = pl.x;
-    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int cl = lane % LPR, v = wave * Sh::G + lane / LPR;
     const InstDesc &d = t.d;
-    const int c = t.tx0 + 4 * lane;
+    const int c = t.tx0 + 4 * cl;
     const bool cok = c < d.pitch;
     for (int k = 0; k < RPW; ++k) {
-        int lr = wave * RPW + k, row = t.ty0 + lr;
+        int lr = v * RPW + k, row = t.ty0 + lr;
         T xv[4], pv[4];
         zero4(xv); zero4(pv);
         if (row < d.nrow && cok) {
@@ -1347,11 +1381,11 @@ __global__ __launch_bounds__(256) void k_rr_iq(const InstDesc *descs, const int 
             ld4(xg + e, xv);
             ld4(pg + e, pv);
         }
-        const int o = (lr + 1) * kLdsStride + kLdsC0 + 4 * lane;
+        const int o = (lr + 1) * S + kLdsC0 + 4 * cl;
 #pragma unroll
         for (int j = 0; j < 4; ++j) { lx[o + j] = Prec<T>::fmaT(al, pv[j], xv[j]); lq[o + j] = pv[j]; }
     }
-    fill_halo<TH>(
+    fill_halo<TH, LPR>(
         t, wave, lane,
         [&](int lr, int lc, long long e, bool ok) {
             T xv[4], pv[4];
@@ -1359,26 +1393,26 @@ __global__ __launch_bounds__(256) void k_rr_iq(const InstDesc *descs, const int 
             if (ok) { ld4(xg + e, xv); ld4(pg + e, pv); }
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                lx[lr * kLdsStride + lc + j] = Prec<T>::fmaT(al, pv[j], xv[j]);
-                lq[lr * kLdsStride + lc + j] = pv[j];
+                lx[lr * S + lc + j] = Prec<T>::fmaT(al, pv[j], xv[j]);
+                lq[lr * S + lc + j] = pv[j];
             }
         },
         [&](int lr, int lc, long long e, bool ok) {
-            lx[lr * kLdsStride + lc] = ok ? Prec<T>::fmaT(al, pg[e], xg[e]) : T(0);
-            lq[lr * kLdsStride + lc] = ok ? pg[e] : T(0);
+            lx[lr * S + lc] = ok ? Prec<T>::fmaT(al, pg[e], xg[e]) : T(0);
+            lq[lr * S + lc] = ok ? pg[e] : T(0);
         });
     __syncthreads();
     double s_t = 0.0, s_r = 0.0;
     T wu[4];
     for (int k = 0; k < RPW; ++k) {
-        int lr = wave * RPW + k, row = t.ty0 + lr;
+        int lr = v * RPW + k, row = t.ty0 + lr;
         if (row >= d.nrow) break;
-        if (k == 0) up_arcs(pl.w2, t, row, cok, lane, wu);
+        if (k == 0) up_arcs(pl.w2, t, row, cok, cl, wu);
         T wd[4], wr[4], wl[4], q[4];
         double y[4];
-        row_arcs(pl.w2, t, row, cok, lane, wd, wr, wl);
-        stencil4_wide(lx, lr, lane, wd, wu, wr, wl, y);
-        stencil4(lq, lr, lane, wd, wu, wr, wl, q);
+        row_arcs<T, LPR>(pl.w2, t, row, cok, cl, wd, wr, wl);
+        stencil4_wide<T, LPR>(lx, lr, cl, wd, wu, wr, wl, y);
+        stencil4<T, LPR>(lq, lr, cl, wd, wu, wr, wl, q);
         if (cok) {
             long long e = d.off + (long long)row * d.pitch + c;
             T b4[4], rt[4], r4[4];
@@ -1403,20 +1437,21 @@ __global__ __launch_bounds__(256) void k_rr_iq(const InstDesc *descs, const int 
 // Deferred x update still pending when a q-free solve stopped (xmode 1 after
 // an odd number of launches): x += alpha_j p_j, p_j in plane set 1 (the
 // input set of the odd-numbered launches).
-template <class T, int RPW>
+template <class T, int RPW, int LPR>
 __global__ __launch_bounds__(256) void k_x_finalize(const InstDesc *descs, const int *blk_inst, Planes<T> pl,
                                                     const SolveScal *scal) {
-    constexpr int TH = 4 * RPW;
-    TileInfo t = get_tile(descs, blk_inst, TH);
+    using Sh = Shape<RPW, LPR>;
+    TileInfo t = get_tile(descs, blk_inst, Sh::TH, Sh::TW);
     const SolveScal *sc = scal + t.inst;
     if (sc->xmode != 1) return;
     const T al = (T)sc->alpha_prev;
-    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int cl = lane % LPR, v = wave * Sh::G + lane / LPR;
     const InstDesc &d = t.d;
-    const int c = t.tx0 + 4 * lane;
+    const int c = t.tx0 + 4 * cl;
     if (c >= d.pitch) return;
     for (int k = 0; k < RPW; ++k) {
-        const int row = t.ty0 + wave * RPW + k;
+        const int row = t.ty0 + v * RPW + k;
         if (row >= d.nrow) break;
         const long long e = d.off + (long long)row * d.pitch + c;
         T x4[4], p4[4];
@@ -1722,22 +1757,48 @@ void launch_ingest(const IngestSrc *d_src, const InstDesc *d_desc, const int *d_
     case 8: hipLaunchKernelGGL((KERNEL<T, 8>), __VA_ARGS__); break;               \
     default: hipLaunchKernelGGL((KERNEL<T, 4>), __VA_ARGS__); break;              \
     }
-#define CUPHU_RPW_DISPATCH_X(rpw, KERNEL, X, ...)                                 \
-    switch (rpw) {                                                                \
-    case 2: hipLaunchKernelGGL((KERNEL<T, 2, X>), __VA_ARGS__); break;            \
-    case 8: hipLaunchKernelGGL((KERNEL<T, 8, X>), __VA_ARGS__); break;            \
-    default: hipLaunchKernelGGL((KERNEL<T, 4, X>), __VA_ARGS__); break;           \
+// Tile shapes of the fused solver's kernels: (rpw, lpr) in {2, 4, 8} x {64}
+// and {2, 4} x {32}; KERNEL<T, RPW, LPR, extra...>.
+static void check_shape(TileShape sh) {
+    const bool ok = (sh.lpr == 64 && (sh.rpw == 2 || sh.rpw == 4 || sh.rpw == 8)) ||
+                    (sh.lpr == 32 && (sh.rpw == 2 || sh.rpw == 4));
+    CUPHU_REQUIRE(ok, "unsupported tile shape");
+}
+#define CUPHU_SHAPE_DISPATCH(sh, KERNEL, ...)                                      \
+    check_shape(sh);                                                               \
+    if (sh.lpr == 32) {                                                            \
+        if (sh.rpw == 2) hipLaunchKernelGGL((KERNEL<T, 2, 32>), __VA_ARGS__);      \
+        else hipLaunchKernelGGL((KERNEL<T, 4, 32>), __VA_ARGS__);                  \
+    } else {                                                                       \
+        switch (sh.rpw) {                                                          \
+        case 2: hipLaunchKernelGGL((KERNEL<T, 2, 64>), __VA_ARGS__); break;        \
+        case 8: hipLaunchKernelGGL((KERNEL<T, 8, 64>), __VA_ARGS__); break;        \
+        default: hipLaunchKernelGGL((KERNEL<T, 4, 64>), __VA_ARGS__); break;       \
+        }                                                                          \
+    }
+#define CUPHU_SHAPE_DISPATCH_X(sh, KERNEL, X, ...)                                 \
+    check_shape(sh);                                                               \
+    if (sh.lpr == 32) {                                                            \
+        if (sh.rpw == 2) hipLaunchKernelGGL((KERNEL<T, 2, 32, X>), __VA_ARGS__);   \
+        else hipLaunchKernelGGL((KERNEL<T, 4, 32, X>), __VA_ARGS__);               \
+    } else {                                                                       \
+        switch (sh.rpw) {                                                          \
+        case 2: hipLaunchKernelGGL((KERNEL<T, 2, 64, X>), __VA_ARGS__); break;     \
+        case 8: hipLaunchKernelGGL((KERNEL<T, 8, 64, X>), __VA_ARGS__); break;     \
+        default: hipLaunchKernelGGL((KERNEL<T, 4, 64, X>), __VA_ARGS__); break;    \
+        }                                                                          \
     }
 
 template <class T>
 void launch_weights(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, int reweight,
-                    T delta2, int rpw, hipStream_t st) {
-    CUPHU_RPW_DISPATCH(rpw, k_weights, dim3(nblk), dim3(256), 0, st, d_desc, d_blk, pl, reweight, delta2);
+                    T delta2, TileShape sh, hipStream_t st) {
+    CUPHU_SHAPE_DISPATCH(sh, k_weights, dim3(nblk), dim3(256), 0, st, d_desc, d_blk, pl, reweight, delta2);
     CUPHU_LAUNCH_CHECK();
 }
 template <class T>
-void launch_assemble(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, int rpw, hipStream_t st) {
-    CUPHU_RPW_DISPATCH(rpw, k_assemble, dim3(nblk), dim3(256), 0, st, d_desc, d_blk, pl);
+void launch_assemble(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, TileShape sh,
+                     hipStream_t st) {
+    CUPHU_SHAPE_DISPATCH(sh, k_assemble, dim3(nblk), dim3(256), 0, st, d_desc, d_blk, pl);
     CUPHU_LAUNCH_CHECK();
 }
 template <class T>
@@ -1784,26 +1845,26 @@ void launch_rr_it(const InstDesc *d_desc, const int *d_blk, int nblk, const Plan
 }
 template <class T>
 void launch_iq(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, const SolveScal *scal,
-               double4 *partials, int rpw, int parity, int start, hipStream_t st) {
+               double4 *partials, TileShape sh, int parity, int start, hipStream_t st) {
     if (nblk >= kStaggerMinGrid) {
-        CUPHU_RPW_DISPATCH_X(rpw, k_iq, true, dim3(nblk), dim3(256), 0, st, d_desc, d_blk, pl, scal, partials, parity,
-                             start);
+        CUPHU_SHAPE_DISPATCH_X(sh, k_iq, true, dim3(nblk), dim3(256), 0, st, d_desc, d_blk, pl, scal, partials,
+                               parity, start);
     } else {
-        CUPHU_RPW_DISPATCH_X(rpw, k_iq, false, dim3(nblk), dim3(256), 0, st, d_desc, d_blk, pl, scal, partials, parity,
-                             start);
+        CUPHU_SHAPE_DISPATCH_X(sh, k_iq, false, dim3(nblk), dim3(256), 0, st, d_desc, d_blk, pl, scal, partials,
+                               parity, start);
     }
     CUPHU_LAUNCH_CHECK();
 }
 template <class T>
 void launch_rr_iq(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, const SolveScal *scal,
-                  double4 *partials, int rpw, hipStream_t st) {
-    CUPHU_RPW_DISPATCH(rpw, k_rr_iq, dim3(nblk), dim3(256), 0, st, d_desc, d_blk, pl, scal, partials);
+                  double4 *partials, TileShape sh, hipStream_t st) {
+    CUPHU_SHAPE_DISPATCH(sh, k_rr_iq, dim3(nblk), dim3(256), 0, st, d_desc, d_blk, pl, scal, partials);
     CUPHU_LAUNCH_CHECK();
 }
 template <class T>
 void launch_x_finalize(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, const SolveScal *scal,
-                       int rpw, hipStream_t st) {
-    CUPHU_RPW_DISPATCH(rpw, k_x_finalize, dim3(nblk), dim3(256), 0, st, d_desc, d_blk, pl, scal);
+                       TileShape sh, hipStream_t st) {
+    CUPHU_SHAPE_DISPATCH(sh, k_x_finalize, dim3(nblk), dim3(256), 0, st, d_desc, d_blk, pl, scal);
     CUPHU_LAUNCH_CHECK();
 }
 void launch_reduce_it(const InstDesc *d_desc, int ninst, const double4 *partials, SolveScal *scal, int mode,
@@ -1860,9 +1921,10 @@ void launch_plain_dot(const T *a, const T *b, size_t n, double *partials, double
 #define CUPHU_INSTANTIATE(T)                                                                                    \
     template void launch_ingest<T>(const IngestSrc *, const InstDesc *, const int *, int, const Planes<T> &, int, \
                                    int, hipStream_t);                                                            \
-    template void launch_weights<T>(const InstDesc *, const int *, int, const Planes<T> &, int, T, int,          \
+    template void launch_weights<T>(const InstDesc *, const int *, int, const Planes<T> &, int, T, TileShape,    \
                                     hipStream_t);                                                                \
-    template void launch_assemble<T>(const InstDesc *, const int *, int, const Planes<T> &, int, hipStream_t);   \
+    template void launch_assemble<T>(const InstDesc *, const int *, int, const Planes<T> &, TileShape,           \
+                                     hipStream_t);                                                               \
     template void launch_rz<T>(const InstDesc *, const int *, int, const Planes<T> &, const SolveScal *, int,    \
                                double2 *, int, hipStream_t);                                                     \
     template void launch_pq<T>(const InstDesc *, const int *, int, const Planes<T> &, const SolveScal *,         \
@@ -1878,11 +1940,11 @@ void launch_plain_dot(const T *a, const T *b, size_t n, double *partials, double
     template void launch_rr_it<T>(const InstDesc *, const int *, int, const Planes<T> &, const SolveScal *,      \
                                   double4 *, int, hipStream_t);                                                  \
     template void launch_iq<T>(const InstDesc *, const int *, int, const Planes<T> &, const SolveScal *,         \
-                               double4 *, int, int, int, hipStream_t);                                           \
+                               double4 *, TileShape, int, int, hipStream_t);                                     \
     template void launch_rr_iq<T>(const InstDesc *, const int *, int, const Planes<T> &, const SolveScal *,      \
-                                  double4 *, int, hipStream_t);                                                  \
+                                  double4 *, TileShape, hipStream_t);                                            \
     template void launch_x_finalize<T>(const InstDesc *, const int *, int, const Planes<T> &, const SolveScal *,  \
-                                       int, hipStream_t);                                                        \
+                                       TileShape, hipStream_t);                                                  \
     template void launch_final<T>(const InstDesc *, const int *, int, const Planes<T> &, float *, int,           \
                                   hipStream_t);                                                                  \
     template void launch_plain_matvec<T>(const InstDesc &, const T *, const T *, T *, hipStream_t);              \

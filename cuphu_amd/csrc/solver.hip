@@ -207,15 +207,15 @@ struct Layout {
     long long total_el = 0;
     int total_rows = 0, max_pitch = 64;
 };
-static Layout make_layout(const std::vector<std::pair<int, int>> &dims, int th) {
+static Layout make_layout(const std::vector<std::pair<int, int>> &dims, TileShape sh) {
     Layout L;
     for (auto &dm : dims) {
         InstDesc d{};
         d.nrow = dm.first;
         d.ncol = dm.second;
         d.pitch = pitch_of(d.ncol);
-        d.tiles_x = (int)ceil_div((size_t)d.pitch, kTileW);
-        d.tiles_y = (int)ceil_div((size_t)d.nrow, (size_t)th);
+        d.tiles_x = (int)ceil_div((size_t)d.pitch, (size_t)sh.tw());
+        d.tiles_y = (int)ceil_div((size_t)d.nrow, (size_t)sh.th());
         d.row_begin = L.total_rows;
         d.off = L.total_el;
         L.total_el += (long long)d.nrow * d.pitch;
@@ -363,8 +363,8 @@ static void solve_group_batched(const Planes<T> &pl, const GroupDev &g, const Cu
                     float df = pass_delta(o, pass);
                     d2 = (sizeof(T) == 4) ? (T)(df * df) : (T)((double)df * (double)df);
                 }
-                launch_weights<T>(g.desc, g.blk, g.nblk, pl, pass > 0 ? 1 : 0, d2, rpw, st);
-                launch_assemble<T>(g.desc, g.blk, g.nblk, pl, rpw, st);
+                launch_weights<T>(g.desc, g.blk, g.nblk, pl, pass > 0 ? 1 : 0, d2, TileShape{rpw, 64}, st);
+                launch_assemble<T>(g.desc, g.blk, g.nblk, pl, TileShape{rpw, 64}, st);
             }
             launch_rz<T>(g.desc, g.blk, g.nblk, pl, g.scal, 1, g.part, rpw, st);
             launch_reduce(g.desc, g.ninst, g.part, g.scal, RED_START, o.max_iter, tol2, st);
@@ -439,7 +439,7 @@ static bool fused_stored_q() {
 
 // Fused solver: one stencil kernel and one reduction per PCG iteration.
 template <class T>
-static void solve_group_fused(const Planes<T> &pl, const GroupDev &g, const CuPhuLaplaceOptions &o, int rpw,
+static void solve_group_fused(const Planes<T> &pl, const GroupDev &g, const CuPhuLaplaceOptions &o, TileShape sh,
                               bool fixed_weights, hipStream_t st, const std::vector<CuPhuSolveStats *> &stats,
                               std::vector<SolveScal> &h) {
     // Chunk length: even, so a chunk (parities 1, 0, 1, ..., 0) leaves the
@@ -452,13 +452,15 @@ static void solve_group_fused(const Planes<T> &pl, const GroupDev &g, const CuPh
     if (prof && !g_prof.ev[0])
         for (auto &e : g_prof.ev) CUPHU_HIP_CHECK(hipEventCreate(&e));
     const bool sq = fused_stored_q();
+    CUPHU_REQUIRE(!sq || sh.lpr == 64, "the stored-q kernels run 256-column tiles only");
+    const int rpw = sh.rpw;
     // values moved per pixel by two consecutive launches: k_it 10 + 10;
     // k_iq 6 (x deferred: reads r, p, arc pair; writes r, p) + 9 (also reads
     // x and the previous p, writes x)
     const double pair_values_per_px = sq ? 20.0 : 15.0;
     auto it = [&](int parity, int start) {
         if (sq) launch_it<T>(g.desc, g.blk, g.nblk, pl, g.scal, g.part4, rpw, parity, start, st);
-        else launch_iq<T>(g.desc, g.blk, g.nblk, pl, g.scal, g.part4, rpw, parity, start, st);
+        else launch_iq<T>(g.desc, g.blk, g.nblk, pl, g.scal, g.part4, sh, parity, start, st);
     };
     auto step = [&](int i) {
         it(1 - (i & 1), 0);
@@ -471,7 +473,7 @@ static void solve_group_fused(const Planes<T> &pl, const GroupDev &g, const CuPh
         for (int i = prof ? 2 : 0; i < CH; ++i) step(i);
         if (o.rr_interval > 0) {
             if (sq) launch_rr_it<T>(g.desc, g.blk, g.nblk, pl, g.scal, g.part4, rpw, st);
-            else launch_rr_iq<T>(g.desc, g.blk, g.nblk, pl, g.scal, g.part4, rpw, st);
+            else launch_rr_iq<T>(g.desc, g.blk, g.nblk, pl, g.scal, g.part4, sh, st);
             launch_reduce_it(g.desc, g.ninst, g.part4, g.scal, RED_RR, o.max_iter, tol2, st);
         }
     };
@@ -500,8 +502,8 @@ static void solve_group_fused(const Planes<T> &pl, const GroupDev &g, const CuPh
                     float df = pass_delta(o, pass);
                     d2 = (sizeof(T) == 4) ? (T)(df * df) : (T)((double)df * (double)df);
                 }
-                launch_weights<T>(g.desc, g.blk, g.nblk, pl, pass > 0 ? 1 : 0, d2, rpw, st);
-                launch_assemble<T>(g.desc, g.blk, g.nblk, pl, rpw, st);
+                launch_weights<T>(g.desc, g.blk, g.nblk, pl, pass > 0 ? 1 : 0, d2, sh, st);
+                launch_assemble<T>(g.desc, g.blk, g.nblk, pl, sh, st);
             }
             it(0, 1);
             launch_reduce_it(g.desc, g.ninst, g.part4, g.scal, RED_START, o.max_iter, tol2, st, sq ? 0 : 1);
@@ -552,7 +554,7 @@ static void solve_group_fused(const Planes<T> &pl, const GroupDev &g, const CuPh
                 ++nch;
                 if ((long long)nch * CH > (long long)o.max_iter + CH) break;   // cannot happen: reduce caps iterations
             }
-            if (!sq) launch_x_finalize<T>(g.desc, g.blk, g.nblk, pl, g.scal, rpw, st);
+            if (!sq) launch_x_finalize<T>(g.desc, g.blk, g.nblk, pl, g.scal, sh, st);
             if (o.verbose)
                 for (int i = 0; i < g.ninst; ++i)
                     fprintf(stderr, "[cuphu-pcg] inst %d pass %d iters %d relres %.3e status %d\n", i, pass, h[i].iters,
@@ -567,11 +569,15 @@ static void solve_group_fused(const Planes<T> &pl, const GroupDev &g, const CuPh
 }
 
 template <class T>
-static void solve_group(const Planes<T> &pl, const GroupDev &g, const CuPhuLaplaceOptions &o, int rpw,
+static void solve_group(const Planes<T> &pl, const GroupDev &g, const CuPhuLaplaceOptions &o, TileShape sh,
                         bool fixed_weights, hipStream_t st, const std::vector<CuPhuSolveStats *> &stats,
                         std::vector<SolveScal> &h) {
-    if (o.solver == CUPHU_SOLVER_FUSED) solve_group_fused<T>(pl, g, o, rpw, fixed_weights, st, stats, h);
-    else solve_group_batched<T>(pl, g, o, rpw, fixed_weights, st, stats, h);
+    if (o.solver == CUPHU_SOLVER_FUSED) {
+        solve_group_fused<T>(pl, g, o, sh, fixed_weights, st, stats, h);
+    } else {
+        CUPHU_REQUIRE(sh.lpr == 64, "the batched solver runs 256-column tiles only");
+        solve_group_batched<T>(pl, g, o, sh.rpw, fixed_weights, st, stats, h);
+    }
 }
 
 // ── plain solve of one instance (reference launch sequence) ─────────────────
@@ -597,8 +603,8 @@ static void solve_instance_plain(const Planes<T> &pl, const GroupDev &g, const I
             float df = pass_delta(o, pass);
             d2 = (sizeof(T) == 4) ? (T)(df * df) : (T)((double)df * (double)df);
         }
-        launch_weights<T>(g.desc, g.blk, g.nblk, pl, pass > 0 ? 1 : 0, d2, rpw, st);
-        launch_assemble<T>(g.desc, g.blk, g.nblk, pl, rpw, st);
+        launch_weights<T>(g.desc, g.blk, g.nblk, pl, pass > 0 ? 1 : 0, d2, TileShape{rpw, 64}, st);
+        launch_assemble<T>(g.desc, g.blk, g.nblk, pl, TileShape{rpw, 64}, st);
         launch_plain_pz<T>(p, z, r, dinv, n, st);
         double rz = dot(r, z), rz0 = rz, rzl = rz;
         int iter = 0, status;
@@ -641,23 +647,73 @@ struct BatchOut {
     unsigned char *mask = nullptr;
 };
 
-// ── tile-height autotuner ───────────────────────────────────────────────────
-// The stencil kernels' tile height (4 * rows-per-wave: 8, 16 or 32 rows)
-// trades halo re-reads against registers and occupancy, and the best value
-// depends on the instance shapes.  On the first batched/fused solve of a
-// shape class (precision, widest pitch, instance count and pixel count
-// classes) each candidate is timed on the real pooled planes with the fused
-// kernel's steady-state step form before the solve starts (scratch scalars;
-// reads plane set 1, writes set 0, which the first assemble resets; x is not
-// touched), 1 warm-up + 3 timed launches, minimum taken; the fastest is
-// cached for the process.  Opt-in (options.autotune
-// or CUPHU_LAPLACE_AUTOTUNE=1): the tiling fixes the order of the FP64
-// partial sums, so a timing-dependent choice can change results in the last
-// bits between processes.  CUPHU_LAPLACE_RPW forces a value.
-static void retile(Layout &L, int th) {
-    for (auto &d : L.desc) d.tiles_y = (int)ceil_div((size_t)d.nrow, (size_t)th);
+// ── tile shape ──────────────────────────────────────────────────────────────
+// Every tile is processed whole, so an instance occupies ceil(pitch / TW) * TW
+// x ceil(nrow / TH) * TH lanes' pixels whatever its size: a pitch 64 past a
+// multiple of 256 leaves 3/4 of its last 256-column tile idle.
+static double tiled_px(const Layout &L, TileShape sh) {
+    double n = 0.0;
+    for (auto &d : L.desc)
+        n += (double)(ceil_div((size_t)d.pitch, (size_t)sh.tw()) * (size_t)sh.tw()) *
+             (double)(ceil_div((size_t)d.nrow, (size_t)sh.th()) * (size_t)sh.th());
+    return n;
 }
 
+// Blocks of a launch that spans 4 resident rounds (3 workgroups x 256 CUs
+// each).
+constexpr size_t kNarrowMinBlocks = 4 * 768;
+
+static size_t tiled_blocks(const Layout &L, TileShape sh) {
+    size_t n = 0;
+    for (auto &d : L.desc)
+        n += ceil_div((size_t)d.pitch, (size_t)sh.tw()) * ceil_div((size_t)d.nrow, (size_t)sh.th());
+    return n;
+}
+
+// Default shape: rows_per_wave() (CUPHU_LAPLACE_RPW) rows per lane group and
+// 256-column tiles, or, on the fused solver, 128-column tiles when they
+// leave exactly as many lanes idle and the launch spans at least 4 resident
+// rounds.  Measured on one GPU, alternating pairs:
+//   - 8192^2 scene in 2048-column tiles (no idle lanes either way): the
+//     128 x 32 tile reads 5 % fewer halo pixels; k_iq 0.4343 -> 0.4324 ms
+//     per launch, 24.86 -> 25.00 Mpixel/s;
+//   - one 2048^2 image (1024 workgroups, 1.3 per slot): 19.85 -> 19.35;
+//   - 64 interferograms of mixed shapes, where 128-column tiles leave fewer
+//     lanes idle: 27.87 -> 27.76 (idle lanes cost little in this HBM-bound
+//     kernel: their loads are never issued).
+// CUPHU_LAPLACE_LPR=64 or 32 forces the width (64 or 32 lanes per tile row).
+static TileShape default_shape(const Layout &L, const CuPhuLaplaceOptions &o) {
+    const int rpw = rows_per_wave();
+    const TileShape wide{rpw, 64}, narrow{rpw, 32};
+    if (o.solver != CUPHU_SOLVER_FUSED || fused_stored_q() || rpw > 4) return wide;
+    int lpr = 0;
+    env_int("CUPHU_LAPLACE_LPR", &lpr);
+    if (lpr == 64) return wide;
+    if (lpr == 32) return narrow;
+    return tiled_px(L, narrow) == tiled_px(L, wide) && tiled_blocks(L, narrow) >= kNarrowMinBlocks ? narrow : wide;
+}
+
+static void retile(Layout &L, TileShape sh) {
+    for (auto &d : L.desc) {
+        d.tiles_x = (int)ceil_div((size_t)d.pitch, (size_t)sh.tw());
+        d.tiles_y = (int)ceil_div((size_t)d.nrow, (size_t)sh.th());
+    }
+}
+
+// ── tile-shape autotuner ────────────────────────────────────────────────────
+// The stencil kernels' tile shape trades halo re-reads and idle lanes against
+// registers and occupancy, and the best value depends on the instance
+// shapes.  On the first batched/fused solve of a shape class (precision,
+// widest pitch, instance count and pixel count classes) each candidate
+// (256-column tiles of 8, 16 or 32 rows; on the fused solver also
+// 128-column tiles of 16 or 32 rows) is timed on the real pooled planes with
+// the fused kernel's steady-state step form before the solve starts (scratch
+// scalars; reads plane set 1, writes set 0, which the first assemble resets;
+// x is not touched), 1 warm-up + 3 timed launches, minimum taken; the
+// fastest is cached for the process.  Opt-in (options.autotune or
+// CUPHU_LAPLACE_AUTOTUNE=1): the tiling fixes the order of the FP64 partial
+// sums, so a timing-dependent choice can change results in the last bits
+// between processes.  CUPHU_LAPLACE_RPW forces the default shape.
 static int ilog2_ceil(long long v) {
     int k = 0;
     while ((1LL << k) < v && k < 62) ++k;
@@ -665,18 +721,25 @@ static int ilog2_ceil(long long v) {
 }
 
 template <class T>
-static int autotune_rpw(Layout L, const Planes<T> &pl, int verbose, DevRes &res, hipStream_t st) {
-    if (std::getenv("CUPHU_LAPLACE_RPW")) return rows_per_wave();
-    static std::map<std::string, int> cache;
+static TileShape autotune_shape(Layout L, const Planes<T> &pl, const CuPhuLaplaceOptions &o, DevRes &res,
+                                hipStream_t st) {
+    if (std::getenv("CUPHU_LAPLACE_RPW")) return default_shape(L, o);
+    static std::map<std::string, TileShape> cache;
     const int ninst = (int)L.desc.size();
+    const bool narrow_ok = o.solver == CUPHU_SOLVER_FUSED && !fused_stored_q();
     const std::string key = std::to_string(sizeof(T)) + "/" + std::to_string(L.max_pitch) + "/" +
-                            std::to_string(ilog2_ceil(ninst)) + "/" + std::to_string(ilog2_ceil(L.total_el));
+                            std::to_string(ilog2_ceil(ninst)) + "/" + std::to_string(ilog2_ceil(L.total_el)) + "/" +
+                            std::to_string((int)narrow_ok);
     auto it = cache.find(key);
     if (it != cache.end()) return it->second;
-    const int cand[3] = {2, 4, 8};
+    std::vector<TileShape> cand = {{2, 64}, {4, 64}, {8, 64}};
+    if (narrow_ok) {
+        cand.push_back({2, 32});
+        cand.push_back({4, 32});
+    }
     size_t maxblk = 0;
-    for (int rpw : cand) {
-        retile(L, 4 * rpw);
+    for (TileShape sh : cand) {
+        retile(L, sh);
         size_t nb = 0;
         for (auto &d : L.desc) nb += (size_t)d.tiles_x * d.tiles_y;
         maxblk = std::max(maxblk, nb);
@@ -694,11 +757,11 @@ static int autotune_rpw(Layout L, const Planes<T> &pl, int verbose, DevRes &res,
     hipEvent_t e0, e1;
     CUPHU_HIP_CHECK(hipEventCreate(&e0));
     CUPHU_HIP_CHECK(hipEventCreate(&e1));
-    int best = 4;
+    TileShape best{4, 64};
     float best_ms = 1e30f;
     try {
-        for (int rpw : cand) {
-            retile(L, 4 * rpw);
+        for (TileShape sh : cand) {
+            retile(L, sh);
             std::vector<InstDesc> gd = L.desc;
             std::vector<int> blk;
             for (int i = 0; i < ninst; ++i) {
@@ -708,19 +771,21 @@ static int autotune_rpw(Layout L, const Planes<T> &pl, int verbose, DevRes &res,
             CUPHU_HIP_CHECK(hipMemcpyAsync(d_desc, gd.data(), gd.size() * sizeof(InstDesc), hipMemcpyHostToDevice, st));
             CUPHU_HIP_CHECK(hipMemcpyAsync(d_blk, blk.data(), blk.size() * sizeof(int), hipMemcpyHostToDevice, st));
             const int nblk = (int)blk.size();
-            launch_iq<T>(d_desc, d_blk, nblk, pl, d_scal, d_part, rpw, 1, 0, st);
+            launch_iq<T>(d_desc, d_blk, nblk, pl, d_scal, d_part, sh, 1, 0, st);
             float ms_min = 1e30f;
             for (int rep = 0; rep < 3; ++rep) {
                 CUPHU_HIP_CHECK(hipEventRecord(e0, st));
-                launch_iq<T>(d_desc, d_blk, nblk, pl, d_scal, d_part, rpw, 1, 0, st);
+                launch_iq<T>(d_desc, d_blk, nblk, pl, d_scal, d_part, sh, 1, 0, st);
                 CUPHU_HIP_CHECK(hipEventRecord(e1, st));
                 CUPHU_HIP_CHECK(hipEventSynchronize(e1));
                 float ms = 0.0f;
                 CUPHU_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
                 ms_min = std::min(ms_min, ms);
             }
-            if (verbose) fprintf(stderr, "[cuphu-autotune] %s rows/wave %d: %.4f ms\n", key.c_str(), rpw, ms_min);
-            if (ms_min < best_ms) { best_ms = ms_min; best = rpw; }
+            if (o.verbose)
+                fprintf(stderr, "[cuphu-autotune] %s rows/lane group %d, lanes/row %d: %.4f ms\n", key.c_str(), sh.rpw,
+                        sh.lpr, ms_min);
+            if (ms_min < best_ms) { best_ms = ms_min; best = sh; }
         }
     } catch (...) {
         (void)hipEventDestroy(e0);
@@ -738,7 +803,7 @@ template <class T, class Ingest>
 static void run_batch(const std::vector<std::pair<int, int>> &dims, const CuPhuLaplaceOptions &o, double minfrac,
                       long maxncomps, bool do_conncomp, DevRes &res, hipStream_t st, CuPhuSolveStats *stats,
                       BatchOut &out, Ingest &&ingest) {
-    Layout L = make_layout(dims, 4 * rows_per_wave());
+    Layout L = make_layout(dims, TileShape{rows_per_wave(), 64});
     int ninst = (int)dims.size();
     PoolSet<T> P = alloc_pools<T>(res, L.total_el);
     InstDesc *d_gdesc = ws<InstDesc>(res, "gdesc", (size_t)ninst);
@@ -749,11 +814,11 @@ static void run_batch(const std::vector<std::pair<int, int>> &dims, const CuPhuL
     CUPHU_HIP_CHECK(hipMemcpyAsync(d_gdesc, L.desc.data(), (size_t)ninst * sizeof(InstDesc), hipMemcpyHostToDevice, st));
     CUPHU_HIP_CHECK(hipMemcpyAsync(d_rowinst, rowinst.data(), rowinst.size() * sizeof(int), hipMemcpyHostToDevice, st));
     ingest(P.pl, L, d_gdesc, d_rowinst);
-    // tile height: tuned for the batched solvers (the element layout does
-    // not depend on it, only the workgroup tiling)
-    const int rpw = (o.solver == CUPHU_SOLVER_PLAIN || !o.autotune) ? rows_per_wave()
-                                                                     : autotune_rpw<T>(L, P.pl, o.verbose, res, st);
-    retile(L, 4 * rpw);
+    // tile shape (the element layout does not depend on it, only the
+    // workgroup tiling)
+    const TileShape sh = (o.solver == CUPHU_SOLVER_PLAIN || !o.autotune) ? default_shape(L, o)
+                                                                          : autotune_shape<T>(L, P.pl, o, res, st);
+    retile(L, sh);
 
     // groups of consecutive instances bounded by the working-set budget
     // bytes per pixel one PCG iteration touches: w2 pair + x, r, p, p2, q, dinv
@@ -811,14 +876,14 @@ static void run_batch(const std::vector<std::pair<int, int>> &dims, const CuPhuL
     if (o.solver == CUPHU_SOLVER_PLAIN) {
         for (int i = 0; i < ninst; ++i) {
             upload_group(i, i + 1);
-            solve_instance_plain<T>(P.pl, g, L.desc[(size_t)i], o, rpw, res, st, sp(i));
+            solve_instance_plain<T>(P.pl, g, L.desc[(size_t)i], o, sh.rpw, res, st, sp(i));
         }
     } else {
         for (auto &gr : groups) {
             upload_group(gr.first, gr.second);
             std::vector<CuPhuSolveStats *> gs;
             for (int i = gr.first; i < gr.second; ++i) gs.push_back(sp(i));
-            solve_group<T>(P.pl, g, o, rpw, false, st, gs, hs);
+            solve_group<T>(P.pl, g, o, sh, false, st, gs, hs);
         }
     }
     launch_final<T>(d_gdesc, d_rowinst, L.total_rows, P.pl, P.out, L.max_pitch, st);
@@ -1119,8 +1184,9 @@ static void wls_impl(const T *d_w, const float *d_phase, const T *d_u, const T *
     o.max_iter = max_iter;
     o.tol = tol;
     o.solver = CUPHU_SOLVER_FUSED;
-    const int rpw = rows_per_wave(), th = 4 * rpw;
-    Layout L = make_layout({{nrow, ncol}}, th);
+    Layout L0 = make_layout({{nrow, ncol}}, TileShape{rows_per_wave(), 64});
+    const TileShape sh = default_shape(L0, o);
+    Layout L = make_layout({{nrow, ncol}}, sh);
     PoolSet<T> P = alloc_pools<T>(res, L.total_el);
     InstDesc d = L.desc[0];
     unsigned gsz = (unsigned)ceil_div((size_t)d.nrow * d.pitch, 256);
@@ -1141,7 +1207,7 @@ static void wls_impl(const T *d_w, const float *d_phase, const T *d_u, const T *
     std::vector<int> blk((size_t)nb, 0);
     CUPHU_HIP_CHECK(hipMemcpyAsync(g.desc, &gd, sizeof(InstDesc), hipMemcpyHostToDevice, st));
     CUPHU_HIP_CHECK(hipMemcpyAsync(g.blk, blk.data(), (size_t)nb * sizeof(int), hipMemcpyHostToDevice, st));
-    launch_assemble<T>(g.desc, g.blk, g.nblk, P.pl, rpw, st);
+    launch_assemble<T>(g.desc, g.blk, g.nblk, P.pl, sh, st);
     if (d_gu) {
         double m = device_mean<T>(d_gu, nrow, ncol, ncol, res, st);
         hipLaunchKernelGGL(k_wls_rhs<T>, dim3(gsz), dim3(256), 0, st, d_gu, m, gd, P.pl);
@@ -1150,7 +1216,7 @@ static void wls_impl(const T *d_w, const float *d_phase, const T *d_u, const T *
     std::vector<SolveScal> h;
     CuPhuSolveStats stt{};
     std::vector<CuPhuSolveStats *> sp = {&stt};
-    solve_group<T>(P.pl, g, o, rpw, true, st, sp, h);
+    solve_group<T>(P.pl, g, o, sh, true, st, sp, h);
     if (iters_out) *iters_out = stt.iters[0];
     if (relres_out) *relres_out = stt.relres[0];
     double mx = device_mean<T>(P.pl.x + d.off, nrow, ncol, d.pitch, res, st);
@@ -1938,8 +2004,8 @@ struct HookInst {
     GroupDev g{};
     InstDesc d{};
 };
-void hook_setup(HookInst &h, DevRes &res, int nrow, int ncol, int rpw, hipStream_t st) {
-    h.L = make_layout({{nrow, ncol}}, 4 * rpw);
+void hook_setup(HookInst &h, DevRes &res, int nrow, int ncol, TileShape sh, hipStream_t st) {
+    h.L = make_layout({{nrow, ncol}}, sh);
     h.P = alloc_pools<float>(res, h.L.total_el);
     h.d = h.L.desc[0];
     int nb = h.d.tiles_x * h.d.tiles_y;
@@ -1993,7 +2059,7 @@ int cuphu_test_ingest(const float *igram_c64, const float *corr, const unsigned 
         srcs[0].mask = mask ? dm.get() : nullptr;
         srcs[0].src_stride = ncol;
         srcs[0].cc = make_cost_consts(*params, params->ncorrlooks);
-        Layout L = make_layout({{nrow, ncol}}, 16);
+        Layout L = make_layout({{nrow, ncol}}, TileShape{4, 64});
         PoolSet<float> P = alloc_pools<float>(res, L.total_el);
         InstDesc *d_desc = ws<InstDesc>(res, "gdesc", 1);
         int *d_rowinst = ws<int>(res, "rowinst", (size_t)nrow);
@@ -2025,9 +2091,11 @@ int cuphu_test_laplace_ops(const float *phase, const short *sigsq, const float *
         CUPHU_REQUIRE(phase && sigsq && x && nrow >= 1 && ncol >= 1, "null argument");
         DevRes &res = devres();
         hipStream_t st = res.stream;
-        const int rpw = rows_per_wave();
+        int lpr = 64;
+        env_int("CUPHU_LAPLACE_LPR", &lpr);   // 32: the fused solver's 128-column tiles
+        const TileShape sh{rows_per_wave(), (lpr == 32 && rows_per_wave() <= 4) ? 32 : 64};
         HookInst h;
-        hook_setup(h, res, nrow, ncol, rpw, st);
+        hook_setup(h, res, nrow, ncol, sh, st);
         const InstDesc &d = h.d;
         size_t na = (size_t)(nrow - 1) * ncol + (size_t)nrow * (ncol - 1);
         DevBuf<short> dsig(na + 1);
@@ -2039,12 +2107,12 @@ int cuphu_test_laplace_ops(const float *phase, const short *sigsq, const float *
                            (const float *)dph.get(), d, h.P.pl.phi, h.P.pl.corr, h.P.pl.mask, h.P.pl.sigsq);
         CUPHU_HIP_CHECK(hipGetLastError());
         to_plane(x, h.P.pl.x, d, st);
-        launch_weights<float>(h.g.desc, h.g.blk, h.g.nblk, h.P.pl, 0, 0.0f, rpw, st);
+        launch_weights<float>(h.g.desc, h.g.blk, h.g.nblk, h.P.pl, 0, 0.0f, sh, st);
         if (w0 && na) {
             hipLaunchKernelGGL(k_planes_to_flat_w<float>, dim3(ga), dim3(256), 0, st, h.P.pl.w2, d, dflat.get());
             CUPHU_HIP_CHECK(hipMemcpyAsync(w0, dflat.get(), na * 4, hipMemcpyDeviceToHost, st));
         }
-        launch_assemble<float>(h.g.desc, h.g.blk, h.g.nblk, h.P.pl, rpw, st);
+        launch_assemble<float>(h.g.desc, h.g.blk, h.g.nblk, h.P.pl, sh, st);
         if (b) from_plane(h.P.pl.b, b, d, st);
         if (dinv) from_plane(h.P.pl.dinv, dinv, d, st);
         if (r) from_plane(h.P.pl.r, r, d, st);
@@ -2053,7 +2121,7 @@ int cuphu_test_laplace_ops(const float *phase, const short *sigsq, const float *
             from_plane(h.P.pl.wv, lx, d, st);
         }
         if (wrw && na) {
-            launch_weights<float>(h.g.desc, h.g.blk, h.g.nblk, h.P.pl, 1, delta * delta, rpw, st);
+            launch_weights<float>(h.g.desc, h.g.blk, h.g.nblk, h.P.pl, 1, delta * delta, sh, st);
             hipLaunchKernelGGL(k_planes_to_flat_w<float>, dim3(ga), dim3(256), 0, st, h.P.pl.w2, d, dflat.get());
             CUPHU_HIP_CHECK(hipMemcpyAsync(wrw, dflat.get(), na * 4, hipMemcpyDeviceToHost, st));
         }
@@ -2080,7 +2148,7 @@ int cuphu_test_pcg_step(const float *w, const float *dinv, int nrow, int ncol, d
         DevRes &res = devres();
         hipStream_t st = res.stream;
         HookInst h;
-        hook_setup(h, res, nrow, ncol, rpw, st);
+        hook_setup(h, res, nrow, ncol, TileShape{rpw, 64}, st);
         const InstDesc &d = h.d;
         size_t na = (size_t)(nrow - 1) * ncol + (size_t)nrow * (ncol - 1);
         DevBuf<float> dw(na + 1);
@@ -2125,11 +2193,15 @@ int cuphu_test_pcg_fused_step(const float *w, const float *dinv, int nrow, int n
     return guarded([&]() {
         std::lock_guard<std::recursive_mutex> lk(g_mu);
         CUPHU_REQUIRE(w && dinv && x && r && p && q && sums && nrow >= 1 && ncol >= 1, "null argument");
-        CUPHU_REQUIRE(rpw == 2 || rpw == 4 || rpw == 8, "rpw must be 2, 4 or 8");
+        // rpw 2, 4, 8: rows per lane group on 256-column tiles; -2, -4: on
+        // 128-column tiles (q-free kernel only)
+        CUPHU_REQUIRE(rpw == 2 || rpw == 4 || rpw == 8 || (!stored_q && (rpw == -2 || rpw == -4)),
+                      "rpw must be 2, 4 or 8 (or -2, -4 without stored q)");
+        const TileShape sh = rpw > 0 ? TileShape{rpw, 64} : TileShape{-rpw, 32};
         DevRes &res = devres();
         hipStream_t st = res.stream;
         HookInst h;
-        hook_setup(h, res, nrow, ncol, rpw, st);
+        hook_setup(h, res, nrow, ncol, sh, st);
         const InstDesc &d = h.d;
         size_t na = (size_t)(nrow - 1) * ncol + (size_t)nrow * (ncol - 1);
         DevBuf<float> dw(na + 1);
@@ -2153,7 +2225,7 @@ int cuphu_test_pcg_fused_step(const float *w, const float *dinv, int nrow, int n
         CUPHU_HIP_CHECK(hipMemcpyAsync(h.g.scal, &sc, sizeof(sc), hipMemcpyHostToDevice, st));
         // parity 0: reads set 0 (r, p, s), writes set 1 (r2, p2, q2)
         if (stored_q) launch_it<float>(h.g.desc, h.g.blk, h.g.nblk, h.P.pl, h.g.scal, h.g.part4, rpw, 0, start ? 1 : 0, st);
-        else launch_iq<float>(h.g.desc, h.g.blk, h.g.nblk, h.P.pl, h.g.scal, h.g.part4, rpw, 0, start ? 1 : 0, st);
+        else launch_iq<float>(h.g.desc, h.g.blk, h.g.nblk, h.P.pl, h.g.scal, h.g.part4, sh, 0, start ? 1 : 0, st);
         std::vector<double4> parts((size_t)h.g.nblk);
         CUPHU_HIP_CHECK(hipMemcpyAsync(parts.data(), h.g.part4, parts.size() * sizeof(double4), hipMemcpyDeviceToHost, st));
         from_plane(h.P.pl.x, x, d, st);

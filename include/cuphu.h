@@ -540,7 +540,9 @@ int cuphu_test_pcg_step(const float *w, const float *dinv, int nrow, int ncol,
  * place; sums[4] = p.q, r.z, q.(dinv r), q.(dinv q).  stored_q != 0 runs
  * the kernel that reads q from memory; 0 runs the default kernel, which
  * recomputes q = L p from p (the q passed in must then be L p) and leaves
- * q unchanged.  dinv must be the assemble kernel's 1/diag of w. */
+ * q unchanged.  rpw: rows per lane group, 2, 4 or 8 on 256-column tiles, or
+ * -2, -4 on the 128-column tiles of the default kernel.  dinv must be the
+ * assemble kernel's 1/diag of w. */
 int cuphu_test_pcg_fused_step(const float *w, const float *dinv, int nrow, int ncol,
                               double alpha, double beta, int start, const float *rt, int rpw,
                               int stored_q, float *x, float *r, float *p, float *q, double *sums);

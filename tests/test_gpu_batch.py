@@ -67,6 +67,29 @@ def test_batch_validation(gpu_ready):
     assert cuphu.unwrap_batch([], [], 1.0) == ([], [])
 
 
+@pytest.mark.parametrize("dims", [[(300, 333)], [(517, 1090)], [(40, 64), (200, 700), (129, 1030), (70, 1)]])
+def test_tile_widths_bit_identical(gpu_ready, monkeypatch, dims):
+    """256- and 128-column tiles (forced with CUPHU_LAPLACE_LPR) give the same
+    unwrapped phase, components and iteration counts: the per-pixel
+    arithmetic does not depend on the tiling, and the FP64 partial sums of
+    the two groupings round to the same FP32 step lengths."""
+    scenes = [synth.make_scene(nr, nc, nbumps=2, npatches=1, nlooks=10.0, seed=70 + i,
+                               patch=max(1, min(20, nr // 3, nc // 3)))
+              for i, (nr, nc) in enumerate(dims)]
+    opts = dict(n_irls=2, max_iter=300, tol=1e-6)
+    res = {}
+    for lpr in ("64", "32"):
+        monkeypatch.setenv("CUPHU_LAPLACE_LPR", lpr)
+        res[lpr] = cuphu.unwrap_batch([s.igram for s in scenes], [s.corr for s in scenes], 10.0, "defo",
+                                      laplace_options=opts, return_stats=True)
+    for i in range(len(scenes)):
+        np.testing.assert_array_equal(res["64"][0][i], res["32"][0][i])
+        np.testing.assert_array_equal(res["64"][1][i], res["32"][1][i])
+        a, b = res["64"][2][i], res["32"][2][i]
+        assert (a["status"], a["iters"], a["pass_status"]) == (b["status"], b["iters"], b["pass_status"])
+        np.testing.assert_allclose(a["relres"], b["relres"], rtol=1e-9)
+
+
 @pytest.mark.parametrize("solver", ["fused", "batched", "plain"])
 def test_degenerate_shapes(gpu_ready, solver):
     """1x1, single rows/columns and 2x2 instances, alone and in one batch

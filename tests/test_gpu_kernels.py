@@ -74,9 +74,12 @@ def test_build_costs_matches_oracle(gpu_ready, defo):
 
 
 @pytest.mark.parametrize("shape", SHAPES)
-@pytest.mark.parametrize("rpw", [2, 4, 8])
-def test_laplace_ops_bitexact(gpu_ready, monkeypatch, shape, rpw):
+@pytest.mark.parametrize("rpw,lpr", [(2, 64), (4, 64), (8, 64), (2, 32), (4, 32)])
+def test_laplace_ops_bitexact(gpu_ready, monkeypatch, shape, rpw, lpr):
+    """Weights, assemble (b, 1/diag, r = b - L x) on both tile widths
+    (lpr 64: 256-column tiles; 32: the fused solver's 128-column tiles)."""
     monkeypatch.setenv("CUPHU_LAPLACE_RPW", str(rpw))
+    monkeypatch.setenv("CUPHU_LAPLACE_LPR", str(lpr))
     rng = np.random.default_rng(shape[0] * 1000 + shape[1])
     nrow, ncol = shape
     na = G.narcs(nrow, ncol)
@@ -140,13 +143,16 @@ def test_pcg_step_kernels_bitexact(gpu_ready, shape, rpw, beta):
 
 
 @pytest.mark.parametrize("shape", SHAPES)
-@pytest.mark.parametrize("rpw", [2, 4, 8])
+@pytest.mark.parametrize("rpw", [2, 4, 8, -2, -4])
 @pytest.mark.parametrize("mode", ["start", "step", "step_beta0", "replace"])
 @pytest.mark.parametrize("stored_q", [0, 1])
 def test_pcg_fused_step_kernel_bitexact(gpu_ready, shape, rpw, mode, stored_q):
     """Fused iteration kernels against the oracle: x, r, p (and q when stored)
     bit-exact; the four sums to FP64 summation-order tolerance.  The q-free
-    kernel recomputes q = L p, so q is passed as the oracle's L p."""
+    kernel recomputes q = L p, so q is passed as the oracle's L p.  rpw < 0:
+    the q-free kernel on 128-column tiles of 8|rpw| rows."""
+    if rpw < 0 and stored_q:
+        pytest.skip("the stored-q kernel has 256-column tiles only")
     rng = np.random.default_rng(shape[0] + 5 * shape[1] + rpw)
     nrow, ncol = shape
     na = G.narcs(nrow, ncol)

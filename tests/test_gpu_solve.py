@@ -118,7 +118,7 @@ def test_graph_and_eager_launches_identical(gpu_ready, solver):
     assert s1[0]["iters"] == s2[0]["iters"] == s3[0]["iters"]
 
 
-def test_autotuned_tile_height_agrees_with_default(gpu_ready, capfd):
+def test_autotuned_tile_shape_agrees_with_default(gpu_ready, capfd):
     scenes = [synth.make_scene(300, 700, nbumps=2, npatches=1, nlooks=10.0, seed=31, patch=40),
               synth.make_scene(90, 200, nbumps=1, npatches=0, nlooks=5.0, seed=32)]
     opts = dict(n_irls=2, max_iter=200, tol=1e-6)
@@ -126,7 +126,11 @@ def test_autotuned_tile_height_agrees_with_default(gpu_ready, capfd):
     u1, c1, s1 = G.batch_solve(scenes, _capi.default_options(autotune=1, verbose=1, **opts))
     err = capfd.readouterr().err
     tuned = [ln for ln in err.splitlines() if ln.startswith("[cuphu-autotune]")]
-    assert len(tuned) == 3, err[-2000:]          # rows/wave 2, 4, 8 timed once
+    # 256-column tiles of 2, 4, 8 rows per lane group and 128-column tiles
+    # of 2, 4, each timed once
+    assert len(tuned) == 5, err[-2000:]
+    assert sorted(ln.split("rows/lane group ")[1].split(":")[0] for ln in tuned) == [
+        "2, lanes/row 32", "2, lanes/row 64", "4, lanes/row 32", "4, lanes/row 64", "8, lanes/row 64"]
     u2, c2, s2 = G.batch_solve(scenes, _capi.default_options(autotune=1, verbose=1, **opts))
     assert "[cuphu-autotune]" not in capfd.readouterr().err   # cached for the shape class
     for i, s in enumerate(scenes):
