@@ -81,8 +81,8 @@ def main() -> int:
                     help="instances of the 64-interferogram batch timed with the plain loop")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (tiles would be sharded across "
-              "GPUs by a per-rank batch; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: the library has no multi-GPU path (tiles would be sharded across "
+              "GPUs as one batch per rank). Run with --gpus 1.", file=sys.stderr)
         return 2
 
     import torch
