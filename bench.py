@@ -277,6 +277,22 @@ def main() -> int:
                       f"is not available (source absent) and is not timed",
         }
         log(f"cpu: {n * n / tc / 1e6:.4f} Mpx/s")
+        # the same oracle on one host thread, on a 256x256 crop
+        n1 = min(256, n)
+        O.set_threads(1)
+        ig1 = np.ascontiguousarray(scene.igram[:n1, :n1])
+        co1 = np.ascontiguousarray(scene.corr[:n1, :n1])
+        ph1 = O.wrapped_phase(ig1)
+        sig1 = O.costs(ph1, co1, scene.nlooks, defo=True)[:, 1]
+        t0 = time.perf_counter()
+        _, cst1 = O.pcg_irls_f64(sig1, ph1, n_irls=args.n_irls, tol=1e-6, max_iter=args.max_iter)
+        t1 = time.perf_counter() - t0
+        O.set_threads(ncore)
+        out["cpu_baseline"]["single_thread"] = {
+            "value": round(n1 * n1 / t1 / 1e6, 5), "unit": "Mpixel/s", "cores": 1,
+            "sample": f"the same oracle and schedule on the {n1}x{n1} top-left crop; iterations {cst1['iters']}; "
+                      f"{t1:.1f} s"}
+        log(f"cpu, one thread: {n1 * n1 / t1 / 1e6:.4f} Mpx/s")
 
     if not args.no_extra:
         out["python_api_scene8192"] = leg_python_api(scene, base_opts, log)
