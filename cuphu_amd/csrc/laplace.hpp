@@ -13,9 +13,13 @@ constexpr int kLdsC0 = 4;
 constexpr int kBlock = 256;
 
 // One instance (a tile or an interferogram) inside the pooled planes.
+// pitch is the row stride; wcols (a multiple of 64, >= ncol) the columns
+// the iteration kernel covers.  The fused solver rounds the stride up to a
+// multiple of 256 elements; the columns from wcols to pitch then hold zeros
+// that only the once-per-pass kernels touch.
 struct InstDesc {
     int nrow, ncol, pitch, tiles_x;
-    int tiles_y, block_begin, row_begin, pad1;
+    int tiles_y, block_begin, row_begin, wcols;
     long long off;   // element offset of pixel (0,0) in every pixel plane
 };
 

@@ -2,9 +2,11 @@
 // solve (reference: src/cxx/cuphu/cuphu_laplace.cu).
 //
 // Layout.  Every instance (a tile or an interferogram) lives in pooled
-// planes with a row pitch that is a multiple of 64 elements; the pad
-// columns and every missing arc hold zero, so a pad pixel is an isolated
-// node with r = u = 0 that drops out of every dot product.  Arc weights are
+// planes with a row pitch that is a multiple of 64 elements (256 on the
+// fused solver, whose iteration kernel covers only the first wcols, the
+// 64-rounded width); the pad columns and every missing arc hold zero, so a
+// pad pixel is an isolated node with r = u = 0 that drops out of every dot
+// product.  Arc weights are
 // stored per pixel as an interleaved {down, right} pair (the pixel's row arc
 // (r,c)->(r+1,c) and column arc (r,c)->(r,c+1)), so one 32-byte load gives
 // the two outgoing arcs of four pixels.
@@ -1129,7 +1131,7 @@ __global__ __launch_bounds__(256) void k_iq(const InstDesc *descs, const int *bl
     const int cl = lane % LPR, v = wave * Sh::G + lane / LPR;   // column group, lane group
     const InstDesc &d = t.d;
     const int c = t.tx0 + 4 * cl;
-    const bool cok = c < d.pitch;
+    const bool cok = c < d.wcols;
     const int lrow0 = v * RPW;               // tile-local first own row of this lane group
     const bool top = v == 0, bot = v == 4 * Sh::G - 1;   // lane groups next to ring rows -1 and TH
     auto grow = [&](int lr) { return t.ty0 + lr; };
@@ -1216,7 +1218,7 @@ __global__ __launch_bounds__(256) void k_iq(const InstDesc *descs, const int *bl
     if (wave == 1 || wave == 2) {
         const int col = (wave == 1) ? t.tx0 - 1 : t.tx0 + TW;
         const int col2 = (wave == 1) ? col - 1 : col + 1;
-        const bool colok = col >= 0 && col < d.pitch, col2ok = col2 >= 0 && col2 < d.pitch;
+        const bool colok = col >= 0 && col < d.wcols, col2ok = col2 >= 0 && col2 < d.wcols;
         const int lcol = (wave == 1) ? -1 : TW, lcol2 = (wave == 1) ? -2 : TW + 1;
         if (lane < TH + 2) {
             const int lr = lane - 1;

@@ -207,13 +207,22 @@ struct Layout {
     long long total_el = 0;
     int total_rows = 0, max_pitch = 64;
 };
-static Layout make_layout(const std::vector<std::pair<int, int>> &dims, TileShape sh) {
+// stride256: round each row stride up to a multiple of 256 elements (the
+// fused solver; the iteration kernel still covers only wcols columns).
+// Measured on k_iq, same pixels per launch, traced averages over 401
+// launches: 1843-column tiles (the reference tile formula at 8192^2) 491.7 ->
+// 460.0 us; 1856 476.4 -> 458.0; 1920 470.1 -> 447.0; 2600 428.1 -> 419.5;
+// 1080 462.7 -> 457.9; 700 447.2 -> 443.6.  Strides that are already
+// multiples of 256 are unchanged (moving them to a larger multiple was
+// slower).
+static Layout make_layout(const std::vector<std::pair<int, int>> &dims, TileShape sh, bool stride256 = false) {
     Layout L;
     for (auto &dm : dims) {
         InstDesc d{};
         d.nrow = dm.first;
         d.ncol = dm.second;
-        d.pitch = pitch_of(d.ncol);
+        d.wcols = pitch_of(d.ncol);
+        d.pitch = stride256 ? (d.wcols + 255) / 256 * 256 : d.wcols;
         d.tiles_x = (int)ceil_div((size_t)d.pitch, (size_t)sh.tw());
         d.tiles_y = (int)ceil_div((size_t)d.nrow, (size_t)sh.th());
         d.row_begin = L.total_rows;
@@ -437,6 +446,16 @@ static bool fused_stored_q() {
     return v;
 }
 
+// Row strides rounded up to 256 elements (make_layout) on the fused q-free
+// solver, whose iteration kernel skips the extra columns (the other solvers
+// would process them every iteration); CUPHU_LAPLACE_STRIDE=64 turns the
+// rounding off.
+static bool wide_stride(const CuPhuLaplaceOptions &o) {
+    int v = 0;
+    env_int("CUPHU_LAPLACE_STRIDE", &v);
+    return v != 64 && o.solver == CUPHU_SOLVER_FUSED && !fused_stored_q();
+}
+
 // Fused solver: one stencil kernel and one reduction per PCG iteration.
 template <class T>
 static void solve_group_fused(const Planes<T> &pl, const GroupDev &g, const CuPhuLaplaceOptions &o, TileShape sh,
@@ -648,13 +667,14 @@ struct BatchOut {
 };
 
 // ── tile shape ──────────────────────────────────────────────────────────────
-// Every tile is processed whole, so an instance occupies ceil(pitch / TW) * TW
-// x ceil(nrow / TH) * TH lanes' pixels whatever its size: a pitch 64 past a
-// multiple of 256 leaves 3/4 of its last 256-column tile idle.
+// Every tile is processed whole, so an instance occupies ceil(wcols / TW) * TW
+// x ceil(nrow / TH) * TH lanes' pixels of the iteration kernel whatever its
+// size: a width 64 past a multiple of 256 leaves 3/4 of its last 256-column
+// tile idle.
 static double tiled_px(const Layout &L, TileShape sh) {
     double n = 0.0;
     for (auto &d : L.desc)
-        n += (double)(ceil_div((size_t)d.pitch, (size_t)sh.tw()) * (size_t)sh.tw()) *
+        n += (double)(ceil_div((size_t)d.wcols, (size_t)sh.tw()) * (size_t)sh.tw()) *
              (double)(ceil_div((size_t)d.nrow, (size_t)sh.th()) * (size_t)sh.th());
     return n;
 }
@@ -803,9 +823,17 @@ template <class T, class Ingest>
 static void run_batch(const std::vector<std::pair<int, int>> &dims, const CuPhuLaplaceOptions &o, double minfrac,
                       long maxncomps, bool do_conncomp, DevRes &res, hipStream_t st, CuPhuSolveStats *stats,
                       BatchOut &out, Ingest &&ingest) {
-    Layout L = make_layout(dims, TileShape{rows_per_wave(), 64});
+    const bool stride256 = wide_stride(o);
+    Layout L = make_layout(dims, TileShape{rows_per_wave(), 64}, stride256);
     int ninst = (int)dims.size();
     PoolSet<T> P = alloc_pools<T>(res, L.total_el);
+    if (stride256) {
+        // the iteration kernel never writes the columns from wcols to pitch of
+        // the second plane set, which the residual monitor and the x finalize
+        // read: they must hold zeros, whatever a previous call left there
+        for (T *q : {P.pl.r2, P.pl.p2, P.pl.q2})
+            CUPHU_HIP_CHECK(hipMemsetAsync(q, 0, (size_t)L.total_el * sizeof(T), st));
+    }
     InstDesc *d_gdesc = ws<InstDesc>(res, "gdesc", (size_t)ninst);
     int *d_rowinst = ws<int>(res, "rowinst", (size_t)L.total_rows);
     std::vector<int> rowinst((size_t)L.total_rows);

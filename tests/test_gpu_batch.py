@@ -69,25 +69,32 @@ def test_batch_validation(gpu_ready):
 
 @pytest.mark.parametrize("dims", [[(300, 333)], [(517, 1090)], [(40, 64), (200, 700), (129, 1030), (70, 1)]])
 def test_tile_widths_bit_identical(gpu_ready, monkeypatch, dims):
-    """256- and 128-column tiles (forced with CUPHU_LAPLACE_LPR) give the same
-    unwrapped phase, components and iteration counts: the per-pixel
-    arithmetic does not depend on the tiling, and the FP64 partial sums of
-    the two groupings round to the same FP32 step lengths."""
+    """256- and 128-column tiles (forced with CUPHU_LAPLACE_LPR) and row
+    strides rounded to 64 or 256 elements (CUPHU_LAPLACE_STRIDE) give the
+    same unwrapped phase, components and iteration counts: the per-pixel
+    arithmetic depends on neither, the extra stride columns stay zero, and
+    the FP64 partial sums of the groupings round to the same FP32 step
+    lengths."""
     scenes = [synth.make_scene(nr, nc, nbumps=2, npatches=1, nlooks=10.0, seed=70 + i,
                                patch=max(1, min(20, nr // 3, nc // 3)))
               for i, (nr, nc) in enumerate(dims)]
     opts = dict(n_irls=2, max_iter=300, tol=1e-6)
     res = {}
-    for lpr in ("64", "32"):
+    # (lanes per tile row, row stride rounding): 256-column tiles on 64- and
+    # 256-element strides, 128-column tiles on 256-element strides
+    for lpr, stride in (("64", "64"), ("64", "256"), ("32", "256")):
         monkeypatch.setenv("CUPHU_LAPLACE_LPR", lpr)
-        res[lpr] = cuphu.unwrap_batch([s.igram for s in scenes], [s.corr for s in scenes], 10.0, "defo",
-                                      laplace_options=opts, return_stats=True)
-    for i in range(len(scenes)):
-        np.testing.assert_array_equal(res["64"][0][i], res["32"][0][i])
-        np.testing.assert_array_equal(res["64"][1][i], res["32"][1][i])
-        a, b = res["64"][2][i], res["32"][2][i]
-        assert (a["status"], a["iters"], a["pass_status"]) == (b["status"], b["iters"], b["pass_status"])
-        np.testing.assert_allclose(a["relres"], b["relres"], rtol=1e-9)
+        monkeypatch.setenv("CUPHU_LAPLACE_STRIDE", stride)
+        res[lpr, stride] = cuphu.unwrap_batch([s.igram for s in scenes], [s.corr for s in scenes], 10.0, "defo",
+                                              laplace_options=opts, return_stats=True)
+    ref = res["64", "64"]
+    for key in (("64", "256"), ("32", "256")):
+        for i in range(len(scenes)):
+            np.testing.assert_array_equal(ref[0][i], res[key][0][i], err_msg=str(key))
+            np.testing.assert_array_equal(ref[1][i], res[key][1][i], err_msg=str(key))
+            a, b = ref[2][i], res[key][2][i]
+            assert (a["status"], a["iters"], a["pass_status"]) == (b["status"], b["iters"], b["pass_status"])
+            np.testing.assert_allclose(a["relres"], b["relres"], rtol=1e-9)
 
 
 @pytest.mark.parametrize("solver", ["fused", "batched", "plain"])
