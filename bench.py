@@ -10,19 +10,20 @@ tiles in one batched launch sequence, tile stitching and conncomp.  Inputs
 are resident in HBM before the timed region; a step is one full unwrap.
 
 Reported beside the headline Mpixel/s:
-* roofline: the fused PCG iteration kernel k_iq (x, r, p updated, q = L p
-  recomputed from p and the next q = L p formed in one pass; 32 B per active
-  pixel) -- algorithmic bytes
-  over its duration, timed live with HIP events on the solver's stream
-  during the timed steps (with --solver batched: the direction+stencil
-  kernel k_pq, 28 B/px, with the update kernel k_xr beside it);
+* roofline: the fused PCG iteration kernel k_iq (r, p updated, q = L p
+  recomputed from p and the next q = L p formed in one pass, x updated on
+  every other launch; 30 B per active pixel on average) -- algorithmic
+  bytes over its duration, timed live with HIP events on the solver's
+  stream during the timed steps (with --solver batched: the
+  direction+stencil kernel k_pq, 28 B/px, with the update kernel k_xr
+  beside it);
 * plain: the reference launch sequence (per-tile loop, 7 launches and 2
   blocking scalar reads per iteration) on the same scene, and the ratio;
 * accuracy: 2-pi ambiguity agreement of the FP32 result with the FP64 mode
   of the same solver (same tiles and schedule), and unwrapped-phase error;
 * cpu_baseline: the FP64 CPU oracle (same algorithm and schedule,
   OpenMP over the host cores this process may use, at most 16) on a
-  1024x1024 crop of the scene.
+  1024x1024 crop of the scene, and on one thread on a 256x256 crop.
 
 Usage: python bench.py [--gpus 1] [--steps 2] [--warmup 1]
 """
