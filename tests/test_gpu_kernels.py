@@ -143,16 +143,14 @@ def test_pcg_step_kernels_bitexact(gpu_ready, shape, rpw, beta):
 
 
 @pytest.mark.parametrize("shape", SHAPES)
-@pytest.mark.parametrize("rpw", [2, 4, 8, -2, -4])
+@pytest.mark.parametrize("rpw,stored_q", [(2, 0), (4, 0), (8, 0), (-2, 0), (-4, 0), (2, 1), (4, 1), (8, 1)])
 @pytest.mark.parametrize("mode", ["start", "step", "step_beta0", "replace"])
-@pytest.mark.parametrize("stored_q", [0, 1])
 def test_pcg_fused_step_kernel_bitexact(gpu_ready, shape, rpw, mode, stored_q):
     """Fused iteration kernels against the oracle: x, r, p (and q when stored)
     bit-exact; the four sums to FP64 summation-order tolerance.  The q-free
     kernel recomputes q = L p, so q is passed as the oracle's L p.  rpw < 0:
-    the q-free kernel on 128-column tiles of 8|rpw| rows."""
-    if rpw < 0 and stored_q:
-        pytest.skip("the stored-q kernel has 256-column tiles only")
+    the q-free kernel on 128-column tiles of 8|rpw| rows (the stored-q
+    kernel has 256-column tiles only)."""
     rng = np.random.default_rng(shape[0] + 5 * shape[1] + rpw)
     nrow, ncol = shape
     na = G.narcs(nrow, ncol)
