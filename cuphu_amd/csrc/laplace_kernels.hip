@@ -124,8 +124,8 @@ __device__ __forceinline__ TileInfo get_tile(const InstDesc *descs, const int *b
 // is 4*LPR columns wide; a wave holds 64/LPR such lane groups, and lane group
 // v = wave * (64/LPR) + lane / LPR of the workgroup owns rows v*RPW ..
 // v*RPW + RPW - 1.  LPR 64 is the 256 x 4*RPW tile; LPR 32 the 128 x 8*RPW
-// tile, for instances whose pitch leaves most of a last 256-column tile idle
-// (same pixels per workgroup).
+// tile (same pixels per workgroup, 5 % fewer halo pixels), which the fused
+// solver takes on large launches that both shapes tile without idle lanes.
 template <int RPW, int LPR>
 struct Shape {
     static_assert(LPR == 64 || LPR == 32, "lanes per tile row");
@@ -1053,13 +1053,15 @@ __global__ __launch_bounds__(256) void k_rr_it(const InstDesc *descs, const int 
 // being written by the previous launch and read back: the stencil of the
 // previous launch and this recomputation see the same p_j and arcs and use
 // the same operation order, so q_j is bit-identical.  Per pixel this moves
-// 32 B (reads x, r, p and the arc pair; writes x, r, p) instead of 40.
+// 24 B (reads r, p and the arc pair; writes r, p), plus 12 B (x and the
+// previous p in, x out) on every other launch, which applies the deferred x
+// updates: 30 B on average instead of k_it's 40.
 //
-// LDS: lp0 holds p_j with a two-pixel halo (rows -2..TH+1, columns -2..257),
-// lp1 holds p_{j+1} with a one-pixel halo.  Phase 1 loads the own rows and
-// stages p_j; wave 0 loads the ring row above the tile (and p_j two rows
-// up), wave 3 the ring row below, waves 1 and 2 the ring columns left and
-// right.  Phase 2 forms q_j, r_{j+1}, z, p_{j+1} at the own pixels and on
+// LDS: lp0 holds p_j with a two-pixel halo (rows -2..TH+1, columns
+// -2..TW+1), lp1 holds p_{j+1} with a one-pixel halo.  Phase 1 loads the own
+// rows and stages p_j; the first lane group loads the ring row above the
+// tile (and p_j two rows up), the last lane group the ring row below, waves
+// 1 and 2 the ring columns left and right (one pixel per lane).  Phase 2 forms q_j, r_{j+1}, z, p_{j+1} at the own pixels and on
 // the ring (whose arcs come from the loads of phase 1, the ring's 1/diag
 // recomputed from its arcs like the own pixels') and stores x, r, p.
 // Phase 3 applies the stencil to p_{j+1} for the sums.  Plane sets: (r, p)
