@@ -2,8 +2,9 @@
 through the device-resident scene path at full size, checked by properties
 that do not depend on size: congruence with the wrapped phase, the anchor
 at pixel (0,0), label range, tile-stitch consistency in the overlaps, and
-bitwise repeatability.  (Full-size parity against the FP64 solve is
-bench.py's accuracy leg.)"""
+bitwise repeatability; and the reference tile formula's 1843-column tiles
+bit-identical on 64- and 256-element row strides.  (Full-size parity
+against the FP64 solve is bench.py's accuracy leg.)"""
 
 import ctypes as C
 
@@ -55,3 +56,36 @@ def test_scene8192_properties(gpu_ready):
                 assert np.ptp(blk) < 1e-3 or np.ptp(np.mod(blk + 0.5, 1.0)) < 1e-3
     assert lab.max() <= prm.maxncomps * 25
     assert (lab > 0).mean() > 0.9
+
+
+def test_scene8192_reference_tiling_strides_bit_identical(gpu_ready, monkeypatch):
+    """The reference tile formula at 8192^2 (5x5 tiles of 1843 columns, the
+    Python API's default tiling): row strides rounded to 256 elements (the
+    fused solver's default; the iteration kernel covers the 1856 work
+    columns of a 2048-element stride) give the same unwrapped phase and
+    labels, bit for bit, as strides rounded to 64."""
+    torch = G.torch_mod()
+    s = synth.config_scene("scene8192")
+    nrow, ncol = s.corr.shape
+    dig = torch.from_numpy(s.igram.view(np.float32).reshape(nrow, 2 * ncol)).to("cuda")
+    dcorr = torch.from_numpy(s.corr).to("cuda")
+    prm = _capi.default_params()
+    prm.nlooks = prm.ncorrlooks = s.nlooks
+    opts = _capi.default_options(n_irls=1, max_iter=60, tol=1e-6)
+    outs = {}
+    for stride in ("64", "256"):
+        monkeypatch.setenv("CUPHU_LAPLACE_STRIDE", stride)
+        du = torch.empty((nrow, ncol), dtype=torch.float32, device="cuda")
+        dl = torch.empty((nrow, ncol), dtype=torch.int32, device="cuda")
+        stats = (_capi.CuPhuSolveStats * 64)()
+        nt = C.c_int()
+        _capi.check(_capi.lib.cuphu_unwrap_scene_gpu(G.p(dig), G.p(dcorr), None, None, nrow, ncol, s.nlooks,
+                                                     _capi.COST["defo"], C.byref(prm), 0, 256, 5, 5, C.byref(opts),
+                                                     G.p(du), G.p(dl), stats, C.byref(nt), G.stream()), "scene")
+        torch.cuda.synchronize()
+        assert nt.value == 25
+        outs[stride] = (du.cpu().numpy(), dl.cpu().numpy(), [stats[i].as_dict()["iters"] for i in range(25)])
+    np.testing.assert_array_equal(outs["64"][0], outs["256"][0])
+    np.testing.assert_array_equal(outs["64"][1], outs["256"][1])
+    assert outs["64"][2] == outs["256"][2]
+    assert np.isfinite(outs["256"][0]).all()
