@@ -691,9 +691,9 @@ static size_t tiled_blocks(const Layout &L, TileShape sh) {
 }
 
 // Default shape: rows_per_wave() (CUPHU_LAPLACE_RPW) rows per lane group and
-// 256-column tiles, or, on the fused solver, 128-column tiles when they
-// leave exactly as many lanes idle and the launch spans at least 4 resident
-// rounds.  Measured on one GPU, alternating pairs:
+// 256-column tiles, or, on the fused solver, 128-column tiles when every
+// instance's width is a multiple of 256, both shapes leave exactly as many
+// lanes idle and the launch spans at least 4 resident rounds.  Measured on one GPU, alternating pairs:
 //   - 8192^2 scene in 2048-column tiles (no idle lanes either way): the
 //     128 x 32 tile reads 5 % fewer halo pixels; k_iq 0.4343 -> 0.4324 ms
 //     per launch, 24.86 -> 25.00 Mpixel/s;
@@ -710,6 +710,12 @@ static TileShape default_shape(const Layout &L, const CuPhuLaplaceOptions &o) {
     env_int("CUPHU_LAPLACE_LPR", &lpr);
     if (lpr == 64) return wide;
     if (lpr == 32) return narrow;
+    // widths that are multiples of 256 (so the row stride has no extra
+    // columns either: a 128-column tile that fits a 128-column instance
+    // exactly ran 355 us on a 128-element stride but 523 us on a 256-element
+    // one)
+    for (auto &d : L.desc)
+        if (d.wcols % 256 != 0) return wide;
     return tiled_px(L, narrow) == tiled_px(L, wide) && tiled_blocks(L, narrow) >= kNarrowMinBlocks ? narrow : wide;
 }
 
