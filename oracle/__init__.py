@@ -36,7 +36,7 @@ class OCostParams(C.Structure):
 
 class OLapOptions(C.Structure):
     _fields_ = [("n_irls", C.c_int), ("delta_max", C.c_float), ("delta_min", C.c_float),
-                ("max_iter", C.c_int), ("tol", C.c_double)]
+                ("max_iter", C.c_int), ("tol", C.c_double), ("continuous", C.c_int)]
 
 
 class OLapStats(C.Structure):
@@ -168,8 +168,9 @@ def congruence(phi, u):
     return u
 
 
-def _opts(n_irls=16, delta_max=0.5, delta_min=0.01, max_iter=1000, tol=1e-3) -> OLapOptions:
-    return OLapOptions(int(n_irls), float(delta_max), float(delta_min), int(max_iter), float(tol))
+def _opts(n_irls=16, delta_max=0.5, delta_min=0.01, max_iter=1000, tol=1e-3, continuous=False) -> OLapOptions:
+    """continuous=True skips the final congruence (anchored WLS solution)."""
+    return OLapOptions(int(n_irls), float(delta_max), float(delta_min), int(max_iter), float(tol), int(continuous))
 
 
 def pcg_irls_f32(sigsq, phi, *, u0=None, **opts):

@@ -14,13 +14,20 @@
  *   o_phase_diffs     closed forms; wrap range; reference ramp fixtures
  *   o_boxcar2d        scipy.ndimage.uniform_filter(mode='mirror')
  *   o_costs           hand-evaluated special cases (rho=0, masked sentinels,
- *                     short clamp, sigsqshortmin floor, defo rounding);
- *                     SNAPHU's own values are NOT available (parity vs SNAPHU
- *                     unpinned; DEF_* constants unverifiable here)
+ *                     short clamp, sigsqshortmin floor, defo rounding) and
+ *                     hand-evaluated sigsq with each rhopow term alone
+ *                     (2 cstd1, cstd2 ln n at n = e^k, cstd3 n) and with the
+ *                     documented defaults at n = 10; SNAPHU's own values are
+ *                     NOT available (parity vs SNAPHU unpinned; DEF_*
+ *                     constants unverifiable here)
  *   o_residues        vortex (+-1), conservative field (0), flow-curl identity
  *   o_assemble/matvec dense numpy Laplacian built from the arc list
  *   o_reweight        numpy closed form
- *   o_pcg_irls_*      dense numpy solve at small sizes; 256^2 ramp closed form
+ *   o_pcg_irls_*      dense float64 IRLS (every pass a dense solve, the delta
+ *                     schedule with a binding floor) on scenes with residues,
+ *                     both drivers, on the continuous solution and the
+ *                     ambiguities; CG iteration bound (<= npix) for the FP32
+ *                     driver; 256^2 ramp closed form through both drivers
  *   o_conncomp        scipy.sparse.csgraph.connected_components; reference
  *                     test_conncomp.py known answers
  *   o_integrate_phase closed form (zero flows, integer flows)
@@ -400,7 +407,7 @@ int o_pcg_irls_f32(const short *sigsq, size_t stride, const float *phi, int nrow
     }
     float off = phi[0] - u[0];
     if (off != 0.0f) for (size_t i = 0; i < n; ++i) u[i] = u[i] + off;
-    o_congruence(phi, u, n);
+    if (!o->continuous) o_congruence(phi, u, n);
     free(w); free(b); free(dinv); free(r); free(z); free(p); free(q);
     return 0;
 }
@@ -531,7 +538,7 @@ int o_pcg_irls_f64(const short *sigsq, size_t stride, const float *phi, int nrow
     double off = (double)phi[0] - u[0];
     for (size_t i = 0; i < n; ++i) {
         double v = u[i] + off, d = v - (double)phi[i];
-        u[i] = fma(O_TWOPI_D, rint(d / O_TWOPI_D), (double)phi[i]);
+        u[i] = o->continuous ? v : fma(O_TWOPI_D, rint(d / O_TWOPI_D), (double)phi[i]);
     }
     free(w); free(b); free(dinv); free(r); free(z); free(p); free(q);
     return 0;

@@ -18,8 +18,10 @@ typedef struct OCostParams {
     long sigsqshortmin;
 } OCostParams;
 
+/* continuous = 1 returns the anchored solution without the final
+ * congruence step, so tests can compare the continuous WLS solution. */
 typedef struct OLapOptions {
-    int n_irls; float delta_max, delta_min; int max_iter; double tol;
+    int n_irls; float delta_max, delta_min; int max_iter; double tol; int continuous;
 } OLapOptions;
 
 typedef struct OLapStats {

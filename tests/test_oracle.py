@@ -125,6 +125,47 @@ def test_cost_power_law_with_chosen_exponent(cstd1, rho, expect):
     assert (c[:, 1] == expect).all()
 
 
+@pytest.mark.parametrize("ncorrlooks,cstd2,cstd3,rho,expect", [
+    # cstd2 alone: rhopow = cstd2 * ln n.  n = e^2 -> rhopow 2 (log10 would
+    # give 0.87 -> 56, log2 2.89 -> 29).
+    (float(np.exp(2.0)), 1.0, 0.0, 0.5, 36),
+    # n = e^3, cstd2 = 1/3 -> rhopow 1: ((0.5)/6 + 0.05) * 400 = 53.3
+    (float(np.exp(3.0)), 1.0 / 3.0, 0.0, 0.5, 53),
+    # n = e, cstd2 = 2 -> rhopow 2 at rho 0.75: (0.0625/6 + 0.05) * 400 = 24.2
+    (float(np.exp(1.0)), 2.0, 0.0, 0.75, 24),
+    # cstd3 alone: rhopow = cstd3 * n.  n = 10, cstd3 = 0.1 -> 1 (cstd3 * ln n
+    # would give 0.23 -> 76)
+    (10.0, 0.0, 0.1, 0.5, 53),
+    # n = 20, cstd3 = 0.1 -> rhopow 2 at rho 0.5 -> 36
+    (20.0, 0.0, 0.1, 0.5, 36),
+    # both: n = e^2, cstd2 = 0.5 (-> 1) + cstd3 = 1/e^2 (-> 1) = 2
+    (float(np.exp(2.0)), 0.5, float(np.exp(-2.0)), 0.5, 36),
+])
+def test_cost_rhopow_log_and_linear_terms(ncorrlooks, cstd2, cstd3, rho, expect):
+    """rhopow = 2 cstd1 + cstd2 ln(n) + cstd3 n (cuphu_cost.cu:44) with cstd1
+    = 0 and the other two chosen so that rhopow is a whole number; each
+    expected sigsq is ((1 - rho)^rhopow / 6 + 0.05) * 200^2 / 100 evaluated
+    by hand.  The rho values stay above the threshold 1.2 (1.3/n + 0.14)."""
+    assert rho >= 1.2 * (1.3 / ncorrlooks + 0.14)
+    phi = _flat_phase(4, 5)
+    corr = np.full((4, 5), rho, np.float32)
+    c = O.costs(phi, corr, ncorrlooks, cstd1=0.0, cstd2=cstd2, cstd3=cstd3)
+    assert (c[:, 1] == expect).all()
+    cd = O.costs(phi, corr, ncorrlooks, cstd1=0.0, cstd2=cstd2, cstd3=cstd3, defo=True)
+    assert (cd[:, 1] == expect).all()
+
+
+def test_cost_default_constants_at_ten_looks():
+    """The documented defaults (cstd 0.4 / 0.35 / 0.06) at n = 10 give
+    rhopow = 0.8 + 0.35 ln 10 + 0.6 = 2.205904.  Evaluated by hand:
+    0.4^2.205904 = 0.132487 -> (0.132487 / 6 + 0.05) * 400 = 28.83 -> 28 at
+    rho = 0.6; 0.6^2.205904 = 0.324055 -> 41.60 -> 41 at rho = 0.4.  A
+    base-10 logarithm would give rhopow 1.75 and 33 / 47."""
+    phi = _flat_phase(3, 3)
+    assert (O.costs(phi, np.full((3, 3), 0.6, np.float32), 10.0)[:, 1] == 28).all()
+    assert (O.costs(phi, np.full((3, 3), 0.4, np.float32), 10.0)[:, 1] == 41).all()
+
+
 def test_cost_threshold_masked_clamp_floor():
     phi = _flat_phase(5, 6)
     nl = 10.0
@@ -276,21 +317,33 @@ def test_pcg_single_solve_matches_dense_solution():
     np.testing.assert_allclose(u, ref, atol=1e-6)
 
 
-def test_irls_schedule_matches_dense_irls():
-    s = synth.make_scene(12, 13, nbumps=3, npatches=0, nlooks=10.0, seed=12)
-    phi = O.wrapped_phase(s.igram).astype(np.float64)
-    sig = O.costs(phi.astype(np.float32), s.corr, 10.0)[:, 1]
-    n_irls, dmax, dmin = 3, 0.5, 0.1
-    u, _ = O.pcg_irls_f64(sig, phi.astype(np.float32), n_irls=n_irls, delta_max=dmax, delta_min=dmin,
-                          max_iter=5000, tol=1e-14)
-    t, h = arc_list(12, 13)
+def _noisy_scene(seed, nrow, ncol):
+    """A small scene with residues (a ramp plus 1.1 rad phase noise), so the
+    IRLS reweighting changes the solution: on a residue-free field every
+    arc's residual is zero and the weights only scale uniformly."""
+    rng = np.random.default_rng(seed)
+    r, c = np.mgrid[0:nrow, 0:ncol]
+    truth = 0.9 * c - 0.4 * r + rng.normal(0, 1.1, (nrow, ncol))
+    phi = np.mod(truth, TWOPI).astype(np.float32)
+    corr = rng.uniform(0.3, 0.95, (nrow, ncol)).astype(np.float32)
+    assert np.abs(O.residues(phi)).sum() >= 10
+    return phi, corr
+
+
+def _dense_irls(sig, phi, n_irls, dmax, dmin):
+    """The IRLS of cuphu_laplace.cu:413-522 as dense float64 solves: each
+    pass's weights come from the previous pass's exact solution with
+    delta_k = max(dmin, dmax 0.5^k).  Returns the anchored continuous
+    solution (u[0] = phi[0], cuphu_laplace.cu:534-539)."""
+    nrow, ncol = phi.shape
+    t, h = arc_list(nrow, ncol)
     g = wrap_pi(phi.ravel()[h] - phi.ravel()[t])
     w0 = np.where((sig <= 0) | (sig >= 32000), 0.0, 1.0 / sig.astype(np.float64))
     w = w0.copy()
-    x = np.zeros(12 * 13)
+    x = np.zeros(nrow * ncol)
     for outer in range(n_irls + 1):
-        L = dense_laplacian(w, 12, 13)
-        b = np.zeros(12 * 13)
+        L = dense_laplacian(w, nrow, ncol)
+        b = np.zeros(nrow * ncol)
         np.add.at(b, h, w * g)
         np.add.at(b, t, -w * g)
         x = np.linalg.solve(L, b)
@@ -298,19 +351,96 @@ def test_irls_schedule_matches_dense_irls():
             d = max(dmin, dmax * 0.5 ** outer)
             e = (x[h] - x[t]) - g
             w = w0 / np.sqrt(e * e + d * d)
-    ref = _anchor_congruence(x.reshape(12, 13), phi)
-    np.testing.assert_allclose(u, ref, atol=1e-5)
+    x = x.reshape(nrow, ncol)
+    return x + (phi[0, 0] - x[0, 0])
+
+
+IRLS_CASES = [                      # the delta_min floor binds in the first and last case
+    (12, (12, 13), 10.0, 4, 0.5, 0.1),
+    (21, (15, 11), 5.0, 4, 0.5, 0.05),
+    (33, (9, 16), 20.0, 3, 0.8, 0.3),
+]
+
+
+@pytest.mark.parametrize("seed,shape,nlooks,n_irls,dmax,dmin", IRLS_CASES)
+def test_irls_schedule_matches_dense_irls(seed, shape, nlooks, n_irls, dmax, dmin):
+    phi, corr = _noisy_scene(seed, *shape)
+    sig = O.costs(phi, corr, nlooks)[:, 1]
+    opts = dict(n_irls=n_irls, delta_max=dmax, delta_min=dmin, max_iter=5000, tol=1e-14)
+    u, _ = O.pcg_irls_f64(sig, phi, continuous=True, **opts)
+    ref = _dense_irls(sig, phi.astype(np.float64), n_irls, dmax, dmin)
+    np.testing.assert_allclose(u, ref, atol=1e-7)
+    uc, _ = O.pcg_irls_f64(sig, phi, **opts)
+    np.testing.assert_allclose(uc, _anchor_congruence(ref, phi.astype(np.float64)), atol=1e-7)
+
+
+def test_irls_reweighting_moves_the_solution():
+    """The IRLS passes change the continuous solution on these scenes by far
+    more than the tolerances above, so a wrong delta schedule (exponent,
+    floor) or a missing reweight is visible."""
+    phi, corr = _noisy_scene(12, 12, 13)
+    sig = O.costs(phi, corr, 10.0)[:, 1]
+    p64 = phi.astype(np.float64)
+    a = _dense_irls(sig, p64, 0, 0.5, 0.1)
+    b = _dense_irls(sig, p64, 3, 0.5, 0.1)
+    c = _dense_irls(sig, p64, 3, 0.5, 0.2)
+    d = _dense_irls(sig, p64, 3, 0.25, 0.1)
+    assert np.abs(a - b).max() > 0.05
+    assert np.abs(b - c).max() > 0.01 and np.abs(b - d).max() > 0.01
+
+
+@pytest.mark.parametrize("seed,shape,nlooks,n_irls,dmax,dmin", IRLS_CASES)
+def test_irls_f32_driver_matches_dense_irls(seed, shape, nlooks, n_irls, dmax, dmin):
+    """The FP32-vector driver (o_pcg_irls_f32, cuphu_laplace.cu:413-522)
+    against the dense float64 IRLS: FP32 vectors converge to ~1e-6
+    relative, so the continuous solution agrees to 1e-3 rad and, after
+    congruence, every 2 pi ambiguity is equal.  Each Jacobi-PCG solve
+    converges within as many iterations as there are pixels, as conjugate
+    gradients must: a wrong beta or step degrades it to a slower
+    iteration."""
+    phi, corr = _noisy_scene(seed, *shape)
+    n = shape[0] * shape[1]
+    sig = O.costs(phi, corr, nlooks)[:, 1]
+    opts = dict(n_irls=n_irls, delta_max=dmax, delta_min=dmin, max_iter=4000, tol=1e-6)
+    u, st = O.pcg_irls_f32(sig, phi, continuous=True, **opts)
+    assert st["npass"] == n_irls + 1
+    assert all(s_ == 0 for s_ in st["status"]), st
+    assert max(st["iters"]) <= n, st
+    phi64 = phi.astype(np.float64)
+    ref = _dense_irls(sig, phi64, n_irls, dmax, dmin)
+    assert np.abs(u - ref).max() < 1e-3
+    frac = (ref - phi64) / TWOPI
+    assert np.abs(np.abs(frac - np.floor(frac)) - 0.5).min() > 1e-2     # no ambiguity near a tie
+    uc, _ = O.pcg_irls_f32(sig, phi, **opts)
+    np.testing.assert_array_equal(np.rint((uc - phi64) / TWOPI), np.rint(frac))
+    np.testing.assert_array_equal(uc, _anchor_congruence_f32(u, phi))
+
+
+def _anchor_congruence_f32(u, phi):
+    """FP32 congruence of an anchored field, cuphu_laplace.cu:256-266."""
+    k = np.rint((u - phi) / np.float32(TWOPI)).astype(np.float32)
+    return (phi + np.float32(TWOPI) * k).astype(np.float32)
 
 
 def test_ramp_closed_form_both_precisions():
+    """Config 1's 256^2 ramp (uniform weights, conservative gradients): the
+    WLS solution is the ramp itself, unw = pi (x + y) + 6 pi, through both
+    the FP64 and the FP32-vector drivers."""
     s = synth.ramp_256()
     phi = O.wrapped_phase(s.igram)
     sig = O.costs(phi, s.corr, 1.0)[:, 1]
     assert (sig == 86).all()
+    amb_truth = np.rint((s.truth - phi) / TWOPI)
     u64, _ = O.pcg_irls_f64(sig, phi, n_irls=0, max_iter=3000, tol=1e-10)
     assert np.abs(u64 - s.truth).max() < 1e-5
-    amb = np.rint((u64 - phi) / TWOPI)
-    np.testing.assert_array_equal(amb, np.rint((s.truth - phi) / TWOPI))
+    np.testing.assert_array_equal(np.rint((u64 - phi) / TWOPI), amb_truth)
+    u32, st = O.pcg_irls_f32(sig, phi, n_irls=0, max_iter=3000, tol=1e-6)
+    assert st["status"][0] in (0, 1)
+    assert np.abs(u32 - s.truth).max() < 1e-4
+    np.testing.assert_array_equal(np.rint((u32 - phi) / TWOPI), amb_truth)
+    # with IRLS passes the ramp stays the fixed point (zero residual arcs)
+    u32i, _ = O.pcg_irls_f32(sig, phi, n_irls=2, max_iter=3000, tol=1e-6)
+    np.testing.assert_array_equal(np.rint((u32i - phi) / TWOPI), amb_truth)
 
 
 def test_all_masked_scene_skips_solve():
