@@ -119,13 +119,15 @@ def config_scene(name: str, scale: float = 1.0) -> Scene:
 
 
 def batch_instances(n: int = 64, seed: int = 4, scale: float = 1.0,
-                    rows: tuple[int, int] = (512, 3000), cols: tuple[int, int] = (512, 1800)) -> list[Scene]:
+                    rows: tuple[int, int] = (512, 3000), cols: tuple[int, int] = (512, 1800),
+                    first: int | None = None) -> list[Scene]:
     """n independent interferograms: nrow ~ U{rows}, ncol ~ U{cols},
     nlooks ~ U{1, 5, 10, 20} drawn from default_rng(seed); instance i is
-    generated with seed + 1 + i (K = 5 bumps, P = 1 patch)."""
+    generated with seed + 1 + i (K = 5 bumps, P = 1 patch).  ``first``
+    generates only the first instances of that same list."""
     rng = np.random.default_rng(seed)
     out = []
-    for i in range(n):
+    for i in range(n if first is None else min(n, first)):
         nr = int(rng.integers(rows[0], rows[1] + 1))
         nc = int(rng.integers(cols[0], cols[1] + 1))
         L = float(rng.choice([1, 5, 10, 20]))

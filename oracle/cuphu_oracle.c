@@ -467,6 +467,110 @@ static double dot_d(const double *a, const double *b, size_t n) {
     for (int t = 0; t < nt; ++t) s += part[t];
     return s;
 }
+/* The two sweeps of an FP64 iteration with their inner products fused in
+ * (q = L p with p.q; x, r, z updates with r.z), so each vector is streamed
+ * once per iteration.  Partial sums run over a static partition per thread
+ * and are added in thread order: deterministic for a given thread count. */
+static int omp_threads_capped(void) {
+    int nt = 1;
+#ifdef _OPENMP
+    nt = omp_get_max_threads();
+#endif
+    return nt > 256 ? 256 : nt;
+}
+/* One output row of the FP64 stencil: arcs in the order down, up, right,
+ * left (as matvec_d); the interior columns run branch-free so the compiler
+ * vectorizes them.  wd/xd (wu/xu) are NULL on the last (first) row. */
+#define O_MV_BODY(c, DN, UP)                                         \
+    {                                                                \
+        double xa = x[c], yv = O_EPS_D * xa;                         \
+        if (DN) yv = fma(wd[c], xa - xd[c], yv);                     \
+        if (UP) yv = fma(wu[c], xa - xu[c], yv);                     \
+        yv = fma(wc[c], xa - x[(c) + 1], yv);                        \
+        yv = fma(wc[(c) - 1], xa - x[(c) - 1], yv);                  \
+        y[c] = yv;                                                   \
+    }
+static void matvec_row_d(const double *restrict wd, const double *restrict wu, const double *restrict wc,
+                         const double *restrict x, const double *restrict xd, const double *restrict xu,
+                         int ncol, double *restrict y) {
+    int dn = wd != NULL, up = wu != NULL;
+    for (int c = 0; c < ncol; c += (ncol > 1 ? ncol - 1 : 1)) {   /* the two edge columns */
+        double xa = x[c], yv = O_EPS_D * xa;
+        if (dn) yv = fma(wd[c], xa - xd[c], yv);
+        if (up) yv = fma(wu[c], xa - xu[c], yv);
+        if (c < ncol - 1) yv = fma(wc[c], xa - x[c + 1], yv);
+        if (c > 0) yv = fma(wc[c - 1], xa - x[c - 1], yv);
+        y[c] = yv;
+    }
+    if (dn && up) { for (int c = 1; c < ncol - 1; ++c) O_MV_BODY(c, 1, 1) }
+    else if (dn)  { for (int c = 1; c < ncol - 1; ++c) O_MV_BODY(c, 1, 0) }
+    else if (up)  { for (int c = 1; c < ncol - 1; ++c) O_MV_BODY(c, 0, 1) }
+    else          { for (int c = 1; c < ncol - 1; ++c) O_MV_BODY(c, 0, 0) }
+}
+#undef O_MV_BODY
+/* a.b over n elements with four interleaved partial sums (a fixed order). */
+static double dot4_d(const double *restrict a, const double *restrict b, size_t n) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    size_t i = 0;
+    for (; i + 4 <= n; i += 4) {
+        s0 += a[i] * b[i]; s1 += a[i + 1] * b[i + 1]; s2 += a[i + 2] * b[i + 2]; s3 += a[i + 3] * b[i + 3];
+    }
+    for (; i < n; ++i) s0 += a[i] * b[i];
+    return (s0 + s1) + (s2 + s3);
+}
+static double matvec_dot_d(const double *w, const double *x, int nrow, int ncol, double *y) {
+    const double *row_w = w, *col_w = w + (size_t)(nrow - 1) * ncol;
+    int nt = omp_threads_capped();
+    double part[256] = {0};
+#pragma omp parallel num_threads(nt)
+    {
+        int t = 0, T = 1;
+#ifdef _OPENMP
+        t = omp_get_thread_num(); T = omp_get_num_threads();
+#endif
+        int rlo = (int)((long long)nrow * t / T), rhi = (int)((long long)nrow * (t + 1) / T);
+        double s = 0.0;
+        for (int r = rlo; r < rhi; ++r) {
+            size_t i = (size_t)r * ncol;
+            matvec_row_d(r < nrow - 1 ? row_w + i : NULL, r > 0 ? row_w + i - ncol : NULL,
+                         col_w + (size_t)r * (ncol - 1), x + i, r < nrow - 1 ? x + i + ncol : NULL,
+                         r > 0 ? x + i - ncol : NULL, ncol, y + i);
+            s += dot4_d(x + i, y + i, (size_t)ncol);
+        }
+        part[t] = s;
+    }
+    double s = 0.0;
+    for (int t = 0; t < nt; ++t) s += part[t];
+    return s;
+}
+static double update_xrz_d(double alpha, const double *restrict p, const double *restrict q,
+                           const double *restrict dinv, double *restrict u, double *restrict r,
+                           double *restrict z, size_t n) {
+    int nt = omp_threads_capped();
+    double part[256] = {0};
+#pragma omp parallel num_threads(nt)
+    {
+        int t = 0, T = 1;
+#ifdef _OPENMP
+        t = omp_get_thread_num(); T = omp_get_num_threads();
+#endif
+        size_t lo = n * (size_t)t / (size_t)T, hi = n * (size_t)(t + 1) / (size_t)T;
+        double s = 0.0;
+        for (size_t a = lo; a < hi; a += 4096) {          /* blocks stay in L1 for the dot */
+            size_t e = a + 4096 < hi ? a + 4096 : hi;
+            for (size_t i = a; i < e; ++i) {
+                u[i] = fma(alpha, p[i], u[i]);
+                r[i] = fma(-alpha, q[i], r[i]);
+                z[i] = r[i] * dinv[i];
+            }
+            s += dot4_d(r + a, z + a, e - a);
+        }
+        part[t] = s;
+    }
+    double s = 0.0;
+    for (int t = 0; t < nt; ++t) s += part[t];
+    return s;
+}
 int o_pcg_irls_f64(const short *sigsq, size_t stride, const float *phi, int nrow, int ncol,
                    const OLapOptions *o, const float *u0, double *u, OLapStats *st) {
     size_t n = (size_t)nrow * ncol, nrowarc = (size_t)(nrow - 1) * ncol,
@@ -496,15 +600,10 @@ int o_pcg_irls_f64(const short *sigsq, size_t stride, const float *phi, int nrow
         else {
             status = 1;
             for (; iter < o->max_iter; ++iter) {
-                matvec_d(w, p, nrow, ncol, q);
-                double pq = dot_d(p, q, n);
+                double pq = matvec_dot_d(w, p, nrow, ncol, q);
                 if (!(pq > 0.0)) { status = isfinite(pq) ? 2 : 4; break; }
                 double alpha = rz / pq;
-#pragma omp parallel for schedule(static)
-                for (long long i = 0; i < (long long)n; ++i) {
-                    u[i] = fma(alpha, p[i], u[i]); r[i] = fma(-alpha, q[i], r[i]); z[i] = r[i] * dinv[i];
-                }
-                double rzn = dot_d(r, z, n);
+                double rzn = update_xrz_d(alpha, p, q, dinv, u, r, z, n);
                 rzl = rzn;
                 if (rzn < o->tol * o->tol * rz0) { iter++; status = 0; break; }
                 double beta = rzn / rz;
