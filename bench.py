@@ -21,9 +21,17 @@ Reported beside the headline Mpixel/s:
   blocking scalar reads per iteration) on the same scene, and the ratio;
 * accuracy: 2-pi ambiguity agreement of the FP32 result with the FP64 mode
   of the same solver (same tiles and schedule), and unwrapped-phase error;
-* cpu_baseline: the FP64 CPU oracle (same algorithm and schedule,
-  OpenMP over the host cores this process may use, at most 16) on a
-  1024x1024 crop of the scene, and on one thread on a 256x256 crop.
+* convergence: per IRLS pass, the min / median / max over tiles of the
+  relative residual each solve ends with, and how many solves stopped at
+  the iteration cap (the workload is iteration-capped: at tol 1e-6 a
+  2048-px tile does not converge within the reference's 1000 iterations);
+* capped_vs_converged: on the 2048x2048 tile at the origin, the capped FP32
+  and FP64 results against the FP64 mode run to convergence (tol 1e-10,
+  cap 20000);
+* cpu_baseline: the FP64 CPU oracle (same algorithm and schedule, OpenMP
+  over every host core this process may use) on the 2048x2048 tile at the
+  origin, converted to the scene-pixel rate, and on one thread on a
+  256x256 crop.
 
 Usage: python bench.py [--gpus 1] [--steps 2] [--warmup 1]
 """
@@ -72,7 +80,9 @@ def main() -> int:
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=1024)
+    ap.add_argument("--no-capped", action="store_true", help="skip the capped-versus-converged leg")
+    ap.add_argument("--cpu-sample", type=int, default=2048,
+                    help="edge of the top-left square the CPU oracle solves (2048: one full tile)")
     ap.add_argument("--group-mb", type=int, default=0)
     # counter runs: fewer iterations (same launch shape), and a copy kernel of
     # known size to calibrate FETCH_SIZE / WRITE_SIZE
@@ -227,6 +237,7 @@ def main() -> int:
         },
         "pcg_iters_per_tile": iters[:3] + (["..."] if len(iters) > 3 else []),
         "pcg_iters_total": int(sum(sum(i) for i in iters)),
+        "convergence": convergence_summary(st, args.max_iter),
     }
 
     if not args.no_plain:
@@ -264,48 +275,11 @@ def main() -> int:
         log(f"agreement with FP64: {agree:.6f}")
         del dunw64
 
-    if not args.no_cpu:
-        import oracle as O
+    if not args.no_capped:
+        out["capped_vs_converged"] = leg_capped_vs_converged(torch, lib, _capi, scene, base_opts, log)
 
-        n = min(args.cpu_sample, nrow, ncol)
-        try:
-            ncore = len(os.sched_getaffinity(0))
-        except AttributeError:
-            ncore = os.cpu_count() or 1
-        ncore = max(1, min(ncore, 16))
-        O.set_threads(ncore)
-        ig = np.ascontiguousarray(scene.igram[:n, :n])
-        co = np.ascontiguousarray(scene.corr[:n, :n])
-        ph = O.wrapped_phase(ig)
-        sig = O.costs(ph, co, scene.nlooks, defo=True)[:, 1]
-        log(f"CPU oracle on {n}x{n} with {ncore} threads")
-        t0 = time.perf_counter()
-        _, cst = O.pcg_irls_f64(sig, ph, n_irls=args.n_irls, tol=1e-6, max_iter=args.max_iter)
-        tc = time.perf_counter() - t0
-        out["cpu_baseline"] = {
-            "value": round(n * n / tc / 1e6, 5), "unit": "Mpixel/s", "cores": ncore, "kind": "port",
-            "sample": f"FP64 C oracle (OpenMP) PCG/IRLS, same schedule ({args.n_irls + 1} solves, tol 1e-6, cap "
-                      f"{args.max_iter}), on the "
-                      f"{n}x{n} top-left crop of the scene; iterations {cst['iters']}; {tc:.1f} s; SNAPHU's CPU MCF "
-                      f"is not available (source absent) and is not timed",
-        }
-        log(f"cpu: {n * n / tc / 1e6:.4f} Mpx/s")
-        # the same oracle on one host thread, on a 256x256 crop
-        n1 = min(256, n)
-        O.set_threads(1)
-        ig1 = np.ascontiguousarray(scene.igram[:n1, :n1])
-        co1 = np.ascontiguousarray(scene.corr[:n1, :n1])
-        ph1 = O.wrapped_phase(ig1)
-        sig1 = O.costs(ph1, co1, scene.nlooks, defo=True)[:, 1]
-        t0 = time.perf_counter()
-        _, cst1 = O.pcg_irls_f64(sig1, ph1, n_irls=args.n_irls, tol=1e-6, max_iter=args.max_iter)
-        t1 = time.perf_counter() - t0
-        O.set_threads(ncore)
-        out["cpu_baseline"]["single_thread"] = {
-            "value": round(n1 * n1 / t1 / 1e6, 5), "unit": "Mpixel/s", "cores": 1,
-            "sample": f"the same oracle and schedule on the {n1}x{n1} top-left crop; iterations {cst1['iters']}; "
-                      f"{t1:.1f} s"}
-        log(f"cpu, one thread: {n1 * n1 / t1 / 1e6:.4f} Mpx/s")
+    if not args.no_cpu:
+        out["cpu_baseline"] = leg_cpu(scene, args, scene_tile_px(scene.corr.shape, tile, ovl, (ntr, ntc)), npx, log)
 
     if not args.no_extra:
         out["python_api_scene8192"] = leg_python_api(scene, base_opts, log)
@@ -315,6 +289,135 @@ def main() -> int:
 
     print(json.dumps(out), flush=True)
     return 0
+
+
+def convergence_summary(stats, max_iter):
+    """Per IRLS pass: min / median / max over tiles of the relative residual
+    sqrt(r.z / r0.z0) at the end of the solve, and how many solves stopped
+    at the iteration cap rather than at the tolerance."""
+    npass = max(s["npass"] for s in stats)
+    by_pass = []
+    for k in range(npass):
+        rr = np.array([s["relres"][k] for s in stats if s["npass"] > k])
+        by_pass.append({"pass": k, "min": float(rr.min()), "median": float(np.median(rr)), "max": float(rr.max())})
+    solves = sum(s["npass"] for s in stats)
+    capped = sum(1 for s in stats for x in s["pass_status"] if x == 1)
+    return {"relres_by_pass": by_pass, "solves": solves, "solves_stopped_at_cap": capped,
+            "note": f"relres = sqrt(r.z / r0.z0); a solve that reaches the cap ({max_iter} iterations, the "
+                    f"reference's default) stops there with this residual, as in the reference"}
+
+
+def leg_capped_vs_converged(torch, lib, _capi, scene, base_opts, log):
+    """The 2048x2048 tile at the scene's origin, solved alone with the
+    headline schedule (FP32, tol 1e-6, cap 1000) and with the FP64 mode run
+    to convergence (tol 1e-10, cap 20000).  The FP64 mode's converged result
+    equals the CPU FP64 oracle's (a GPU test checks this on a 512x512
+    crop, and the capped FP64 mode against the oracle on this tile)."""
+    n = min(2048, *scene.corr.shape)
+    sub = type(scene)(igram=np.ascontiguousarray(scene.igram[:n, :n]), corr=np.ascontiguousarray(scene.corr[:n, :n]),
+                      truth=scene.truth[:n, :n], nlooks=scene.nlooks, seed=scene.seed)
+    keep, insts = _upload(torch, [sub])
+    t_cap, st_cap = _solve(torch, lib, _capi, insts, _capi.default_options(**base_opts))
+    u_cap = insts[0][4].cpu().numpy()
+    _, ins64 = _upload(torch, [sub])
+    t_cap64, _ = _solve(torch, lib, _capi, ins64, _capi.default_options(**{**base_opts, "precision": 64}))
+    u_cap64 = ins64[0][4].cpu().numpy()
+    conv = {**base_opts, "precision": 64, "tol": 1e-10, "max_iter": 20000}
+    _, insc = _upload(torch, [sub])
+    t_conv, st_conv = _solve(torch, lib, _capi, insc, _capi.default_options(**conv))
+    u_conv = insc[0][4].cpu().numpy().astype(np.float64)
+    phi = np.mod(np.angle(sub.igram.astype(np.complex128)), 2 * np.pi)
+    k_conv = np.rint((u_conv - phi) / (2 * np.pi))
+
+    def cmp(u):
+        d = u.astype(np.float64) - u_conv
+        return {"ambiguity_agreement": float((np.rint((u - phi) / (2 * np.pi)) == k_conv).mean()),
+                "max_abs_rad": float(np.abs(d).max()), "rms_rad": float(np.sqrt(np.mean(d * d)))}
+
+    r = {"workload": f"{n}x{n} tile at the scene origin, solved alone",
+         "capped_fp32": {**cmp(u_cap), "relres": [float(x) for x in st_cap[0]["relres"]], "seconds": round(t_cap, 3)},
+         "capped_fp64": {**cmp(u_cap64), "seconds": round(t_cap64, 3)},
+         "converged_fp64": {"tol": 1e-10, "max_iter": 20000, "iters": st_conv[0]["iters"],
+                            "relres": [float(x) for x in st_conv[0]["relres"]],
+                            "all_converged": all(x == 0 for x in st_conv[0]["pass_status"]),
+                            "seconds": round(t_conv, 3)}}
+    log(f"capped vs converged on {n}^2: FP32 capped agrees on {r['capped_fp32']['ambiguity_agreement']:.6f} "
+        f"(converged in {st_conv[0]['iters']} iterations)")
+    del keep, ins64, insc
+    return r
+
+
+def leg_cpu(scene, args, tile_px_total, npx, log):
+    """The FP64 C oracle (same algorithm and schedule) on every host core
+    this process may use, on the top-left square of the scene (by default
+    2048x2048: one full tile of the fixed tiling), and on one thread on the
+    256x256 crop.  Every tile solve of the workload runs the same capped
+    iteration count, so CPU time scales with tile pixels: ``value`` is the
+    sample's rate converted to the headline's scene-pixel rate (scene pixels
+    / (sample seconds x tile pixels / sample pixels))."""
+    import oracle as O
+
+    n = min(args.cpu_sample, *scene.corr.shape)
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count() or 1
+    O.set_threads(affinity)
+    ig = np.ascontiguousarray(scene.igram[:n, :n])
+    co = np.ascontiguousarray(scene.corr[:n, :n])
+    ph = O.wrapped_phase(ig)
+    sig = O.costs(ph, co, scene.nlooks, defo=True)[:, 1]
+    log(f"CPU oracle on {n}x{n} with {affinity} threads (1 warm-up + median of 3)")
+    times = []
+    for rep in range(4):
+        t0 = time.perf_counter()
+        _, cst = O.pcg_irls_f64(sig, ph, n_irls=args.n_irls, tol=1e-6, max_iter=args.max_iter)
+        times.append(time.perf_counter() - t0)
+    tc = float(np.median(times[1:]))
+    total_tile_px = tile_px_total
+    scene_rate = npx / (tc * total_tile_px / (n * n)) / 1e6
+    r = {"value": round(scene_rate, 5), "unit": "Mpixel/s", "cores": affinity, "kind": "port",
+         "host_cpu_count": os.cpu_count(), "sched_affinity": affinity,
+         "sample": f"FP64 C oracle (OpenMP, {affinity} threads = every core in this process's affinity; "
+                   f"os.cpu_count() {os.cpu_count()}) running the same PCG/IRLS schedule ({args.n_irls + 1} solves, "
+                   f"tol 1e-6, cap {args.max_iter}) on the {n}x{n} top-left tile of the scene in {tc:.2f} s (median of 3 "
+                   f"after a warm-up; runs {[round(x, 2) for x in times]} s) "
+                   f"(iterations {cst['iters']}); value = scene pixels per second at that rate over the workload's "
+                   f"{total_tile_px / 1e6:.2f} M tile pixels. SNAPHU's CPU MCF is not available (source absent) and "
+                   f"is not timed",
+         "sample_mpix_s": round(n * n / tc / 1e6, 5), "sample_seconds": round(tc, 2)}
+    log(f"cpu: {n * n / tc / 1e6:.4f} Mpx/s on the sample, {scene_rate:.4f} Mpx/s scene-equivalent")
+    n1 = min(256, n)
+    O.set_threads(1)
+    ig1 = np.ascontiguousarray(scene.igram[:n1, :n1])
+    co1 = np.ascontiguousarray(scene.corr[:n1, :n1])
+    ph1 = O.wrapped_phase(ig1)
+    sig1 = O.costs(ph1, co1, scene.nlooks, defo=True)[:, 1]
+    t0 = time.perf_counter()
+    _, cst1 = O.pcg_irls_f64(sig1, ph1, n_irls=args.n_irls, tol=1e-6, max_iter=args.max_iter)
+    t1 = time.perf_counter() - t0
+    O.set_threads(affinity)
+    r["single_thread"] = {
+        "value": round(n1 * n1 / t1 / 1e6, 5), "unit": "Mpixel/s", "cores": 1,
+        "sample": f"the same oracle and schedule on one thread on the {n1}x{n1} top-left crop; iterations "
+                  f"{cst1['iters']}; {t1:.1f} s (rate of the crop itself)"}
+    log(f"cpu, one thread: {n1 * n1 / t1 / 1e6:.4f} Mpx/s")
+    return r
+
+
+def scene_tile_px(shape, tile, ovl, ntiles):
+    """Sum of tile pixels of the scene's tiling (the library's own tile
+    geometry: fixed ragged tiles of edge ``tile``, or the reference formula
+    with ``ntiles`` tiles per axis when tile is 0)."""
+    from cuphu_amd import _capi
+
+    tot = 1
+    for n, nt in zip(shape, ntiles):
+        first = (C.c_int * 4096)()
+        size = (C.c_int * 4096)()
+        k = _capi.check(_capi.lib.cuphu_tile_axis(n, tile, ovl, nt, first, size, 4096), "cuphu_tile_axis")
+        tot *= sum(size[i] for i in range(k))
+    return tot
 
 
 def _upload(torch, scenes):
@@ -469,7 +572,7 @@ def leg_single(torch, lib, _capi, synth, base_opts, log):
          "mpix_s": round(npx / t / 1e6, 3), "seconds": round(t, 3),
          "plain_mpix_s": round(npx / tp / 1e6, 3), "plain_seconds": round(tp, 3),
          "fused_vs_plain": round(tp / t, 3), "ambiguity_agreement_with_plain": _amb(torch, u_f, insts_p[0][4], s.igram),
-         "pcg_iters": st[0]["iters"]}
+         "pcg_iters": st[0]["iters"], "relres": [float(x) for x in st[0]["relres"]]}
     log(f"config2: fused {r['mpix_s']} Mpx/s, plain {r['plain_mpix_s']} Mpx/s")
     del keep
     return r
@@ -511,7 +614,8 @@ def leg_batch64(torch, lib, _capi, synth, base_opts, nsample, log):
          "batched_vs_plain": round((npx / t) / (spx / tp), 3),
          "ambiguity_agreement_with_plain": agree,
          "sample_ambiguity_agreement_with_fp64": {"batched": ag64, "plain": ag64p},
-         "pcg_iters_total": int(sum(sum(x["iters"]) for x in st))}
+         "pcg_iters_total": int(sum(sum(x["iters"]) for x in st)),
+         "convergence": convergence_summary(st, opts.max_iter)}
     log(f"config4: batched {r['mpix_s']} Mpx/s, plain sample {r['plain_sample_mpix_s']} Mpx/s")
     del keep
     return r
