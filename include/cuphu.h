@@ -11,8 +11,10 @@
  *     thread-local message of the last failure;
  *   - streams are passed as an opaque cuphu_stream_t (a hipStream_t), so the
  *     header needs no HIP include;
- *   - the network-flow initialisers (MCF/MST, SNAPHU TreeSolve) are not part
- *     of this library: requesting them returns CUPHU_ERR_UNSUPPORTED.
+ *   - the network-flow initialisers (MCF/MST, SNAPHU's CPU TreeSolve) are
+ *     not part of this library: init 'mcf'/'mst' runs the reference's GPU
+ *     stages around a host solver that the caller registers with
+ *     cuphu_set_flow_solver, and returns CUPHU_ERR_UNSUPPORTED without one.
  *
  * Array conventions (same as the reference): pixels are row-major
  * [r*ncol + c]; arc arrays are flat, row arcs first ((nrow-1)*ncol entries,
@@ -125,7 +127,7 @@ typedef struct CuPhuTileParams {
     int fix_cycle_spikes;
 } CuPhuTileParams;
 
-/* ── phase-bridging parameters (ref cuphu.h:142-162; accepted, not run) ───── */
+/* ── phase-bridging parameters (ref cuphu.h:142-162) ─────────────────────── */
 typedef enum {
     CUPHU_RAMP_NONE = 0,
     CUPHU_RAMP_LINEAR,
@@ -174,13 +176,20 @@ void cuphu_default_tile_params(CuPhuTileParams *tp);
 void cuphu_default_bridge_params(CuPhuBridgeParams *bp);
 
 /* Full pipeline on host arrays (ref cuphu.h:218-234, cuphu_solver.cpp:1674).
- * Only init_meth == CUPHU_INIT_LAPLACE is implemented; cost modes smooth,
- * defo and topo (topo = defo kernels, ref cuphu_solver.cpp:441-450) are all
- * accepted for Laplace (they give identical sigsq).  Multi-tile requests use
- * the reference tile formula (cuphu_solver.cpp:1759-1766) and stitching
- * (:1846-2031); all tiles are solved in one batched launch sequence.
- * bridge->enabled, single_tile_reoptimize, laplace_neighbor_feedback and
- * fix_cycle_spikes return CUPHU_ERR_UNSUPPORTED. */
+ * init_meth CUPHU_INIT_LAPLACE: cost modes smooth, defo and topo (topo =
+ * defo kernels, ref cuphu_solver.cpp:441-450) give identical sigsq.
+ * Multi-tile requests use the reference tile formula
+ * (cuphu_solver.cpp:1759-1766) and stitching (:1846-2031); all tiles are
+ * solved in one batched launch sequence.
+ * init_meth CUPHU_INIT_MCF / CUPHU_INIT_MST: the reference's GPU stages
+ * (costs, incremental costs, flow integration, gated conncomp) around the
+ * host solver registered with cuphu_set_flow_solver; CUPHU_ERR_UNSUPPORTED
+ * without one, and with more than one tile.
+ * Post-passes, as in the reference (:2037-2098): laplace_neighbor_feedback
+ * (Laplace) and fix_cycle_spikes run on multi-tile results (the single-tile
+ * path skips them, :1714-1723); bridge->enabled runs on every result.
+ * single_tile_reoptimize is a full-scene CPU TreeSolve: with more than one
+ * tile it returns CUPHU_ERR_UNSUPPORTED, on one tile it has nothing to do. */
 int cuphu_unwrap(const float *igram_r, const float *igram_i, const float *corr,
                  const float *mag, const unsigned char *mask,
                  int nrow, int ncol,
