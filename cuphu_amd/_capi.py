@@ -132,6 +132,7 @@ _SIGS = {
     "cuphu_build_defo_costs_gpu": (_i, [_P, _P, _P, _P, _i, _i, C.POINTER(CuPhuParams), _i, _i,
                                         C.POINTER(C.c_void_p), _P]),
     "cuphu_laplace_unwrap_gpu": (_i, [_P, _P, _i, _i, _d, _i, C.c_float, _i, _P, _P]),
+    "cuphu_laplace_unwrap_gpu_noalloc": (_i, [_P, _P, _i, _i, _d, _i, C.c_float, _i, _P, _P]),
     "cuphu_residues_gpu": (_i, [_P, _i, _i, _P, _P]),
     "cuphu_laplace_batch_gpu": (_i, [C.POINTER(CuPhuInstance), _i, _i, C.POINTER(CuPhuParams),
                                      C.POINTER(CuPhuLaplaceOptions), C.POINTER(CuPhuSolveStats), _P]),

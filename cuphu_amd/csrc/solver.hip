@@ -22,7 +22,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
+#include <string>
 #include <thread>
 #include <type_traits>
 
@@ -48,15 +50,49 @@ struct DevRes {
     hipEvent_t stage_ev[2] = {nullptr, nullptr};   // staged host copies
     std::map<std::string, DevBuf<unsigned char>> bufs;
     std::map<std::string, PinBuf<unsigned char>> pins;
+    // A fixed device region (the no-allocation entry point): while `arena` is
+    // set, get() hands out 256-byte aligned pieces of it in request order
+    // and never allocates.
+    unsigned char *arena = nullptr;
+    size_t arena_cap = 0, arena_used = 0;
+    std::map<std::string, std::pair<void *, size_t>> arena_bufs;
     void *get(const std::string &name, size_t bytes) {
+        if (arena) {
+            auto it = arena_bufs.find(name);
+            if (it != arena_bufs.end() && it->second.second >= bytes) return it->second.first;
+            size_t off = (arena_used + 255) & ~(size_t)255;
+            if (off > arena_cap || bytes > arena_cap - off)
+                throw Error(CUPHU_ERR_INVALID, "scratch region exhausted (" + std::to_string(off + bytes) + " of " +
+                                                   std::to_string(arena_cap) + " bytes)");
+            arena_used = off + bytes;
+            arena_bufs[name] = {arena + off, bytes};
+            return arena + off;
+        }
         DevBuf<unsigned char> &b = bufs[name];
         if (b.n < bytes) b.alloc(bytes);
         return b.p;
     }
     void *pin(const std::string &name, size_t bytes) {
+        if (arena) throw Error(CUPHU_ERR_INTERNAL, "pinned staging requested inside the no-allocation path");
         PinBuf<unsigned char> &b = pins[name];
         if (b.n < bytes) b.alloc(bytes);
         return b.p;
+    }
+};
+// Routes every workspace request of one call to a fixed region; restores
+// the allocating workspace when the call ends (normally or by exception).
+struct ArenaScope {
+    DevRes &r;
+    ArenaScope(DevRes &res, unsigned char *base, size_t cap) : r(res) {
+        r.arena = base;
+        r.arena_cap = cap;
+        r.arena_used = 0;
+        r.arena_bufs.clear();
+    }
+    ~ArenaScope() {
+        r.arena = nullptr;
+        r.arena_cap = r.arena_used = 0;
+        r.arena_bufs.clear();
     }
 };
 static DevRes &devres() {
@@ -1954,10 +1990,45 @@ int cuphu_conncomp_gpu(const float *d_unw, const float *d_corr, const unsigned c
     });
 }
 
-int cuphu_laplace_unwrap_gpu(const CuPhuSmoothCost *d_costs, const float *d_phase, int nrow, int ncol,
-                             double nshortcycle, int max_iter, float tol, int verbose, float *d_unw,
-                             cuphu_stream_t stream) {
-    (void)nshortcycle;   // unused by the reference as well (cuphu_laplace.cu:365)
+// ── the device Laplace unwrap, allocating or from the fixed scratch ──────────
+// Scratch of the no-allocation entry point: the solver's pooled planes and
+// its small per-group buffers for one instance of up to
+// CUPHU_NOALLOC_MAX_ELEMENTS row-pitched elements in FP32.  Every GPU the
+// library runs on holds its own copy until the process ends.
+constexpr size_t kNoAllocSmallBytes = (size_t)4 << 20;
+// Scratch bytes one instance needs (run_batch's requests): 17 pooled planes
+// of row-pitched elements (phi, corr, sigsq, out, lab 4 B, mask 1 B; w2 two
+// values and b, dinv, x, r, p, s, wv, p2, r2, q2 one value each: 21 + 12
+// sizeof(T) bytes per element, 69 in FP32), each padded by up to 512
+// bytes, plus the per-group buffers.  With at most
+// CUPHU_NOALLOC_MAX_ELEMENTS elements an instance has at most 16384 rows
+// and 11265 workgroup tiles (256 x 8 or 128 x 16 at the smallest), so the
+// per-group buffers (row map 4 B per row; 52 B per tile; the plain
+// solver's 8 KB of dot partials) stay far below kNoAllocSmallBytes.
+constexpr size_t noalloc_need(long long el, size_t tsize) {
+    return (size_t)el * (21 + 12 * tsize) + 17 * 512 + kNoAllocSmallBytes;
+}
+constexpr size_t kNoAllocBytes = noalloc_need(CUPHU_NOALLOC_MAX_ELEMENTS, 4);
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+static std::map<int, unsigned char *> g_scratch_dev;   // device -> address of its copy (under g_mu)
+
+// Device address of the current device's scratch (the array's name is a
+// host-side stand-in and must never reach a kernel or a copy).
+static unsigned char *scratch_address() {
+    int dev = 0;
+    CUPHU_HIP_CHECK(hipGetDevice(&dev));
+    auto it = g_scratch_dev.find(dev);
+    if (it != g_scratch_dev.end()) return it->second;
+    void *ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        throw Error(CUPHU_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_dev[dev] = (unsigned char *)ptr;
+    return (unsigned char *)ptr;
+}
+
+static int laplace_unwrap_dev(const CuPhuSmoothCost *d_costs, const float *d_phase, int nrow, int ncol, int max_iter,
+                              float tol, int verbose, float *d_unw, cuphu_stream_t stream, bool noalloc) {
     return guarded([&]() {
         std::lock_guard<std::recursive_mutex> lk(g_mu);
         CUPHU_REQUIRE(d_costs && d_phase && d_unw && nrow >= 1 && ncol >= 1, "invalid argument");
@@ -1968,8 +2039,29 @@ int cuphu_laplace_unwrap_gpu(const CuPhuSmoothCost *d_costs, const float *d_phas
         o.verbose = verbose;
         check_options(o);
         DevRes &res = devres();
+        unsigned char *base = nullptr;
+        if (noalloc) {
+            // eager launches (no graph objects to instantiate) and the default
+            // tile shape (the tuner creates events); both give the same bits
+            o.use_graph = 0;
+            o.autotune = 0;
+            int pitch = pitch_of(ncol);
+            if (wide_stride(o)) pitch = (pitch + 255) / 256 * 256;
+            long long el = (long long)nrow * pitch;
+            size_t need = noalloc_need(el, o.precision == 64 ? 8 : 4);
+            if (el > (long long)CUPHU_NOALLOC_MAX_ELEMENTS || need > kNoAllocBytes)
+                throw Error(CUPHU_ERR_INVALID,
+                            "instance of " + std::to_string(nrow) + " x " + std::to_string(ncol) + " needs " +
+                                std::to_string(el) + " row-pitched elements (" + std::to_string(need) +
+                                " scratch bytes); the no-allocation entry point holds " +
+                                std::to_string(CUPHU_NOALLOC_MAX_ELEMENTS) + " elements (" +
+                                std::to_string(kNoAllocBytes) + " bytes, FP32)");
+            base = scratch_address();
+        }
         StreamJoin join(res, (hipStream_t)stream);
         hipStream_t st = res.stream;
+        std::unique_ptr<ArenaScope> arena;
+        if (noalloc) arena.reset(new ArenaScope(res, base, kNoAllocBytes));
         auto ing = [&](auto &pl, const Layout &L, const InstDesc *, const int *) {
             const InstDesc &d = L.desc[0];
             unsigned gsz = (unsigned)ceil_div((size_t)d.nrow * d.pitch, 256);
@@ -1989,6 +2081,20 @@ int cuphu_laplace_unwrap_gpu(const CuPhuSmoothCost *d_costs, const float *d_phas
         CUPHU_HIP_CHECK(hipStreamSynchronize(st));
         return 0;
     });
+}
+
+int cuphu_laplace_unwrap_gpu(const CuPhuSmoothCost *d_costs, const float *d_phase, int nrow, int ncol,
+                             double nshortcycle, int max_iter, float tol, int verbose, float *d_unw,
+                             cuphu_stream_t stream) {
+    (void)nshortcycle;   // unused by the reference as well (cuphu_laplace.cu:365)
+    return laplace_unwrap_dev(d_costs, d_phase, nrow, ncol, max_iter, tol, verbose, d_unw, stream, false);
+}
+
+int cuphu_laplace_unwrap_gpu_noalloc(const CuPhuSmoothCost *d_costs, const float *d_phase, int nrow, int ncol,
+                                     double nshortcycle, int max_iter, float tol, int verbose, float *d_unw,
+                                     cuphu_stream_t stream) {
+    (void)nshortcycle;
+    return laplace_unwrap_dev(d_costs, d_phase, nrow, ncol, max_iter, tol, verbose, d_unw, stream, true);
 }
 
 int cuphu_wls_solve_gpu(const void *d_w, const void *d_phase, int nrow, int ncol, int precision, int max_iter,

@@ -292,6 +292,27 @@ int cuphu_laplace_unwrap_gpu(const CuPhuSmoothCost *d_smooth_costs,
  * Additions (no counterpart in the reference)
  * ════════════════════════════════════════════════════════════════════════════ */
 
+/* cuphu_laplace_unwrap_gpu without any device allocation.  No counterpart
+ * in the reference: it is for callers that cannot allocate (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, kernels and results as cuphu_laplace_unwrap_gpu; the
+ * solver's workspace is a fixed region inside the library (one per GPU,
+ * held until the process ends, CUPHU_NOALLOC_MAX_ELEMENTS * 69 bytes plus
+ * 4 MiB), and the launches are eager (no graph objects).  An instance may
+ * have at most CUPHU_NOALLOC_MAX_ELEMENTS row-pitched elements (nrow x
+ * ncol rounded up to 256 columns, FP32; e.g. 1024 x 1024); larger ones
+ * return CUPHU_ERR_INVALID before anything runs.  Calls on one GPU share
+ * the region, which keeps the last call's contents (the solver initialises
+ * everything it reads); calls that overlapped on one GPU would race on it.
+ * Within a process the library's lock serialises its calls and each call
+ * returns only after its GPU work has finished. */
+#define CUPHU_NOALLOC_MAX_ELEMENTS 1048576
+int cuphu_laplace_unwrap_gpu_noalloc(const CuPhuSmoothCost *d_smooth_costs,
+                                     const float *d_phase, int nrow, int ncol,
+                                     double nshortcycle, int max_iter, float tol,
+                                     int verbose, float *d_unw,
+                                     cuphu_stream_t stream);
+
 /* Thread-local message of the last failed call ("" if none). */
 const char *cuphu_last_error(void);
 int cuphu_gpu_count(void);
