@@ -104,6 +104,23 @@ def test_dilate_mask_grows_by_exact_distance():
     assert grown.sum() == 1 + 4 + 8 + 12
 
 
+@pytest.mark.parametrize("pixels", [1, 2, 5])
+def test_dilate_mask_matches_brute_force_manhattan_distance(pixels):
+    """Every pixel within Manhattan distance `pixels` of a valid pixel, by
+    brute force over all valid pixels, on random sparse masks (the edges
+    included) and on an empty one."""
+    rng = np.random.default_rng(pixels)
+    for m in (rng.random((17, 23)) > 0.93, np.zeros((6, 7), bool)):
+        ys, xs = np.nonzero(m)
+        rr, cc = np.mgrid[0:m.shape[0], 0:m.shape[1]]
+        if ys.size:
+            d = np.min(np.abs(rr[..., None] - ys) + np.abs(cc[..., None] - xs), axis=-1)
+            expect = d <= pixels
+        else:
+            expect = np.zeros_like(m)
+        np.testing.assert_array_equal(_dilate_mask(m, pixels), expect)
+
+
 def test_dilate_mask_zero_is_noop():
     m = np.random.default_rng(0).random((30, 30)) > 0.5
     np.testing.assert_array_equal(_dilate_mask(m, 0), m)
