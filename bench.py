@@ -73,10 +73,13 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="scene8192", choices=["scene8192", "scene2048"])
+    ap.add_argument("--workload", default="scene8192", choices=["scene8192", "scene2048", "scene1024", "batch64"],
+                    help="scene8192: the headline; scene2048 / scene1024: one interferogram; batch64: the 64 "
+                         "uneven interferograms in one launch sequence (kernel profiling of that workload)")
     ap.add_argument("--tiling", default="fixed", choices=["fixed", "reference"],
                     help="scene8192 tiles: fixed 2048/256 (the headline) or the reference tile formula "
                          "(target 2048, overlap 256: 5x5 tiles of 1843 columns, the Python API's default)")
+    ap.add_argument("--tile-size", type=int, default=2048, help="edge of the fixed tiles (scene8192, --tiling fixed)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
@@ -109,6 +112,8 @@ def main() -> int:
         return 1
     name = C.create_string_buffer(256)
     lib.cuphu_gpu_name(0, name, 256)
+    if args.workload == "batch64":
+        return main_batch64(args, torch, lib, _capi, synth, name.value.decode())
 
     ntr = ntc = 1
     if args.workload == "scene8192" and args.tiling == "reference":
@@ -117,7 +122,7 @@ def main() -> int:
         tile, ovl = 0, 256
         ntr, ntc = _auto_ntiles_by_target_size(8192, 8192, 2048, 256, 256)
     elif args.workload == "scene8192":
-        tile, ovl = 2048, 256
+        tile, ovl = args.tile_size, 256
     else:
         tile, ovl = 0, 0
     log(f"generating {args.workload}")
@@ -136,13 +141,7 @@ def main() -> int:
     base_opts = dict(n_irls=args.n_irls, tol=1e-6, max_iter=args.max_iter, rr_interval=50,
                      group_bytes_mb=args.group_mb, solver=args.solver)
     if args.calib_copy:
-        # 512 MiB read + 512 MiB written per launch (torch.neg: one elementwise pass)
-        src = torch.ones(128 << 20, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            torch.neg(src, out=dst)
-        torch.cuda.synchronize()
-        del src, dst
+        calib_copy(torch)
     torch.cuda.synchronize()
     stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
 
@@ -196,7 +195,8 @@ def main() -> int:
                                    "achieved": round(xr_gbs, 1) if xr_gbs else None,
                                    "frac": round(xr_gbs / HBM_PEAK_GBPS, 4) if xr_gbs else None}}
     # the PMC-measured traffic is for the fixed 2048-px tiling
-    traffic = round(k_bytes / bpp * tpp) if (tpp and args.workload == "scene8192" and args.tiling == "fixed") else None
+    traffic = round(k_bytes / bpp * tpp) if (tpp and args.workload == "scene8192" and args.tiling == "fixed"
+                                             and tile == 2048) else None
 
     out = {
         "metric": METRIC,
@@ -288,6 +288,59 @@ def main() -> int:
         out["config5_autograd1024"] = leg_autograd(torch, synth, log)
 
     print(json.dumps(out), flush=True)
+    return 0
+
+
+def calib_copy(torch):
+    """A copy of known size for the HBM counters: 512 MiB read + 512 MiB
+    written per launch (torch.neg: one elementwise pass), three launches."""
+    src = torch.ones(128 << 20, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        torch.neg(src, out=dst)
+    torch.cuda.synchronize()
+    del src, dst
+
+
+def main_batch64(args, torch, lib, _capi, synth, gpu):
+    """The 64 uneven interferograms (nrow U{512..3000}, ncol U{512..1800},
+    nlooks U{1,5,10,20}) solved in one launch sequence per step, with the
+    live k_iq timing; a profiling workload, reported as its own JSON line."""
+    log("generating batch64")
+    scenes = synth.batch_instances(64)
+    npx = sum(s.corr.size for s in scenes)
+    keep, insts = _upload(torch, scenes)
+    if args.calib_copy:
+        calib_copy(torch)
+    opts = _capi.default_options(n_irls=args.n_irls, tol=1e-6, max_iter=args.max_iter, rr_interval=50,
+                                 group_bytes_mb=args.group_mb, solver=args.solver)
+    for _ in range(args.warmup):
+        _solve(torch, lib, _capi, insts, opts)
+    lib.cuphu_set_profiling(1)
+    t_total = 0.0
+    for _ in range(args.steps):
+        t, st = _solve(torch, lib, _capi, insts, opts)
+        t_total += t
+    prof = _capi.CuPhuKernelProfile()
+    lib.cuphu_get_profile(C.byref(prof))
+    lib.cuphu_set_profiling(0)
+    achieved = (prof.it_bytes_sum / 1e9) / (prof.it_ms_sum / 1e3) if prof.it_ms_sum > 0 else None
+    out = {"metric": METRIC, "value": round(npx * args.steps / t_total / 1e6, 4), "unit": "Mpixel/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1000.0 * t_total / args.steps, 2),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic (seeded ramp+bumps+coherence generator, cuphu_amd/synth.py)",
+           "config": {"workload": f"batch64: 64 interferograms, {npx / 1e6:.2f} Mpx, defo, {args.n_irls} IRLS "
+                                  f"re-solves, tol 1e-6, max_iter {args.max_iter}, {args.solver} solver",
+                      "gpu": gpu},
+           "roofline": {"bound": "hbm", "kernel": "k_iq", "achieved": round(achieved, 1) if achieved else None,
+                        "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBPS, 4) if achieved else None, "traffic": None,
+                        "bytes_per_px": BYTES_PER_PX_IT,
+                        "avg_launch_ms": round(prof.it_ms_sum / max(prof.samples, 1), 4),
+                        "timed_launches": prof.samples},
+           "convergence": convergence_summary(st, args.max_iter)}
+    print(json.dumps(out), flush=True)
+    del keep
     return 0
 
 
