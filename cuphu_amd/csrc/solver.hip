@@ -766,7 +766,8 @@ static void retile(Layout &L, TileShape sh) {
 // The stencil kernels' tile shape trades halo re-reads and idle lanes against
 // registers and occupancy, and the best value depends on the instance
 // shapes.  On the first batched/fused solve of a shape class (precision,
-// widest pitch, instance count and pixel count classes) each candidate
+// widest pitch, stride rounding, instance count, stored and work pixel
+// count classes) each candidate
 // (256-column tiles of 8, 16 or 32 rows; on the fused solver also
 // 128-column tiles of 16 or 32 rows) is timed on the real pooled planes with
 // the fused kernel's steady-state step form before the solve starts (scratch
@@ -788,9 +789,21 @@ static TileShape autotune_shape(Layout L, const Planes<T> &pl, const CuPhuLaplac
     if (std::getenv("CUPHU_LAPLACE_RPW")) return default_shape(L, o);
     static std::map<std::string, TileShape> cache;
     const int ninst = (int)L.desc.size();
-    const bool narrow_ok = o.solver == CUPHU_SOLVER_FUSED && !fused_stored_q();
+    // 128-column tiles only where default_shape would consider them: every
+    // work width a multiple of 256 (a 128-column tile on a wider row stride
+    // ran 523 us against 355 us on its own stride)
+    bool widths256 = true;
+    long long work_px = 0;
+    for (auto &d : L.desc) {
+        widths256 = widths256 && d.wcols % 256 == 0;
+        work_px += (long long)d.nrow * d.wcols;
+    }
+    const bool narrow_ok = o.solver == CUPHU_SOLVER_FUSED && !fused_stored_q() && widths256;
+    // shape class: precision, widest stride, stride rounding, instance count,
+    // stored and work pixel counts (log2 classes) and the 128-column choice
     const std::string key = std::to_string(sizeof(T)) + "/" + std::to_string(L.max_pitch) + "/" +
-                            std::to_string(ilog2_ceil(ninst)) + "/" + std::to_string(ilog2_ceil(L.total_el)) + "/" +
+                            std::to_string((int)wide_stride(o)) + "/" + std::to_string(ilog2_ceil(ninst)) + "/" +
+                            std::to_string(ilog2_ceil(L.total_el)) + "/" + std::to_string(ilog2_ceil(work_px)) + "/" +
                             std::to_string((int)narrow_ok);
     auto it = cache.find(key);
     if (it != cache.end()) return it->second;
@@ -860,6 +873,21 @@ static TileShape autotune_shape(Layout L, const Planes<T> &pl, const CuPhuLaplac
     return best;
 }
 
+// Zeroes columns wcols..pitch-1 of two pixel planes: one thread per (row,
+// gap column) of the pooled rows, max_gap = the widest gap.
+template <class T>
+__global__ void k_zero_gaps(const InstDesc *descs, const int *rowinst, int nrows_total, int max_gap, T *a, T *b) {
+    long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (long long)nrows_total * max_gap) return;
+    const int row = (int)(i / max_gap), j = (int)(i % max_gap);
+    const InstDesc d = descs[rowinst[row]];
+    const int c = d.wcols + j;
+    if (c >= d.pitch) return;
+    const long long e = d.off + (long long)(row - d.row_begin) * d.pitch + c;
+    a[e] = T(0);
+    b[e] = T(0);
+}
+
 // ingest(planes, layout, d_desc, d_rowinst) fills phi/corr/mask/sigsq and x.
 template <class T, class Ingest>
 static void run_batch(const std::vector<std::pair<int, int>> &dims, const CuPhuLaplaceOptions &o, double minfrac,
@@ -869,13 +897,6 @@ static void run_batch(const std::vector<std::pair<int, int>> &dims, const CuPhuL
     Layout L = make_layout(dims, TileShape{rows_per_wave(), 64}, stride256);
     int ninst = (int)dims.size();
     PoolSet<T> P = alloc_pools<T>(res, L.total_el);
-    if (stride256) {
-        // the iteration kernel never writes the columns from wcols to pitch of
-        // the second plane set, which the residual monitor and the x finalize
-        // read: they must hold zeros, whatever a previous call left there
-        for (T *q : {P.pl.r2, P.pl.p2, P.pl.q2})
-            CUPHU_HIP_CHECK(hipMemsetAsync(q, 0, (size_t)L.total_el * sizeof(T), st));
-    }
     InstDesc *d_gdesc = ws<InstDesc>(res, "gdesc", (size_t)ninst);
     int *d_rowinst = ws<int>(res, "rowinst", (size_t)L.total_rows);
     std::vector<int> rowinst((size_t)L.total_rows);
@@ -883,6 +904,22 @@ static void run_batch(const std::vector<std::pair<int, int>> &dims, const CuPhuL
         for (int rr = 0; rr < L.desc[(size_t)i].nrow; ++rr) rowinst[(size_t)(L.desc[(size_t)i].row_begin + rr)] = i;
     CUPHU_HIP_CHECK(hipMemcpyAsync(d_gdesc, L.desc.data(), (size_t)ninst * sizeof(InstDesc), hipMemcpyHostToDevice, st));
     CUPHU_HIP_CHECK(hipMemcpyAsync(d_rowinst, rowinst.data(), rowinst.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    if (stride256) {
+        // The q-free iteration kernel covers columns below wcols only, so it
+        // never writes the stride gap (wcols..pitch-1) of the second plane
+        // set (r2, p2), which the residual monitor and the x finalize read
+        // over the whole pitch: the gap must hold zeros, whatever a previous
+        // call left there.  (Plane set 0 is rewritten over the whole pitch by
+        // the assemble kernel every pass; q2 is not used by this solver.)
+        int max_gap = 0;
+        for (auto &d : L.desc) max_gap = std::max(max_gap, d.pitch - d.wcols);
+        if (max_gap > 0) {
+            long long n = (long long)L.total_rows * max_gap;
+            hipLaunchKernelGGL(k_zero_gaps<T>, dim3((unsigned)ceil_div((size_t)n, 256)), dim3(256), 0, st, d_gdesc,
+                               d_rowinst, L.total_rows, max_gap, P.pl.r2, P.pl.p2);
+            CUPHU_HIP_CHECK(hipGetLastError());
+        }
+    }
     ingest(P.pl, L, d_gdesc, d_rowinst);
     // tile shape (the element layout does not depend on it, only the
     // workgroup tiling)
@@ -1004,53 +1041,22 @@ __global__ void k_copy_rows(const float *src, int spitch, float *dst, int nrow, 
     dst[i] = src[(i / ncol) * spitch + (i % ncol)];
 }
 
-template <class T>
-static void scene_impl(const float2 *d_igram, const float *d_corr, const unsigned char *d_mask,
-                       const unsigned char *d_conf, int nrow, int ncol, double nlooks, const CuPhuParams &prm,
-                       int tile_size, int rovl, int covl, int ntr_req, int ntc_req, const CuPhuLaplaceOptions &o,
-                       float *d_unw, uint32_t *d_cc, CuPhuSolveStats *stats, int *ntiles_out, DevRes &res,
-                       hipStream_t st) {
-    std::vector<int> rf(4096), rs(4096), cf(4096), cs(4096);
-    int ntr = tile_axis(nrow, tile_size, rovl, ntr_req, rf.data(), rs.data(), 4096);
-    int ntc = tile_axis(ncol, tile_size, covl, ntc_req, cf.data(), cs.data(), 4096);
-    CUPHU_REQUIRE(ntr > 0 && ntc > 0 && ntr <= 4096 && ntc <= 4096, "invalid tiling");
-    for (int t = 0; t < ntr; ++t) CUPHU_REQUIRE(rs[(size_t)t] >= 1, "tiling produced an empty tile (rows)");
-    for (int t = 0; t < ntc; ++t) CUPHU_REQUIRE(cs[(size_t)t] >= 1, "tiling produced an empty tile (cols)");
-    int nt = ntr * ntc;
-    if (ntiles_out) *ntiles_out = nt;
-    CostConsts cc = make_cost_consts(prm, nlooks);
-    std::vector<IngestSrc> srcs((size_t)nt);
-    std::vector<std::pair<int, int>> dims((size_t)nt);
-    for (int tr = 0; tr < ntr; ++tr)
-        for (int tc = 0; tc < ntc; ++tc) {
-            IngestSrc &s = srcs[(size_t)(tr * ntc + tc)];
-            size_t o0 = (size_t)rf[(size_t)tr] * ncol + cf[(size_t)tc];
-            std::memset(&s, 0, sizeof(s));
-            s.igram = d_igram + o0;
-            s.corr = d_corr + o0;
-            s.mask = d_mask ? d_mask + o0 : nullptr;
-            s.src_stride = ncol;
-            s.cc = cc;
-            dims[(size_t)(tr * ntc + tc)] = {rs[(size_t)tr], cs[(size_t)tc]};
-        }
-    BatchOut bo;
-    // merge_labels: components are labelled once on the stitched scene
-    const bool scene_cc = d_cc != nullptr && o.merge_labels != 0 && nt > 1;
-    run_batch<T>(dims, o, prm.minconncompfrac, prm.maxncomps, d_cc != nullptr && !scene_cc, res, st, stats, bo,
-                 src_ingest<T>(srcs, false, res, st));
-    if (nt == 1) {
-        const InstDesc &d = bo.lay.desc[0];
-        CUPHU_HIP_CHECK(hipMemcpy2DAsync(d_unw, (size_t)ncol * 4, bo.unw + d.off, (size_t)d.pitch * 4, (size_t)ncol * 4,
-                                         (size_t)nrow, hipMemcpyDeviceToDevice, st));
-        if (d_cc)
-            CUPHU_HIP_CHECK(hipMemcpy2DAsync(d_cc, (size_t)ncol * 4, bo.lab + d.off, (size_t)d.pitch * 4,
-                                             (size_t)ncol * 4, (size_t)nrow, hipMemcpyDeviceToDevice, st));
-        CUPHU_HIP_CHECK(hipStreamSynchronize(st));
-        return;
-    }
+// Registration, offsets and merge of solved tiles held in pooled planes
+// (bo: pitched unw / labels / coherence / solve-mask pools and the tile
+// layout, tiles in row-major order): each pair's coherence-weighted median
+// overlap difference on the GPU (cuphu_solver.cpp:1867-1959), the BFS over
+// the tile grid on the host, whole-cycle offsets when `integer`, overlap
+// halves trimmed and labels offset by the running component count
+// (:1969-2031).  d_conf: scene-level confidence mask (or NULL: the pooled
+// solve mask when pooled_mask, else none).  d_cc NULL skips the labels.
+static void stitch_tiles(const BatchOut &bo, int nrow, int ncol, int ntr, int ntc, const std::vector<int> &rf,
+                         const std::vector<int> &rs, const std::vector<int> &cf, const std::vector<int> &cs, int rovl,
+                         int covl, const unsigned char *d_conf, bool pooled_mask, bool integer, float *d_unw,
+                         uint32_t *d_cc, DevRes &res, hipStream_t st) {
     // pair registrations on the GPU (weighted medians of the overlap
     // differences), tile offsets by the BFS over the tile grid on the host
-    const unsigned char *conf = d_conf ? d_conf : (d_mask ? bo.mask : nullptr);
+    const int nt = ntr * ntc;
+    const unsigned char *conf = d_conf ? d_conf : (pooled_mask ? bo.mask : nullptr);
     std::vector<PairDesc> pairs;
     std::vector<std::pair<int, bool>> pair_of;
     int max_chunks = 0;
@@ -1112,7 +1118,7 @@ static void scene_impl(const float2 *d_igram, const float *d_corr, const unsigne
             // cycles: fractional for the Laplace path (:1928), whole cycles
             // when the congruent stitch is asked for
             double cyc = (double)sel[(size_t)i].value / (2.0 * M_PI);
-            if (o.stitch_integer) cyc = std::round(cyc);
+            if (integer) cyc = std::round(cyc);
             (pair_of[(size_t)i].second ? hk : vk)[(size_t)pair_of[(size_t)i].first] = cyc;
         }
     }
@@ -1157,9 +1163,57 @@ static void scene_impl(const float2 *d_igram, const float *d_corr, const unsigne
     CUPHU_HIP_CHECK(hipMemcpyAsync(d_coff, coff.data(), (size_t)nt * 4, hipMemcpyHostToDevice, st));
     long long npx = (long long)nrow * ncol;
     hipLaunchKernelGGL(k_merge, dim3((unsigned)ceil_div((size_t)npx, 256)), dim3(256), 0, st, bo.unw, bo.lab, d_toff,
-                       d_tpitch, d_rtab, d_rloc, d_ctab, d_cloc, ntc, d_add, d_coff, nrow, ncol, d_unw,
-                       scene_cc ? nullptr : d_cc);
+                       d_tpitch, d_rtab, d_rloc, d_ctab, d_cloc, ntc, d_add, d_coff, nrow, ncol, d_unw, d_cc);
     CUPHU_HIP_CHECK(hipGetLastError());
+}
+
+
+template <class T>
+static void scene_impl(const float2 *d_igram, const float *d_corr, const unsigned char *d_mask,
+                       const unsigned char *d_conf, int nrow, int ncol, double nlooks, const CuPhuParams &prm,
+                       int tile_size, int rovl, int covl, int ntr_req, int ntc_req, const CuPhuLaplaceOptions &o,
+                       float *d_unw, uint32_t *d_cc, CuPhuSolveStats *stats, int *ntiles_out, DevRes &res,
+                       hipStream_t st) {
+    std::vector<int> rf(4096), rs(4096), cf(4096), cs(4096);
+    int ntr = tile_axis(nrow, tile_size, rovl, ntr_req, rf.data(), rs.data(), 4096);
+    int ntc = tile_axis(ncol, tile_size, covl, ntc_req, cf.data(), cs.data(), 4096);
+    CUPHU_REQUIRE(ntr > 0 && ntc > 0 && ntr <= 4096 && ntc <= 4096, "invalid tiling");
+    for (int t = 0; t < ntr; ++t) CUPHU_REQUIRE(rs[(size_t)t] >= 1, "tiling produced an empty tile (rows)");
+    for (int t = 0; t < ntc; ++t) CUPHU_REQUIRE(cs[(size_t)t] >= 1, "tiling produced an empty tile (cols)");
+    int nt = ntr * ntc;
+    if (ntiles_out) *ntiles_out = nt;
+    CostConsts cc = make_cost_consts(prm, nlooks);
+    std::vector<IngestSrc> srcs((size_t)nt);
+    std::vector<std::pair<int, int>> dims((size_t)nt);
+    for (int tr = 0; tr < ntr; ++tr)
+        for (int tc = 0; tc < ntc; ++tc) {
+            IngestSrc &s = srcs[(size_t)(tr * ntc + tc)];
+            size_t o0 = (size_t)rf[(size_t)tr] * ncol + cf[(size_t)tc];
+            std::memset(&s, 0, sizeof(s));
+            s.igram = d_igram + o0;
+            s.corr = d_corr + o0;
+            s.mask = d_mask ? d_mask + o0 : nullptr;
+            s.src_stride = ncol;
+            s.cc = cc;
+            dims[(size_t)(tr * ntc + tc)] = {rs[(size_t)tr], cs[(size_t)tc]};
+        }
+    BatchOut bo;
+    // merge_labels: components are labelled once on the stitched scene
+    const bool scene_cc = d_cc != nullptr && o.merge_labels != 0 && nt > 1;
+    run_batch<T>(dims, o, prm.minconncompfrac, prm.maxncomps, d_cc != nullptr && !scene_cc, res, st, stats, bo,
+                 src_ingest<T>(srcs, false, res, st));
+    if (nt == 1) {
+        const InstDesc &d = bo.lay.desc[0];
+        CUPHU_HIP_CHECK(hipMemcpy2DAsync(d_unw, (size_t)ncol * 4, bo.unw + d.off, (size_t)d.pitch * 4, (size_t)ncol * 4,
+                                         (size_t)nrow, hipMemcpyDeviceToDevice, st));
+        if (d_cc)
+            CUPHU_HIP_CHECK(hipMemcpy2DAsync(d_cc, (size_t)ncol * 4, bo.lab + d.off, (size_t)d.pitch * 4,
+                                             (size_t)ncol * 4, (size_t)nrow, hipMemcpyDeviceToDevice, st));
+        CUPHU_HIP_CHECK(hipStreamSynchronize(st));
+        return;
+    }
+    stitch_tiles(bo, nrow, ncol, ntr, ntc, rf, rs, cf, cs, rovl, covl, d_conf, d_mask != nullptr,
+                 o.stitch_integer != 0, d_unw, scene_cc ? nullptr : d_cc, res, st);
     if (scene_cc)
         conncomp_run(d_unw, d_corr, d_mask, nrow, ncol, ncol, prm.minconncompfrac, prm.maxncomps, d_cc, st);
     CUPHU_HIP_CHECK(hipStreamSynchronize(st));
@@ -1665,9 +1719,9 @@ __global__ void k_mask_arc_weights(const unsigned char *mask, int nrow, int ncol
 // on the GPU the incremental costs at the solved flows (smooth costs), the
 // flow integration and the connected components gated by the incremental
 // costs.  Inputs and outputs are device arrays.
-static void mcf_tile(const float2 *d_ig, const float *d_corr, const unsigned char *d_mask, int nrow, int ncol,
-                     CuPhuCostMode cost_mode, CuPhuInitMethod init_meth, const CuPhuParams &prm, float *d_unw,
-                     uint32_t *d_cc, DevRes &res, hipStream_t st) {
+static int mcf_tile(const float2 *d_ig, const float *d_corr, const unsigned char *d_mask, int nrow, int ncol,
+                    CuPhuCostMode cost_mode, CuPhuInitMethod init_meth, const CuPhuParams &prm, float *d_unw,
+                    uint32_t *d_cc, DevRes &res, hipStream_t st) {
     size_t n = (size_t)nrow * ncol;
     size_t nra = (size_t)(nrow - 1) * ncol, na = nra + (size_t)nrow * (ncol - 1);
     int defo = cost_mode != CUPHU_COST_SMOOTH;
@@ -1701,8 +1755,62 @@ static void mcf_tile(const float2 *d_ig, const float *d_corr, const unsigned cha
         neg = d_neg.get();
     }
     util_integrate(d_ph.get(), d_flows.get(), d_flows.get() + nra, nrow, ncol, d_unw, st);
-    conncomp_run(d_unw, d_corr, d_mask, nrow, ncol, ncol, prm.minconncompfrac, prm.maxncomps, d_cc, st, 0.05f, pos, neg,
-                 prm.conncompthresh);
+    int ncomp = conncomp_run(d_unw, d_corr, d_mask, nrow, ncol, ncol, prm.minconncompfrac, prm.maxncomps, d_cc, st,
+                             0.05f, pos, neg, prm.conncompthresh);
+    CUPHU_HIP_CHECK(hipStreamSynchronize(st));
+    return ncomp;
+}
+
+// Tiled 'mcf'/'mst' (cuphu_solver.cpp:1753-1808 with solve_tile's network-
+// flow branch): the reference tile formula, each tile copied out of the
+// scene and solved through mcf_tile (costs -> registered solver ->
+// incremental costs -> integration -> gated conncomp, one tile after the
+// other: the host solver is the serial part), then stitched by whole 2 pi
+// cycles (:1928: network-flow tiles differ by exact integer cycles) with
+// the labels concatenated.
+static void mcf_scene(const float2 *d_ig, const float *d_corr, const unsigned char *d_mask,
+                      const unsigned char *d_conf, int nrow, int ncol, int rovl, int covl, int ntr_req, int ntc_req,
+                      CuPhuCostMode cost_mode, CuPhuInitMethod init_meth, const CuPhuParams &prm, float *d_unw,
+                      uint32_t *d_cc, DevRes &res, hipStream_t st) {
+    std::vector<int> rf(4096), rs(4096), cf(4096), cs(4096);
+    int ntr = tile_axis(nrow, 0, rovl, ntr_req, rf.data(), rs.data(), 4096);
+    int ntc = tile_axis(ncol, 0, covl, ntc_req, cf.data(), cs.data(), 4096);
+    CUPHU_REQUIRE(ntr > 0 && ntc > 0 && ntr <= 4096 && ntc <= 4096, "invalid tiling");
+    for (int t = 0; t < ntr; ++t) CUPHU_REQUIRE(rs[(size_t)t] >= 1, "tiling produced an empty tile (rows)");
+    for (int t = 0; t < ntc; ++t) CUPHU_REQUIRE(cs[(size_t)t] >= 1, "tiling produced an empty tile (cols)");
+    const int nt = ntr * ntc;
+    BatchOut bo;
+    for (int t = 0; t < nt; ++t) {
+        InstDesc d{};
+        d.nrow = rs[(size_t)(t / ntc)];
+        d.ncol = cs[(size_t)(t % ntc)];
+        d.pitch = d.wcols = d.ncol;
+        d.off = bo.lay.total_el;
+        bo.lay.total_el += (long long)d.nrow * d.ncol;
+        bo.lay.desc.push_back(d);
+    }
+    const size_t tot = (size_t)bo.lay.total_el;
+    float2 *ig = ws<float2>(res, "mcf_ig", tot);
+    bo.unw = ws<float>(res, "mcf_unw", tot);
+    bo.lab = ws<uint32_t>(res, "mcf_lab", tot);
+    bo.corr = ws<float>(res, "mcf_corr", tot);
+    bo.mask = d_mask ? ws<unsigned char>(res, "mcf_mask", tot) : nullptr;
+    bo.ncomp.assign((size_t)nt, 0);
+    for (int t = 0; t < nt; ++t) {
+        const InstDesc &d = bo.lay.desc[(size_t)t];
+        const size_t o0 = (size_t)rf[(size_t)(t / ntc)] * ncol + cf[(size_t)(t % ntc)];
+        CUPHU_HIP_CHECK(hipMemcpy2DAsync(ig + d.off, (size_t)d.ncol * 8, d_ig + o0, (size_t)ncol * 8, (size_t)d.ncol * 8,
+                                         (size_t)d.nrow, hipMemcpyDeviceToDevice, st));
+        CUPHU_HIP_CHECK(hipMemcpy2DAsync(bo.corr + d.off, (size_t)d.ncol * 4, d_corr + o0, (size_t)ncol * 4,
+                                         (size_t)d.ncol * 4, (size_t)d.nrow, hipMemcpyDeviceToDevice, st));
+        if (d_mask)
+            CUPHU_HIP_CHECK(hipMemcpy2DAsync(bo.mask + d.off, (size_t)d.ncol, d_mask + o0, (size_t)ncol, (size_t)d.ncol,
+                                             (size_t)d.nrow, hipMemcpyDeviceToDevice, st));
+        bo.ncomp[(size_t)t] = mcf_tile(ig + d.off, bo.corr + d.off, d_mask ? bo.mask + d.off : nullptr, d.nrow, d.ncol,
+                                       cost_mode, init_meth, prm, bo.unw + d.off, bo.lab + d.off, res, st);
+    }
+    stitch_tiles(bo, nrow, ncol, ntr, ntc, rf, rs, cf, cs, rovl, covl, d_conf, d_mask != nullptr, true, d_unw, d_cc,
+                 res, st);
     CUPHU_HIP_CHECK(hipStreamSynchronize(st));
 }
 
@@ -1737,8 +1845,6 @@ int cuphu_unwrap_ex(const float *igram_r, const float *igram_i, const float *igr
             throw Error(CUPHU_ERR_UNSUPPORTED,
                         "init 'mcf'/'mst' needs a network-flow solver (SNAPHU's CPU solver is not part of this "
                         "library); register one with cuphu_set_flow_solver");
-        if (mcf && tile->ntilerow * tile->ntilecol > 1)
-            throw Error(CUPHU_ERR_UNSUPPORTED, "init 'mcf'/'mst' runs on one tile here (ntiles = (1, 1))");
         CUPHU_REQUIRE(cost_mode >= CUPHU_COST_SMOOTH && cost_mode <= CUPHU_COST_TOPO, "invalid cost mode");
         if (tile->single_tile_reoptimize && tile->ntilerow * tile->ntilecol > 1)
             throw Error(CUPHU_ERR_UNSUPPORTED, "single_tile_reoptimize needs the CPU network-flow solver");
@@ -1783,9 +1889,13 @@ int cuphu_unwrap_ex(const float *igram_r, const float *igram_i, const float *igr
             const unsigned char *conf = orig_mask ? d_orig.get() : nullptr;
             int single = (tile->ntilerow == 1 && tile->ntilecol == 1);
             int rov = single ? 0 : tile->rowovrlp, cov = single ? 0 : tile->colovrlp;
-            if (mcf)
+            if (mcf && single)
                 mcf_tile(d_ig.get(), d_corr.get(), mask ? d_mask.get() : nullptr, nrow, ncol, cost_mode, init_meth,
                          *params, d_unw.get(), d_cc.get(), res, st);
+            else if (mcf)
+                mcf_scene(d_ig.get(), d_corr.get(), mask ? d_mask.get() : nullptr, conf, nrow, ncol, rov, cov,
+                          tile->ntilerow, tile->ntilecol, cost_mode, init_meth, *params, d_unw.get(), d_cc.get(), res,
+                          st);
             else if (o.precision == 64)
                 scene_impl<double>(d_ig.get(), d_corr.get(), mask ? d_mask.get() : nullptr, conf, nrow, ncol,
                                    params->ncorrlooks, *params, 0, rov, cov, tile->ntilerow, tile->ntilecol, o,

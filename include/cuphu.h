@@ -183,8 +183,9 @@ void cuphu_default_bridge_params(CuPhuBridgeParams *bp);
  * solved in one batched launch sequence.
  * init_meth CUPHU_INIT_MCF / CUPHU_INIT_MST: the reference's GPU stages
  * (costs, incremental costs, flow integration, gated conncomp) around the
- * host solver registered with cuphu_set_flow_solver; CUPHU_ERR_UNSUPPORTED
- * without one, and with more than one tile.
+ * host solver registered with cuphu_set_flow_solver, per tile, the tiles
+ * stitched by whole 2 pi cycles (cuphu_solver.cpp:1928) with their labels
+ * concatenated; CUPHU_ERR_UNSUPPORTED without a solver.
  * Post-passes, as in the reference (:2037-2098): laplace_neighbor_feedback
  * (Laplace) and fix_cycle_spikes run on multi-tile results (the single-tile
  * path skips them, :1714-1723); bridge->enabled runs on every result.
@@ -248,7 +249,7 @@ int cuphu_incrcost_gpu(const CuPhuSmoothCost *d_costs, const short *d_flows,
  * phase: wrapped phase [nrow*ncol]; init_method: CUPHU_INIT_MCF or
  * CUPHU_INIT_MST; flows_out: narcs shorts, same arc layout.  Return 0 on
  * success.  With a solver registered, cuphu_unwrap/cuphu_unwrap_ex with
- * init 'mcf'/'mst' (one tile) run the reference's GPU sequence around it:
+ * init 'mcf'/'mst' run the reference's GPU sequence around it on each tile:
  * costs -> solver -> incremental costs at the solved flows (smooth) ->
  * flow integration -> connected components gated by the incremental costs
  * (min(poscost, negcost) >= params->conncompthresh).  Without one they

@@ -127,14 +127,76 @@ def test_mcf_defo_cost_and_mst_reach_the_solver(gpu_ready, solver_slot):
     assert seen == [(4, "mst")]
 
 
-def test_mcf_without_solver_or_with_tiles_raises(gpu_ready, solver_slot):
+def test_mcf_without_solver_raises(gpu_ready, solver_slot):
     s = synth.make_scene(40, 50, nbumps=1, npatches=0, nlooks=10.0, seed=9)
     cuphu.set_flow_solver(None)
     with pytest.raises(ValueError, match="network-flow solver"):
         cuphu.unwrap(s.igram, s.corr, 10.0, init="mcf")
-    cuphu.set_flow_solver(lambda c, p, i: np.zeros(c.shape[0], np.int16))
-    with pytest.raises(ValueError, match="one tile"):
+    with pytest.raises(ValueError, match="network-flow solver"):
         cuphu.unwrap(s.igram, s.corr, 10.0, init="mcf", ntiles=(2, 2))
+
+
+def _tile_flows(phase, seed):
+    """A deterministic integer flow field for one tile (about 2 % of arcs
+    nonzero, in -2..2), from the tile's own shape and a seed."""
+    nrow, ncol = phase.shape
+    na = O.narcs(nrow, ncol)
+    rng = np.random.default_rng(seed * 100003 + nrow * 1009 + ncol)
+    return np.where(rng.uniform(size=na) < 0.02, rng.integers(-2, 3, na), 0).astype(np.int16)
+
+
+@pytest.mark.parametrize("ntiles,overlap,masked", [((2, 2), 24, False), ((2, 3), (20, 30), True), ((3, 1), 0, False)])
+def test_tiled_mcf_matches_oracle_per_tile_and_integer_stitch(gpu_ready, solver_slot, ntiles, overlap, masked):
+    """init='mcf' on a tiled scene (cuphu_solver.cpp:1753-1808, :1928): each
+    tile of the reference tile formula goes to the registered solver with
+    its own phase and costs; the result is the oracle's integration of each
+    tile's flows, components gated by that tile's incremental costs, tiles
+    stitched by whole cycles with the labels concatenated."""
+    s = synth.make_scene(150, 190, nbumps=2, npatches=1, nlooks=5.0, seed=12, patch=40)
+    nrow, ncol = s.corr.shape
+    mask = None
+    if masked:
+        mask = np.ones((nrow, ncol), np.uint8)
+        mask[60:75, 20:90] = 0
+    calls = []
+
+    def solver(costs, phase, init):
+        f = _tile_flows(phase, len(calls))
+        calls.append((costs.copy(), phase.copy(), f))
+        return f
+
+    cuphu.set_flow_solver(solver)
+    unw, cc = cuphu.unwrap(s.igram, s.corr, 5.0, cost="smooth", init="mcf", ntiles=ntiles, tile_overlap=overlap,
+                           mask=mask, reference_pixel=None, min_conncomp_frac=0.0)
+    rov, cov = (overlap, overlap) if np.ndim(overlap) == 0 else overlap
+    rf, rs = O.tile_axis(nrow, 0, rov, ntiles[0])
+    cf, cs = O.tile_axis(ncol, 0, cov, ntiles[1])
+    assert len(calls) == len(rf) * len(cf)
+    ph = O.wrapped_phase(s.igram, correctly_rounded=True)
+    prm = _capi.default_params()
+    tu, tc, tr, tm = [], [], [], []
+    for t, (costs, phase, flows) in enumerate(calls):
+        r0, c0 = rf[t // len(cf)], cf[t % len(cf)]
+        nr, nc = rs[t // len(cf)], cs[t % len(cf)]
+        np.testing.assert_array_equal(phase, ph[r0:r0 + nr, c0:c0 + nc])
+        crop_corr = s.corr[r0:r0 + nr, c0:c0 + nc]
+        crop_mask = None if mask is None else mask[r0:r0 + nr, c0:c0 + nc]
+        ref_costs = G.sigsq_from_mask(phase, crop_corr, 5.0, crop_mask)
+        d = np.abs(costs.astype(np.int32) - ref_costs)
+        assert d.max() <= 1 and (d > 0).sum() <= 2
+        nra = (nr - 1) * nc
+        u = O.integrate_phase(phase, flows[:nra].reshape(nr - 1, nc), flows[nra:].reshape(nr, nc - 1))
+        pc, ng, _ = O.incrcost_smooth(costs, flows, nshortcycle=int(prm.nshortcycle))
+        tu.append(u)
+        tc.append(O.conncomp(u, crop_corr, crop_mask, 0.0, prm.maxncomps, poscost=pc, negcost=ng,
+                             conncompthresh=prm.conncompthresh))
+        tr.append(crop_corr)
+        tm.append(crop_mask)
+    ref_unw, ref_cc, tk = O.stitch(nrow, ncol, rf, rs, cf, cs, rov, cov, tu, tc, tr,
+                                   tiles_conf=tm if mask is not None else None, laplace=False)
+    assert np.all(tk == np.rint(tk))
+    np.testing.assert_array_equal(unw, ref_unw)
+    np.testing.assert_array_equal(cc, ref_cc)
 
 
 def test_mcf_failing_solver_reports_error(gpu_ready, solver_slot):
@@ -146,3 +208,21 @@ def test_mcf_failing_solver_reports_error(gpu_ready, solver_slot):
     cuphu.set_flow_solver(bad)
     with pytest.raises(RuntimeError, match="network-flow solver failed"):
         cuphu.unwrap(s.igram, s.corr, 10.0, init="mcf")
+
+
+def test_mcf_auto_tiling_with_nproc(gpu_ready, solver_slot):
+    """unwrap(init='mcf', nproc > 1) auto-tiles toward target_tile_size (the
+    reference front door's rule, _unwrap.py:484-512) and runs each tile
+    through the registered solver: the same result as the explicit tile
+    counts of that rule."""
+    from cuphu_amd._unwrap import _auto_ntiles_by_target_size
+
+    s = synth.make_scene(300, 420, nbumps=2, npatches=0, nlooks=10.0, seed=14)
+    counts = []
+    cuphu.set_flow_solver(lambda c, p, i: (counts.append(p.shape), np.zeros(c.shape[0], np.int16))[1])
+    u1, c1 = cuphu.unwrap(s.igram, s.corr, 10.0, init="mcf", nproc=4, target_tile_size=200)
+    ntr, ntc = _auto_ntiles_by_target_size(300, 420, 200, 0, 0)
+    assert ntr * ntc > 1 and len(counts) == ntr * ntc
+    u2, c2 = cuphu.unwrap(s.igram, s.corr, 10.0, init="mcf", ntiles=(ntr, ntc))
+    np.testing.assert_array_equal(u1, u2)
+    np.testing.assert_array_equal(c1, c2)

@@ -126,17 +126,25 @@ def test_autotuned_tile_shape_agrees_with_default(gpu_ready, capfd):
     u1, c1, s1 = G.batch_solve(scenes, _capi.default_options(autotune=1, verbose=1, **opts))
     err = capfd.readouterr().err
     tuned = [ln for ln in err.splitlines() if ln.startswith("[cuphu-autotune]")]
-    # 256-column tiles of 2, 4, 8 rows per lane group and 128-column tiles
-    # of 2, 4, each timed once
-    assert len(tuned) == 5, err[-2000:]
+    # widths 700 and 200 are not multiples of 256, so only the 256-column
+    # tiles of 2, 4, 8 rows per lane group are timed (a 128-column tile on
+    # a wider row stride is slow), each once
     assert sorted(ln.split("rows/lane group ")[1].split(":")[0] for ln in tuned) == [
-        "2, lanes/row 32", "2, lanes/row 64", "4, lanes/row 32", "4, lanes/row 64", "8, lanes/row 64"]
+        "2, lanes/row 64", "4, lanes/row 64", "8, lanes/row 64"], err[-2000:]
     u2, c2, s2 = G.batch_solve(scenes, _capi.default_options(autotune=1, verbose=1, **opts))
     assert "[cuphu-autotune]" not in capfd.readouterr().err   # cached for the shape class
     for i, s in enumerate(scenes):
         ph = O.wrapped_phase(s.igram, correctly_rounded=True)
         assert (amb(u1[i], ph) == amb(u0[i], ph)).mean() >= 0.999
         np.testing.assert_array_equal(u2[i], u1[i])
+    # widths that are multiples of 256: the 128-column tiles of 2, 4 rows are
+    # candidates as well
+    wide = [synth.make_scene(200, 512, nbumps=1, npatches=0, nlooks=10.0, seed=33),
+            synth.make_scene(120, 256, nbumps=1, npatches=0, nlooks=5.0, seed=34)]
+    G.batch_solve(wide, _capi.default_options(autotune=1, verbose=1, **opts))
+    tuned = [ln for ln in capfd.readouterr().err.splitlines() if ln.startswith("[cuphu-autotune]")]
+    assert sorted(ln.split("rows/lane group ")[1].split(":")[0] for ln in tuned) == [
+        "2, lanes/row 32", "2, lanes/row 64", "4, lanes/row 32", "4, lanes/row 64", "8, lanes/row 64"]
 
 
 def test_conncomp_of_batched_output_matches_oracle(gpu_ready):
