@@ -17,7 +17,9 @@ __all__ = [
     "SOLVER_PLAIN", "SOLVER_BATCHED", "SOLVER_FUSED", "COST", "INIT", "EXPORTED",
 ]
 
-LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libcuphu_hip.so")
+# CUPHU_LIB selects another build of the library (the CPU sanitizer lane's
+# build with an instrumented host stitch)
+LIB_PATH = os.environ.get("CUPHU_LIB") or os.path.join(os.path.dirname(os.path.abspath(__file__)), "libcuphu_hip.so")
 
 COST = {"smooth": 0, "defo": 1, "topo": 2}
 INIT = {"mst": 0, "mcf": 1, "laplace": 2}

@@ -19,7 +19,8 @@ import subprocess
 import numpy as np
 
 _DIR = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_DIR, "_build", "libcuphu_oracle.so")
+# CUPHU_ORACLE_LIB selects another build of the oracle (the sanitizer lane)
+LIB_PATH = os.environ.get("CUPHU_ORACLE_LIB") or os.path.join(_DIR, "_build", "libcuphu_oracle.so")
 
 
 def build(force: bool = False) -> None:
