@@ -6,6 +6,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <cstdint>
 #include <cstdio>
@@ -112,6 +113,15 @@ struct DeviceGuard {
         int cur = -1;
         if (hipGetDevice(&cur) == hipSuccess && cur != prev && prev >= 0) (void)hipSetDevice(prev);
     }
+};
+
+// A named host range for `rocprofv3 --marker-trace` (roctx; costs a call
+// into the roctx library when no profiler is attached).
+struct TraceRange {
+    explicit TraceRange(const char *name) { roctxRangePushA(name); }
+    ~TraceRange() { roctxRangePop(); }
+    TraceRange(const TraceRange &) = delete;
+    TraceRange &operator=(const TraceRange &) = delete;
 };
 
 inline size_t ceil_div(size_t a, size_t b) { return (a + b - 1) / b; }

@@ -551,6 +551,7 @@ static void solve_group_fused(const Planes<T> &pl, const GroupDev &g, const CuPh
     const int npass = fixed_weights ? 1 : o.n_irls + 1;
     try {
         for (int pass = 0; pass < npass; ++pass) {
+            TraceRange tr("cuphu:pcg_solve");
             if (!fixed_weights) {
                 T d2 = T(0);
                 if (pass > 0) {
@@ -920,7 +921,10 @@ static void run_batch(const std::vector<std::pair<int, int>> &dims, const CuPhuL
             CUPHU_HIP_CHECK(hipGetLastError());
         }
     }
-    ingest(P.pl, L, d_gdesc, d_rowinst);
+    {
+        TraceRange tr("cuphu:ingest");
+        ingest(P.pl, L, d_gdesc, d_rowinst);
+    }
     // tile shape (the element layout does not depend on it, only the
     // workgroup tiling)
     const TileShape sh = (o.solver == CUPHU_SOLVER_PLAIN || !o.autotune) ? default_shape(L, o)
@@ -980,6 +984,7 @@ static void run_batch(const std::vector<std::pair<int, int>> &dims, const CuPhuL
         for (int i = a; i < b; ++i) g.px.push_back((long long)L.desc[(size_t)i].nrow * L.desc[(size_t)i].ncol);
         CUPHU_HIP_CHECK(hipStreamSynchronize(st));   // host vectors die at scope end
     };
+    TraceRange tr_solve("cuphu:pcg_irls");
     if (o.solver == CUPHU_SOLVER_PLAIN) {
         for (int i = 0; i < ninst; ++i) {
             upload_group(i, i + 1);
@@ -995,6 +1000,7 @@ static void run_batch(const std::vector<std::pair<int, int>> &dims, const CuPhuL
     }
     launch_final<T>(d_gdesc, d_rowinst, L.total_rows, P.pl, P.out, L.max_pitch, st);
     out.ncomp.assign((size_t)ninst, 0);
+    TraceRange tr_cc("cuphu:conncomp");
     if (do_conncomp)
         for (int i = 0; i < ninst; ++i) {
             const InstDesc &d = L.desc[(size_t)i];
@@ -1053,6 +1059,7 @@ static void stitch_tiles(const BatchOut &bo, int nrow, int ncol, int ntr, int nt
                          const std::vector<int> &rs, const std::vector<int> &cf, const std::vector<int> &cs, int rovl,
                          int covl, const unsigned char *d_conf, bool pooled_mask, bool integer, float *d_unw,
                          uint32_t *d_cc, DevRes &res, hipStream_t st) {
+    TraceRange tr("cuphu:stitch");
     // pair registrations on the GPU (weighted medians of the overlap
     // differences), tile offsets by the BFS over the tile grid on the host
     const int nt = ntr * ntc;
@@ -1607,6 +1614,7 @@ int cuphu_laplace_batch_gpu(const CuPhuInstance *inst, int ninst, CuPhuCostMode 
                             const CuPhuLaplaceOptions *opt, CuPhuSolveStats *stats, cuphu_stream_t stream) {
     return guarded([&]() {
         std::lock_guard<std::recursive_mutex> lk(g_mu);
+        TraceRange tr_entry("cuphu:laplace_batch");
         CUPHU_REQUIRE(inst && ninst > 0 && base, "instances and params required");
         CUPHU_REQUIRE(cost_mode >= CUPHU_COST_SMOOTH && cost_mode <= CUPHU_COST_TOPO, "invalid cost mode");
         CuPhuLaplaceOptions o = opts_or_default(opt);
@@ -1663,6 +1671,7 @@ int cuphu_unwrap_scene_gpu(const float *d_igram, const float *d_corr, const unsi
                            uint32_t *d_conncomp, CuPhuSolveStats *stats, int *ntiles_out, cuphu_stream_t stream) {
     return guarded([&]() {
         std::lock_guard<std::recursive_mutex> lk(g_mu);
+        TraceRange tr_entry("cuphu:unwrap_scene");
         CUPHU_REQUIRE(d_igram && d_corr && d_unw && params, "igram, corr, unw and params required");
         CUPHU_REQUIRE(nrow >= 1 && ncol >= 1, "dimensions must be >= 1");
         CUPHU_REQUIRE(nlooks >= 1.0, "nlooks must be >= 1");
@@ -1743,8 +1752,12 @@ static int mcf_tile(const float2 *d_ig, const float *d_corr, const unsigned char
     if (na) CUPHU_HIP_CHECK(hipMemcpyAsync(h_costs.data(), d_costs.get(), na * esz, hipMemcpyDeviceToHost, st));
     CUPHU_HIP_CHECK(hipMemcpyAsync(h_ph.data(), d_ph.get(), n * 4, hipMemcpyDeviceToHost, st));
     CUPHU_HIP_CHECK(hipStreamSynchronize(st));
-    int rc = g_flow_solver(h_costs.data(), (int)cost_mode, h_ph.data(), nrow, ncol, (int)init_meth, h_flows.data(),
+    int rc;
+    {
+        TraceRange tr("cuphu:host_flow_solver");
+        rc = g_flow_solver(h_costs.data(), (int)cost_mode, h_ph.data(), nrow, ncol, (int)init_meth, h_flows.data(),
                            g_flow_solver_user);
+    }
     if (rc != 0) throw Error(CUPHU_ERR_INTERNAL, "the registered network-flow solver failed (" + std::to_string(rc) + ")");
     if (na) CUPHU_HIP_CHECK(hipMemcpyAsync(d_flows.get(), h_flows.data(), na * 2, hipMemcpyHostToDevice, st));
     const short *pos = nullptr, *neg = nullptr;
@@ -1833,6 +1846,7 @@ int cuphu_unwrap_ex(const float *igram_r, const float *igram_i, const float *igr
                     CuPhuResult *result) {
     return guarded([&]() {
         std::lock_guard<std::recursive_mutex> lk(g_mu);
+        TraceRange tr_entry("cuphu:unwrap");
         CUPHU_REQUIRE(result && corr && params && tile, "null argument");
         CUPHU_REQUIRE(igram_c64 || (igram_r && igram_i), "igram required");
         CUPHU_REQUIRE(nrow >= 1 && ncol >= 1, "dimensions must be >= 1");
