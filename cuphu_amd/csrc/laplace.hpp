@@ -116,13 +116,25 @@ void launch_it(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<
 template <class T>
 void launch_rr_it(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, const SolveScal *scal,
                   double4 *partials, int rpw, hipStream_t st);
+// scal_in (default: scal) is the state the reduction starts from; the new
+// state goes to scal.
 void launch_reduce_it(const InstDesc *d_desc, int ninst, const double4 *partials, SolveScal *scal, int mode,
-                      int max_iter, double tol2, hipStream_t st, int defer_x = 0);
+                      int max_iter, double tol2, hipStream_t st, int defer_x = 0,
+                      const SolveScal *scal_in = nullptr);
 // The same iteration without a stored q (q_j = A p_j recomputed from p_j):
 // plane sets (r, p) and (r2, p2); and its true-residual kernel.
 template <class T>
 void launch_iq(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, const SolveScal *scal,
                double4 *partials, TileShape sh, int parity, int start, hipStream_t st);
+// The same iteration with the previous launch's reduction folded in (for
+// grids of a few resident rounds, where the reduction's own launch is a
+// large part of an iteration): reduces part_prev in the reduction's order,
+// advances scal_prev to the next state exactly as the reduction would
+// (written to scal_next by the instance's first tile) and runs on it.
+template <class T>
+void launch_iq_fold(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl,
+                    const SolveScal *scal_prev, SolveScal *scal_next, const double4 *part_prev, double4 *partials,
+                    TileShape sh, int parity, int max_iter, double tol2, hipStream_t st);
 template <class T>
 void launch_rr_iq(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, const SolveScal *scal,
                   double4 *partials, TileShape sh, hipStream_t st);

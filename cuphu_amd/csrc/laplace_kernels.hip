@@ -811,6 +811,77 @@ __device__ __forceinline__ void block_sum4(double a, double b, double c, double 
                             ((s4[2][0] + s4[2][1]) + s4[2][2]) + s4[2][3], ((s4[3][0] + s4[3][1]) + s4[3][2]) + s4[3][3]);
 }
 
+// Arguments of the folded iteration kernel: the previous launch's partial
+// sums (reduced here), where the new scalar state goes, and the stopping
+// rule of the reduction.
+struct FoldArgs {
+    const double4 *part_prev;
+    SolveScal *scal_next;
+    int max_iter;
+    double tol2;
+};
+
+// Sum of nb workgroup partials in the reductions' fixed order: thread t adds
+// partials t, t + 256, t + 512, ... (four loads in flight before their
+// sums), then the block butterfly; *res holds the sums for every thread
+// after the closing barrier.
+__device__ __forceinline__ void sum_partials4(const double4 *pp, int nb, double4 *res) {
+    double a = 0.0, b = 0.0, c = 0.0, e = 0.0;
+    int i = threadIdx.x;
+    for (; i + 768 < nb; i += 1024) {
+        const double4 v0 = pp[i], v1 = pp[i + 256], v2 = pp[i + 512], v3 = pp[i + 768];
+        a += v0.x; b += v0.y; c += v0.z; e += v0.w;
+        a += v1.x; b += v1.y; c += v1.z; e += v1.w;
+        a += v2.x; b += v2.y; c += v2.z; e += v2.w;
+        a += v3.x; b += v3.y; c += v3.z; e += v3.w;
+    }
+    for (; i < nb; i += 256) {
+        const double4 v = pp[i];
+        a += v.x; b += v.y; c += v.z; e += v.w;
+    }
+    block_sum4(a, b, c, e, res);
+    __syncthreads();
+}
+
+// alpha_j = rz / p.q and beta from the rz expansion (see k_it); stops on
+// p.q <= 0 (cuphu_laplace.cu:460).  v = (p.q, r.z, q.(D r), q.(D q)).
+__device__ __forceinline__ void fused_alpha_beta(SolveScal &s, const double4 v) {
+    const double pq = v.x, rz = v.y, qr = v.z, qq = v.w;
+    if (!(pq > 0.0)) {
+        s.status = isfinite(pq) ? CUPHU_SOLVE_BREAKDOWN : CUPHU_SOLVE_NONFINITE;
+        s.active = 0;
+        return;
+    }
+    const double al = rz / pq;
+    double rzn = rz - 2.0 * al * qr + al * al * qq;
+    if (!(rzn > 0.0)) rzn = 0.0;
+    s.alpha = al;
+    s.beta = rzn / rz;
+    s.status = CUPHU_SOLVE_MAXITER;
+    s.active = 1;
+}
+
+// The scalar recurrence after one fused iteration (the reduction's RED_IT
+// mode; the folded iteration kernel runs the same code): one iteration
+// counted, converged if rz < tol^2 rz_0 (:478), the cap, then alpha / beta.
+__device__ __forceinline__ void fused_it_scalars(SolveScal &s, const double4 v, int max_iter, double tol2) {
+    const double pq = v.x, rz = v.y;
+    s.replace = 0;
+    s.iters += 1;
+    // the launch just done applied xmode 0 (deferred alpha_j p_j) or 1
+    // (applied both); alpha_prev = alpha_j for the finalize / next launch
+    s.xmode = (s.xmode == 0) ? 1 : (s.xmode == 1 ? 0 : 2);
+    s.alpha_prev = s.alpha;
+    s.relres = (s.gamma0 > 0.0) ? sqrt(fabs(rz) / s.gamma0) : 0.0;
+    s.gamma_prev = s.gamma;
+    s.gamma = rz;
+    s.delta = pq;
+    if (rz < tol2 * s.gamma0) { s.status = CUPHU_SOLVE_CONVERGED; s.active = 0; return; }
+    if (s.iters >= max_iter) { s.status = CUPHU_SOLVE_MAXITER; s.active = 0; return; }
+    fused_alpha_beta(s, v);
+}
+
+
 template <class T>
 struct ItSets {
     const T *r_in, *p_in, *q_in;
@@ -1100,16 +1171,44 @@ __device__ __forceinline__ void stagger_first_round() {
     }
 }
 
-template <class T, int RPW, int LPR, bool STAGGER>
-__global__ __launch_bounds__(256) void k_iq(const InstDesc *descs, const int *blk_inst, Planes<T> pl,
-                                            const SolveScal *scal, double4 *partials, int parity, int start) {
+// The iteration body; FOLD: the previous launch's reduction first (see
+// launch_iq_fold).  Two kernels wrap it (k_iq, k_iq_fold) so that each keeps
+// its own register budget.
+template <class T, int RPW, int LPR, bool STAGGER, bool FOLD>
+__device__ __forceinline__ void iq_body(const InstDesc *descs, const int *blk_inst, const Planes<T> &pl,
+                                        const SolveScal *scal, double4 *partials, int parity, int start,
+                                        const FoldArgs &fa) {
     using Sh = Shape<RPW, LPR>;
     constexpr int TH = Sh::TH, S = Sh::S, TW = Sh::TW;
     __shared__ T lp0[(TH + 4) * S];   // p_j: LDS row = row + 2, col = col + kLdsC0
     __shared__ T lp1[(TH + 2) * S];   // p_{j+1}: LDS row = row + 1
     TileInfo t = get_tile(descs, blk_inst, TH, TW);
-    const SolveScal *sc = scal + t.inst;
-    if (!start && !sc->active) return;
+    const SolveScal *sc;
+    if constexpr (FOLD) {
+        // the previous launch's reduction, done here (every workgroup of the
+        // instance forms the same state; its first tile stores it)
+        __shared__ SolveScal ls;
+        __shared__ double4 sums;
+        const SolveScal prev = scal[t.inst];
+        const bool lead = t.blk == t.d.block_begin;
+        if (!prev.active) {
+            if (lead && threadIdx.x == 0) fa.scal_next[t.inst] = prev;
+            return;
+        }
+        sum_partials4(fa.part_prev + t.d.block_begin, t.d.tiles_x * t.d.tiles_y, &sums);
+        if (threadIdx.x == 0) {
+            SolveScal sn = prev;
+            fused_it_scalars(sn, sums, fa.max_iter, fa.tol2);
+            ls = sn;
+            if (lead) fa.scal_next[t.inst] = sn;
+        }
+        __syncthreads();
+        if (!ls.active) return;
+        sc = &ls;
+    } else {
+        sc = scal + t.inst;
+        if (!start && !sc->active) return;
+    }
     if (STAGGER) stagger_first_round();
     const bool upd = !start;                          // x += a p
     const bool rep = upd && sc->replace != 0;         // r from the true-residual plane
@@ -1354,6 +1453,21 @@ __global__ __launch_bounds__(256) void k_iq(const InstDesc *descs, const int *bl
     block_sum4(s_pq, s_rz, s_qr, s_qq, partials + t.blk);
 }
 
+template <class T, int RPW, int LPR, bool STAGGER>
+__global__ __launch_bounds__(256) void k_iq(const InstDesc *descs, const int *blk_inst, Planes<T> pl,
+                                            const SolveScal *scal, double4 *partials, int parity, int start) {
+    const FoldArgs none{nullptr, nullptr, 0, 0.0};
+    iq_body<T, RPW, LPR, STAGGER, false>(descs, blk_inst, pl, scal, partials, parity, start, none);
+}
+// The folded FP32 kernel with 4 rows per lane group is held to 3 resident
+// workgroups per CU (168 registers), like the unfolded one.
+template <class T, int RPW, int LPR>
+__global__ __launch_bounds__(256, (RPW == 4 && sizeof(T) == 4) ? 3 : 1) void k_iq_fold(
+    const InstDesc *descs, const int *blk_inst, Planes<T> pl, const SolveScal *scal_prev, double4 *partials,
+    int parity, FoldArgs fa) {
+    iq_body<T, RPW, LPR, false, true>(descs, blk_inst, pl, scal_prev, partials, parity, 0, fa);
+}
+
 // True residual for the q-free fused solver at the end of an even chunk
 // (state in set 1 = (r2, p2) with alpha_j pending): x' = x + alpha_j p_j and
 // q_j = A p_j from LDS tiles with a one-pixel halo; r_t = b - A x' into wv;
@@ -1483,26 +1597,23 @@ __global__ __launch_bounds__(256) void k_x_finalize(const InstDesc *descs, const
 //          1000-iteration cap it left up to 5 % of pixels a cycle away from
 //          the FP64 solve, whose iterates the unreplaced FP32 recursion
 //          tracks to 99.98 %.
-__global__ __launch_bounds__(256) void k_reduce_it(const InstDesc *descs, const double4 *partials, SolveScal *scal,
-                                                   int mode, int max_iter, double tol2, int defer_x) {
+__global__ __launch_bounds__(256) void k_reduce_it(const InstDesc *descs, const double4 *partials, const SolveScal *scal_in,
+                                                   SolveScal *scal, int mode, int max_iter, double tol2, int defer_x) {
     int inst = blockIdx.x;
     const InstDesc d = descs[inst];
+    const SolveScal *sp_in = scal_in + inst;
     SolveScal *sp = scal + inst;
-    if (mode != RED_START && !sp->active) return;
-    int nb = d.tiles_x * d.tiles_y;
-    double a = 0.0, b = 0.0, c = 0.0, e = 0.0;
-    for (int i = threadIdx.x; i < nb; i += 256) {
-        double4 v = partials[d.block_begin + i];
-        a += v.x; b += v.y; c += v.z; e += v.w;
+    if (mode != RED_START && !sp_in->active) {
+        if (threadIdx.x == 0 && sp_in != sp) *sp = *sp_in;   // a frozen state moves to the output buffer
+        return;
     }
     __shared__ double4 res;
-    block_sum4(a, b, c, e, &res);
-    __syncthreads();
+    sum_partials4(partials + d.block_begin, d.tiles_x * d.tiles_y, &res);
     if (threadIdx.x != 0) return;
-    SolveScal s = *sp;
+    SolveScal s = *sp_in;
     const double4 v = res;
-    s.replace = 0;
     if (mode == RED_RR) {
+        s.replace = 0;
         const double rt = v.x, rr = v.y;
         s.delta = rt / rr;   // true over recursive r.z, kept until the next reduction
         if (isfinite(rt) && rt < tol2 * s.gamma0) {
@@ -1514,39 +1625,18 @@ __global__ __launch_bounds__(256) void k_reduce_it(const InstDesc *descs, const 
         *sp = s;
         return;
     }
-    const double pq = v.x, rz = v.y, qr = v.z, qq = v.w;
     if (mode == RED_START) {
+        const double pq = v.x, rz = v.y;
+        s.replace = 0;
         s.iters = 0; s.gamma0 = rz; s.gamma_prev = rz; s.relres = 1.0; s.active = 0;
         s.alpha = 0.0; s.alpha_prev = 0.0; s.beta = 0.0; s.delta = pq; s.gamma = rz;
         s.xmode = defer_x ? 0 : 2;
         if (!(rz > 0.0)) { s.status = (rz == 0.0) ? CUPHU_SOLVE_EMPTY : CUPHU_SOLVE_NONFINITE; s.relres = 0.0; *sp = s; return; }
         if (max_iter <= 0) { s.status = CUPHU_SOLVE_MAXITER; *sp = s; return; }
+        fused_alpha_beta(s, v);
     } else {
-        s.iters += 1;
-        // the launch just done applied xmode 0 (deferred alpha_j p_j) or 1
-        // (applied both); alpha_prev = alpha_j for the finalize / next launch
-        s.xmode = (s.xmode == 0) ? 1 : (s.xmode == 1 ? 0 : 2);
-        s.alpha_prev = s.alpha;
-        s.relres = (s.gamma0 > 0.0) ? sqrt(fabs(rz) / s.gamma0) : 0.0;
-        s.gamma_prev = s.gamma;
-        s.gamma = rz;
-        s.delta = pq;
-        if (rz < tol2 * s.gamma0) { s.status = CUPHU_SOLVE_CONVERGED; s.active = 0; *sp = s; return; }
-        if (s.iters >= max_iter) { s.status = CUPHU_SOLVE_MAXITER; s.active = 0; *sp = s; return; }
+        fused_it_scalars(s, v, max_iter, tol2);
     }
-    if (!(pq > 0.0)) {
-        s.status = isfinite(pq) ? CUPHU_SOLVE_BREAKDOWN : CUPHU_SOLVE_NONFINITE;
-        s.active = 0;
-        *sp = s;
-        return;
-    }
-    const double al = rz / pq;
-    double rzn = rz - 2.0 * al * qr + al * al * qq;
-    if (!(rzn > 0.0)) rzn = 0.0;
-    s.alpha = al;
-    s.beta = rzn / rz;
-    s.status = CUPHU_SOLVE_MAXITER;
-    s.active = 1;
     *sp = s;
 }
 
@@ -1570,7 +1660,13 @@ __global__ __launch_bounds__(256) void k_reduce(const InstDesc *descs, const dou
     if (mode != RED_START && !sp->active) return;
     int nb = d.tiles_x * d.tiles_y;
     double a = 0.0;
-    for (int i = threadIdx.x; i < nb; i += 256) a += partials[d.block_begin + i].x;
+    const double2 *pp = partials + d.block_begin;
+    int i = threadIdx.x;
+    for (; i + 768 < nb; i += 1024) {                 // four loads before their sums (same order)
+        const double v0 = pp[i].x, v1 = pp[i + 256].x, v2 = pp[i + 512].x, v3 = pp[i + 768].x;
+        a += v0; a += v1; a += v2; a += v3;
+    }
+    for (; i < nb; i += 256) a += pp[i].x;
     __shared__ double2 res;
     block_sum2(a, 0.0, &res);
     __syncthreads();
@@ -1860,6 +1956,15 @@ void launch_iq(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<
     CUPHU_LAUNCH_CHECK();
 }
 template <class T>
+void launch_iq_fold(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl,
+                    const SolveScal *scal_prev, SolveScal *scal_next, const double4 *part_prev, double4 *partials,
+                    TileShape sh, int parity, int max_iter, double tol2, hipStream_t st) {
+    const FoldArgs fa{part_prev, scal_next, max_iter, tol2};
+    CUPHU_SHAPE_DISPATCH(sh, k_iq_fold, dim3(nblk), dim3(256), 0, st, d_desc, d_blk, pl, scal_prev, partials, parity,
+                         fa);
+    CUPHU_LAUNCH_CHECK();
+}
+template <class T>
 void launch_rr_iq(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, const SolveScal *scal,
                   double4 *partials, TileShape sh, hipStream_t st) {
     CUPHU_SHAPE_DISPATCH(sh, k_rr_iq, dim3(nblk), dim3(256), 0, st, d_desc, d_blk, pl, scal, partials);
@@ -1872,9 +1977,9 @@ void launch_x_finalize(const InstDesc *d_desc, const int *d_blk, int nblk, const
     CUPHU_LAUNCH_CHECK();
 }
 void launch_reduce_it(const InstDesc *d_desc, int ninst, const double4 *partials, SolveScal *scal, int mode,
-                      int max_iter, double tol2, hipStream_t st, int defer_x) {
-    hipLaunchKernelGGL(k_reduce_it, dim3(ninst), dim3(256), 0, st, d_desc, partials, scal, mode, max_iter, tol2,
-                       defer_x);
+                      int max_iter, double tol2, hipStream_t st, int defer_x, const SolveScal *scal_in) {
+    hipLaunchKernelGGL(k_reduce_it, dim3(ninst), dim3(256), 0, st, d_desc, partials, scal_in ? scal_in : scal, scal,
+                       mode, max_iter, tol2, defer_x);
     CUPHU_LAUNCH_CHECK();
 }
 void launch_reduce(const InstDesc *d_desc, int ninst, const double2 *partials, SolveScal *scal, int mode,
@@ -1945,6 +2050,9 @@ void launch_plain_dot(const T *a, const T *b, size_t n, double *partials, double
                                   double4 *, int, hipStream_t);                                                  \
     template void launch_iq<T>(const InstDesc *, const int *, int, const Planes<T> &, const SolveScal *,         \
                                double4 *, TileShape, int, int, hipStream_t);                                     \
+    template void launch_iq_fold<T>(const InstDesc *, const int *, int, const Planes<T> &, const SolveScal *,    \
+                                    SolveScal *, const double4 *, double4 *, TileShape, int, int, double,        \
+                                    hipStream_t);                                                                \
     template void launch_rr_iq<T>(const InstDesc *, const int *, int, const Planes<T> &, const SolveScal *,      \
                                   double4 *, TileShape, hipStream_t);                                            \
     template void launch_x_finalize<T>(const InstDesc *, const int *, int, const Planes<T> &, const SolveScal *,  \

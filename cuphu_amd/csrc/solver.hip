@@ -333,6 +333,8 @@ struct GroupDev {
     SolveScal *scal;
     double2 *part;
     double4 *part4;   // fused solver's four partial sums per workgroup
+    SolveScal *scal2;    // the folded iteration's second state buffer
+    double4 *part4b;     // and its second partial-sum buffer
     int ninst, nblk;
     std::vector<long long> px;   // host: logical pixels per instance
 };
@@ -492,6 +494,22 @@ static bool wide_stride(const CuPhuLaplaceOptions &o) {
     return v != 64 && o.solver == CUPHU_SOLVER_FUSED && !fused_stored_q();
 }
 
+// Folded reductions (launch_iq_fold) for grids of at most this many
+// workgroups, where the reduction launch it replaces is a large part of an
+// iteration and every workgroup's extra read of its instance's partial sums
+// is short.  Measured (same GPU, alternating runs, fused solver with and
+// without folding): 1024^2 (256 workgroups; k_iq 9.0 us + k_reduce_it
+// 4.5 us per iteration unfolded) 13.07 -> 15.17 Mpixel/s; 2048^2 (1024
+// workgroups; 29.0 + 4.7 us) 20.13 -> 20.59.  CUPHU_FUSED_FOLD=0 turns it
+// off, =1 forces it on.
+constexpr int kFoldMaxBlocks = 1024;
+static bool fold_reductions(int nblk) {
+    int v = -1;
+    env_int("CUPHU_FUSED_FOLD", &v);
+    if (v == 0 || v == 1) return v == 1;
+    return nblk <= kFoldMaxBlocks;
+}
+
 // Fused solver: one stencil kernel and one reduction per PCG iteration.
 template <class T>
 static void solve_group_fused(const Planes<T> &pl, const GroupDev &g, const CuPhuLaplaceOptions &o, TileShape sh,
@@ -521,11 +539,38 @@ static void solve_group_fused(const Planes<T> &pl, const GroupDev &g, const CuPh
         it(1 - (i & 1), 0);
         launch_reduce_it(g.desc, g.ninst, g.part4, g.scal, RED_IT, o.max_iter, tol2, st);
     };
+    // Steps first..last-1 of a chunk with the reductions folded: step `first`
+    // runs on the state in g.scal and leaves its sums in g.part4; each later
+    // step k (counted from first) reduces step k-1's sums, advancing the
+    // state from one buffer to the other (state after step k-2 in g.scal when
+    // k is odd, g.scal2 when even; it writes the other), while writing its own
+    // sums to the other partial buffer; one reduction after the last step
+    // brings the state back to g.scal.  The same states, bit for bit, as one
+    // reduction launch per step.
+    CUPHU_REQUIRE(g.scal2 && g.part4b, "fused solver group without its second state / partial buffers");
+    const bool fold = !sq && fold_reductions(g.nblk);
+    auto folded_steps = [&](int first, int last) {
+        if (first >= last) return;
+        double4 *pb[2] = {g.part4, g.part4b};
+        launch_iq<T>(g.desc, g.blk, g.nblk, pl, g.scal, pb[0], sh, 1 - (first & 1), 0, st);
+        for (int i = first + 1; i < last; ++i) {
+            const int k = i - first;
+            const SolveScal *sin = (k & 1) ? g.scal : g.scal2;
+            SolveScal *sout = (k & 1) ? g.scal2 : g.scal;
+            launch_iq_fold<T>(g.desc, g.blk, g.nblk, pl, sin, sout, pb[(k - 1) & 1], pb[k & 1], sh, 1 - (i & 1),
+                              o.max_iter, tol2, st);
+        }
+        const int k = last - 1 - first;   // the last step: state after step k-1 is in (k odd ? scal2 : scal)
+        launch_reduce_it(g.desc, g.ninst, pb[k & 1], g.scal, RED_IT, o.max_iter, tol2, st, 0,
+                         k == 0 ? g.scal : ((k & 1) ? g.scal2 : g.scal));
+    };
     // With profiling on, the first two steps of a chunk run outside the
     // graph, each between two events (a deferral pair: together they move
     // the average bytes of the steady state); the graph holds CH-2.
     auto chunk = [&]() {
-        for (int i = prof ? 2 : 0; i < CH; ++i) step(i);
+        if (fold) folded_steps(prof ? 2 : 0, CH);
+        else
+            for (int i = prof ? 2 : 0; i < CH; ++i) step(i);
         if (o.rr_interval > 0) {
             if (sq) launch_rr_it<T>(g.desc, g.blk, g.nblk, pl, g.scal, g.part4, rpw, st);
             else launch_rr_iq<T>(g.desc, g.blk, g.nblk, pl, g.scal, g.part4, sh, st);
@@ -962,6 +1007,8 @@ static void run_batch(const std::vector<std::pair<int, int>> &dims, const CuPhuL
     g.scal = ws<SolveScal>(res, "grpscal", maxinst);
     g.part = ws<double2>(res, "grppart", maxblk);
     g.part4 = ws<double4>(res, "grppart4", maxblk);
+    g.scal2 = ws<SolveScal>(res, "grpscal2", maxinst);
+    g.part4b = ws<double4>(res, "grppart4b", maxblk);
     std::vector<SolveScal> hs;
     std::vector<CuPhuSolveStats> dummy((size_t)ninst);
     auto sp = [&](int i) { return stats ? stats + i : &dummy[(size_t)i]; };
@@ -1330,6 +1377,8 @@ static void wls_impl(const T *d_w, const float *d_phase, const T *d_u, const T *
     g.scal = ws<SolveScal>(res, "grpscal", 1);
     g.part = ws<double2>(res, "grppart", (size_t)nb);
     g.part4 = ws<double4>(res, "grppart4", (size_t)nb);
+    g.scal2 = ws<SolveScal>(res, "grpscal2", 1);
+    g.part4b = ws<double4>(res, "grppart4b", (size_t)nb);
     g.ninst = 1;
     g.nblk = nb;
     g.px = {(long long)nrow * ncol};
@@ -2127,8 +2176,9 @@ constexpr size_t kNoAllocSmallBytes = (size_t)4 << 20;
 // bytes, plus the per-group buffers.  With at most
 // CUPHU_NOALLOC_MAX_ELEMENTS elements an instance has at most 16384 rows
 // and 11265 workgroup tiles (256 x 8 or 128 x 16 at the smallest), so the
-// per-group buffers (row map 4 B per row; 52 B per tile; the plain
-// solver's 8 KB of dot partials) stay far below kNoAllocSmallBytes.
+// per-group buffers (row map 4 B per row; 84 B per tile; two scalar states
+// per instance; the plain solver's 8 KB of dot partials) stay far below
+// kNoAllocSmallBytes.
 constexpr size_t noalloc_need(long long el, size_t tsize) {
     return (size_t)el * (21 + 12 * tsize) + 17 * 512 + kNoAllocSmallBytes;
 }
@@ -2278,6 +2328,8 @@ void hook_setup(HookInst &h, DevRes &res, int nrow, int ncol, TileShape sh, hipS
     h.g.scal = ws<SolveScal>(res, "grpscal", 1);
     h.g.part = ws<double2>(res, "grppart", (size_t)nb);
     h.g.part4 = ws<double4>(res, "grppart4", (size_t)nb);
+    h.g.scal2 = ws<SolveScal>(res, "grpscal2", 1);
+    h.g.part4b = ws<double4>(res, "grppart4b", (size_t)nb);
     h.g.ninst = 1;
     h.g.nblk = nb;
     std::vector<int> blk((size_t)nb, 0);

@@ -403,3 +403,37 @@ def test_repeat_runs_are_deterministic(gpu_ready):
     a, _, _ = G.batch_solve([s], o)
     b, _, _ = G.batch_solve([s], o)
     np.testing.assert_array_equal(a[0], b[0])
+
+
+@pytest.mark.parametrize("opts", [
+    dict(n_irls=3, max_iter=200, tol=1e-6),                    # every solve to the cap
+    dict(n_irls=2, max_iter=37, tol=1e-6, rr_interval=10),     # odd cap, short chunks
+    dict(n_irls=2, max_iter=400, tol=3e-3),                    # converges inside a chunk
+    dict(n_irls=1, max_iter=200, tol=1e-6, use_graph=0),       # eager launches
+])
+def test_folded_reductions_bit_identical(gpu_ready, monkeypatch, opts):
+    """The fused solver's iteration kernel with the previous launch's
+    reduction folded in (small grids) gives the states, iteration counts and
+    unwrapped phase of one reduction launch per iteration, bit for bit;
+    alone and batched, with the live kernel timing on and off."""
+    scenes = [synth.make_scene(70, 90, nbumps=2, npatches=1, nlooks=10.0, seed=41, patch=15),
+              synth.make_scene(300, 520, nbumps=3, npatches=1, nlooks=5.0, seed=42, patch=60),
+              synth.make_scene(33, 200, nbumps=1, npatches=0, nlooks=20.0, seed=43)]
+    o = _capi.default_options(solver="fused", **opts)
+    res = {}
+    for fold in ("0", "1"):
+        monkeypatch.setenv("CUPHU_FUSED_FOLD", fold)
+        res[fold] = [G.batch_solve(scenes, o)]
+        res[fold] += [G.batch_solve([sc], o) for sc in scenes[:2]]
+        _capi.lib.cuphu_set_profiling(1)
+        try:
+            res[fold].append(G.batch_solve(scenes[1:2], o))
+        finally:
+            _capi.lib.cuphu_set_profiling(0)
+    for a, b in zip(res["0"], res["1"]):
+        for ua, ub in zip(a[0], b[0]):
+            np.testing.assert_array_equal(ua, ub)
+        for ca, cb in zip(a[1], b[1]):
+            np.testing.assert_array_equal(ca, cb)
+        assert [x["iters"] for x in a[2]] == [x["iters"] for x in b[2]]
+        assert [x["relres"] for x in a[2]] == [x["relres"] for x in b[2]]
