@@ -20,6 +20,8 @@ constexpr int kBlock = 256;
 struct InstDesc {
     int nrow, ncol, pitch, tiles_x;
     int tiles_y, block_begin, row_begin, wcols;
+    int band;        // tile columns per band of the launch order (0: raster order)
+    int pad_;
     long long off;   // element offset of pixel (0,0) in every pixel plane
 };
 

@@ -109,14 +109,34 @@ struct TileInfo {
     InstDesc d;
     int inst, blk, tx0, ty0;
 };
+// Tile of this workgroup.  The launch order (after the XCD remap) walks an
+// instance in raster order, or, when the instance is wider than `band` tile
+// columns, band by band: `band` columns wide, row by row within a band (the
+// last band narrower), so that vertically adjacent tiles stay close in the
+// order and meet in the same L2.  t.blk is the tile's raster slot, where its
+// partial sums go: the reductions add them in raster order whatever the
+// launch order, so the order does not change any result.
 __device__ __forceinline__ TileInfo get_tile(const InstDesc *descs, const int *blk_inst, int th, int tw = kTileW) {
     TileInfo t;
-    t.blk = xcd_block((int)blockIdx.x, (int)gridDim.x);
-    t.inst = blk_inst[t.blk];
+    const int b = xcd_block((int)blockIdx.x, (int)gridDim.x);
+    t.inst = blk_inst[b];
     t.d = descs[t.inst];
-    int local = t.blk - t.d.block_begin;
-    t.tx0 = (local % t.d.tiles_x) * tw;
-    t.ty0 = (local / t.d.tiles_x) * th;
+    const int local = b - t.d.block_begin;
+    int tx, ty;
+    if (t.d.band > 0) {
+        const int bsz = t.d.band * t.d.tiles_y;          // tiles of a full band
+        const int k = local / bsz;
+        const int bw = min(t.d.band, t.d.tiles_x - k * t.d.band);
+        const int rem = local - k * bsz;
+        ty = rem / bw;
+        tx = k * t.d.band + rem % bw;
+    } else {
+        tx = local % t.d.tiles_x;
+        ty = local / t.d.tiles_x;
+    }
+    t.blk = t.d.block_begin + ty * t.d.tiles_x + tx;
+    t.tx0 = tx * tw;
+    t.ty0 = ty * th;
     return t;
 }
 

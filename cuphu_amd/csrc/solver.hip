@@ -251,6 +251,23 @@ struct Layout {
 // 1080 462.7 -> 457.9; 700 447.2 -> 443.6.  Strides that are already
 // multiples of 256 are unchanged (moving them to a larger multiple was
 // slower).
+// Band width (tile columns) of an instance's launch order.  Instances
+// wider than 4096 columns are walked in bands of equal width of at most
+// 2048 columns, like 2048-wide instances, whose halo rows meet in L2 (a
+// single 8192-wide instance read 5.6 % more DRAM bytes in raster order).
+// Measured on k_iq, same box, alternating: one 8192-wide instance 0.390 ->
+// 0.371 ms per launch; 4096-wide tiles 0.436 -> 0.438 (two bands); 2560-wide
+// tiles 0.464 -> 0.478 with bands of 16 and 4 tiles, hence no bands up to
+// 4096 columns.  CUPHU_LAPLACE_BANDS=0 keeps raster order.
+static int band_of(int tiles_x, TileShape sh) {
+    int on = 1;
+    env_int("CUPHU_LAPLACE_BANDS", &on);
+    const int b = 2048 / sh.tw();
+    if (!on || tiles_x <= 2 * b) return 0;
+    const int nbands = (tiles_x + b - 1) / b;
+    return (tiles_x + nbands - 1) / nbands;
+}
+
 static Layout make_layout(const std::vector<std::pair<int, int>> &dims, TileShape sh, bool stride256 = false) {
     Layout L;
     for (auto &dm : dims) {
@@ -261,6 +278,7 @@ static Layout make_layout(const std::vector<std::pair<int, int>> &dims, TileShap
         d.pitch = stride256 ? (d.wcols + 255) / 256 * 256 : d.wcols;
         d.tiles_x = (int)ceil_div((size_t)d.pitch, (size_t)sh.tw());
         d.tiles_y = (int)ceil_div((size_t)d.nrow, (size_t)sh.th());
+        d.band = band_of(d.tiles_x, sh);
         d.row_begin = L.total_rows;
         d.off = L.total_el;
         L.total_el += (long long)d.nrow * d.pitch;
@@ -805,6 +823,7 @@ static void retile(Layout &L, TileShape sh) {
     for (auto &d : L.desc) {
         d.tiles_x = (int)ceil_div((size_t)d.pitch, (size_t)sh.tw());
         d.tiles_y = (int)ceil_div((size_t)d.nrow, (size_t)sh.th());
+        d.band = band_of(d.tiles_x, sh);
     }
 }
 

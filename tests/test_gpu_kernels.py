@@ -21,7 +21,9 @@ from tests import gpu_util as G
 
 pytestmark = pytest.mark.gpu
 TWOPI = 2 * np.pi
-SHAPES = [(1, 7), (5, 1), (2, 2), (37, 301), (64, 256), (130, 257), (300, 70)]
+# (19, 5000): wider than 4096 columns, so the tile kernels walk it in
+# bands of at most 2048 columns (launch order only)
+SHAPES = [(1, 7), (5, 1), (2, 2), (37, 301), (64, 256), (130, 257), (300, 70), (19, 5000)]
 
 
 def _rand_igram(rng, nrow, ncol):
