@@ -19,11 +19,17 @@ around which the MCF path's GPU stages run).
 
 from cuphu_amd._unwrap import unwrap
 from cuphu_amd._conncomp import regrow_conncomp
-from cuphu_amd._gpu import gpu_count, gpu_info, gpu_name
+from cuphu_amd._ext import _cuphu_ext
 from cuphu_amd._looks import get_effective_looks
 from cuphu_amd.mcf import set_flow_solver
 
 __version__ = "0.1.0"
+
+# device queries, straight from the extension: gpu_count(), gpu_name(device_id=0),
+# gpu_info(device_id=0) -> {name, total_memory, sm_count (compute units), compute_cap (gfx arch)}
+gpu_count = _cuphu_ext.gpu_count
+gpu_name = _cuphu_ext.gpu_name
+gpu_info = _cuphu_ext.gpu_info
 
 __all__ = [
     "__version__",
