@@ -837,8 +837,9 @@ static void retile(Layout &L, TileShape sh) {
 // 128-column tiles of 16 or 32 rows) is timed on the real pooled planes with
 // the fused kernel's steady-state step form before the solve starts (scratch
 // scalars; reads plane set 1, writes set 0, which the first assemble resets;
-// x is not touched), 1 warm-up + 3 timed launches, minimum taken; the
-// fastest is cached for the process.  Opt-in (options.autotune or
+// x is not touched): one warm-up launch each, then four rounds of one timed
+// launch per candidate, minimum per candidate; the fastest is taken unless
+// the default shape is within 3 % of it, and cached for the process.  Opt-in (options.autotune or
 // CUPHU_LAPLACE_AUTOTUNE=1): the tiling fixes the order of the FP64 partial
 // sums, so a timing-dependent choice can change results in the last bits
 // between processes.  CUPHU_LAPLACE_RPW forces the default shape.
@@ -877,15 +878,33 @@ static TileShape autotune_shape(Layout L, const Planes<T> &pl, const CuPhuLaplac
         cand.push_back({2, 32});
         cand.push_back({4, 32});
     }
+    // one descriptor set per candidate, so that the candidates can be timed
+    // in interleaved rounds
+    const int ncand = (int)cand.size();
+    std::vector<InstDesc *> d_desc((size_t)ncand);
+    std::vector<int *> d_blk((size_t)ncand);
+    std::vector<int> nblk((size_t)ncand);
     size_t maxblk = 0;
-    for (TileShape sh : cand) {
-        retile(L, sh);
-        size_t nb = 0;
-        for (auto &d : L.desc) nb += (size_t)d.tiles_x * d.tiles_y;
-        maxblk = std::max(maxblk, nb);
+    for (int ci = 0; ci < ncand; ++ci) {
+        retile(L, cand[(size_t)ci]);
+        std::vector<InstDesc> gd = L.desc;
+        std::vector<int> blk;
+        for (int i = 0; i < ninst; ++i) {
+            gd[(size_t)i].block_begin = (int)blk.size();
+            for (int k = 0; k < gd[(size_t)i].tiles_x * gd[(size_t)i].tiles_y; ++k) blk.push_back(i);
+        }
+        const std::string sfx = std::to_string(ci);
+        d_desc[(size_t)ci] = ws<InstDesc>(res, ("tunedesc" + sfx).c_str(), (size_t)ninst);
+        d_blk[(size_t)ci] = ws<int>(res, ("tuneblk" + sfx).c_str(), blk.size());
+        nblk[(size_t)ci] = (int)blk.size();
+        maxblk = std::max(maxblk, blk.size());
+        CUPHU_HIP_CHECK(hipMemcpyAsync(d_desc[(size_t)ci], gd.data(), gd.size() * sizeof(InstDesc),
+                                       hipMemcpyHostToDevice, st));
+        CUPHU_HIP_CHECK(hipMemcpyAsync(d_blk[(size_t)ci], blk.data(), blk.size() * sizeof(int), hipMemcpyHostToDevice,
+                                       st));
+        // the host vectors go out of scope: finish the copies first
+        CUPHU_HIP_CHECK(hipStreamSynchronize(st));
     }
-    InstDesc *d_desc = ws<InstDesc>(res, "tunedesc", (size_t)ninst);
-    int *d_blk = ws<int>(res, "tuneblk", maxblk);
     double4 *d_part = ws<double4>(res, "tunepart", maxblk);
     SolveScal *d_scal = ws<SolveScal>(res, "tunescal", (size_t)ninst);
     // scratch scalars for the steady-state step form: q recomputed, r and p
@@ -897,36 +916,27 @@ static TileShape autotune_shape(Layout L, const Planes<T> &pl, const CuPhuLaplac
     hipEvent_t e0, e1;
     CUPHU_HIP_CHECK(hipEventCreate(&e0));
     CUPHU_HIP_CHECK(hipEventCreate(&e1));
-    TileShape best{4, 64};
-    float best_ms = 1e30f;
+    // 1 warm-up launch per candidate, then kRounds rounds of one timed launch
+    // per candidate, minimum per candidate: a clock or power transient during
+    // the timing then reaches every candidate alike instead of deciding
+    // against the one it happened to meet.
+    constexpr int kRounds = 4;
+    std::vector<float> ms_min((size_t)ncand, 1e30f);
     try {
-        for (TileShape sh : cand) {
-            retile(L, sh);
-            std::vector<InstDesc> gd = L.desc;
-            std::vector<int> blk;
-            for (int i = 0; i < ninst; ++i) {
-                gd[(size_t)i].block_begin = (int)blk.size();
-                for (int k = 0; k < gd[(size_t)i].tiles_x * gd[(size_t)i].tiles_y; ++k) blk.push_back(i);
-            }
-            CUPHU_HIP_CHECK(hipMemcpyAsync(d_desc, gd.data(), gd.size() * sizeof(InstDesc), hipMemcpyHostToDevice, st));
-            CUPHU_HIP_CHECK(hipMemcpyAsync(d_blk, blk.data(), blk.size() * sizeof(int), hipMemcpyHostToDevice, st));
-            const int nblk = (int)blk.size();
-            launch_iq<T>(d_desc, d_blk, nblk, pl, d_scal, d_part, sh, 1, 0, st);
-            float ms_min = 1e30f;
-            for (int rep = 0; rep < 3; ++rep) {
+        for (int ci = 0; ci < ncand; ++ci)
+            launch_iq<T>(d_desc[(size_t)ci], d_blk[(size_t)ci], nblk[(size_t)ci], pl, d_scal, d_part,
+                         cand[(size_t)ci], 1, 0, st);
+        for (int rep = 0; rep < kRounds; ++rep)
+            for (int ci = 0; ci < ncand; ++ci) {
                 CUPHU_HIP_CHECK(hipEventRecord(e0, st));
-                launch_iq<T>(d_desc, d_blk, nblk, pl, d_scal, d_part, sh, 1, 0, st);
+                launch_iq<T>(d_desc[(size_t)ci], d_blk[(size_t)ci], nblk[(size_t)ci], pl, d_scal, d_part,
+                             cand[(size_t)ci], 1, 0, st);
                 CUPHU_HIP_CHECK(hipEventRecord(e1, st));
                 CUPHU_HIP_CHECK(hipEventSynchronize(e1));
                 float ms = 0.0f;
                 CUPHU_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-                ms_min = std::min(ms_min, ms);
+                ms_min[(size_t)ci] = std::min(ms_min[(size_t)ci], ms);
             }
-            if (o.verbose)
-                fprintf(stderr, "[cuphu-autotune] %s rows/lane group %d, lanes/row %d: %.4f ms\n", key.c_str(), sh.rpw,
-                        sh.lpr, ms_min);
-            if (ms_min < best_ms) { best_ms = ms_min; best = sh; }
-        }
     } catch (...) {
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
@@ -934,6 +944,24 @@ static TileShape autotune_shape(Layout L, const Planes<T> &pl, const CuPhuLaplac
     }
     CUPHU_HIP_CHECK(hipEventDestroy(e0));
     CUPHU_HIP_CHECK(hipEventDestroy(e1));
+    // The default shape is kept unless another candidate is at least
+    // kMargin faster: the candidates often differ by less than the timing's
+    // own spread, and a shape that only wins on noise can lose several per
+    // cent over the solve.
+    constexpr float kMargin = 0.03f;
+    const TileShape def = default_shape(L, o);
+    int ibest = -1, idef = -1;
+    for (int ci = 0; ci < ncand; ++ci) {
+        if (cand[(size_t)ci] == def) idef = ci;
+        if (ibest < 0 || ms_min[(size_t)ci] < ms_min[(size_t)ibest]) ibest = ci;
+    }
+    if (idef >= 0 && ms_min[(size_t)ibest] > (1.0f - kMargin) * ms_min[(size_t)idef]) ibest = idef;
+    const TileShape best = cand[(size_t)ibest];
+    if (o.verbose)
+        for (int ci = 0; ci < ncand; ++ci)
+            fprintf(stderr, "[cuphu-autotune] %s rows/lane group %d, lanes/row %d: %.4f ms%s\n", key.c_str(),
+                    cand[(size_t)ci].rpw, cand[(size_t)ci].lpr, ms_min[(size_t)ci],
+                    ci == ibest ? " (chosen)" : (ci == idef ? " (default)" : ""));
     cache[key] = best;
     return best;
 }

@@ -118,6 +118,18 @@ def test_graph_and_eager_launches_identical(gpu_ready, solver):
     assert s1[0]["iters"] == s2[0]["iters"] == s3[0]["iters"]
 
 
+def _check_choice(tuned):
+    """One candidate is chosen; a candidate other than the default shape only
+    when it is at least 3 % faster than the default."""
+    ms = {ln: float(ln.split(": ")[1].split(" ms")[0]) for ln in tuned}
+    chosen = [ln for ln in tuned if ln.endswith("(chosen)")]
+    assert len(chosen) == 1, tuned
+    assert ms[chosen[0]] == min(ms.values()) or all(not ln.endswith("(default)") for ln in tuned), tuned
+    default = [ln for ln in tuned if ln.endswith("(default)")]
+    if default:      # the default was timed but lost
+        assert ms[chosen[0]] <= 0.97 * ms[default[0]], tuned
+
+
 def test_autotuned_tile_shape_agrees_with_default(gpu_ready, capfd):
     scenes = [synth.make_scene(300, 700, nbumps=2, npatches=1, nlooks=10.0, seed=31, patch=40),
               synth.make_scene(90, 200, nbumps=1, npatches=0, nlooks=5.0, seed=32)]
@@ -131,6 +143,7 @@ def test_autotuned_tile_shape_agrees_with_default(gpu_ready, capfd):
     # a wider row stride is slow), each once
     assert sorted(ln.split("rows/lane group ")[1].split(":")[0] for ln in tuned) == [
         "2, lanes/row 64", "4, lanes/row 64", "8, lanes/row 64"], err[-2000:]
+    _check_choice(tuned)
     u2, c2, s2 = G.batch_solve(scenes, _capi.default_options(autotune=1, verbose=1, **opts))
     assert "[cuphu-autotune]" not in capfd.readouterr().err   # cached for the shape class
     for i, s in enumerate(scenes):
@@ -145,6 +158,7 @@ def test_autotuned_tile_shape_agrees_with_default(gpu_ready, capfd):
     tuned = [ln for ln in capfd.readouterr().err.splitlines() if ln.startswith("[cuphu-autotune]")]
     assert sorted(ln.split("rows/lane group ")[1].split(":")[0] for ln in tuned) == [
         "2, lanes/row 32", "2, lanes/row 64", "4, lanes/row 32", "4, lanes/row 64", "8, lanes/row 64"]
+    _check_choice(tuned)
 
 
 def test_conncomp_of_batched_output_matches_oracle(gpu_ready):
