@@ -10,7 +10,8 @@
 // and a per-instance reduction kernel) closed by a true-residual
 // replacement.  Each chunk is captured once in a HIP graph and replayed;
 // the host reads the per-instance scalars (a few hundred bytes) once per
-// chunk.  Converged instances are masked inside the kernels.
+// chunk, the fused solver with the next chunk already issued.  Converged
+// instances are masked inside the kernels.
 //
 // Plain solver: the reference's launch sequence for one instance at a
 // time (matvec, dot, axpy, axpy, vmul, dot, axpby with two blocking scalar
@@ -353,6 +354,7 @@ struct GroupDev {
     double4 *part4;   // fused solver's four partial sums per workgroup
     SolveScal *scal2;    // the folded iteration's second state buffer
     double4 *part4b;     // and its second partial-sum buffer
+    SolveScal *hstate[2] = {nullptr, nullptr};   // pinned host copies of the state (chunk pipelining), or null
     int ninst, nblk;
     std::vector<long long> px;   // host: logical pixels per instance
 };
@@ -361,6 +363,7 @@ struct GroupDev {
 struct KernelProf {
     bool on = false;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev2[4] = {nullptr, nullptr, nullptr, nullptr};   // the second chunk in flight (fused solver)
     CuPhuKernelProfile acc{};
 };
 static KernelProf g_prof;
@@ -542,6 +545,8 @@ static void solve_group_fused(const Planes<T> &pl, const GroupDev &g, const CuPh
     const bool prof = g_prof.on;
     if (prof && !g_prof.ev[0])
         for (auto &e : g_prof.ev) CUPHU_HIP_CHECK(hipEventCreate(&e));
+    if (prof && !g_prof.ev2[0])
+        for (auto &e : g_prof.ev2) CUPHU_HIP_CHECK(hipEventCreate(&e));
     const bool sq = fused_stored_q();
     CUPHU_REQUIRE(!sq || sh.lpr == 64, "the stored-q kernels run 256-column tiles only");
     const int rpw = sh.rpw;
@@ -610,7 +615,83 @@ static void solve_group_fused(const Planes<T> &pl, const GroupDev &g, const CuPh
         CUPHU_HIP_CHECK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
         CUPHU_HIP_CHECK(hipGraphDestroy(graph));
     }
+    // Chunk pipelining: with pinned state buffers, the next chunk is issued
+    // before the host has read the state after the current one, so the GPU
+    // does not wait for the host between chunks.  Once every instance has
+    // stopped, the chunk still in flight is a no-op (its kernels skip
+    // stopped instances and the reductions keep their states), so the
+    // results are those of the one-chunk-at-a-time loop.
+    const bool pipe = g.hstate[0] && g.hstate[1];
+    struct Done {   // waits for a chunk still in flight before its event and state buffer go
+        hipEvent_t e[2] = {nullptr, nullptr};
+        ~Done() {
+            for (auto &x : e)
+                if (x) {
+                    (void)hipEventSynchronize(x);
+                    (void)hipEventDestroy(x);
+                }
+        }
+    } done;
+    if (pipe)
+        for (auto &x : done.e) CUPHU_HIP_CHECK(hipEventCreateWithFlags(&x, hipEventDisableTiming));
     h.resize((size_t)g.ninst);
+    // Issues one chunk into slot `slot` (profiled pair first when profiling
+    // is on) and the copy of the state after it.
+    auto issue = [&](int slot) {
+        if (prof) {
+            hipEvent_t *ev = slot ? g_prof.ev2 : g_prof.ev;
+            CUPHU_HIP_CHECK(hipEventRecord(ev[0], st));
+            it(1, 0);
+            CUPHU_HIP_CHECK(hipEventRecord(ev[1], st));
+            launch_reduce_it(g.desc, g.ninst, g.part4, g.scal, RED_IT, o.max_iter, tol2, st);
+            CUPHU_HIP_CHECK(hipEventRecord(ev[2], st));
+            it(0, 0);
+            CUPHU_HIP_CHECK(hipEventRecord(ev[3], st));
+            launch_reduce_it(g.desc, g.ninst, g.part4, g.scal, RED_IT, o.max_iter, tol2, st);
+        }
+        if (gexec) CUPHU_HIP_CHECK(hipGraphLaunch(gexec, st));
+        else chunk();
+        if (pipe) {
+            CUPHU_HIP_CHECK(hipMemcpyAsync(g.hstate[slot], g.scal, (size_t)g.ninst * sizeof(SolveScal),
+                                           hipMemcpyDeviceToHost, st));
+            CUPHU_HIP_CHECK(hipEventRecord(done.e[slot], st));
+        } else {
+            CUPHU_HIP_CHECK(hipMemcpyAsync(h.data(), g.scal, h.size() * sizeof(SolveScal), hipMemcpyDeviceToHost, st));
+        }
+    };
+    // Waits for the chunk in `slot` and reads the state after it into h;
+    // `before` is the state it started from (the profiled pair's bytes).
+    auto collect = [&](int slot, const std::vector<SolveScal> &before, int pass) {
+        if (pipe) {
+            CUPHU_HIP_CHECK(hipEventSynchronize(done.e[slot]));
+            std::memcpy(h.data(), g.hstate[slot], h.size() * sizeof(SolveScal));
+        } else {
+            CUPHU_HIP_CHECK(hipStreamSynchronize(st));
+        }
+        if (o.verbose > 1 && o.rr_interval > 0)
+            for (int i = 0; i < g.ninst; ++i)
+                if (h[(size_t)i].active)
+                    fprintf(stderr, "[cuphu-rr] inst %d pass %d iters %d relres %.3e true/recursive r.z %.4g %s\n", i,
+                            pass, h[i].iters, h[i].relres, h[i].delta, h[i].replace ? "replaced" : "kept");
+        if (prof) {
+            // both launches of a deferral pair (the first defers the x
+            // update, the second applies two), on the instances active
+            // when the chunk started
+            hipEvent_t *ev = slot ? g_prof.ev2 : g_prof.ev;
+            double active_px = 0.0;
+            for (int i = 0; i < g.ninst; ++i)
+                if (before[(size_t)i].active) active_px += (double)g.px[(size_t)i];
+            float ms0 = 0.0f, ms1 = 0.0f;
+            CUPHU_HIP_CHECK(hipEventElapsedTime(&ms0, ev[0], ev[1]));
+            CUPHU_HIP_CHECK(hipEventElapsedTime(&ms1, ev[2], ev[3]));
+            CuPhuKernelProfile &a = g_prof.acc;
+            a.samples += 2;
+            a.px_sum += 2.0 * active_px;
+            a.it_ms_sum += (double)ms0 + (double)ms1;
+            a.it_bytes_sum += active_px * pair_values_per_px * sizeof(T);
+            a.iterations += CH;
+        }
+    };
     const int npass = fixed_weights ? 1 : o.n_irls + 1;
     try {
         for (int pass = 0; pass < npass; ++pass) {
@@ -628,50 +709,32 @@ static void solve_group_fused(const Planes<T> &pl, const GroupDev &g, const CuPh
             launch_reduce_it(g.desc, g.ninst, g.part4, g.scal, RED_START, o.max_iter, tol2, st, sq ? 0 : 1);
             CUPHU_HIP_CHECK(hipMemcpyAsync(h.data(), g.scal, h.size() * sizeof(SolveScal), hipMemcpyDeviceToHost, st));
             CUPHU_HIP_CHECK(hipStreamSynchronize(st));
-            int nch = 0;
             auto any_active = [&]() {
                 for (auto &s : h) if (s.active) return true;
                 return false;
             };
-            while (any_active()) {
-                double active_px = 0.0;
-                if (prof) {
-                    for (int i = 0; i < g.ninst; ++i)
-                        if (h[(size_t)i].active) active_px += (double)g.px[(size_t)i];
-                    CUPHU_HIP_CHECK(hipEventRecord(g_prof.ev[0], st));
-                    it(1, 0);
-                    CUPHU_HIP_CHECK(hipEventRecord(g_prof.ev[1], st));
-                    launch_reduce_it(g.desc, g.ninst, g.part4, g.scal, RED_IT, o.max_iter, tol2, st);
-                    CUPHU_HIP_CHECK(hipEventRecord(g_prof.ev[2], st));
-                    it(0, 0);
-                    CUPHU_HIP_CHECK(hipEventRecord(g_prof.ev[3], st));
-                    launch_reduce_it(g.desc, g.ninst, g.part4, g.scal, RED_IT, o.max_iter, tol2, st);
+            // chunks issued and collected; `before` is the state the chunk
+            // being collected started from
+            int issued = 0, collected = 0;
+            std::vector<SolveScal> before = h;
+            auto can_issue = [&]() { return (long long)issued * CH <= (long long)o.max_iter + CH; };
+            if (any_active()) {
+                issue(issued & 1);
+                ++issued;
+            }
+            while (collected < issued) {
+                if (pipe && can_issue()) {   // the next chunk, before this one's state is known
+                    issue(issued & 1);
+                    ++issued;
                 }
-                if (gexec) CUPHU_HIP_CHECK(hipGraphLaunch(gexec, st));
-                else chunk();
-                CUPHU_HIP_CHECK(hipMemcpyAsync(h.data(), g.scal, h.size() * sizeof(SolveScal), hipMemcpyDeviceToHost, st));
-                CUPHU_HIP_CHECK(hipStreamSynchronize(st));
-                if (o.verbose > 1 && o.rr_interval > 0)
-                    for (int i = 0; i < g.ninst; ++i)
-                        if (h[(size_t)i].active)
-                            fprintf(stderr, "[cuphu-rr] inst %d pass %d iters %d relres %.3e true/recursive r.z %.4g %s\n",
-                                    i, pass, h[i].iters, h[i].relres, h[i].delta,
-                                    h[i].replace ? "replaced" : "kept");
-                if (prof) {
-                    // both launches of a deferral pair (the first defers the
-                    // x update, the second applies two)
-                    float ms0 = 0.0f, ms1 = 0.0f;
-                    CUPHU_HIP_CHECK(hipEventElapsedTime(&ms0, g_prof.ev[0], g_prof.ev[1]));
-                    CUPHU_HIP_CHECK(hipEventElapsedTime(&ms1, g_prof.ev[2], g_prof.ev[3]));
-                    CuPhuKernelProfile &a = g_prof.acc;
-                    a.samples += 2;
-                    a.px_sum += 2.0 * active_px;
-                    a.it_ms_sum += (double)ms0 + (double)ms1;
-                    a.it_bytes_sum += active_px * pair_values_per_px * sizeof(T);
-                    a.iterations += CH;
+                collect(collected & 1, before, pass);
+                ++collected;
+                before = h;
+                if (!any_active()) break;   // a chunk still in flight finds every instance stopped: a no-op
+                if (!pipe && can_issue()) {
+                    issue(issued & 1);
+                    ++issued;
                 }
-                ++nch;
-                if ((long long)nch * CH > (long long)o.max_iter + CH) break;   // cannot happen: reduce caps iterations
             }
             if (!sq) launch_x_finalize<T>(g.desc, g.blk, g.nblk, pl, g.scal, sh, st);
             if (o.verbose)
@@ -1056,6 +1119,10 @@ static void run_batch(const std::vector<std::pair<int, int>> &dims, const CuPhuL
     g.part4 = ws<double4>(res, "grppart4", maxblk);
     g.scal2 = ws<SolveScal>(res, "grpscal2", maxinst);
     g.part4b = ws<double4>(res, "grppart4b", maxblk);
+    if (!res.arena) {   // pinned state copies for the pipelined chunk loop (none in the no-allocation path)
+        g.hstate[0] = (SolveScal *)res.pin("grpstate0", maxinst * sizeof(SolveScal));
+        g.hstate[1] = (SolveScal *)res.pin("grpstate1", maxinst * sizeof(SolveScal));
+    }
     std::vector<SolveScal> hs;
     std::vector<CuPhuSolveStats> dummy((size_t)ninst);
     auto sp = [&](int i) { return stats ? stats + i : &dummy[(size_t)i]; };
@@ -1426,6 +1493,8 @@ static void wls_impl(const T *d_w, const float *d_phase, const T *d_u, const T *
     g.part4 = ws<double4>(res, "grppart4", (size_t)nb);
     g.scal2 = ws<SolveScal>(res, "grpscal2", 1);
     g.part4b = ws<double4>(res, "grppart4b", (size_t)nb);
+    g.hstate[0] = (SolveScal *)res.pin("grpstate0", sizeof(SolveScal));
+    g.hstate[1] = (SolveScal *)res.pin("grpstate1", sizeof(SolveScal));
     g.ninst = 1;
     g.nblk = nb;
     g.px = {(long long)nrow * ncol};

@@ -108,3 +108,54 @@ def test_gpu_info(gpu_ready):
     info = cuphu.gpu_info(0)
     assert "name" in info and "total_memory" in info and info["total_memory"] > 0
     assert cuphu.gpu_count() >= 1 and isinstance(cuphu.gpu_name(0), str)
+
+
+def test_unwrap_with_mask(gpu_ready, small_igram, small_corr):
+    """test_unwrap.py:129-138: masked pixels get conncomp label 0."""
+    mask = np.ones((50, 60), dtype=np.uint8)
+    mask[20:30, 20:40] = 0
+    unw, cc = cuphu.unwrap(small_igram, small_corr, nlooks=10.0, init="laplace", mask=mask)
+    assert unw.shape == (50, 60)
+    assert (cc[20:30, 20:40] == 0).all()
+    assert (cc[mask != 0] == 1).all()
+
+
+def test_unwrap_cost_modes(gpu_ready, small_igram, small_corr):
+    """test_unwrap.py:141-147 with init='laplace'; 'topo' runs the defo
+    kernels in the reference (cuphu_solver.cpp:441-450), so its result equals
+    'defo' bit for bit."""
+    out = {}
+    for mode in ("smooth", "defo", "topo"):
+        out[mode], _ = cuphu.unwrap(small_igram, small_corr, nlooks=10.0, cost=mode, init="laplace")
+        assert np.isfinite(out[mode]).all()
+    np.testing.assert_array_equal(out["topo"], out["defo"])
+
+
+def test_reopt_single_tile_is_noop(gpu_ready, small_igram, small_corr):
+    """test_unwrap.py:243-254 with init='laplace': single_tile_reoptimize has
+    no effect on a (1, 1) tiling."""
+    u_off, c_off = cuphu.unwrap(small_igram, small_corr, nlooks=10.0, init="laplace", ntiles=(1, 1),
+                                single_tile_reoptimize=False)
+    u_on, c_on = cuphu.unwrap(small_igram, small_corr, nlooks=10.0, init="laplace", ntiles=(1, 1),
+                              single_tile_reoptimize=True)
+    np.testing.assert_array_equal(u_off, u_on)
+    np.testing.assert_array_equal(c_off, c_on)
+
+
+def test_reopt_default_off_unchanged(gpu_ready, small_igram, small_corr):
+    """test_unwrap.py:274-286 with init='laplace': omitting
+    single_tile_reoptimize equals passing False."""
+    u_d, c_d = cuphu.unwrap(small_igram, small_corr, nlooks=10.0, init="laplace", ntiles=(2, 2), tile_overlap=4)
+    u_e, c_e = cuphu.unwrap(small_igram, small_corr, nlooks=10.0, init="laplace", ntiles=(2, 2), tile_overlap=4,
+                            single_tile_reoptimize=False)
+    np.testing.assert_array_equal(u_d, u_e)
+    np.testing.assert_array_equal(c_d, c_e)
+
+
+def test_reopt_tiled_is_unsupported(gpu_ready, small_igram, small_corr):
+    """Tiled single_tile_reoptimize is a full-scene CPU TreeSolve in the
+    reference (SNAPHU), which this build does not have: it raises instead
+    of returning an unrefined result."""
+    with pytest.raises(ValueError, match="single_tile_reoptimize"):
+        cuphu.unwrap(small_igram, small_corr, nlooks=10.0, init="laplace", ntiles=(2, 2), tile_overlap=4,
+                     single_tile_reoptimize=True)
