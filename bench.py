@@ -283,9 +283,9 @@ def main() -> int:
 
     if not args.no_extra:
         out["python_api_scene8192"] = leg_python_api(scene, base_opts, log)
-        out["config2_scene2048"] = leg_single(torch, lib, _capi, synth, base_opts, log)
-        out["config4_batch64"] = leg_batch64(torch, lib, _capi, synth, base_opts, args.plain_sample, log)
-        out["config5_autograd1024"] = leg_autograd(torch, synth, log)
+        out["single_scene2048"] = leg_single(torch, lib, _capi, synth, base_opts, log)
+        out["batch64_uneven"] = leg_batch64(torch, lib, _capi, synth, base_opts, args.plain_sample, log)
+        out["differentiable_scene1024"] = leg_autograd(torch, synth, log)
 
     print(json.dumps(out), flush=True)
     return 0
@@ -605,7 +605,7 @@ def leg_autograd(torch, synth, log):
                      "tol 1e-6",
          "seconds_per_step": round(t, 4), "mpix_s": round(nrow * ncol / t / 1e6, 3), "loss": float(loss.detach()),
          "grads": [float(a.grad), float(b.grad), float(eps.grad)]}
-    log(f"config5: {r['seconds_per_step'] * 1e3:.1f} ms per forward+backward step ({r['mpix_s']} Mpx/s)")
+    log(f"differentiable 1024^2: {r['seconds_per_step'] * 1e3:.1f} ms per forward+backward step ({r['mpix_s']} Mpx/s)")
     return r
 
 
@@ -626,7 +626,7 @@ def leg_single(torch, lib, _capi, synth, base_opts, log):
          "plain_mpix_s": round(npx / tp / 1e6, 3), "plain_seconds": round(tp, 3),
          "fused_vs_plain": round(tp / t, 3), "ambiguity_agreement_with_plain": _amb(torch, u_f, insts_p[0][4], s.igram),
          "pcg_iters": st[0]["iters"], "relres": [float(x) for x in st[0]["relres"]]}
-    log(f"config2: fused {r['mpix_s']} Mpx/s, plain {r['plain_mpix_s']} Mpx/s")
+    log(f"single 2048^2: fused {r['mpix_s']} Mpx/s, plain {r['plain_mpix_s']} Mpx/s")
     del keep
     return r
 
@@ -637,7 +637,7 @@ def leg_batch64(torch, lib, _capi, synth, base_opts, nsample, log):
     on the first nsample instances)."""
     t0 = time.perf_counter()
     scenes = synth.batch_instances(64)
-    log(f"config4: generated 64 instances in {time.perf_counter() - t0:.1f}s")
+    log(f"batch of 64: generated 64 instances in {time.perf_counter() - t0:.1f}s")
     npx = sum(s.corr.size for s in scenes)
     keep, insts = _upload(torch, scenes)
     opts = _capi.default_options(**base_opts)
@@ -669,7 +669,7 @@ def leg_batch64(torch, lib, _capi, synth, base_opts, nsample, log):
          "sample_ambiguity_agreement_with_fp64": {"batched": ag64, "plain": ag64p},
          "pcg_iters_total": int(sum(sum(x["iters"]) for x in st)),
          "convergence": convergence_summary(st, opts.max_iter)}
-    log(f"config4: batched {r['mpix_s']} Mpx/s, plain sample {r['plain_sample_mpix_s']} Mpx/s")
+    log(f"batch of 64: batched {r['mpix_s']} Mpx/s, plain sample {r['plain_sample_mpix_s']} Mpx/s")
     del keep
     return r
 
