@@ -515,6 +515,10 @@ static bool wide_stride(const CuPhuLaplaceOptions &o) {
     return v != 64 && o.solver == CUPHU_SOLVER_FUSED && !fused_stored_q();
 }
 
+// Chunk pipelining (solve_group_fused) for grids of at most this many
+// workgroups (16 M pixels).
+constexpr int kPipeMaxBlocks = 4096;
+
 // Folded reductions (launch_iq_fold) for grids of at most this many
 // workgroups, where the reduction launch it replaces is a large part of an
 // iteration and every workgroup's extra read of its instance's partial sums
@@ -620,8 +624,12 @@ static void solve_group_fused(const Planes<T> &pl, const GroupDev &g, const CuPh
     // does not wait for the host between chunks.  Once every instance has
     // stopped, the chunk still in flight is a no-op (its kernels skip
     // stopped instances and the reductions keep their states), so the
-    // results are those of the one-chunk-at-a-time loop.
-    const bool pipe = g.hstate[0] && g.hstate[1];
+    // results are those of the one-chunk-at-a-time loop.  Only on grids of
+    // at most kPipeMaxBlocks workgroups, where a chunk lasts a few ms or
+    // less and the host's turnaround is a visible part of it (1024^2:
+    // 14.7 -> 15.2 Mpixel/s); a 50-iteration chunk of the 8192^2 scene
+    // lasts 22 ms, and there the no-op chunk would only add launches.
+    const bool pipe = g.hstate[0] && g.hstate[1] && g.nblk <= kPipeMaxBlocks;
     struct Done {   // waits for a chunk still in flight before its event and state buffer go
         hipEvent_t e[2] = {nullptr, nullptr};
         ~Done() {
