@@ -595,7 +595,7 @@ def leg_autograd(torch, synth, log):
 
     step()
     torch.cuda.synchronize()
-    n = 3
+    n = 5
     t0 = time.perf_counter()
     for _ in range(n):
         loss = step()

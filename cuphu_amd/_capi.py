@@ -150,7 +150,6 @@ _SIGS = {
                                    C.POINTER(_d), _P]),
     "cuphu_set_profiling": (_i, [_i]),
     "cuphu_get_profile": (_i, [C.POINTER(CuPhuKernelProfile)]),
-    "cuphu_test_recip_f32": (_i, [C.c_uint, C.c_uint, C.POINTER(C.c_ulonglong)]),
     "cuphu_test_ingest": (_i, [_P, _P, _P, _i, _i, C.POINTER(CuPhuParams), _P, _P]),
     "cuphu_test_laplace_ops": (_i, [_P, _P, _P, _i, _i, C.c_float, _P, _P, _P, _P, _P, _P, _P]),
     "cuphu_test_pcg_step": (_i, [_P, _P, _i, _i, _d, _d, _i, _P, _P, _P, _P, C.POINTER(_d), C.POINTER(_d),

@@ -157,23 +157,9 @@ __device__ __forceinline__ float phase_of(float re, float im) {
     if (a < 0.0f) a += 6.28318530717958648f;
     return wrap_to_2pi(a);
 }
-// 1/d in FP32, correctly rounded.  On [2^-46, 2^46) one FMA Newton step
-// from the hardware reciprocal (v_rcp_f32, within about 1 ulp) gives the
-// correctly rounded value for every float: all 771,751,936 of them were
-// checked against IEEE division on gfx950 (test hook cuphu_test_recip_f32).
-// Elsewhere (and for non-finite or negative d) it is IEEE division.  Three
-// instructions instead of the ten of the general division sequence.
-__device__ __forceinline__ float recip_f(float d) {
-    if (d >= 0x1p-46f && d < 0x1p46f) {
-        const float r = __builtin_amdgcn_rcpf(d);
-        const float e = __builtin_fmaf(-d, r, 1.0f);
-        return __builtin_fmaf(e, r, r);
-    }
-    return 1.0f / d;
-}
 __device__ __forceinline__ float arc_weight_f(short s) {
     if (s <= 0 || s >= kLargeShort) return 0.0f;
-    return recip_f((float)s);
+    return 1.0f / (float)s;
 }
 __device__ __forceinline__ double arc_weight_d(short s) {
     if (s <= 0 || s >= kLargeShort) return 0.0;
@@ -189,7 +175,6 @@ template <> struct Prec<float> {
     static __device__ __forceinline__ float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
     static __device__ __forceinline__ float wrap(float x) { return wrap_pi(x); }
     static __device__ __forceinline__ float weight(short s) { return arc_weight_f(s); }
-    static __device__ __forceinline__ float recip(float x) { return recip_f(x); }
     static __device__ __forceinline__ float sqrtT(float x) { return __builtin_sqrtf(x); }
     static constexpr float eps = kEpsF;
 };
@@ -198,7 +183,6 @@ template <> struct Prec<double> {
     static __device__ __forceinline__ double fmaT(double a, double b, double c) { return __builtin_fma(a, b, c); }
     static __device__ __forceinline__ double wrap(double x) { return wrap_pi_d(x); }
     static __device__ __forceinline__ double weight(short s) { return arc_weight_d(s); }
-    static __device__ __forceinline__ double recip(double x) { return 1.0 / x; }
     static __device__ __forceinline__ double sqrtT(double x) { return __builtin_sqrt(x); }
     static constexpr double eps = kEpsD;
 };

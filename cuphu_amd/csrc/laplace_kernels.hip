@@ -484,7 +484,7 @@ __global__ __launch_bounds__(256) void k_assemble(const InstDesc *descs, const i
             if (wr[j] > T(0)) { bv = Prec<T>::fmaT(-wr[j], Prec<T>::wrap((T)pc[j + 1] - pa), bv); ws += wr[j]; }
             if (wl[j] > T(0)) { bv = Prec<T>::fmaT(wl[j], Prec<T>::wrap(pa - (T)pc[j - 1]), bv); ws += wl[j]; }
             b[j] = bv;
-            di[j] = Prec<T>::recip(ws + Prec<T>::eps);
+            di[j] = T(1) / (ws + Prec<T>::eps);
             rr[j] = bv - y[j];
             z4[j] = T(0);
         }
@@ -1012,7 +1012,7 @@ __global__ __launch_bounds__(256) void k_it(const InstDesc *descs, const int *bl
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             T ws = ((wd[k][j] + wuk[j]) + wr[k][j]) + wl[k][j];
-            di[k][j] = Prec<T>::recip(ws + Prec<T>::eps);
+            di[k][j] = T(1) / (ws + Prec<T>::eps);
         }
     }
     // own pixels: updates, stores, LDS
@@ -1169,7 +1169,7 @@ __device__ __forceinline__ T stencil1(T a, T dn, T up, T rt, T lf, T wd, T wu, T
 }
 template <class T>
 __device__ __forceinline__ T diag_inv(T wd, T wu, T wr, T wl) {
-    return Prec<T>::recip((((wd + wu) + wr) + wl) + Prec<T>::eps);
+    return T(1) / ((((wd + wu) + wr) + wl) + Prec<T>::eps);
 }
 
 // Workgroups that start together on one CU run identical instruction streams

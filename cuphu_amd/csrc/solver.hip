@@ -2478,39 +2478,6 @@ void from_plane(const float *dev, float *host, const InstDesc &d, hipStream_t st
 }
 }  // namespace
 
-}  // extern "C"
-
-// recip_f against IEEE division on the float bit patterns first..first+count-1
-// (results that are both NaN count as equal).
-__global__ void k_recip_check(unsigned first, unsigned long long count, unsigned long long *bad) {
-    unsigned long long local = 0;
-    const unsigned long long step = (unsigned long long)gridDim.x * 256;
-    for (unsigned long long i = (unsigned long long)blockIdx.x * 256 + threadIdx.x; i < count; i += step) {
-        const float x = __uint_as_float(first + (unsigned)i);
-        const float a = recip_f(x), b = 1.0f / x;
-        if (__float_as_uint(a) != __float_as_uint(b) && !(a != a && b != b)) ++local;
-    }
-    if (local) atomicAdd(bad, local);
-}
-
-extern "C" {
-
-int cuphu_test_recip_f32(unsigned first_bits, unsigned count, unsigned long long *mismatches) {
-    return guarded([&]() {
-        std::lock_guard<std::recursive_mutex> lk(g_mu);
-        CUPHU_REQUIRE(mismatches && (unsigned long long)first_bits + count <= (1ULL << 32), "invalid argument");
-        DevRes &res = devres();
-        hipStream_t st = res.stream;
-        unsigned long long *d_bad = ws<unsigned long long>(res, "recipbad", 1);
-        CUPHU_HIP_CHECK(hipMemsetAsync(d_bad, 0, sizeof(unsigned long long), st));
-        hipLaunchKernelGGL(k_recip_check, dim3(4096), dim3(256), 0, st, first_bits, (unsigned long long)count, d_bad);
-        CUPHU_HIP_CHECK(hipGetLastError());
-        CUPHU_HIP_CHECK(hipMemcpyAsync(mismatches, d_bad, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
-        CUPHU_HIP_CHECK(hipStreamSynchronize(st));
-        return 0;
-    });
-}
-
 int cuphu_test_ingest(const float *igram_c64, const float *corr, const unsigned char *mask, int nrow, int ncol,
                       const CuPhuParams *params, float *phase_out, short *sigsq_out) {
     return guarded([&]() {

@@ -545,12 +545,6 @@ int cuphu_test_ingest(const float *igram_c64, const float *corr, const unsigned 
                       int nrow, int ncol, const CuPhuParams *params,
                       float *phase_out, short *sigsq_out);
 
-/* The library's FP32 reciprocal (one FMA Newton step from the hardware
- * reciprocal on [2^-46, 2^46), IEEE division elsewhere) against IEEE
- * division, on the float bit patterns first_bits .. first_bits + count - 1
- * (first_bits + count <= 2^32); *mismatches = the number that differ. */
-int cuphu_test_recip_f32(unsigned first_bits, unsigned count, unsigned long long *mismatches);
-
 /* Laplace operator kernels.  Inputs: phase, flat sigsq [narcs], x, delta.
  * Outputs (each nullable): w0 flat [narcs] (1/sigsq), b, dinv, r = b - L x
  * (assemble kernel, LDS stencil), lx = L x (plain global-gather stencil),

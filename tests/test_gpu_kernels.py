@@ -282,19 +282,3 @@ def test_integrate_phase(gpu_ready):
     ref = O.integrate_phase(phi, hfl, vfl)
     assert _ulps(got, ref).max() <= 1
     np.testing.assert_allclose(got, u - u[0, 0] + p64[0, 0], atol=1e-2)
-
-
-def test_fp32_reciprocal_is_correctly_rounded(gpu_ready):
-    """The kernels' FP32 reciprocal (1/diag, 1/sigsq) is one FMA Newton step
-    from the hardware reciprocal on [2^-46, 2^46): it equals IEEE division on
-    every one of those floats, and outside that range (zero, subnormals,
-    huge, infinite, NaN, negative) it falls back to IEEE division."""
-    def mismatches(first, count):
-        n = C.c_ulonglong(0)
-        _capi.check(_capi.lib.cuphu_test_recip_f32(first, count, C.byref(n)), "recip")
-        return n.value
-    lo, hi = (127 - 46) << 23, (127 + 46) << 23
-    assert mismatches(lo, hi - lo) == 0
-    assert mismatches(0, lo) == 0                          # +0, subnormals, small normals
-    assert mismatches(hi, (1 << 31) - hi) == 0             # large normals, +inf, NaN
-    assert mismatches(1 << 31, 1 << 31) == 0               # the negative half
