@@ -41,6 +41,14 @@
  *                     (constant offset, local gradient, sparse boundary,
  *                     far-from-data fade, single tile); hand-evaluated
  *                     profile (moving median, gap fill, feather)
+ *   o_pcg_step_f32 / o_pcg_fused_step_f32  textbook dense float64 Jacobi-PCG
+ *   o_incrcost_smooth hand-evaluated cases, exact integer arithmetic
+ *   o_conncomp_gated  scipy.sparse.csgraph on the gated arc set
+ *   o_bridge*         reference test_bridge.py cases; scipy.ndimage label
+ *                     (8-connected), binary_erosion, maximum/minimum_filter;
+ *                     numpy brute force for the nearest boundary pairs
+ * The outputs of one small seeded scene are also frozen in
+ * tests/golden/scene64x80.npz, which the oracle must keep reproducing.
  */
 #include "cuphu_oracle.h"
 
