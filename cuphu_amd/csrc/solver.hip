@@ -2051,59 +2051,63 @@ int cuphu_unwrap_ex(const float *igram_r, const float *igram_i, const float *igr
             return (int)CUPHU_ERR_NOMEM;
         }
         try {
-            DevBuf<float> d_corr(n), d_unw(n);
-            DevBuf<float2> d_ig(n);
-            DevBuf<uint32_t> d_cc(n);
-            DevBuf<unsigned char> d_mask(mask ? n : 0), d_orig(orig_mask ? n : 0);
+            // the device copies of the caller's arrays live in the per-device
+            // workspace, like the solver's planes, so repeated calls do not
+            // allocate and free 20 B per pixel each time
+            float *d_corr = ws<float>(res, "ex_corr", n), *d_unw = ws<float>(res, "ex_unw", n);
+            float2 *d_ig = ws<float2>(res, "ex_ig", n);
+            uint32_t *d_cc = ws<uint32_t>(res, "ex_cc", n);
+            unsigned char *d_mask = mask ? ws<unsigned char>(res, "ex_mask", n) : nullptr;
+            unsigned char *d_orig = orig_mask ? ws<unsigned char>(res, "ex_orig", n) : nullptr;
             if (igram_c64) {
-                copy_h2d_staged(d_ig.get(), igram_c64, n * 8, res, st);
+                copy_h2d_staged(d_ig, igram_c64, n * 8, res, st);
             } else {
                 DevBuf<float> d_re(n), d_im(n);
                 copy_h2d_staged(d_re.get(), igram_r, n * 4, res, st);
                 copy_h2d_staged(d_im.get(), igram_i, n * 4, res, st);
                 hipLaunchKernelGGL(k_interleave, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st, d_re.get(),
-                                   d_im.get(), d_ig.get(), n);
+                                   d_im.get(), d_ig, n);
                 CUPHU_HIP_CHECK(hipGetLastError());
                 CUPHU_HIP_CHECK(hipStreamSynchronize(st));
             }
-            copy_h2d_staged(d_corr.get(), corr, n * 4, res, st);
-            if (mask) copy_h2d_staged(d_mask.get(), mask, n, res, st);
-            if (orig_mask) copy_h2d_staged(d_orig.get(), orig_mask, n, res, st);
-            hipLaunchKernelGGL(k_scrub_inputs, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st, d_ig.get(),
-                               d_corr.get(), n);
+            copy_h2d_staged(d_corr, corr, n * 4, res, st);
+            if (mask) copy_h2d_staged(d_mask, mask, n, res, st);
+            if (orig_mask) copy_h2d_staged(d_orig, orig_mask, n, res, st);
+            hipLaunchKernelGGL(k_scrub_inputs, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st, d_ig,
+                               d_corr, n);
             CUPHU_HIP_CHECK(hipGetLastError());
             // tile-stitching confidence mask: orig_mask, else mask (cuphu_solver.cpp:1696)
-            const unsigned char *conf = orig_mask ? d_orig.get() : nullptr;
+            const unsigned char *conf = orig_mask ? d_orig : nullptr;
             int single = (tile->ntilerow == 1 && tile->ntilecol == 1);
             int rov = single ? 0 : tile->rowovrlp, cov = single ? 0 : tile->colovrlp;
             if (mcf && single)
-                mcf_tile(d_ig.get(), d_corr.get(), mask ? d_mask.get() : nullptr, nrow, ncol, cost_mode, init_meth,
-                         *params, d_unw.get(), d_cc.get(), res, st);
+                mcf_tile(d_ig, d_corr, mask ? d_mask : nullptr, nrow, ncol, cost_mode, init_meth,
+                         *params, d_unw, d_cc, res, st);
             else if (mcf)
-                mcf_scene(d_ig.get(), d_corr.get(), mask ? d_mask.get() : nullptr, conf, nrow, ncol, rov, cov,
-                          tile->ntilerow, tile->ntilecol, cost_mode, init_meth, *params, d_unw.get(), d_cc.get(), res,
+                mcf_scene(d_ig, d_corr, mask ? d_mask : nullptr, conf, nrow, ncol, rov, cov,
+                          tile->ntilerow, tile->ntilecol, cost_mode, init_meth, *params, d_unw, d_cc, res,
                           st);
             else if (o.precision == 64)
-                scene_impl<double>(d_ig.get(), d_corr.get(), mask ? d_mask.get() : nullptr, conf, nrow, ncol,
+                scene_impl<double>(d_ig, d_corr, mask ? d_mask : nullptr, conf, nrow, ncol,
                                    params->ncorrlooks, *params, 0, rov, cov, tile->ntilerow, tile->ntilecol, o,
-                                   d_unw.get(), d_cc.get(), stats, nullptr, res, st);
+                                   d_unw, d_cc, stats, nullptr, res, st);
             else
-                scene_impl<float>(d_ig.get(), d_corr.get(), mask ? d_mask.get() : nullptr, conf, nrow, ncol,
+                scene_impl<float>(d_ig, d_corr, mask ? d_mask : nullptr, conf, nrow, ncol,
                                   params->ncorrlooks, *params, 0, rov, cov, tile->ntilerow, tile->ntilecol, o,
-                                  d_unw.get(), d_cc.get(), stats, nullptr, res, st);
+                                  d_unw, d_cc, stats, nullptr, res, st);
             // optional post-passes in the reference's order (cuphu_solver.cpp:2037-2095); like the
             // reference, the single-tile path skips both (it returns before them, :1714-1723)
             if (!mcf && tile->laplace_neighbor_feedback && !single)
-                neighbor_feedback_dev(d_unw.get(), mask ? d_mask.get() : nullptr, nrow, ncol,
+                neighbor_feedback_dev(d_unw, mask ? d_mask : nullptr, nrow, ncol,
                                       seams(ncol, cov, tile->ntilecol), seams(nrow, rov, tile->ntilerow),
                                       tile->laplace_neighbor_feedback_feather, st);
             if (tile->fix_cycle_spikes && !single)
-                fix_cycle_spikes_dev(d_unw.get(), mask ? d_mask.get() : nullptr, nrow, ncol, st);
+                fix_cycle_spikes_dev(d_unw, mask ? d_mask : nullptr, nrow, ncol, st);
             // phase bridging, on single tiles too (:1721-1722, :2097-2098)
             if (bridge && bridge->enabled)
-                bridge_dev(d_unw.get(), d_cc.get(), mask ? d_mask.get() : nullptr, nrow, ncol, *bridge, st);
-            copy_d2h_staged(result->unw, d_unw.get(), n * 4, res, st);
-            copy_d2h_staged(result->conncomp, d_cc.get(), n * 4, res, st);
+                bridge_dev(d_unw, d_cc, mask ? d_mask : nullptr, nrow, ncol, *bridge, st);
+            copy_d2h_staged(result->unw, d_unw, n * 4, res, st);
+            copy_d2h_staged(result->conncomp, d_cc, n * 4, res, st);
         } catch (...) {
             std::free(result->unw); std::free(result->conncomp);
             result->unw = nullptr; result->conncomp = nullptr;
