@@ -31,7 +31,11 @@ Reported beside the headline Mpixel/s:
 * cpu_baseline: the FP64 CPU oracle (same algorithm and schedule, OpenMP
   over every host core this process may use) on the 2048x2048 tile at the
   origin, converted to the scene-pixel rate, and on one thread on a
-  256x256 crop.
+  256x256 crop;
+* next_stages_scene8192: the stages either side of the path at the same
+  size -- the three post-passes (cycle spikes, tile-seam feedback,
+  bridging) switched on in cuphu_amd.unwrap against off, and the MCF-path
+  incremental costs over every arc.
 
 Usage: python bench.py [--gpus 1] [--steps 2] [--warmup 1]
 """
@@ -94,6 +98,8 @@ def main() -> int:
     ap.add_argument("--calib-copy", action="store_true")
     ap.add_argument("--solver", default="fused", choices=["fused", "batched"])
     ap.add_argument("--no-extra", action="store_true", help="skip the 2048^2 and 64-interferogram legs")
+    ap.add_argument("--legs", default="api,next,single,batch64,autograd",
+                    help="comma list of the extra legs to run (api, next, single, batch64, autograd)")
     ap.add_argument("--plain-sample", type=int, default=8,
                     help="instances of the 64-interferogram batch timed with the plain loop")
     args = ap.parse_args()
@@ -281,10 +287,17 @@ def main() -> int:
     if not args.no_cpu:
         out["cpu_baseline"] = leg_cpu(scene, args, scene_tile_px(scene.corr.shape, tile, ovl, (ntr, ntc)), npx, log)
 
-    if not args.no_extra:
-        out["python_api_scene8192"] = leg_python_api(scene, base_opts, log)
+    legs = set(args.legs.split(",")) if not args.no_extra else set()
+    if args.workload == "scene8192":
+        if "api" in legs:
+            out["python_api_scene8192"] = leg_python_api(scene, base_opts, log)
+        if "next" in legs:
+            out["next_stages_scene8192"] = leg_next_stages(scene, base_opts, log)
+    if "single" in legs:
         out["single_scene2048"] = leg_single(torch, lib, _capi, synth, base_opts, log)
+    if "batch64" in legs:
         out["batch64_uneven"] = leg_batch64(torch, lib, _capi, synth, base_opts, args.plain_sample, log)
+    if "autograd" in legs:
         out["differentiable_scene1024"] = leg_autograd(torch, synth, log)
 
     print(json.dumps(out), flush=True)
@@ -563,6 +576,87 @@ def leg_python_api(scene, base_opts, log):
          "ncomponents": int(cc.max())}
     log(f"python api: {r['mpix_s']} Mpx/s ({t:.2f} s; device-resident same tiling {t_dev:.2f} s)")
     return r
+
+
+def leg_next_stages(scene, base_opts, log):
+    """The stages either side of the PCG path, at the headline size.
+
+    * post-passes: cuphu_amd.unwrap as in the python-api leg (reference tile
+      formula, same PCG/IRLS options, workspace warm) with
+      fix_cycle_spikes, laplace_neighbor_feedback and bridge switched on,
+      against the same call with them off; the calls alternate off/on three
+      times and the median of each is kept (reference hooks:
+      cuphu_solver.cpp fix_cycle_spikes / laplace_neighbor_feedback /
+      bridge);
+    * MCF-path incremental costs: cuphu_incrcost_gpu (ref
+      cuphu_incrcost.cu:55-191) over every arc of the scene's smooth costs
+      (cuphu_build_smooth_costs_gpu) with flows in {-1, 0, 1}; 20 calls
+      timed on the host (each call ends with the stream sync that reads its
+      two minima back, as the reference's early-exit flag needs).  Per arc
+      the kernel reads the cost pair (4 B) and the flow (2 B) and writes
+      both incremental costs (4 B): 10 B."""
+    import ctypes as C
+
+    import torch
+
+    import cuphu_amd as cuphu
+    from cuphu_amd import _capi
+
+    lo = {k: v for k, v in base_opts.items() if k != "group_bytes_mb"}
+    kw = dict(cost="defo", init="laplace", target_tile_size=2048, tile_overlap=256, laplace_options=lo)
+    post = dict(fix_cycle_spikes=True, laplace_neighbor_feedback=True, bridge=True)
+    cuphu.unwrap(scene.igram, scene.corr, scene.nlooks, **kw, **post)   # warm both workspaces
+    times = {"off": [], "on": []}
+    for _ in range(3):
+        for mode in ("off", "on"):
+            t0 = time.perf_counter()
+            cuphu.unwrap(scene.igram, scene.corr, scene.nlooks, **kw, **(post if mode == "on" else {}))
+            times[mode].append(time.perf_counter() - t0)
+    t_off, t_on = float(np.median(times["off"])), float(np.median(times["on"]))
+    log(f"post-passes: {t_on:.3f} s on, {t_off:.3f} s off ({1e3 * (t_on - t_off):.0f} ms for the three passes)")
+
+    nrow, ncol = scene.corr.shape
+    narcs = (nrow - 1) * ncol + nrow * (ncol - 1)
+    ph = torch.from_numpy(np.angle(scene.igram).astype(np.float32)).to("cuda")
+    dcorr = torch.from_numpy(scene.corr).to("cuda")
+    prm = _capi.default_params()
+    prm.nlooks = prm.ncorrlooks = scene.nlooks
+    st = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    dcost = C.c_void_p()
+    _capi.check(_capi.lib.cuphu_build_smooth_costs_gpu(C.c_void_p(ph.data_ptr()), C.c_void_p(dcorr.data_ptr()),
+                                                       None, None, nrow, ncol, C.byref(prm), 7, 7, C.byref(dcost),
+                                                       st), "smooth costs")
+    g = torch.Generator(device="cuda").manual_seed(5)
+    flows = torch.randint(-1, 2, (narcs,), dtype=torch.int16, device="cuda", generator=g)
+    pos = torch.empty(narcs, dtype=torch.int16, device="cuda")
+    neg = torch.empty(narcs, dtype=torch.int16, device="cuda")
+
+    def incr():
+        _capi.check(_capi.lib.cuphu_incrcost_gpu(dcost, C.c_void_p(flows.data_ptr()), narcs, int(prm.nshortcycle), 1,
+                                                 C.c_void_p(pos.data_ptr()), C.c_void_p(neg.data_ptr()), None, st),
+                    "incrcost")
+
+    incr()
+    torch.cuda.synchronize()
+    reps = 20
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        incr()          # returns after the stream sync that reads the two minima back
+    us = 1e6 * (time.perf_counter() - t0) / reps
+    rt = C.CDLL(None)
+    rt.hipFree.argtypes = [C.c_void_p]
+    rt.hipFree.restype = C.c_int
+    if rt.hipFree(dcost) != 0:
+        raise RuntimeError("hipFree of the device costs failed")
+    gbs = 10.0 * narcs / (us * 1e-6) / 1e9
+    log(f"incrcost: {us:.1f} us per call over {narcs} arcs ({gbs:.0f} GB/s at 10 B per arc, call included)")
+    return {"workload": "scene8192 through cuphu_amd.unwrap (reference tile formula, same PCG/IRLS options); "
+                        "incremental costs over every arc of its smooth costs",
+            "post_passes": {"passes": "fix_cycle_spikes + laplace_neighbor_feedback + bridge",
+                            "seconds_on": round(t_on, 3), "seconds_off": round(t_off, 3),
+                            "added_ms": round(1e3 * (t_on - t_off), 1)},
+            "incrcost": {"arcs": narcs, "us_per_call": round(us, 1), "bytes_per_arc": 10,
+                         "achieved_gb_s": round(gbs, 1), "frac_of_8tb_s": round(gbs / 8000.0, 4)}}
 
 
 def leg_autograd(torch, synth, log):

@@ -4,10 +4,11 @@
 // bridge_unwrapped_phase()).
 //
 // Pixel-scale stages run on the GPU:
-//   regions   8-connected components of "unw != 0 and not masked": lock-free
-//             union-find (atomicMin links, the smaller index stays root),
-//             flatten, root flags + device scan -> labels 1..N numbered by
-//             each region's minimum raster index
+//   regions   8-connected components of "unw != 0 and not masked":
+//             union-find in 64 x 16 LDS tiles, then lock-free global unions
+//             (atomicMin links, the smaller index stays root) across tile
+//             borders, flatten, root flags + device scan -> labels 1..N
+//             numbered by each region's minimum raster index
 //   pruning   regions below min_num_pixel, then regions with no pixel left
 //             after a binary erosion by a square and then a circular element
 //   ramp      the optional ramp fit's samples are gathered in raster order
@@ -64,27 +65,85 @@ __device__ void br_union(uint32_t *par, uint32_t a, uint32_t b) {
     }
 }
 
-__global__ void k_br_init(const float *unw, const unsigned char *mask, size_t n, uint32_t *par) {
-    size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    const bool valid = !(mask && !mask[i]) && unw[i] != 0.0f;
-    par[i] = valid ? (uint32_t)i : kBrNone;
+// Regions are labelled in two levels, as the connected components are:
+// union-find inside a 64 x 16 tile in LDS, then global unions over the
+// forward 8-neighbour edges that cross a tile border.  Tile-local order is
+// raster order, and every union links the larger root under the smaller, so
+// each component's root is its smallest raster index at both levels.
+constexpr int kBrTW = 64, kBrTH = 16, kBrTN = kBrTW * kBrTH;
+
+__device__ __forceinline__ bool br_valid(const float *unw, const unsigned char *mask, size_t i) {
+    return !(mask && !mask[i]) && unw[i] != 0.0f;
+}
+__device__ __forceinline__ int br_lds_find(const int *lp, int x) {
+    int p = lp[x];
+    while (p != x) { x = p; p = lp[x]; }
+    return x;
+}
+__device__ void br_lds_union(int *lp, int a, int b) {
+    for (;;) {
+        a = br_lds_find(lp, a);
+        b = br_lds_find(lp, b);
+        if (a == b) return;
+        if (a < b) { int t = a; a = b; b = t; }
+        int old = atomicMin(&lp[a], b);
+        if (old == a) return;
+        a = old;
+    }
 }
 
-// every 8-neighbour edge once: east, and the three neighbours below
-__global__ void k_br_union(uint32_t *par, int nrow, int ncol) {
-    size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= (size_t)nrow * ncol) return;
-    if (br_load(par + i) == kBrNone) return;
-    const int r = (int)(i / ncol), c = (int)(i % ncol);
-    if (c + 1 < ncol && br_load(par + i + 1) != kBrNone) br_union(par, (uint32_t)i, (uint32_t)(i + 1));
-    if (r + 1 < nrow) {
-        for (int dc = -1; dc <= 1; ++dc) {
-            const int cc = c + dc;
-            if (cc < 0 || cc >= ncol) continue;
-            const size_t j = (size_t)(r + 1) * ncol + cc;
-            if (br_load(par + j) != kBrNone) br_union(par, (uint32_t)i, (uint32_t)j);
+// Every valid pixel's entry := the raster index of its tile-local root;
+// invalid pixels kBrNone.  Forward edges inside the tile: E, SW, S, SE.
+__global__ __launch_bounds__(256) void k_br_local(const float *unw, const unsigned char *mask, int nrow, int ncol,
+                                                  uint32_t *par) {
+    __shared__ int lp[kBrTN];
+    const int c0 = (int)blockIdx.x * kBrTW, r0 = (int)blockIdx.y * kBrTH;
+    for (int l = threadIdx.x; l < kBrTN; l += 256) {
+        const int r = r0 + l / kBrTW, c = c0 + l % kBrTW;
+        lp[l] = (r < nrow && c < ncol && br_valid(unw, mask, (size_t)r * ncol + c)) ? l : -1;
+    }
+    __syncthreads();
+    // an entry that is -1 stays -1 (unions touch valid entries only)
+    for (int l = threadIdx.x; l < kBrTN; l += 256) {
+        if (lp[l] < 0) continue;
+        const int lr = l / kBrTW, lc = l % kBrTW;
+        if (lc + 1 < kBrTW && lp[l + 1] >= 0) br_lds_union(lp, l, l + 1);
+        if (lr + 1 < kBrTH) {
+            if (lc > 0 && lp[l + kBrTW - 1] >= 0) br_lds_union(lp, l, l + kBrTW - 1);
+            if (lp[l + kBrTW] >= 0) br_lds_union(lp, l, l + kBrTW);
+            if (lc + 1 < kBrTW && lp[l + kBrTW + 1] >= 0) br_lds_union(lp, l, l + kBrTW + 1);
         }
+    }
+    __syncthreads();
+    for (int l = threadIdx.x; l < kBrTN; l += 256) {
+        const int r = r0 + l / kBrTW, c = c0 + l % kBrTW;
+        if (r >= nrow || c >= ncol) continue;
+        uint32_t out = kBrNone;
+        if (lp[l] >= 0) {
+            const int root = br_lds_find(lp, l);
+            out = (uint32_t)((size_t)(r0 + root / kBrTW) * ncol + (size_t)(c0 + root % kBrTW));
+        }
+        par[(size_t)r * ncol + c] = out;
+    }
+}
+
+// The forward 8-neighbour edges that leave the pixel's tile: E across the
+// right border, S across the bottom, SE across either, SW across the left
+// border or the bottom.
+__global__ void k_br_border(uint32_t *par, int nrow, int ncol) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (size_t)nrow * ncol) return;
+    const int r = (int)(i / ncol), c = (int)(i % ncol);
+    const bool right = (c % kBrTW) == kBrTW - 1, left = (c % kBrTW) == 0, bottom = (r % kBrTH) == kBrTH - 1;
+    if (!right && !left && !bottom) return;
+    if (br_load(par + i) == kBrNone) return;
+    const uint32_t a = (uint32_t)i;
+    if (right && c + 1 < ncol && br_load(par + i + 1) != kBrNone) br_union(par, a, (uint32_t)(i + 1));
+    if (r + 1 < nrow) {
+        const size_t d = i + (size_t)ncol;
+        if (bottom && br_load(par + d) != kBrNone) br_union(par, a, (uint32_t)d);
+        if ((right || bottom) && c + 1 < ncol && br_load(par + d + 1) != kBrNone) br_union(par, a, (uint32_t)(d + 1));
+        if ((left || bottom) && c > 0 && br_load(par + d - 1) != kBrNone) br_union(par, a, (uint32_t)(d - 1));
     }
 }
 
@@ -422,10 +481,12 @@ void bridge_dev(float *d_unw, uint32_t *d_cc, const unsigned char *d_mask, int n
     CUPHU_REQUIRE(n < 0x7FFFFFF0ull, "bridging: image too large for 31-bit pixel indices");
     CUPHU_REQUIRE(bp.ramp_type >= CUPHU_RAMP_NONE && bp.ramp_type <= CUPHU_RAMP_QUADRATIC_AZIMUTH,
                   "bridging: invalid ramp type");
+    TraceRange tr("cuphu:bridge");
     // ── regions of (unw != 0 and not masked), 8-connected ──
     DevBuf<uint32_t> d_par(n), d_flag(n), d_rank(n), d_lab(n);
-    hipLaunchKernelGGL(k_br_init, dim3(grid1(n)), dim3(256), 0, st, d_unw, d_mask, n, d_par.get());
-    hipLaunchKernelGGL(k_br_union, dim3(grid1(n)), dim3(256), 0, st, d_par.get(), nrow, ncol);
+    hipLaunchKernelGGL(k_br_local, dim3((unsigned)ceil_div((size_t)ncol, kBrTW), (unsigned)ceil_div((size_t)nrow, kBrTH)),
+                       dim3(256), 0, st, d_unw, d_mask, nrow, ncol, d_par.get());
+    hipLaunchKernelGGL(k_br_border, dim3(grid1(n)), dim3(256), 0, st, d_par.get(), nrow, ncol);
     hipLaunchKernelGGL(k_br_flatten, dim3(grid1(n)), dim3(256), 0, st, d_par.get(), n, d_flag.get());
     CUPHU_HIP_CHECK(hipGetLastError());
     int N = (int)scan_flags(d_flag.get(), d_rank.get(), n, st);

@@ -124,6 +124,24 @@ __global__ void k_feather(float *unw, const unsigned char *mask, int nrow, int n
     unw[g] += (float)(diff[k] * (double)w);
 }
 
+// The two pixel lines either side of a seam, contiguous: by_col, columns
+// seam - 1 and seam (n = nrow entries); else rows seam - 1 and seam (n =
+// ncol).  lines = [a (n floats), b (n floats)], valid = [ma (n), mb (n)]
+// (1 where the pixel is valid; all 1 without a mask).  A strided column
+// copied to the host row by row costs a transfer per row.
+__global__ void k_seam_lines(const float *unw, const unsigned char *mask, int nrow, int ncol, int seam, int by_col,
+                             float *lines, unsigned char *valid) {
+    const int n = by_col ? nrow : ncol;
+    const int i = (int)(blockIdx.x * 256 + threadIdx.x);
+    if (i >= n) return;
+    const long long ga = by_col ? (long long)i * ncol + (seam - 1) : (long long)(seam - 1) * ncol + i;
+    const long long gb = by_col ? (long long)i * ncol + seam : (long long)seam * ncol + i;
+    lines[i] = unw[ga];
+    lines[n + i] = unw[gb];
+    valid[i] = mask ? (unsigned char)(mask[ga] != 0) : (unsigned char)1;
+    valid[n + i] = mask ? (unsigned char)(mask[gb] != 0) : (unsigned char)1;
+}
+
 // ── host side of the passes ─────────────────────────────────────────────────
 
 void spikes_1d(float *d_unw, const unsigned char *d_mask, int nrow, int ncol, bool by_row, hipStream_t st) {
@@ -215,33 +233,19 @@ void feedback_seam(float *d_unw, const unsigned char *d_mask, int nrow, int ncol
                    hipStream_t st) {
     const int n = by_col ? nrow : ncol, lim = by_col ? ncol : nrow;
     if (seam - 1 < 0 || seam + 1 >= lim) return;   // :1587, :1624
-    // the pixel lines just before and at the seam
-    std::vector<float> a((size_t)n), b((size_t)n);
-    std::vector<unsigned char> ma((size_t)n, 1), mb((size_t)n, 1);
-    if (by_col) {
-        CUPHU_HIP_CHECK(hipMemcpy2DAsync(a.data(), 4, d_unw + (seam - 1), (size_t)ncol * 4, 4, (size_t)nrow,
-                                         hipMemcpyDeviceToHost, st));
-        CUPHU_HIP_CHECK(hipMemcpy2DAsync(b.data(), 4, d_unw + seam, (size_t)ncol * 4, 4, (size_t)nrow,
-                                         hipMemcpyDeviceToHost, st));
-        if (d_mask) {
-            CUPHU_HIP_CHECK(hipMemcpy2DAsync(ma.data(), 1, d_mask + (seam - 1), (size_t)ncol, 1, (size_t)nrow,
-                                             hipMemcpyDeviceToHost, st));
-            CUPHU_HIP_CHECK(hipMemcpy2DAsync(mb.data(), 1, d_mask + seam, (size_t)ncol, 1, (size_t)nrow,
-                                             hipMemcpyDeviceToHost, st));
-        }
-    } else {
-        CUPHU_HIP_CHECK(hipMemcpyAsync(a.data(), d_unw + (size_t)(seam - 1) * ncol, (size_t)ncol * 4,
-                                       hipMemcpyDeviceToHost, st));
-        CUPHU_HIP_CHECK(hipMemcpyAsync(b.data(), d_unw + (size_t)seam * ncol, (size_t)ncol * 4, hipMemcpyDeviceToHost,
-                                       st));
-        if (d_mask) {
-            CUPHU_HIP_CHECK(hipMemcpyAsync(ma.data(), d_mask + (size_t)(seam - 1) * ncol, (size_t)ncol,
-                                           hipMemcpyDeviceToHost, st));
-            CUPHU_HIP_CHECK(hipMemcpyAsync(mb.data(), d_mask + (size_t)seam * ncol, (size_t)ncol,
-                                           hipMemcpyDeviceToHost, st));
-        }
-    }
+    // the pixel lines just before and at the seam, gathered on the device
+    DevBuf<float> d_lines(2 * (size_t)n);
+    DevBuf<unsigned char> d_valid(2 * (size_t)n);
+    hipLaunchKernelGGL(k_seam_lines, dim3((unsigned)ceil_div((size_t)n, 256)), dim3(256), 0, st, d_unw, d_mask, nrow,
+                       ncol, seam, by_col ? 1 : 0, d_lines.get(), d_valid.get());
+    CUPHU_HIP_CHECK(hipGetLastError());
+    std::vector<float> lines(2 * (size_t)n);
+    std::vector<unsigned char> valid(2 * (size_t)n);
+    CUPHU_HIP_CHECK(hipMemcpyAsync(lines.data(), d_lines.get(), lines.size() * 4, hipMemcpyDeviceToHost, st));
+    CUPHU_HIP_CHECK(hipMemcpyAsync(valid.data(), d_valid.get(), valid.size(), hipMemcpyDeviceToHost, st));
     CUPHU_HIP_CHECK(hipStreamSynchronize(st));
+    const float *a = lines.data(), *b = lines.data() + n;
+    const unsigned char *ma = valid.data(), *mb = valid.data() + n;
     std::vector<double> diff((size_t)n, 0.0);
     std::vector<unsigned char> have((size_t)n, 0);
     for (int i = 0; i < n; ++i) {
@@ -267,6 +271,7 @@ void feedback_seam(float *d_unw, const unsigned char *d_mask, int nrow, int ncol
 }  // namespace
 
 void fix_cycle_spikes_dev(float *d_unw, const unsigned char *d_mask, int nrow, int ncol, hipStream_t st) {
+    TraceRange tr("cuphu:fix_cycle_spikes");
     spikes_1d(d_unw, d_mask, nrow, ncol, true, st);
     spikes_1d(d_unw, d_mask, nrow, ncol, false, st);
 }
@@ -274,6 +279,7 @@ void fix_cycle_spikes_dev(float *d_unw, const unsigned char *d_mask, int nrow, i
 void neighbor_feedback_dev(float *d_unw, const unsigned char *d_mask, int nrow, int ncol, const std::vector<int> &col_seams,
                            const std::vector<int> &row_seams, int feather, hipStream_t st) {
     if (feather <= 0) return;
+    TraceRange tr("cuphu:neighbor_feedback");
     for (int s : col_seams) feedback_seam(d_unw, d_mask, nrow, ncol, s, true, feather, st);
     for (int s : row_seams) feedback_seam(d_unw, d_mask, nrow, ncol, s, false, feather, st);
 }

@@ -281,39 +281,109 @@ __global__ void k_fill_i32(int *p, int n, int v) {
     if (i < n) p[i] = v;
 }
 
-__global__ __launch_bounds__(256) void k_incrcost(const short *costs, const short *flows, size_t narcs,
-                                                  long long nsc, int nflow, short *pos, short *neg, int *mins) {
-    int mp = 32000, mn = 32000;
-    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < narcs; i += (size_t)gridDim.x * 256) {
-        const long long off = costs[2 * i], sig = costs[2 * i + 1], f = flows[i];
-        long long pc = 0, nc = 0;
-        if (sig != 32000) {
-            const long long nfl = nflow;
-            long long idz1 = f * nsc + off, idp = (f + nfl) * nsc + off, idn = (f - nfl) * nsc + off;
-            idz1 = idz1 < 0 ? -idz1 : idz1;
-            idp = idp < 0 ? -idp : idp;
-            idn = idn < 0 ? -idn : idn;
+// One arc (SNAPHU CalcCostSmooth as cuphu_incrcost.cu:55-191 applies it):
+// the squared distances d^2 = (flow * nshortcycle + offset)^2 at the flow
+// and at flow +- nflow, c1 = d0^2 / sigsq, pr / nr = d+-^2 / sigsq - c1,
+// each rounded away from zero over nflow^2, clamped to +-32000; a masked
+// arc (sigsq 32000) costs 0.
+//
+// Exact FP64 form, taken while every |d| < 2^25 and sigsq >= 1: the
+// squares are integers below 2^50, exact in FP64, and so are the
+// quotients' remainders.  trunc(a * (1/b)) carries a relative error of at
+// most 2^-52 (two roundings) on a quotient below 2^50, so it is within one
+// of floor(a / b), and the remainder a - q b, an integer of magnitude below
+// 2b formed exactly by one fma, fixes it.  Otherwise (|d| >= 2^25, or
+// sigsq <= 0) the 64-bit integer divisions.  Both give the same integers.
+__device__ __forceinline__ double ic_floor_div_f64(double a, double b, double rb) {   // a >= 0, b >= 1
+    double q = trunc(a * rb);
+    const double r = fma(-q, b, a);
+    if (r < 0.0) q -= 1.0;
+    else if (r >= b) q += 1.0;
+    return q;
+}
+__device__ __forceinline__ long long ic_away_div_f64(double a, double b, double rb) {   // a integer-valued
+    const double m = fabs(a);
+    double q = ic_floor_div_f64(m, b, rb);
+    if (fma(-q, b, m) > 0.0) q += 1.0;
+    return (long long)(a < 0.0 ? -q : q);
+}
+
+struct IncrArgs {
+    long long nsc, nfl;
+    double nsq, rnsq;   // nflow^2 and its reciprocal
+};
+
+__device__ __forceinline__ void incr_arc(long long off, long long sig, long long f, const IncrArgs &a, short &pos,
+                                         short &neg, int &mp, int &mn) {
+    long long pc = 0, nc = 0;
+    if (sig != 32000) {
+        long long idz1 = f * a.nsc + off, idp = (f + a.nfl) * a.nsc + off, idn = (f - a.nfl) * a.nsc + off;
+        idz1 = idz1 < 0 ? -idz1 : idz1;
+        idp = idp < 0 ? -idp : idp;
+        idn = idn < 0 ? -idn : idn;
+        const long long lim = 1LL << 25;
+        if (sig >= 1 && idz1 < lim && idp < lim && idn < lim) {
+            const double b = (double)sig, rb = 1.0 / b;
+            const double z1 = (double)idz1, zp = (double)idp, zn = (double)idn;
+            const double c1 = ic_floor_div_f64(z1 * z1, b, rb);
+            const double pr = ic_floor_div_f64(zp * zp, b, rb) - c1;
+            const double nr = ic_floor_div_f64(zn * zn, b, rb) - c1;
+            pc = ic_away_div_f64(pr, a.nsq, a.rnsq);
+            nc = ic_away_div_f64(nr, a.nsq, a.rnsq);
+        } else {
             const long long c1 = (idz1 * idz1) / sig;
             const long long pr = (idp * idp) / sig - c1, nr = (idn * idn) / sig - c1;
-            const long long nsq = nfl * nfl;
+            const long long nsq = a.nfl * a.nfl;
             pc = pr > 0 ? ic_ceil_div(pr, nsq) : ic_floor_div(pr, nsq);
             nc = nr > 0 ? ic_ceil_div(nr, nsq) : ic_floor_div(nr, nsq);
-            pc = pc > 32000 ? 32000 : (pc < -32000 ? -32000 : pc);
-            nc = nc > 32000 ? 32000 : (nc < -32000 ? -32000 : nc);
         }
-        pos[i] = (short)pc;
-        neg[i] = (short)nc;
-        mp = min(mp, (int)pc);
-        mn = min(mn, (int)nc);
+        pc = pc > 32000 ? 32000 : (pc < -32000 ? -32000 : pc);
+        nc = nc > 32000 ? 32000 : (nc < -32000 ? -32000 : nc);
+    }
+    pos = (short)pc;
+    neg = (short)nc;
+    mp = min(mp, (int)pc);
+    mn = min(mn, (int)nc);
+}
+
+// Four consecutive arcs per thread, grid-stride over groups of four; VEC:
+// 16-byte cost and 8-byte flow / output accesses (the host checks the
+// alignment).  The minima are reduced per workgroup, one atomic each.
+template <bool VEC>
+__global__ __launch_bounds__(256) void k_incrcost(const short *costs, const short *flows, size_t narcs, IncrArgs a,
+                                                  short *pos, short *neg, int *mins) {
+    __shared__ int sm[2][4];
+    int mp = 32000, mn = 32000;
+    const size_t nq = (narcs + 3) / 4;
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < nq; q += (size_t)gridDim.x * 256) {
+        const size_t i0 = 4 * q;
+        if (VEC && i0 + 4 <= narcs) {
+            const short4 c01 = reinterpret_cast<const short4 *>(costs)[2 * q];
+            const short4 c23 = reinterpret_cast<const short4 *>(costs)[2 * q + 1];
+            const short4 fl = reinterpret_cast<const short4 *>(flows)[q];
+            short4 po, ne;
+            incr_arc(c01.x, c01.y, fl.x, a, po.x, ne.x, mp, mn);
+            incr_arc(c01.z, c01.w, fl.y, a, po.y, ne.y, mp, mn);
+            incr_arc(c23.x, c23.y, fl.z, a, po.z, ne.z, mp, mn);
+            incr_arc(c23.z, c23.w, fl.w, a, po.w, ne.w, mp, mn);
+            reinterpret_cast<short4 *>(pos)[q] = po;
+            reinterpret_cast<short4 *>(neg)[q] = ne;
+        } else {
+            for (size_t i = i0; i < i0 + 4 && i < narcs; ++i)
+                incr_arc(costs[2 * i], costs[2 * i + 1], flows[i], a, pos[i], neg[i], mp, mn);
+        }
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
         mp = min(mp, __shfl_xor(mp, o, 64));
         mn = min(mn, __shfl_xor(mn, o, 64));
     }
-    if ((threadIdx.x & 63) == 0) {
-        atomicMin(mins, mp);
-        atomicMin(mins + 1, mn);
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { sm[0][wave] = mp; sm[1][wave] = mn; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        atomicMin(mins, min(min(sm[0][0], sm[0][1]), min(sm[0][2], sm[0][3])));
+        atomicMin(mins + 1, min(min(sm[1][0], sm[1][1]), min(sm[1][2], sm[1][3])));
     }
 }
 
@@ -322,9 +392,20 @@ int util_incrcost(const short *costs, const short *flows, size_t narcs, long nsh
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, mins, 2, 32000);
     CUPHU_HIP_CHECK(hipGetLastError());
     if (narcs) {
-        unsigned g = (unsigned)std::min<size_t>(ceil_div(narcs, 256), 65535);
-        hipLaunchKernelGGL(k_incrcost, dim3(g), dim3(256), 0, st, costs, flows, narcs, (long long)nshortcycle, nflow,
-                           poscost, negcost, mins);
+        IncrArgs a;
+        a.nsc = (long long)nshortcycle;
+        a.nfl = nflow;
+        a.nsq = (double)nflow * (double)nflow;
+        a.rnsq = 1.0 / a.nsq;
+        // 8 workgroups per CU at most: every thread takes several groups of four
+        const unsigned g = (unsigned)std::min<size_t>(ceil_div(ceil_div(narcs, 4), 256), 2048);
+        auto al = [](const void *p, size_t b) { return ((uintptr_t)p % b) == 0; };
+        if (al(costs, 16) && al(flows, 8) && al(poscost, 8) && al(negcost, 8))
+            hipLaunchKernelGGL(k_incrcost<true>, dim3(g), dim3(256), 0, st, costs, flows, narcs, a, poscost, negcost,
+                               mins);
+        else
+            hipLaunchKernelGGL(k_incrcost<false>, dim3(g), dim3(256), 0, st, costs, flows, narcs, a, poscost, negcost,
+                               mins);
         CUPHU_HIP_CHECK(hipGetLastError());
     }
     int h[2] = {0, 0};

@@ -118,6 +118,63 @@ def test_bridge_random_scenes_bitexact(gpu_ready, seed, shape, use_mask):
     _bridge(src, m, bridge_max_boundary_samples=7, bridge_radius=500)
 
 
+def _diagonal_lines():
+    """One-pixel diagonal (rows 0-47) and anti-diagonal (rows 48-95) lines:
+    8-connected but not 4-connected (7 regions against 336), each with its
+    own whole-cycle offset on a background of zeros.  They cross the region
+    labeller's 64 x 16 tile borders at tile corners ((15, 63) -> (16, 64),
+    (15, 127) -> (16, 128), (63, 128) -> (64, 127), (63, 192) -> (64, 191)),
+    across right borders only ((4, 63) -> (5, 64), (21, 191) -> (22, 192)),
+    bottom borders only and a left border only ((66, 64) -> (67, 63))."""
+    nrow, ncol = 96, 256
+    r, c = np.mgrid[0:nrow, 0:ncol]
+    truth = 0.3 + 0.004 * r - 0.002 * c
+    reg = np.zeros((nrow, ncol), np.int32)
+    k = 0
+    for off in (48, 59, 112, 170):
+        k += 1
+        for rr in range(48):
+            if rr + off < ncol:
+                reg[rr, rr + off] = k
+    for off in (130, 191, 255):
+        k += 1
+        for rr in range(48, 96):
+            if 0 <= off - rr < ncol:
+                reg[rr, off - rr] = k
+    jumps = np.array([0, 3, -2, 1, 4, -1, 2, -3], np.float64) * TWO_PI
+    return np.where(reg > 0, truth + jumps[reg], 0.0).astype(np.float32), reg, truth
+
+
+def test_bridge_regions_diagonal_across_tile_borders(gpu_ready):
+    unw, reg, truth = _diagonal_lines()
+    assert ndimage.label(reg > 0, structure=np.ones((3, 3)))[1] == 7
+    u, cc, n = _bridge(unw, None, bridge_erosion_size=0, bridge_min_num_pixel=14, bridge_radius=60)
+    assert n == 7
+    # regions are numbered by their first raster pixel, the order the lines were drawn in
+    np.testing.assert_array_equal(cc, reg.astype(cc.dtype))
+    # every line is moved onto the cycle of region 1 (the largest, lowest label on a tie: +3 cycles)
+    v = reg > 0
+    np.testing.assert_allclose(u[v] - truth[v], 3 * TWO_PI, atol=1e-3)
+
+
+@pytest.mark.parametrize("use_mask", [False, True])
+def test_bridge_regions_sparse_random(gpu_ready, use_mask):
+    """Random pixels at density 0.45, just above the 8-neighbour site
+    percolation threshold: 216 regions joined mostly through diagonal
+    links, 14 of at least 14 pixels."""
+    rng = np.random.default_rng(11)
+    valid = rng.random((80, 300)) < 0.45
+    lab, nlab = ndimage.label(valid, structure=np.ones((3, 3)))
+    r, c = np.mgrid[0:80, 0:300]
+    jumps = rng.integers(-3, 4, nlab + 1) * TWO_PI
+    val = (0.2 + 0.003 * r + 0.001 * c + jumps[lab]).astype(np.float32)
+    if use_mask:
+        _bridge(val, valid.astype(np.uint8), bridge_erosion_size=0, bridge_min_num_pixel=14, bridge_radius=60)
+    else:
+        _bridge(np.where(valid, val, 0).astype(np.float32), None, bridge_erosion_size=0, bridge_min_num_pixel=14,
+                bridge_radius=60)
+
+
 @pytest.mark.parametrize("ramp", RAMPS)
 def test_bridge_ramp_types_bitexact(gpu_ready, ramp):
     unw, isl, _ = _islands(7, (180, 240))

@@ -47,6 +47,33 @@ def test_incremental_costs_bitexact(gpu_ready, nflow):
     assert f2
 
 
+@pytest.mark.parametrize("nshortcycle,nflow", [(200, 1), (200, 3), (5000, 2)])
+@pytest.mark.parametrize("aligned", [True, False])
+def test_incremental_costs_full_range_bitexact(gpu_ready, nshortcycle, nflow, aligned):
+    """Offsets, flows and sigsq over the whole int16 range (sigsq >= 1, plus
+    the masked sentinel), a fifth of the flows small.  At nshortcycle 5000
+    most arcs have |flow * nshortcycle + offset| >= 2^25 and take the 64-bit
+    integer division instead of the exact FP64 form.  Unaligned: the
+    inputs start one element into their buffers (the scalar-access kernel)."""
+    torch = G.torch_mod()
+    rng = np.random.default_rng(nshortcycle + nflow)
+    n = 300_007
+    costs = np.stack([rng.integers(-32768, 32768, n), rng.integers(1, 32768, n)], 1).astype(np.int16)
+    costs[::97, 1] = 32000
+    flows = rng.integers(-32768, 32768, n).astype(np.int16)
+    flows[::5] = rng.integers(-3, 4, flows[::5].size)
+    k = 0 if aligned else 1
+    dc = G.dev(np.concatenate([np.zeros((k, 2), np.int16), costs]))[k:]
+    df = G.dev(np.concatenate([np.zeros(k, np.int16), flows]))[k:]
+    assert (dc.data_ptr() % 16 == 0) == aligned and (df.data_ptr() % 8 == 0) == aligned
+    pos, neg, flag = incremental_costs(dc, df, nshortcycle=nshortcycle, nflow=nflow)
+    torch.cuda.synchronize()
+    rp, rn, rflag = O.incrcost_smooth(costs, flows, nshortcycle=nshortcycle, nflow=nflow)
+    np.testing.assert_array_equal(G.host(pos), rp)
+    np.testing.assert_array_equal(G.host(neg), rn)
+    assert flag == rflag
+
+
 @pytest.mark.parametrize("shape", [(23, 31), (130, 300)])
 def test_conncomp_gated_bitexact(gpu_ready, shape):
     torch = G.torch_mod()
