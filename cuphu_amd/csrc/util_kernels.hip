@@ -301,49 +301,70 @@ __device__ __forceinline__ double ic_floor_div_f64(double a, double b, double rb
     else if (r >= b) q += 1.0;
     return q;
 }
-__device__ __forceinline__ long long ic_away_div_f64(double a, double b, double rb) {   // a integer-valued
+__device__ __forceinline__ double ic_away_div_f64(double a, double b, double rb) {   // a integer-valued
     const double m = fabs(a);
     double q = ic_floor_div_f64(m, b, rb);
     if (fma(-q, b, m) > 0.0) q += 1.0;
-    return (long long)(a < 0.0 ? -q : q);
+    return a < 0.0 ? -q : q;
 }
 
 struct IncrArgs {
     long long nsc, nfl;
-    double nsq, rnsq;   // nflow^2 and its reciprocal
+    double dnsc, dnfl;   // the same as doubles (exact)
+    double nsq, rnsq;    // nflow^2 and its reciprocal
+    int unit;            // nflow == 1: the division by nflow^2 is the identity
 };
 
-__device__ __forceinline__ void incr_arc(long long off, long long sig, long long f, const IncrArgs &a, short &pos,
-                                         short &neg, int &mp, int &mn) {
-    long long pc = 0, nc = 0;
+// The 64-bit integer form, for the arcs the FP64 form does not cover.
+__device__ __noinline__ void incr_arc_int(long long off, long long sig, long long f, const IncrArgs &a,
+                                          long long &pc, long long &nc) {
+    long long idz1 = f * a.nsc + off, idp = (f + a.nfl) * a.nsc + off, idn = (f - a.nfl) * a.nsc + off;
+    idz1 = idz1 < 0 ? -idz1 : idz1;
+    idp = idp < 0 ? -idp : idp;
+    idn = idn < 0 ? -idn : idn;
+    const long long c1 = (idz1 * idz1) / sig;
+    const long long pr = (idp * idp) / sig - c1, nr = (idn * idn) / sig - c1;
+    const long long nsq = a.nfl * a.nfl;
+    pc = pr > 0 ? ic_ceil_div(pr, nsq) : ic_floor_div(pr, nsq);
+    nc = nr > 0 ? ic_ceil_div(nr, nsq) : ic_floor_div(nr, nsq);
+    pc = pc > 32000 ? 32000 : (pc < -32000 ? -32000 : pc);
+    nc = nc > 32000 ? 32000 : (nc < -32000 ? -32000 : nc);
+}
+
+// The distances |(flow + k nflow) nshortcycle + offset| are formed as
+// fabs(fma(.)) in FP64: exact while the true value is below 2^53, and a
+// larger one fails the 2^25 test either way.
+__device__ __forceinline__ void incr_arc(int off, int sig, int f, const IncrArgs &a, short &pos, short &neg, int &mp,
+                                         int &mn) {
+    int pc = 0, nc = 0;
     if (sig != 32000) {
-        long long idz1 = f * a.nsc + off, idp = (f + a.nfl) * a.nsc + off, idn = (f - a.nfl) * a.nsc + off;
-        idz1 = idz1 < 0 ? -idz1 : idz1;
-        idp = idp < 0 ? -idp : idp;
-        idn = idn < 0 ? -idn : idn;
-        const long long lim = 1LL << 25;
-        if (sig >= 1 && idz1 < lim && idp < lim && idn < lim) {
+        const double d0 = (double)f, doff = (double)off;
+        const double z1 = fabs(fma(d0, a.dnsc, doff));
+        const double zp = fabs(fma(d0 + a.dnfl, a.dnsc, doff));
+        const double zn = fabs(fma(d0 - a.dnfl, a.dnsc, doff));
+        const double lim = 33554432.0;   // 2^25
+        if (sig >= 1 && z1 < lim && zp < lim && zn < lim) {
             const double b = (double)sig, rb = 1.0 / b;
-            const double z1 = (double)idz1, zp = (double)idp, zn = (double)idn;
             const double c1 = ic_floor_div_f64(z1 * z1, b, rb);
-            const double pr = ic_floor_div_f64(zp * zp, b, rb) - c1;
-            const double nr = ic_floor_div_f64(zn * zn, b, rb) - c1;
-            pc = ic_away_div_f64(pr, a.nsq, a.rnsq);
-            nc = ic_away_div_f64(nr, a.nsq, a.rnsq);
+            double pr = ic_floor_div_f64(zp * zp, b, rb) - c1;
+            double nr = ic_floor_div_f64(zn * zn, b, rb) - c1;
+            if (!a.unit) {
+                pr = ic_away_div_f64(pr, a.nsq, a.rnsq);
+                nr = ic_away_div_f64(nr, a.nsq, a.rnsq);
+            }
+            pc = (int)fmin(fmax(pr, -32000.0), 32000.0);
+            nc = (int)fmin(fmax(nr, -32000.0), 32000.0);
         } else {
-            const long long c1 = (idz1 * idz1) / sig;
-            const long long pr = (idp * idp) / sig - c1, nr = (idn * idn) / sig - c1;
-            const long long nsq = a.nfl * a.nfl;
-            pc = pr > 0 ? ic_ceil_div(pr, nsq) : ic_floor_div(pr, nsq);
-            nc = nr > 0 ? ic_ceil_div(nr, nsq) : ic_floor_div(nr, nsq);
+            long long p64, n64;
+            incr_arc_int(off, sig, f, a, p64, n64);
+            pc = (int)p64;
+            nc = (int)n64;
         }
-        pc = pc > 32000 ? 32000 : (pc < -32000 ? -32000 : pc);
-        nc = nc > 32000 ? 32000 : (nc < -32000 ? -32000 : nc);
     }
     pos = (short)pc;
     neg = (short)nc;
-    mp = min(mp, (int)pc);
-    mn = min(mn, (int)nc);
+    mp = min(mp, pc);
+    mn = min(mn, nc);
 }
 
 // Four consecutive arcs per thread, grid-stride over groups of four; VEC:
@@ -395,8 +416,11 @@ int util_incrcost(const short *costs, const short *flows, size_t narcs, long nsh
         IncrArgs a;
         a.nsc = (long long)nshortcycle;
         a.nfl = nflow;
+        a.dnsc = (double)nshortcycle;
+        a.dnfl = (double)nflow;
         a.nsq = (double)nflow * (double)nflow;
         a.rnsq = 1.0 / a.nsq;
+        a.unit = nflow == 1 ? 1 : 0;
         // 8 workgroups per CU at most: every thread takes several groups of four
         const unsigned g = (unsigned)std::min<size_t>(ceil_div(ceil_div(narcs, 4), 256), 2048);
         auto al = [](const void *p, size_t b) { return ((uintptr_t)p % b) == 0; };
