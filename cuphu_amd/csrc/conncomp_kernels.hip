@@ -132,12 +132,14 @@ __global__ __launch_bounds__(256) void k_cc_border(const float *unw, int nrow, i
     long long i = (long long)blockIdx.x * 256 + threadIdx.x;
     if (i >= (long long)nrow * ncol) return;
     int r = (int)(i / ncol), c = (int)(i % ncol);
+    const bool right = (c % kCcTW) == kCcTW - 1, bottom = (r % kCcTH) == kCcTH - 1;
+    if (!right && !bottom) return;   // 1 pixel in 13: no label load for the others
     long long g = (long long)r * stride + c;
     if (g_load(lab + g) == kInvalidLabel) return;
-    if ((c % kCcTW) == kCcTW - 1 && c + 1 < ncol && g_load(lab + g + 1) != kInvalidLabel && cc_join(unw, g, g + 1) &&
+    if (right && c + 1 < ncol && g_load(lab + g + 1) != kInvalidLabel && cc_join(unw, g, g + 1) &&
         gate.right(r, c))
         g_union(lab, (uint32_t)g, (uint32_t)(g + 1));
-    if ((r % kCcTH) == kCcTH - 1 && r + 1 < nrow && g_load(lab + g + stride) != kInvalidLabel &&
+    if (bottom && r + 1 < nrow && g_load(lab + g + stride) != kInvalidLabel &&
         cc_join(unw, g, g + stride) && gate.down(r, c))
         g_union(lab, (uint32_t)g, (uint32_t)(g + stride));
 }
@@ -231,40 +233,61 @@ __global__ void k_cc_count(const uint32_t *rank, const uint32_t *flags, size_t n
     *out = (int)((long)total < maxncomps ? (long)total : (maxncomps > 0 ? maxncomps : 0));
 }
 
+size_t conncomp_scan_bytes(size_t n) {
+    size_t tmp_bytes = 0;
+    CUPHU_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, (const uint32_t *)nullptr,
+                                                     (uint32_t *)nullptr, (int)n, (hipStream_t)0));
+    return tmp_bytes + 16;
+}
+
 int conncomp_run(const float *unw, const float *corr, const unsigned char *mask, int nrow, int ncol, int stride,
                  double minfrac, long maxncomps, uint32_t *labels, hipStream_t st, float corr_thresh,
-                 const short *poscost, const short *negcost, long conncompthresh) {
+                 const short *poscost, const short *negcost, long conncompthresh, const CcScratch *scr) {
     if (nrow <= 0 || ncol <= 0) return 0;
     size_t n = (size_t)nrow * stride;
     size_t npx = (size_t)nrow * ncol;
     CUPHU_REQUIRE(n < 0xFFFFFFF0ull, "conncomp: image too large for 32-bit labels");
-    DevBuf<uint32_t> cnt(n), flags(n), rank(n);
-    DevBuf<int> dcount(1);
+    CUPHU_REQUIRE(!scr || n <= scr->n, "conncomp: scratch smaller than the image");
+    // scratch: the caller's, else allocated for this call
+    DevBuf<uint32_t> own_cnt, own_flags, own_rank;
+    DevBuf<int> own_count;
+    auto mk = [](auto &b, size_t k) { b.alloc(k); return b.get(); };
+    uint32_t *cnt = scr ? scr->cnt : mk(own_cnt, n);
+    uint32_t *flags = scr ? scr->flags : mk(own_flags, n);
+    uint32_t *rank = scr ? scr->rank : mk(own_rank, n);
+    int *dcount = scr ? scr->count : mk(own_count, 1);
     unsigned g = (unsigned)ceil_div(n, 256), gp = (unsigned)ceil_div(npx, 256);
     dim3 tiles((unsigned)ceil_div((size_t)ncol, kCcTW), (unsigned)ceil_div((size_t)nrow, kCcTH));
-    hipLaunchKernelGGL(k_cc_fill, dim3(g), dim3(256), 0, st, cnt.get(), n, 0u);
+    hipLaunchKernelGGL(k_cc_fill, dim3(g), dim3(256), 0, st, cnt, n, 0u);
     if (stride != ncol) hipLaunchKernelGGL(k_cc_fill, dim3(g), dim3(256), 0, st, labels, n, kInvalidLabel);
     CcGate gate{poscost, negcost, conncompthresh, (long long)(nrow - 1) * ncol, ncol};
     hipLaunchKernelGGL(k_cc_local, tiles, dim3(256), 0, st, unw, corr, mask, nrow, ncol, stride, corr_thresh, gate,
-                       labels, flags.get());
+                       labels, flags);
     hipLaunchKernelGGL(k_cc_border, dim3(gp), dim3(256), 0, st, unw, nrow, ncol, stride, gate, labels);
-    hipLaunchKernelGGL(k_cc_flatten, dim3(gp), dim3(256), 0, st, nrow, ncol, stride, flags.get(), labels);
+    hipLaunchKernelGGL(k_cc_flatten, dim3(gp), dim3(256), 0, st, nrow, ncol, stride, flags, labels);
     hipLaunchKernelGGL(k_cc_compress, dim3((unsigned)ceil_div(npx, (size_t)256 * kCcRun)), dim3(256), 0, st, nrow, ncol,
-                       stride, labels, cnt.get());
+                       stride, labels, cnt);
     int min_size = (int)(minfrac * (double)(int)npx);
     if (min_size < 1) min_size = 1;
-    hipLaunchKernelGGL(k_cc_keep, dim3(g), dim3(256), 0, st, cnt.get(), n, (uint32_t)min_size, flags.get());
+    hipLaunchKernelGGL(k_cc_keep, dim3(g), dim3(256), 0, st, cnt, n, (uint32_t)min_size, flags);
     CUPHU_HIP_CHECK(hipGetLastError());
     size_t tmp_bytes = 0;
-    CUPHU_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, flags.get(), rank.get(), (int)n, st));
-    DevBuf<unsigned char> tmp(tmp_bytes + 16);
-    CUPHU_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp.get(), tmp_bytes, flags.get(), rank.get(), (int)n, st));
-    hipLaunchKernelGGL(k_cc_relabel, dim3(gp), dim3(256), 0, st, nrow, ncol, stride, rank.get(), flags.get(),
+    CUPHU_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, flags, rank, (int)n, st));
+    DevBuf<unsigned char> own_tmp;
+    unsigned char *tmp = nullptr;
+    if (scr) {
+        CUPHU_REQUIRE(tmp_bytes <= scr->tmp_bytes, "conncomp: scan scratch too small");
+        tmp = scr->tmp;
+    } else {
+        tmp = mk(own_tmp, tmp_bytes + 16);
+    }
+    CUPHU_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, flags, rank, (int)n, st));
+    hipLaunchKernelGGL(k_cc_relabel, dim3(gp), dim3(256), 0, st, nrow, ncol, stride, rank, flags,
                        maxncomps, labels);
-    hipLaunchKernelGGL(k_cc_count, dim3(1), dim3(1), 0, st, rank.get(), flags.get(), n, maxncomps, dcount.get());
+    hipLaunchKernelGGL(k_cc_count, dim3(1), dim3(1), 0, st, rank, flags, n, maxncomps, dcount);
     CUPHU_HIP_CHECK(hipGetLastError());
     int k = 0;
-    CUPHU_HIP_CHECK(hipMemcpyAsync(&k, dcount.get(), sizeof(int), hipMemcpyDeviceToHost, st));
+    CUPHU_HIP_CHECK(hipMemcpyAsync(&k, dcount, sizeof(int), hipMemcpyDeviceToHost, st));
     CUPHU_HIP_CHECK(hipStreamSynchronize(st));
     return k;
 }

@@ -112,6 +112,20 @@ template <class T> static T *ws(DevRes &r, const char *name, size_t count) {
     return reinterpret_cast<T *>(r.get(name, std::max<size_t>(count, 1) * sizeof(T) + 256));
 }
 
+// Connected-component scratch for images of up to n = nrow * stride
+// entries, kept in the workspace between calls.
+static CcScratch cc_scratch(DevRes &res, size_t n) {
+    CcScratch scr;
+    scr.n = n;
+    scr.cnt = ws<uint32_t>(res, "cc_cnt", n);
+    scr.flags = ws<uint32_t>(res, "cc_flags", n);
+    scr.rank = ws<uint32_t>(res, "cc_rank", n);
+    scr.tmp_bytes = conncomp_scan_bytes(n);
+    scr.tmp = ws<unsigned char>(res, "cc_tmp", scr.tmp_bytes);
+    scr.count = ws<int>(res, "cc_count", 1);
+    return scr;
+}
+
 // ── host <-> device copies of caller buffers through pinned staging ────────
 // Pageable device-to-host copies run at a few GB/s; staging through two
 // pinned chunks runs at the link rate.  The DMA of one chunk overlaps the
@@ -1170,12 +1184,19 @@ static void run_batch(const std::vector<std::pair<int, int>> &dims, const CuPhuL
     launch_final<T>(d_gdesc, d_rowinst, L.total_rows, P.pl, P.out, L.max_pitch, st);
     out.ncomp.assign((size_t)ninst, 0);
     TraceRange tr_cc("cuphu:conncomp");
-    if (do_conncomp)
+    if (do_conncomp) {
+        // one scratch set in the workspace, sized for the largest instance, serves every call
+        size_t maxn = 1;
+        for (int i = 0; i < ninst; ++i)
+            maxn = std::max(maxn, (size_t)L.desc[(size_t)i].nrow * (size_t)L.desc[(size_t)i].pitch);
+        const CcScratch scr = cc_scratch(res, maxn);
         for (int i = 0; i < ninst; ++i) {
             const InstDesc &d = L.desc[(size_t)i];
             out.ncomp[(size_t)i] = conncomp_run(P.out + d.off, P.pl.corr + d.off, P.pl.mask + d.off, d.nrow, d.ncol,
-                                                d.pitch, minfrac, maxncomps, P.lab + d.off, st);
+                                                d.pitch, minfrac, maxncomps, P.lab + d.off, st, 0.05f, nullptr,
+                                                nullptr, 0, &scr);
         }
+    }
     out.lay = L;
     out.unw = P.out;
     out.lab = P.lab;
@@ -1390,8 +1411,11 @@ static void scene_impl(const float2 *d_igram, const float *d_corr, const unsigne
     }
     stitch_tiles(bo, nrow, ncol, ntr, ntc, rf, rs, cf, cs, rovl, covl, d_conf, d_mask != nullptr,
                  o.stitch_integer != 0, d_unw, scene_cc ? nullptr : d_cc, res, st);
-    if (scene_cc)
-        conncomp_run(d_unw, d_corr, d_mask, nrow, ncol, ncol, prm.minconncompfrac, prm.maxncomps, d_cc, st);
+    if (scene_cc) {
+        const CcScratch scr = cc_scratch(res, (size_t)nrow * ncol);
+        conncomp_run(d_unw, d_corr, d_mask, nrow, ncol, ncol, prm.minconncompfrac, prm.maxncomps, d_cc, st, 0.05f,
+                     nullptr, nullptr, 0, &scr);
+    }
     CUPHU_HIP_CHECK(hipStreamSynchronize(st));
 }
 

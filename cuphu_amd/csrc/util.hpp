@@ -18,12 +18,24 @@ void util_integrate(const float *ph, const short *hflows, const short *vflows, i
 
 // Connected components of one image (stride = elements per row of the unw,
 // corr, mask and label arrays).  Returns the number of labels assigned
-// (labels are 1..K).  Scratch is allocated internally.
+// (labels are 1..K).
 // poscost/negcost (flat arc layout, both or neither): the MCF path's arc gate
 // min(poscost, negcost) >= conncompthresh.
+// scr (nullable): device scratch of a caller that keeps it between calls --
+// cnt, flags, rank of n >= nrow * stride entries each, tmp of tmp_bytes >=
+// conncomp_scan_bytes(n) and one int; without it the call allocates its own.
+struct CcScratch {
+    uint32_t *cnt, *flags, *rank;
+    unsigned char *tmp;
+    size_t tmp_bytes;
+    int *count;
+    size_t n;
+};
+size_t conncomp_scan_bytes(size_t n);
 int conncomp_run(const float *unw, const float *corr, const unsigned char *mask, int nrow, int ncol, int stride,
                  double minfrac, long maxncomps, uint32_t *labels, hipStream_t st, float corr_thresh = 0.05f,
-                 const short *poscost = nullptr, const short *negcost = nullptr, long conncompthresh = 0);
+                 const short *poscost = nullptr, const short *negcost = nullptr, long conncompthresh = 0,
+                 const CcScratch *scr = nullptr);
 
 // Incremental costs of smooth-cost arcs at the given flows (SNAPHU
 // CalcCostSmooth for a +-nflow change; costs are {offset, sigsq} pairs).
