@@ -531,7 +531,6 @@ static bool wide_stride(const CuPhuLaplaceOptions &o) {
 
 // Chunk pipelining (solve_group_fused) for grids of at most this many
 // workgroups (16 M pixels).
-constexpr int kPipeMaxBlocks = 4096;
 
 // Folded reductions (launch_iq_fold) for grids of at most this many
 // workgroups, where the reduction launch it replaces is a large part of an
@@ -635,15 +634,18 @@ static void solve_group_fused(const Planes<T> &pl, const GroupDev &g, const CuPh
     }
     // Chunk pipelining: with pinned state buffers, the next chunk is issued
     // before the host has read the state after the current one, so the GPU
-    // does not wait for the host between chunks.  Once every instance has
-    // stopped, the chunk still in flight is a no-op (its kernels skip
-    // stopped instances and the reductions keep their states), so the
-    // results are those of the one-chunk-at-a-time loop.  Only on grids of
-    // at most kPipeMaxBlocks workgroups, where a chunk lasts a few ms or
-    // less and the host's turnaround is a visible part of it (1024^2:
-    // 14.7 -> 15.2 Mpixel/s); a 50-iteration chunk of the 8192^2 scene
-    // lasts 22 ms, and there the no-op chunk would only add launches.
-    const bool pipe = g.hstate[0] && g.hstate[1] && g.nblk <= kPipeMaxBlocks;
+    // does not wait for the host between chunks (1024^2: 14.7 -> 15.2
+    // Mpixel/s; 8192^2: the 120 chunk boundaries of an unwrap left the GPU
+    // idle about 130 us each).  It is issued early only while an instance
+    // active in the last state read can still be active after the chunk in
+    // flight (iterations + chunk < max_iter), so a solve that runs to the
+    // cap ends without an extra chunk.  One that converges inside a chunk
+    // leaves the chunk after it a no-op (its kernels skip stopped instances
+    // and the reductions keep their states).  Whenever instances are still
+    // active after a read with nothing in flight, the next chunk is issued
+    // then, as in the serial loop, so the results are those of the
+    // one-chunk-at-a-time loop whatever the prediction.
+    const bool pipe = g.hstate[0] && g.hstate[1];
     struct Done {   // waits for a chunk still in flight before its event and state buffer go
         hipEvent_t e[2] = {nullptr, nullptr};
         ~Done() {
@@ -740,12 +742,18 @@ static void solve_group_fused(const Planes<T> &pl, const GroupDev &g, const CuPh
             int issued = 0, collected = 0;
             std::vector<SolveScal> before = h;
             auto can_issue = [&]() { return (long long)issued * CH <= (long long)o.max_iter + CH; };
+            // h is the state after the last chunk read; one chunk is in flight
+            auto may_run_after = [&]() {
+                for (auto &s : h)
+                    if (s.active && (long long)s.iters + CH < (long long)o.max_iter) return true;
+                return false;
+            };
             if (any_active()) {
                 issue(issued & 1);
                 ++issued;
             }
             while (collected < issued) {
-                if (pipe && can_issue()) {   // the next chunk, before this one's state is known
+                if (pipe && can_issue() && may_run_after()) {   // the next chunk, before this one's state is known
                     issue(issued & 1);
                     ++issued;
                 }
@@ -753,7 +761,7 @@ static void solve_group_fused(const Planes<T> &pl, const GroupDev &g, const CuPh
                 ++collected;
                 before = h;
                 if (!any_active()) break;   // a chunk still in flight finds every instance stopped: a no-op
-                if (!pipe && can_issue()) {
+                if ((!pipe || collected == issued) && can_issue()) {
                     issue(issued & 1);
                     ++issued;
                 }
