@@ -175,6 +175,35 @@ def test_bridge_regions_sparse_random(gpu_ready, use_mask):
                 bridge_radius=60)
 
 
+def test_bridge_regions_at_scale_known_answer(gpu_ready):
+    """4096 x 4096: five bands separated by one-pixel zero rows, each crossed
+    by diagonal zero lines (c - r = 301 k + 37 - 4096, c + r = 419 k + 53).
+    Those lines cut a band into 415 pieces under 4-connectivity but not
+    under 8-connectivity, so the bands stay whole only if every diagonal link
+    across the labeller's 64 x 16 tile borders is found (about 16,000 tiles,
+    one global forest).  Answer by construction: labels = band numbers (in
+    raster order), and every band moved onto band 1's cycle (the largest),
+    each pixel gaining float32(-J 2 pi) for its band's jump J."""
+    n = 4096
+    heights, jumps = [1100, 700, 900, 800, 592], [0, 3, -2, 5, -1]
+    band = np.zeros((n, n), np.int32)
+    r0 = 0
+    for k, h in enumerate(heights):
+        band[r0:r0 + h] = k + 1
+        r0 += h + 1
+    r, c = np.mgrid[0:n, 0:n]
+    band[(np.mod(c - r + n - 37, 301) == 0) | (np.mod(c + r - 53, 419) == 0)] = 0
+    J = np.array([0] + jumps, np.float64)
+    unw = np.where(band > 0, 0.5 + 1e-4 * r + 2e-4 * c + J[band] * TWO_PI, 0.0).astype(np.float32)
+    del r, c
+    u, cc = _cuphu_ext._bridge_apply_test(unw, mask=None, bridge_radius=500, bridge_min_num_pixel=14,
+                                          bridge_erosion_size=2, bridge_max_boundary_samples=4096,
+                                          bridge_ramp_type=None, bridge_ramp_max_num_sample=1000000)
+    np.testing.assert_array_equal(cc, band.astype(cc.dtype))
+    shift = np.array([np.float32(-j * TWO_PI) for j in J], np.float32)
+    np.testing.assert_array_equal(u, np.where(band > 0, unw + shift[band], unw).astype(np.float32))
+
+
 @pytest.mark.parametrize("ramp", RAMPS)
 def test_bridge_ramp_types_bitexact(gpu_ready, ramp):
     unw, isl, _ = _islands(7, (180, 240))
