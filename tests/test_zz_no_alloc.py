@@ -2,7 +2,9 @@
 callers that cannot allocate) against cuphu_laplace_unwrap_gpu, through the
 C ABI: the same unwrapped phase bit for bit on consecutive calls of
 decreasing size (the fixed scratch keeps the previous call's contents),
-up to the largest instance it holds; no change in free device memory
+on solves that converge inside a chunk or stop at a cap that is not a
+multiple of it (the allocating entry pipelines its chunks, this one does
+not), up to the largest instance it holds; no change in free device memory
 across its calls; and a clear error, before anything runs, for an instance
 beyond CUPHU_NOALLOC_MAX_ELEMENTS."""
 
@@ -85,6 +87,24 @@ def test_noalloc_equals_allocating_entry_on_two_calls(gpu_ready, monkeypatch):
     finally:
         big.close()
         small.close()
+
+
+@pytest.mark.parametrize("tol,max_iter", [(3e-2, 3000), (1e-2, 3000), (1e-6, 130)])
+def test_noalloc_equals_allocating_entry_on_converging_and_odd_capped_solves(gpu_ready, monkeypatch, tol, max_iter):
+    """The allocating entry pipelines its 50-iteration chunks (the next one is
+    issued before the state after the current one is read); the
+    no-allocation entry reads the state after every chunk first.  Solves that
+    converge inside a chunk (the chunk issued after it is a no-op) and a cap
+    that is not a multiple of the chunk must give the same phase bit for
+    bit."""
+    monkeypatch.setenv("CUPHU_N_IRLS", "2")
+    c = _Case(300, 517, 65)
+    try:
+        _capi.check(c.run(False, max_iter=max_iter, tol=tol), "cuphu_laplace_unwrap_gpu")
+        _capi.check(c.run(True, max_iter=max_iter, tol=tol), "cuphu_laplace_unwrap_gpu_noalloc")
+        np.testing.assert_array_equal(G.host(c.u_na), G.host(c.u_ref))
+    finally:
+        c.close()
 
 
 def test_noalloc_at_its_largest_instance(gpu_ready, monkeypatch):
