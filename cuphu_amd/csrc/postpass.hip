@@ -185,16 +185,44 @@ void spikes_1d(float *d_unw, const unsigned char *d_mask, int nrow, int ncol, bo
 // measured entries (rank m/2), then linear gap fill between measurements at
 // most 2*max_fill apart and a linear fade to 0 over max_fill entries
 // elsewhere (:1492-1583).
+// The median of each window is the entry of rank m/2 among its m measured
+// entries, the value nth_element selects.  The windows only slide forward,
+// so a sorted copy of the current window is kept and updated by one removal
+// and one insertion per position; a profile with a NaN entry (no order)
+// takes the per-window selection instead.
 void profile_smooth(std::vector<double> &x, std::vector<unsigned char> &have, int w, int max_fill) {
     const int n = (int)x.size(), half = w / 2;
-    std::vector<double> out((size_t)n, 0.0), seg;
-    for (int i = 0; i < n; ++i) {
-        if (!have[(size_t)i]) continue;
-        seg.clear();
-        for (int j = std::max(0, i - half); j < std::min(n, i + half + 1); ++j)
-            if (have[(size_t)j]) seg.push_back(x[(size_t)j]);
-        std::nth_element(seg.begin(), seg.begin() + (long)(seg.size() / 2), seg.end());
-        out[(size_t)i] = seg[seg.size() / 2];
+    std::vector<double> out((size_t)n, 0.0);
+    bool has_nan = false;
+    for (int i = 0; i < n; ++i)
+        if (have[(size_t)i] && std::isnan(x[(size_t)i])) has_nan = true;
+    if (has_nan) {
+        std::vector<double> seg;
+        for (int i = 0; i < n; ++i) {
+            if (!have[(size_t)i]) continue;
+            seg.clear();
+            for (int j = std::max(0, i - half); j < std::min(n, i + half + 1); ++j)
+                if (have[(size_t)j]) seg.push_back(x[(size_t)j]);
+            std::nth_element(seg.begin(), seg.begin() + (long)(seg.size() / 2), seg.end());
+            out[(size_t)i] = seg[seg.size() / 2];
+        }
+    } else {
+        std::vector<double> win;
+        win.reserve((size_t)w + 2);
+        int lo = 0, hi = -1;   // win holds the measured entries of [lo, hi], sorted
+        for (int i = 0; i < n; ++i) {
+            const int a = std::max(0, i - half), b = std::min(n - 1, i + half);
+            while (hi < b) {
+                ++hi;
+                if (have[(size_t)hi])
+                    win.insert(std::upper_bound(win.begin(), win.end(), x[(size_t)hi]), x[(size_t)hi]);
+            }
+            while (lo < a) {
+                if (have[(size_t)lo]) win.erase(std::lower_bound(win.begin(), win.end(), x[(size_t)lo]));
+                ++lo;
+            }
+            if (have[(size_t)i]) out[(size_t)i] = win[win.size() / 2];
+        }
     }
     x.swap(out);
     std::vector<int> idx;
@@ -229,20 +257,29 @@ void profile_smooth(std::vector<double> &x, std::vector<unsigned char> &have, in
     have.swap(fh);
 }
 
+// Device buffers of one feedback call, sized for the longer seam line and
+// shared by its seams (each seam's work ends with a stream sync).
+struct SeamBufs {
+    DevBuf<float> lines;
+    DevBuf<unsigned char> valid, have;
+    DevBuf<double> diff;
+    explicit SeamBufs(size_t n) : lines(2 * n), valid(2 * n), have(n), diff(n) {}
+};
+
 void feedback_seam(float *d_unw, const unsigned char *d_mask, int nrow, int ncol, int seam, bool by_col, int feather,
-                   hipStream_t st) {
+                   SeamBufs &bufs, hipStream_t st) {
     const int n = by_col ? nrow : ncol, lim = by_col ? ncol : nrow;
     if (seam - 1 < 0 || seam + 1 >= lim) return;   // :1587, :1624
     // the pixel lines just before and at the seam, gathered on the device
-    DevBuf<float> d_lines(2 * (size_t)n);
-    DevBuf<unsigned char> d_valid(2 * (size_t)n);
+    float *d_lines = bufs.lines.get();
+    unsigned char *d_valid = bufs.valid.get();
     hipLaunchKernelGGL(k_seam_lines, dim3((unsigned)ceil_div((size_t)n, 256)), dim3(256), 0, st, d_unw, d_mask, nrow,
-                       ncol, seam, by_col ? 1 : 0, d_lines.get(), d_valid.get());
+                       ncol, seam, by_col ? 1 : 0, d_lines, d_valid);
     CUPHU_HIP_CHECK(hipGetLastError());
     std::vector<float> lines(2 * (size_t)n);
     std::vector<unsigned char> valid(2 * (size_t)n);
-    CUPHU_HIP_CHECK(hipMemcpyAsync(lines.data(), d_lines.get(), lines.size() * 4, hipMemcpyDeviceToHost, st));
-    CUPHU_HIP_CHECK(hipMemcpyAsync(valid.data(), d_valid.get(), valid.size(), hipMemcpyDeviceToHost, st));
+    CUPHU_HIP_CHECK(hipMemcpyAsync(lines.data(), d_lines, lines.size() * 4, hipMemcpyDeviceToHost, st));
+    CUPHU_HIP_CHECK(hipMemcpyAsync(valid.data(), d_valid, valid.size(), hipMemcpyDeviceToHost, st));
     CUPHU_HIP_CHECK(hipStreamSynchronize(st));
     const float *a = lines.data(), *b = lines.data() + n;
     const unsigned char *ma = valid.data(), *mb = valid.data() + n;
@@ -257,13 +294,13 @@ void feedback_seam(float *d_unw, const unsigned char *d_mask, int nrow, int ncol
     profile_smooth(diff, have, smooth, 2 * smooth);
     const int extent = std::min(lim, seam + feather) - seam;
     if (extent <= 0) return;
-    DevBuf<double> d_diff((size_t)n);
-    DevBuf<unsigned char> d_have((size_t)n);
-    CUPHU_HIP_CHECK(hipMemcpyAsync(d_diff.get(), diff.data(), (size_t)n * 8, hipMemcpyHostToDevice, st));
-    CUPHU_HIP_CHECK(hipMemcpyAsync(d_have.get(), have.data(), (size_t)n, hipMemcpyHostToDevice, st));
+    double *d_diff = bufs.diff.get();
+    unsigned char *d_have = bufs.have.get();
+    CUPHU_HIP_CHECK(hipMemcpyAsync(d_diff, diff.data(), (size_t)n * 8, hipMemcpyHostToDevice, st));
+    CUPHU_HIP_CHECK(hipMemcpyAsync(d_have, have.data(), (size_t)n, hipMemcpyHostToDevice, st));
     const size_t total = by_col ? (size_t)nrow * extent : (size_t)extent * ncol;
     hipLaunchKernelGGL(k_feather, dim3((unsigned)ceil_div(total, 256)), dim3(256), 0, st, d_unw, d_mask, nrow, ncol, seam,
-                       by_col ? 1 : 0, feather, extent, d_diff.get(), d_have.get());
+                       by_col ? 1 : 0, feather, extent, d_diff, d_have);
     CUPHU_HIP_CHECK(hipGetLastError());
     CUPHU_HIP_CHECK(hipStreamSynchronize(st));   // host profile buffers are freed on return
 }
@@ -280,8 +317,10 @@ void neighbor_feedback_dev(float *d_unw, const unsigned char *d_mask, int nrow, 
                            const std::vector<int> &row_seams, int feather, hipStream_t st) {
     if (feather <= 0) return;
     TraceRange tr("cuphu:neighbor_feedback");
-    for (int s : col_seams) feedback_seam(d_unw, d_mask, nrow, ncol, s, true, feather, st);
-    for (int s : row_seams) feedback_seam(d_unw, d_mask, nrow, ncol, s, false, feather, st);
+    if (col_seams.empty() && row_seams.empty()) return;
+    SeamBufs bufs((size_t)std::max(nrow, ncol));
+    for (int s : col_seams) feedback_seam(d_unw, d_mask, nrow, ncol, s, true, feather, bufs, st);
+    for (int s : row_seams) feedback_seam(d_unw, d_mask, nrow, ncol, s, false, feather, bufs, st);
 }
 
 }  // namespace cuphu
