@@ -35,8 +35,9 @@ __device__ __forceinline__ float pp_value(uint32_t k) {
 
 // Line l's valid elements are base(l) + j*estride, j < len, base = l*lstride.
 // med[l] = the valid element of rank count/2 (the reference's nth_element
-// at size/2), have[l] = count > 20.  Workgroups are spread so that each XCD
-// takes a contiguous run of lines (neighbouring columns share cache lines).
+// at size/2), have[l] = count > 20.  Used for rows (contiguous elements;
+// k_col_median below takes the columns).  Workgroups are spread so that each
+// XCD takes a contiguous run of lines.
 __global__ __launch_bounds__(256) void k_line_median(const float *unw, const unsigned char *mask, int nlines, int len,
                                                      long long lstride, long long estride, double *med,
                                                      unsigned char *have) {
@@ -96,6 +97,87 @@ __global__ __launch_bounds__(256) void k_line_median(const float *unw, const uns
     }
 }
 
+// The same selection for the columns of an image (elements ncol apart), 32
+// adjacent columns per workgroup.  Each wave reads two rows of 32 consecutive
+// floats (one 128-byte line each) per step, so every line fetched serves 32
+// columns; with one workgroup per column each line fetched served one float,
+// and the column pass moved 16.6x its bytes.  Per-column counts in LDS (rows
+// of 257 words: the same bucket of neighbouring columns falls in different
+// banks); thread j < 32 selects column c0 + j's next 8 bits.
+constexpr int kColBlk = 32;
+__global__ __launch_bounds__(256) void k_col_median(const float *unw, const unsigned char *mask, int nrow, int ncol,
+                                                    double *med, unsigned char *have) {
+    __shared__ unsigned cnt[kColBlk][257];
+    __shared__ unsigned s_prefix[kColBlk], s_rank[kColBlk], s_stop[kColBlk];
+    const int c0 = (int)blockIdx.x * kColBlk;
+    const int j = (int)threadIdx.x % kColBlk, g = (int)threadIdx.x / kColBlk;   // column in the block, row group
+    const int c = c0 + j;
+    if (threadIdx.x < kColBlk) { s_prefix[threadIdx.x] = 0u; s_rank[threadIdx.x] = 0u; s_stop[threadIdx.x] = 0u; }
+    for (int pass = 0; pass < 4; ++pass) {
+        const int shift = 24 - 8 * pass;
+        for (int k = (int)threadIdx.x; k < kColBlk * 257; k += 256) (&cnt[0][0])[k] = 0u;
+        __syncthreads();
+        const unsigned prefix = s_prefix[j];
+        if (c < ncol && !s_stop[j]) {
+            // rows g, g + RG, g + 2 RG, ...; eight of them loaded before any is
+            // counted, so eight loads are in flight per thread
+            constexpr int RG = 256 / kColBlk, U = 8;
+            for (int r0 = g; r0 < nrow; r0 += U * RG) {
+                float v[U];
+                bool ok[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int r = r0 + u * RG;
+                    ok[u] = r < nrow;
+                    v[u] = 0.0f;
+                    if (ok[u]) {
+                        const long long i = (long long)r * ncol + c;
+                        v[u] = unw[i];
+                        if (mask) ok[u] = mask[i] != 0;
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    if (!ok[u]) continue;
+                    const uint32_t key = pp_key(v[u]);
+                    if (pass > 0 && (key >> (shift + 8)) != prefix) continue;
+                    atomicAdd(&cnt[j][(key >> shift) & 255u], 1u);
+                }
+            }
+        }
+        __syncthreads();
+        if ((int)threadIdx.x < kColBlk && c0 + (int)threadIdx.x < ncol && !s_stop[threadIdx.x]) {
+            const int jj = (int)threadIdx.x;
+            unsigned rank = s_rank[jj];
+            bool stop = false;
+            if (pass == 0) {
+                unsigned tot = 0u;
+                for (int k = 0; k < 256; ++k) tot += cnt[jj][k];
+                if (tot <= 20u) stop = true;
+                else rank = tot / 2u;
+            }
+            if (stop) {
+                s_stop[jj] = 1u;
+            } else {
+                unsigned below = 0u;
+                int k = 0;
+                for (; k < 255; ++k) {
+                    if (rank < below + cnt[jj][k]) break;
+                    below += cnt[jj][k];
+                }
+                s_prefix[jj] = (s_prefix[jj] << 8) | (unsigned)k;
+                s_rank[jj] = rank - below;
+            }
+        }
+        __syncthreads();
+    }
+    if ((int)threadIdx.x < kColBlk && c0 + (int)threadIdx.x < ncol) {
+        const int jj = (int)threadIdx.x;
+        med[c0 + jj] = s_stop[jj] ? 0.0 : (double)pp_value(s_prefix[jj]);
+        have[c0 + jj] = s_stop[jj] ? 0 : 1;
+    }
+}
+
 // unw -= shift[line] on the valid pixels of every line with a nonzero shift.
 __global__ void k_line_shift(float *unw, const unsigned char *mask, int nrow, int ncol, int by_row, const float *shift) {
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
@@ -150,8 +232,12 @@ void spikes_1d(float *d_unw, const unsigned char *d_mask, int nrow, int ncol, bo
     DevBuf<double> d_med((size_t)nl);
     DevBuf<unsigned char> d_have((size_t)nl);
     DevBuf<float> d_shift((size_t)nl);
-    hipLaunchKernelGGL(k_line_median, dim3((unsigned)nl), dim3(256), 0, st, d_unw, d_mask, nl, len,
-                       by_row ? (long long)ncol : 1LL, by_row ? 1LL : (long long)ncol, d_med.get(), d_have.get());
+    if (by_row)
+        hipLaunchKernelGGL(k_line_median, dim3((unsigned)nl), dim3(256), 0, st, d_unw, d_mask, nl, len, (long long)ncol,
+                           1LL, d_med.get(), d_have.get());
+    else
+        hipLaunchKernelGGL(k_col_median, dim3((unsigned)ceil_div((size_t)ncol, (size_t)kColBlk)), dim3(256), 0, st,
+                           d_unw, d_mask, nrow, ncol, d_med.get(), d_have.get());
     CUPHU_HIP_CHECK(hipGetLastError());
     std::vector<double> med((size_t)nl);
     std::vector<unsigned char> have((size_t)nl);
