@@ -101,7 +101,7 @@ __global__ __launch_bounds__(256) void k_line_median(const float *unw, const uns
 // adjacent columns per workgroup.  Each wave reads two rows of 32 consecutive
 // floats (one 128-byte line each) per step, so every line fetched serves 32
 // columns; with one workgroup per column each line fetched served one float,
-// and the column pass moved 16.6x its bytes.  Per-column counts in LDS (rows
+// and the column pass moved 17x its bytes.  Per-column counts in LDS (rows
 // of 257 words: the same bucket of neighbouring columns falls in different
 // banks); thread j < 32 selects column c0 + j's next 8 bits.
 constexpr int kColBlk = 32;
