@@ -246,10 +246,16 @@ static void check_options(const CuPhuLaplaceOptions &o) {
     CUPHU_REQUIRE(o.tol >= 0.0 && o.max_iter >= 0, "tol and max_iter must be >= 0");
 }
 
-static int rows_per_wave() {
-    int rpw = 4;
+// Rows per lane group of the stencil tiles: CUPHU_LAPLACE_RPW (2, 4 or 8),
+// else 4 in FP32 and 2 in FP64.  The FP64 iteration kernel holds twice the
+// registers (292 VGPRs at 4 rows: one workgroup per CU; about 198 at 2):
+// the 8192^2 scene's FP64 solve took 6.23 s in 128 x 32 tiles and 5.80 s
+// in 128 x 16 ones (256 x 8: 5.85 s).
+static int rows_per_wave(int precision) {
+    int rpw = 0;
     env_int("CUPHU_LAPLACE_RPW", &rpw);
-    return (rpw == 2 || rpw == 8) ? rpw : 4;
+    if (rpw == 2 || rpw == 4 || rpw == 8) return rpw;
+    return precision == 64 ? 2 : 4;
 }
 
 // ── layout ──────────────────────────────────────────────────────────────────
@@ -883,7 +889,7 @@ static size_t tiled_blocks(const Layout &L, TileShape sh) {
     return n;
 }
 
-// Default shape: rows_per_wave() (CUPHU_LAPLACE_RPW) rows per lane group and
+// Default shape: rows_per_wave() (CUPHU_LAPLACE_RPW; 4 in FP32, 2 in FP64) rows per lane group and
 // 256-column tiles, or, on the fused solver, 128-column tiles when every
 // instance's width is a multiple of 256, both shapes leave exactly as many
 // lanes idle and the launch spans at least 4 resident rounds.  Measured on one GPU, alternating pairs:
@@ -896,7 +902,7 @@ static size_t tiled_blocks(const Layout &L, TileShape sh) {
 //     kernel: their loads are never issued).
 // CUPHU_LAPLACE_LPR=64 or 32 forces the width (64 or 32 lanes per tile row).
 static TileShape default_shape(const Layout &L, const CuPhuLaplaceOptions &o) {
-    const int rpw = rows_per_wave();
+    const int rpw = rows_per_wave(o.precision);
     const TileShape wide{rpw, 64}, narrow{rpw, 32};
     if (o.solver != CUPHU_SOLVER_FUSED || fused_stored_q() || rpw > 4) return wide;
     int lpr = 0;
@@ -1080,7 +1086,7 @@ static void run_batch(const std::vector<std::pair<int, int>> &dims, const CuPhuL
                       long maxncomps, bool do_conncomp, DevRes &res, hipStream_t st, CuPhuSolveStats *stats,
                       BatchOut &out, Ingest &&ingest) {
     const bool stride256 = wide_stride(o);
-    Layout L = make_layout(dims, TileShape{rows_per_wave(), 64}, stride256);
+    Layout L = make_layout(dims, TileShape{rows_per_wave(o.precision), 64}, stride256);
     int ninst = (int)dims.size();
     PoolSet<T> P = alloc_pools<T>(res, L.total_el);
     InstDesc *d_gdesc = ws<InstDesc>(res, "gdesc", (size_t)ninst);
@@ -1516,7 +1522,8 @@ static void wls_impl(const T *d_w, const float *d_phase, const T *d_u, const T *
     o.max_iter = max_iter;
     o.tol = tol;
     o.solver = CUPHU_SOLVER_FUSED;
-    Layout L0 = make_layout({{nrow, ncol}}, TileShape{rows_per_wave(), 64});
+    o.precision = sizeof(T) == 8 ? 64 : 32;   // picks the default tile shape
+    Layout L0 = make_layout({{nrow, ncol}}, TileShape{rows_per_wave(o.precision), 64});
     const TileShape sh = default_shape(L0, o);
     Layout L = make_layout({{nrow, ncol}}, sh);
     PoolSet<T> P = alloc_pools<T>(res, L.total_el);
@@ -2569,7 +2576,7 @@ int cuphu_test_laplace_ops(const float *phase, const short *sigsq, const float *
         hipStream_t st = res.stream;
         int lpr = 64;
         env_int("CUPHU_LAPLACE_LPR", &lpr);   // 32: the fused solver's 128-column tiles
-        const TileShape sh{rows_per_wave(), (lpr == 32 && rows_per_wave() <= 4) ? 32 : 64};
+        const TileShape sh{rows_per_wave(32), (lpr == 32 && rows_per_wave(32) <= 4) ? 32 : 64};
         HookInst h;
         hook_setup(h, res, nrow, ncol, sh, st);
         const InstDesc &d = h.d;
