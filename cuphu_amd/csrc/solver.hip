@@ -535,9 +535,6 @@ static bool wide_stride(const CuPhuLaplaceOptions &o) {
     return v != 64 && o.solver == CUPHU_SOLVER_FUSED && !fused_stored_q();
 }
 
-// Chunk pipelining (solve_group_fused) for grids of at most this many
-// workgroups (16 M pixels).
-
 // Folded reductions (launch_iq_fold) for grids of at most this many
 // workgroups, where the reduction launch it replaces is a large part of an
 // iteration and every workgroup's extra read of its instance's partial sums
