@@ -65,7 +65,7 @@ BYTES_PER_PX_XR = 28.0
 # (profiles/k_iq_scene8192_hbm_traffic.md, profiles/k_pq_k_xr_scene8192_hbm_traffic.md).
 # Re-measure after kernel changes.
 TRAFFIC_PER_PX_PQ = 2770.6e6 / 84.93e6
-TRAFFIC_PER_PX_IT = 2557.0e6 / 84.93e6   # mean over alternating x-deferred / x-applying launches
+TRAFFIC_PER_PX_IT = 2557.1e6 / 84.93e6   # mean over alternating x-deferred / x-applying launches
 
 
 def log(msg):
