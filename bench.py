@@ -626,28 +626,31 @@ def leg_next_stages(scene, base_opts, log):
     _capi.check(_capi.lib.cuphu_build_smooth_costs_gpu(C.c_void_p(ph.data_ptr()), C.c_void_p(dcorr.data_ptr()),
                                                        None, None, nrow, ncol, C.byref(prm), 7, 7, C.byref(dcost),
                                                        st), "smooth costs")
-    g = torch.Generator(device="cuda").manual_seed(5)
-    flows = torch.randint(-1, 2, (narcs,), dtype=torch.int16, device="cuda", generator=g)
-    pos = torch.empty(narcs, dtype=torch.int16, device="cuda")
-    neg = torch.empty(narcs, dtype=torch.int16, device="cuda")
-
-    def incr():
-        _capi.check(_capi.lib.cuphu_incrcost_gpu(dcost, C.c_void_p(flows.data_ptr()), narcs, int(prm.nshortcycle), 1,
-                                                 C.c_void_p(pos.data_ptr()), C.c_void_p(neg.data_ptr()), None, st),
-                    "incrcost")
-
-    incr()
-    torch.cuda.synchronize()
-    reps = 20
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        incr()          # returns after the stream sync that reads the two minima back
-    us = 1e6 * (time.perf_counter() - t0) / reps
     rt = C.CDLL(None)
     rt.hipFree.argtypes = [C.c_void_p]
     rt.hipFree.restype = C.c_int
-    if rt.hipFree(dcost) != 0:
-        raise RuntimeError("hipFree of the device costs failed")
+    try:
+        g = torch.Generator(device="cuda").manual_seed(5)
+        flows = torch.randint(-1, 2, (narcs,), dtype=torch.int16, device="cuda", generator=g)
+        pos = torch.empty(narcs, dtype=torch.int16, device="cuda")
+        neg = torch.empty(narcs, dtype=torch.int16, device="cuda")
+
+        def incr():
+            _capi.check(_capi.lib.cuphu_incrcost_gpu(dcost, C.c_void_p(flows.data_ptr()), narcs,
+                                                     int(prm.nshortcycle), 1, C.c_void_p(pos.data_ptr()),
+                                                     C.c_void_p(neg.data_ptr()), None, st), "incrcost")
+
+        incr()
+        torch.cuda.synchronize()
+        reps = 20
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            incr()          # returns after the stream sync that reads the two minima back
+        us = 1e6 * (time.perf_counter() - t0) / reps
+    finally:
+        torch.cuda.synchronize()
+        if rt.hipFree(dcost) != 0:
+            raise RuntimeError("hipFree of the device costs failed")
     gbs = 10.0 * narcs / (us * 1e-6) / 1e9
     log(f"incrcost: {us:.1f} us per call over {narcs} arcs ({gbs:.0f} GB/s at 10 B per arc, call included)")
     return {"workload": "scene8192 through cuphu_amd.unwrap (reference tile formula, same PCG/IRLS options); "
