@@ -128,6 +128,11 @@ void launch_reduce_it(const InstDesc *d_desc, int ninst, const double4 *partials
 template <class T>
 void launch_iq(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, const SolveScal *scal,
                double4 *partials, TileShape sh, int parity, int start, hipStream_t st);
+// The same iteration in the merged tiling (k_iq_mix): d_tiles lists the
+// launch's ntiles workgroups as {instance, tile column, tile row, tall}.
+template <class T>
+void launch_iq_mix(const InstDesc *d_desc, const int4 *d_tiles, int ntiles, const Planes<T> &pl,
+                   const SolveScal *scal, double4 *partials, int rpw, int parity, int start, hipStream_t st);
 // The same iteration with the previous launch's reduction folded in (for
 // grids of a few resident rounds, where the reduction's own launch is a
 // large part of an iteration): reduces part_prev in the reduction's order,

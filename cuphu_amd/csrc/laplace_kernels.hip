@@ -146,14 +146,17 @@ __device__ __forceinline__ TileInfo get_tile(const InstDesc *descs, const int *b
 // v*RPW + RPW - 1.  LPR 64 is the 256 x 4*RPW tile; LPR 32 the 128 x 8*RPW
 // tile (same pixels per workgroup, 5 % fewer halo pixels), which the fused
 // solver takes on large launches that both shapes tile without idle lanes.
+// LPR 16, the 64 x 16*RPW tile, is the iteration kernel's tall tile for an
+// instance's last 64 columns (k_iq_mix).
 template <int RPW, int LPR>
 struct Shape {
-    static_assert(LPR == 64 || LPR == 32, "lanes per tile row");
+    static_assert(LPR == 64 || LPR == 32 || LPR == 16, "lanes per tile row");
     static constexpr int G = 64 / LPR;      // lane groups per wave
     static constexpr int TW = 4 * LPR;      // tile width
     static constexpr int TH = 4 * G * RPW;  // tile height
     static constexpr int S = TW + 8;        // LDS row stride; tile col j -> LDS col kLdsC0 + j
-    static_assert(TH + 2 <= 64, "ring columns: one lane per row");
+    static constexpr int L0 = (TH + 4) * S; // the iteration kernel's LDS: p_j with a two-pixel halo
+    static constexpr int L1 = (TH + 2) * S; // and p_{j+1} with a one-pixel halo
 };
 
 // Halo fill: tile rows -1 and TH (lane group 0 of waves 0 and 3, 4 pixels
@@ -163,6 +166,7 @@ struct Shape {
 // stores 0.
 template <int TH, int LPR = 64, class P4, class P1>
 __device__ __forceinline__ void fill_halo(const TileInfo &t, int wave, int lane, P4 &&put4, P1 &&put1) {
+    static_assert(TH + 2 <= 64, "ring columns: one lane per row");
     constexpr int TW = 4 * LPR;
     const InstDesc &d = t.d;
     if (wave == 0 || wave == 3) {
@@ -1192,17 +1196,28 @@ __device__ __forceinline__ void stagger_first_round() {
 }
 
 // The iteration body; FOLD: the previous launch's reduction first (see
-// launch_iq_fold).  Two kernels wrap it (k_iq, k_iq_fold) so that each keeps
-// its own register budget.
-template <class T, int RPW, int LPR, bool STAGGER, bool FOLD>
-__device__ __forceinline__ void iq_body(const InstDesc *descs, const int *blk_inst, const Planes<T> &pl,
+// launch_iq_fold).  Three kernels wrap it (k_iq, k_iq_fold, k_iq_mix) so that
+// each keeps its own register budget.  lp0 (Sh::L0 elements) holds p_j, LDS
+// row = row + 2, col = col + kLdsC0; lp1 (Sh::L1) p_{j+1}, LDS row = row + 1.
+// EXT (k_iq_mix, two tile shapes in one kernel): the tile is *ext_t and the
+// LDS the caller's lx0 / lx1, one allocation for both shapes; otherwise the
+// body declares its LDS and looks its tile up itself (a tile passed in from
+// the wrapper lost the compiler some index range facts: 0.5 % longer k_iq
+// launches).
+template <class T, int RPW, int LPR, bool STAGGER, bool FOLD, bool EXT = false>
+__device__ __forceinline__ void iq_body(const InstDesc *descs, const int *blk_inst, const TileInfo *ext_t,
+                                        T *__restrict__ lx0, T *__restrict__ lx1, const Planes<T> &pl,
                                         const SolveScal *scal, double4 *partials, int parity, int start,
                                         const FoldArgs &fa) {
     using Sh = Shape<RPW, LPR>;
     constexpr int TH = Sh::TH, S = Sh::S, TW = Sh::TW;
-    __shared__ T lp0[(TH + 4) * S];   // p_j: LDS row = row + 2, col = col + kLdsC0
-    __shared__ T lp1[(TH + 2) * S];   // p_{j+1}: LDS row = row + 1
-    TileInfo t = get_tile(descs, blk_inst, TH, TW);
+    __shared__ __align__(16) T own0[EXT ? 4 : Sh::L0];
+    __shared__ __align__(16) T own1[EXT ? 4 : Sh::L1];
+    T *__restrict__ lp0 = EXT ? lx0 : own0;
+    T *__restrict__ lp1 = EXT ? lx1 : own1;
+    TileInfo t;
+    if constexpr (EXT) t = *ext_t;
+    else t = get_tile(descs, blk_inst, TH, TW);
     const SolveScal *sc;
     if constexpr (FOLD) {
         // the previous launch's reduction, done here (every workgroup of the
@@ -1333,19 +1348,29 @@ __device__ __forceinline__ void iq_body(const InstDesc *descs, const int *blk_in
         }
     }
     // ring columns (wave 1: column -1, wave 2: column TW): lane l < TH owns
-    // pixel (l, col); lanes l < TH + 2 stage p_j at (l - 1, col) and lanes
-    // l < TH p_j at (l, col -/+ 1) beyond it
+    // pixel (l, col); lanes l < TH + 2 stage p_j at (l - 1, col) (the 64-row
+    // tall tile stages its last two rows in a second pass) and lanes l < TH
+    // p_j at (l, col -/+ 1) beyond it
+    static_assert(TH <= 64, "ring columns: one lane per own row");
     T cr = T(0), cp = T(0), cwd = T(0), cwr = T(0), cwu = T(0), cwl = T(0);
     if (wave == 1 || wave == 2) {
         const int col = (wave == 1) ? t.tx0 - 1 : t.tx0 + TW;
         const int col2 = (wave == 1) ? col - 1 : col + 1;
         const bool colok = col >= 0 && col < d.wcols, col2ok = col2 >= 0 && col2 < d.wcols;
         const int lcol = (wave == 1) ? -1 : TW, lcol2 = (wave == 1) ? -2 : TW + 1;
-        if (lane < TH + 2) {
-            const int lr = lane - 1;
-            T v = T(0);
-            if (upd && colok && row_ok(lr)) v = p_in[elem(lr, col)];
-            if (upd) lp0[(lr + 2) * S + kLdsC0 + lcol] = v;
+        if constexpr (TH + 2 <= 64) {
+            if (lane < TH + 2) {
+                const int lr = lane - 1;
+                T v = T(0);
+                if (upd && colok && row_ok(lr)) v = p_in[elem(lr, col)];
+                if (upd) lp0[(lr + 2) * S + kLdsC0 + lcol] = v;
+            }
+        } else {
+            for (int lr = lane - 1; lr <= TH; lr += 64) {
+                T v = T(0);
+                if (upd && colok && row_ok(lr)) v = p_in[elem(lr, col)];
+                if (upd) lp0[(lr + 2) * S + kLdsC0 + lcol] = v;
+            }
         }
         if (lane < TH) {
             const int lr = lane;
@@ -1477,7 +1502,8 @@ template <class T, int RPW, int LPR, bool STAGGER>
 __global__ __launch_bounds__(256) void k_iq(const InstDesc *descs, const int *blk_inst, Planes<T> pl,
                                             const SolveScal *scal, double4 *partials, int parity, int start) {
     const FoldArgs none{nullptr, nullptr, 0, 0.0};
-    iq_body<T, RPW, LPR, STAGGER, false>(descs, blk_inst, pl, scal, partials, parity, start, none);
+    iq_body<T, RPW, LPR, STAGGER, false>(descs, blk_inst, nullptr, nullptr, nullptr, pl, scal, partials, parity,
+                                         start, none);
 }
 // The folded FP32 kernel with 4 rows per lane group is held to 3 resident
 // workgroups per CU (168 registers), like the unfolded one.
@@ -1485,7 +1511,49 @@ template <class T, int RPW, int LPR>
 __global__ __launch_bounds__(256, (RPW == 4 && sizeof(T) == 4) ? 3 : 1) void k_iq_fold(
     const InstDesc *descs, const int *blk_inst, Planes<T> pl, const SolveScal *scal_prev, double4 *partials,
     int parity, FoldArgs fa) {
-    iq_body<T, RPW, LPR, false, true>(descs, blk_inst, pl, scal_prev, partials, parity, 0, fa);
+    iq_body<T, RPW, LPR, false, true>(descs, blk_inst, nullptr, nullptr, nullptr, pl, scal_prev, partials, parity,
+                                      0, fa);
+}
+
+// The iteration on 256 x TH tiles (TH = 16 RPW) where an instance's last tile
+// column holds 64 work columns (a row stride of 256 k + 64 columns, as the
+// reference tile formula's 1843-column tiles have): that column runs in
+// 64 x 4TH tall tiles, one workgroup for four of the 256-column tiles, which
+// would each keep three quarters of their lanes idle.  Workgroups carry a
+// fixed cost (tile lookup, halo ring, LDS phases, block sums) that the other
+// workgroups' memory traffic does not hide, so four quarter-filled ones cost
+// about 2.2 full ones and the tall one about 1.2.  tiles[b] = {instance, tile
+// column, tile row, tall} in the launch order (b after the XCD remap), tile
+// coordinates in 256 x TH units.  A tall tile's sums go to the raster slot of
+// the first 256 x TH tile it covers and the slots of the other three hold
+// zeros, so the reductions read the same slots as after k_iq.
+template <class T, int RPW, bool STAGGER>
+__global__ __launch_bounds__(256) void k_iq_mix(const InstDesc *descs, const int4 *tiles, Planes<T> pl,
+                                                const SolveScal *scal, double4 *partials, int parity, int start) {
+    using Sw = Shape<RPW, 64>;
+    using Sn = Shape<RPW, 16>;
+    static_assert(Sn::TH == 4 * Sw::TH && Sn::TW == Sw::TW / 4, "a tall tile covers four wide tiles' rows");
+    constexpr int kWide = Sw::L0 + Sw::L1, kTall = Sn::L0 + Sn::L1;
+    __shared__ __align__(16) T lds[kWide > kTall ? kWide : kTall];
+    const int b = xcd_block((int)blockIdx.x, (int)gridDim.x);
+    const int4 q = tiles[b];
+    TileInfo t;
+    t.inst = q.x;
+    t.d = descs[q.x];
+    t.blk = t.d.block_begin + q.z * t.d.tiles_x + q.y;
+    t.tx0 = q.y * Sw::TW;
+    t.ty0 = q.z * Sw::TH;
+    const FoldArgs none{nullptr, nullptr, 0, 0.0};
+    if (q.w) {
+        iq_body<T, RPW, 16, STAGGER, false, true>(descs, nullptr, &t, lds, lds + Sn::L0, pl, scal, partials,
+                                                  parity, start, none);
+        const int k = (int)threadIdx.x;
+        if (k >= 1 && k < 4 && q.z + k < t.d.tiles_y)
+            partials[t.blk + k * t.d.tiles_x] = make_double4(0.0, 0.0, 0.0, 0.0);
+    } else {
+        iq_body<T, RPW, 64, STAGGER, false, true>(descs, nullptr, &t, lds, lds + Sw::L0, pl, scal, partials,
+                                                  parity, start, none);
+    }
 }
 
 // True residual for the q-free fused solver at the end of an even chunk
@@ -1976,6 +2044,23 @@ void launch_iq(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<
     CUPHU_LAUNCH_CHECK();
 }
 template <class T>
+void launch_iq_mix(const InstDesc *d_desc, const int4 *d_tiles, int ntiles, const Planes<T> &pl,
+                   const SolveScal *scal, double4 *partials, int rpw, int parity, int start, hipStream_t st) {
+    CUPHU_REQUIRE(rpw == 2 || rpw == 4, "the merged iteration launch runs 2 or 4 rows per lane group");
+    if (ntiles >= kStaggerMinGrid) {
+        if (rpw == 2) hipLaunchKernelGGL((k_iq_mix<T, 2, true>), dim3(ntiles), dim3(256), 0, st, d_desc, d_tiles, pl,
+                                         scal, partials, parity, start);
+        else hipLaunchKernelGGL((k_iq_mix<T, 4, true>), dim3(ntiles), dim3(256), 0, st, d_desc, d_tiles, pl, scal,
+                                partials, parity, start);
+    } else {
+        if (rpw == 2) hipLaunchKernelGGL((k_iq_mix<T, 2, false>), dim3(ntiles), dim3(256), 0, st, d_desc, d_tiles,
+                                         pl, scal, partials, parity, start);
+        else hipLaunchKernelGGL((k_iq_mix<T, 4, false>), dim3(ntiles), dim3(256), 0, st, d_desc, d_tiles, pl, scal,
+                                partials, parity, start);
+    }
+    CUPHU_LAUNCH_CHECK();
+}
+template <class T>
 void launch_iq_fold(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl,
                     const SolveScal *scal_prev, SolveScal *scal_next, const double4 *part_prev, double4 *partials,
                     TileShape sh, int parity, int max_iter, double tol2, hipStream_t st) {
@@ -2070,6 +2155,8 @@ void launch_plain_dot(const T *a, const T *b, size_t n, double *partials, double
                                   double4 *, int, hipStream_t);                                                  \
     template void launch_iq<T>(const InstDesc *, const int *, int, const Planes<T> &, const SolveScal *,         \
                                double4 *, TileShape, int, int, hipStream_t);                                     \
+    template void launch_iq_mix<T>(const InstDesc *, const int4 *, int, const Planes<T> &, const SolveScal *,    \
+                                   double4 *, int, int, int, hipStream_t);                                       \
     template void launch_iq_fold<T>(const InstDesc *, const int *, int, const Planes<T> &, const SolveScal *,    \
                                     SolveScal *, const double4 *, double4 *, TileShape, int, int, double,        \
                                     hipStream_t);                                                                \

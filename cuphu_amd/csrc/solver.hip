@@ -375,7 +375,9 @@ struct GroupDev {
     SolveScal *scal2;    // the folded iteration's second state buffer
     double4 *part4b;     // and its second partial-sum buffer
     SolveScal *hstate[2] = {nullptr, nullptr};   // pinned host copies of the state (chunk pipelining), or null
+    int4 *mix = nullptr;   // the merged iteration launch's tile list (k_iq_mix), or null
     int ninst, nblk;
+    int nmix = 0;
     std::vector<long long> px;   // host: logical pixels per instance
 };
 
@@ -576,6 +578,7 @@ static void solve_group_fused(const Planes<T> &pl, const GroupDev &g, const CuPh
     const double pair_values_per_px = sq ? 20.0 : 15.0;
     auto it = [&](int parity, int start) {
         if (sq) launch_it<T>(g.desc, g.blk, g.nblk, pl, g.scal, g.part4, rpw, parity, start, st);
+        else if (g.nmix) launch_iq_mix<T>(g.desc, g.mix, g.nmix, pl, g.scal, g.part4, rpw, parity, start, st);
         else launch_iq<T>(g.desc, g.blk, g.nblk, pl, g.scal, g.part4, sh, parity, start, st);
     };
     auto step = [&](int i) {
@@ -923,6 +926,46 @@ static void retile(Layout &L, TileShape sh) {
     }
 }
 
+// ── merged tiling of the iteration kernel (k_iq_mix) ────────────────────────
+// On 256-column tiles, an instance whose last tile column holds 64 work
+// columns (wcols = 256 k + 64, as the reference tile formula's 1843-column
+// tiles have) runs that column in 64 x 4TH tall tiles: one workgroup instead
+// of four quarter-filled ones.  CUPHU_LAPLACE_MERGE=0 turns it off.
+static bool tall_column(const InstDesc &d, TileShape sh) {
+    return sh.lpr == 64 && d.wcols - (d.tiles_x - 1) * sh.tw() == 64;
+}
+static bool merge_allowed(TileShape sh) {
+    int v = 1;
+    env_int("CUPHU_LAPLACE_MERGE", &v);
+    return v != 0 && sh.lpr == 64 && (sh.rpw == 2 || sh.rpw == 4) && !fused_stored_q();
+}
+// Appends instance i's workgroups (descriptor d, tile coordinates in
+// 256 x TH units) in the launch order of get_tile (raster, or band by band),
+// with the tall column's four-tile groups at the place of their first tile.
+static void mix_tiles(const InstDesc &d, int i, TileShape sh, std::vector<int4> &out) {
+    const bool tall = tall_column(d, sh);
+    const int n = d.tiles_x * d.tiles_y;
+    for (int local = 0; local < n; ++local) {
+        int tx, ty;
+        if (d.band > 0) {
+            const int bsz = d.band * d.tiles_y;
+            const int k = local / bsz;
+            const int bw = std::min(d.band, d.tiles_x - k * d.band);
+            const int rem = local - k * bsz;
+            ty = rem / bw;
+            tx = k * d.band + rem % bw;
+        } else {
+            tx = local % d.tiles_x;
+            ty = local / d.tiles_x;
+        }
+        if (tall && tx == d.tiles_x - 1) {
+            if (ty % 4 == 0) out.push_back(make_int4(i, tx, ty, 1));
+        } else {
+            out.push_back(make_int4(i, tx, ty, 0));
+        }
+    }
+}
+
 // ── tile-shape autotuner ────────────────────────────────────────────────────
 // The stencil kernels' tile shape trades halo re-reads and idle lanes against
 // registers and occupancy, and the best value depends on the instance
@@ -1161,17 +1204,30 @@ static void run_batch(const std::vector<std::pair<int, int>> &dims, const CuPhuL
     auto sp = [&](int i) { return stats ? stats + i : &dummy[(size_t)i]; };
     for (int i = 0; i < ninst; ++i) std::memset(sp(i), 0, sizeof(CuPhuSolveStats));
 
+    // the merged iteration launch, where some instance has a tall column of
+    // at least two tall tiles' worth of rows
+    const bool merge = o.solver == CUPHU_SOLVER_FUSED && merge_allowed(sh);
+    if (merge) g.mix = ws<int4>(res, "grpmix", maxblk);
     auto upload_group = [&](int a, int b) {
         std::vector<InstDesc> gd;
         std::vector<int> blk;
+        std::vector<int4> mix;
+        bool merges = false;
         for (int i = a; i < b; ++i) {
             InstDesc d = L.desc[(size_t)i];
             d.block_begin = (int)blk.size();
             for (int k = 0; k < d.tiles_x * d.tiles_y; ++k) blk.push_back(i - a);
+            if (merge) {
+                mix_tiles(d, i - a, sh, mix);
+                merges = merges || (tall_column(d, sh) && d.tiles_y > 1);
+            }
             gd.push_back(d);
         }
         CUPHU_HIP_CHECK(hipMemcpyAsync(g.desc, gd.data(), gd.size() * sizeof(InstDesc), hipMemcpyHostToDevice, st));
         CUPHU_HIP_CHECK(hipMemcpyAsync(g.blk, blk.data(), blk.size() * sizeof(int), hipMemcpyHostToDevice, st));
+        g.nmix = merges ? (int)mix.size() : 0;
+        if (g.nmix)
+            CUPHU_HIP_CHECK(hipMemcpyAsync(g.mix, mix.data(), mix.size() * sizeof(int4), hipMemcpyHostToDevice, st));
         g.ninst = b - a;
         g.nblk = (int)blk.size();
         g.px.clear();
@@ -2703,8 +2759,19 @@ int cuphu_test_pcg_fused_step(const float *w, const float *dinv, int nrow, int n
         sc.active = 1;
         sc.xmode = 2;   // apply this step's x update (no deferral)
         CUPHU_HIP_CHECK(hipMemcpyAsync(h.g.scal, &sc, sizeof(sc), hipMemcpyHostToDevice, st));
+        // the merged tiling wherever the solver would take it (also on a
+        // single tile row)
+        std::vector<int4> mix;
+        if (!stored_q && merge_allowed(sh) && tall_column(d, sh)) {
+            mix_tiles(d, 0, sh, mix);
+            h.g.mix = ws<int4>(res, "grpmix", mix.size());
+            h.g.nmix = (int)mix.size();
+            CUPHU_HIP_CHECK(hipMemcpyAsync(h.g.mix, mix.data(), mix.size() * sizeof(int4), hipMemcpyHostToDevice, st));
+        }
         // parity 0: reads set 0 (r, p, s), writes set 1 (r2, p2, q2)
         if (stored_q) launch_it<float>(h.g.desc, h.g.blk, h.g.nblk, h.P.pl, h.g.scal, h.g.part4, rpw, 0, start ? 1 : 0, st);
+        else if (h.g.nmix)
+            launch_iq_mix<float>(h.g.desc, h.g.mix, h.g.nmix, h.P.pl, h.g.scal, h.g.part4, sh.rpw, 0, start ? 1 : 0, st);
         else launch_iq<float>(h.g.desc, h.g.blk, h.g.nblk, h.P.pl, h.g.scal, h.g.part4, sh, 0, start ? 1 : 0, st);
         std::vector<double4> parts((size_t)h.g.nblk);
         CUPHU_HIP_CHECK(hipMemcpyAsync(parts.data(), h.g.part4, parts.size() * sizeof(double4), hipMemcpyDeviceToHost, st));

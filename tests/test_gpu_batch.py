@@ -97,6 +97,33 @@ def test_tile_widths_bit_identical(gpu_ready, monkeypatch, dims):
             np.testing.assert_allclose(a["relres"], b["relres"], rtol=1e-9)
 
 
+@pytest.mark.parametrize("precision", [32, 64])
+def test_merged_tiling_bit_identical(gpu_ready, monkeypatch, precision):
+    """Instances 256 k + 64 columns wide run their last 64 columns in tall
+    tiles (one workgroup for four 256-column tiles, CUPHU_LAPLACE_MERGE=0
+    turns it off): the same unwrapped phase, components and iteration
+    counts as the plain 256-column tiling, beside instances it leaves alone.
+    The tall tiles group their rows' FP64 partial sums differently; the
+    sums round to the same FP32 step lengths."""
+    dims = [(300, 1843), (130, 4160), (77, 50), (200, 700), (40, 257)]
+    scenes = [synth.make_scene(nr, nc, nbumps=2, npatches=1, nlooks=10.0, seed=90 + i,
+                               patch=max(1, min(20, nr // 3, nc // 3)))
+              for i, (nr, nc) in enumerate(dims)]
+    opts = dict(n_irls=2, max_iter=300, tol=1e-6, precision=precision)
+    monkeypatch.setenv("CUPHU_LAPLACE_LPR", "64")
+    res = {}
+    for merge in ("0", "1"):
+        monkeypatch.setenv("CUPHU_LAPLACE_MERGE", merge)
+        res[merge] = cuphu.unwrap_batch([s.igram for s in scenes], [s.corr for s in scenes], 10.0, "defo",
+                                        laplace_options=opts, return_stats=True)
+    for i in range(len(scenes)):
+        np.testing.assert_array_equal(res["0"][0][i], res["1"][0][i], err_msg=str(dims[i]))
+        np.testing.assert_array_equal(res["0"][1][i], res["1"][1][i], err_msg=str(dims[i]))
+        a, b = res["0"][2][i], res["1"][2][i]
+        assert (a["status"], a["iters"], a["pass_status"]) == (b["status"], b["iters"], b["pass_status"])
+        np.testing.assert_allclose(a["relres"], b["relres"], rtol=1e-9)
+
+
 @pytest.mark.parametrize("solver", ["fused", "batched", "plain"])
 def test_degenerate_shapes(gpu_ready, solver):
     """1x1, single rows/columns and 2x2 instances, alone and in one batch

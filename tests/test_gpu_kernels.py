@@ -144,7 +144,13 @@ def test_pcg_step_kernels_bitexact(gpu_ready, shape, rpw, beta):
     assert rz.value == pytest.approx(ref[6], rel=1e-12)
 
 
-@pytest.mark.parametrize("shape", SHAPES)
+# widths of 256 k + 64 columns: the q-free kernel on 256-column tiles runs
+# the last 64 columns in tall tiles (k_iq_mix); several tall tiles, and a
+# width over 4096 columns whose tiles are walked band by band
+MERGED_SHAPES = [(261, 1843), (70, 4160)]
+
+
+@pytest.mark.parametrize("shape", SHAPES + MERGED_SHAPES)
 @pytest.mark.parametrize("rpw,stored_q", [(2, 0), (4, 0), (8, 0), (-2, 0), (-4, 0), (2, 1), (4, 1), (8, 1)])
 @pytest.mark.parametrize("mode", ["start", "step", "step_beta0", "replace"])
 def test_pcg_fused_step_kernel_bitexact(gpu_ready, shape, rpw, mode, stored_q):
@@ -152,7 +158,8 @@ def test_pcg_fused_step_kernel_bitexact(gpu_ready, shape, rpw, mode, stored_q):
     bit-exact; the four sums to FP64 summation-order tolerance.  The q-free
     kernel recomputes q = L p, so q is passed as the oracle's L p.  rpw < 0:
     the q-free kernel on 128-column tiles of 8|rpw| rows (the stored-q
-    kernel has 256-column tiles only)."""
+    kernel has 256-column tiles only).  Widths of 256 k + 64 columns (1, 2,
+    7, 301, 257, 1843, 4160 here) take the merged tiling with rpw 2 and 4."""
     rng = np.random.default_rng(shape[0] + 5 * shape[1] + rpw)
     nrow, ncol = shape
     na = G.narcs(nrow, ncol)
