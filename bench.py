@@ -186,8 +186,9 @@ def main() -> int:
     ns = max(prof.samples, 1)
     gbs = lambda b, ms: (b / 1e9) / (ms / 1e3) if ms > 0 else None
     if args.solver == "fused":
-        kname, bpp, tpp = "k_iq (fused PCG iteration: x, r, p updates + two weighted 5-point stencils + 4 sums)", \
-            BYTES_PER_PX_IT, TRAFFIC_PER_PX_IT
+        widths = scene_tile_widths(ncol, tile, ovl, ntc) if args.workload == "scene8192" else [ncol]
+        kname, bpp, tpp = f"{iq_kernel(widths)} (fused PCG iteration: x, r, p updates + two weighted 5-point " \
+                          "stencils + 4 sums)", BYTES_PER_PX_IT, TRAFFIC_PER_PX_IT
         k_ms, k_bytes = prof.it_ms_sum / ns, prof.it_bytes_sum / ns
         achieved = gbs(prof.it_bytes_sum, prof.it_ms_sum)
         extra = {}
@@ -345,7 +346,8 @@ def main_batch64(args, torch, lib, _capi, synth, gpu):
            "config": {"workload": f"batch64: 64 interferograms, {npx / 1e6:.2f} Mpx, defo, {args.n_irls} IRLS "
                                   f"re-solves, tol 1e-6, max_iter {args.max_iter}, {args.solver} solver",
                       "gpu": gpu},
-           "roofline": {"bound": "hbm", "kernel": "k_iq", "achieved": round(achieved, 1) if achieved else None,
+           "roofline": {"bound": "hbm", "kernel": iq_kernel([s.corr.shape[1] for s in scenes]),
+                        "achieved": round(achieved, 1) if achieved else None,
                         "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                         "frac": round(achieved / HBM_PEAK_GBPS, 4) if achieved else None, "traffic": None,
                         "bytes_per_px": BYTES_PER_PX_IT,
@@ -469,6 +471,28 @@ def leg_cpu(scene, args, tile_px_total, npx, log):
                   f"{cst1['iters']}; {t1:.1f} s (rate of the crop itself)"}
     log(f"cpu, one thread: {n1 * n1 / t1 / 1e6:.4f} Mpx/s")
     return r
+
+
+def iq_kernel(widths):
+    """The fused iteration kernel that runs on instances of these column
+    counts: the merged launch k_iq_mix when some width rounds up (to a
+    multiple of 64) to 256 k + 64 columns, whose last 64 columns then run in
+    64 x 64 tall tiles (off with CUPHU_LAPLACE_MERGE=0 or 128-column tiles
+    forced by CUPHU_LAPLACE_LPR=32), else k_iq."""
+    if os.environ.get("CUPHU_LAPLACE_MERGE", "1") == "0" or os.environ.get("CUPHU_LAPLACE_LPR") == "32":
+        return "k_iq"
+    return "k_iq_mix" if any((-(-w // 64) * 64) % 256 == 64 for w in widths) else "k_iq"
+
+
+def scene_tile_widths(ncol, tile, ovl, ntc):
+    """Column counts of the scene's tiles along one row of tiles (the
+    library's own tile geometry, as scene_tile_px)."""
+    from cuphu_amd import _capi
+
+    first = (C.c_int * 4096)()
+    size = (C.c_int * 4096)()
+    k = _capi.check(_capi.lib.cuphu_tile_axis(ncol, tile, ovl, ntc, first, size, 4096), "cuphu_tile_axis")
+    return [size[i] for i in range(k)]
 
 
 def scene_tile_px(shape, tile, ovl, ntiles):
