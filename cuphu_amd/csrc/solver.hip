@@ -2396,9 +2396,10 @@ constexpr size_t kNoAllocSmallBytes = (size_t)4 << 20;
 // bytes, plus the per-group buffers.  With at most
 // CUPHU_NOALLOC_MAX_ELEMENTS elements an instance has at most 16384 rows
 // and 11265 workgroup tiles (256 x 8 or 128 x 16 at the smallest), so the
-// per-group buffers (row map 4 B per row; 84 B per tile; two scalar states
-// per instance; the plain solver's 8 KB of dot partials) stay far below
-// kNoAllocSmallBytes.
+// per-group buffers (row map 4 B per row; 100 B per tile: instance map,
+// three partial-sum buffers and the merged launch's tile list; two scalar
+// states per instance; the plain solver's 8 KB of dot partials) stay far
+// below kNoAllocSmallBytes.
 constexpr size_t noalloc_need(long long el, size_t tsize) {
     return (size_t)el * (21 + 12 * tsize) + 17 * 512 + kNoAllocSmallBytes;
 }

@@ -429,7 +429,12 @@ def test_folded_reductions_bit_identical(gpu_ready, monkeypatch, opts):
     """The fused solver's iteration kernel with the previous launch's
     reduction folded in (small grids) gives the states, iteration counts and
     unwrapped phase of one reduction launch per iteration, bit for bit;
-    alone and batched, with the live kernel timing on and off."""
+    alone and batched, with the live kernel timing on and off.  Both
+    schedules run the same tiling: the 520-column scene (512 + 64 columns)
+    would otherwise run its last columns in tall tiles when unfolded, which
+    groups the FP64 sums differently (that tiling is compared with the plain
+    one in test_merged_tiling_bit_identical)."""
+    monkeypatch.setenv("CUPHU_LAPLACE_MERGE", "0")
     scenes = [synth.make_scene(70, 90, nbumps=2, npatches=1, nlooks=10.0, seed=41, patch=15),
               synth.make_scene(300, 520, nbumps=3, npatches=1, nlooks=5.0, seed=42, patch=60),
               synth.make_scene(33, 200, nbumps=1, npatches=0, nlooks=20.0, seed=43)]
