@@ -476,12 +476,13 @@ def leg_cpu(scene, args, tile_px_total, npx, log):
 def iq_kernel(widths):
     """The fused iteration kernel that runs on instances of these column
     counts: the merged launch k_iq_mix when some width rounds up (to a
-    multiple of 64) to 256 k + 64 columns, whose last 64 columns then run in
-    64 x 64 tall tiles (off with CUPHU_LAPLACE_MERGE=0 or 128-column tiles
-    forced by CUPHU_LAPLACE_LPR=32), else k_iq."""
+    multiple of 64) to 256 k + 64 or 256 k + 128 columns, whose last 64 (128)
+    columns then run in 64 x 64 (128 x 32) tiles (off with
+    CUPHU_LAPLACE_MERGE=0 or 128-column tiles forced by
+    CUPHU_LAPLACE_LPR=32), else k_iq."""
     if os.environ.get("CUPHU_LAPLACE_MERGE", "1") == "0" or os.environ.get("CUPHU_LAPLACE_LPR") == "32":
         return "k_iq"
-    return "k_iq_mix" if any((-(-w // 64) * 64) % 256 == 64 for w in widths) else "k_iq"
+    return "k_iq_mix" if any((-(-w // 64) * 64) % 256 in (64, 128) for w in widths) else "k_iq"
 
 
 def scene_tile_widths(ncol, tile, ovl, ntc):

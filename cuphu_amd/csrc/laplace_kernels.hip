@@ -1516,25 +1516,29 @@ __global__ __launch_bounds__(256, (RPW == 4 && sizeof(T) == 4) ? 3 : 1) void k_i
 }
 
 // The iteration on 256 x TH tiles (TH = 16 RPW) where an instance's last tile
-// column holds 64 work columns (a row stride of 256 k + 64 columns, as the
-// reference tile formula's 1843-column tiles have): that column runs in
-// 64 x 4TH tall tiles, one workgroup for four of the 256-column tiles, which
-// would each keep three quarters of their lanes idle.  Workgroups carry a
-// fixed cost (tile lookup, halo ring, LDS phases, block sums) that the other
-// workgroups' memory traffic does not hide, so four quarter-filled ones cost
-// about 2.2 full ones and the tall one about 1.2.  tiles[b] = {instance, tile
-// column, tile row, tall} in the launch order (b after the XCD remap), tile
-// coordinates in 256 x TH units.  A tall tile's sums go to the raster slot of
-// the first 256 x TH tile it covers and the slots of the other three hold
-// zeros, so the reductions read the same slots as after k_iq.
+// column holds 64 or 128 work columns (a width of 256 k + 64 columns, as the
+// reference tile formula's 1843-column tiles have, or 256 k + 128): that
+// column runs in 64 x 4TH (or 128 x 2TH) tall tiles, one workgroup for four
+// (two) of the 256-column tiles, which would each keep three quarters (half)
+// of their lanes idle.  Workgroups carry a fixed cost (tile lookup, halo
+// ring, LDS phases, block sums) that the other workgroups' memory traffic
+// does not hide, so four quarter-filled ones cost about 2.2 full ones and the
+// tall one about 1.2.  tiles[b] = {instance, tile column, tile row, kind} in
+// the launch order (b after the XCD remap), tile coordinates in 256 x TH
+// units, kind 0 = 256 x TH, 1 = 64 x 4TH, 2 = 128 x 2TH.  A tall tile's sums
+// go to the raster slot of the first 256 x TH tile it covers and the slots of
+// the others hold zeros, so the reductions read the same slots as after k_iq.
 template <class T, int RPW, bool STAGGER>
 __global__ __launch_bounds__(256) void k_iq_mix(const InstDesc *descs, const int4 *tiles, Planes<T> pl,
                                                 const SolveScal *scal, double4 *partials, int parity, int start) {
     using Sw = Shape<RPW, 64>;
     using Sn = Shape<RPW, 16>;
+    using Sh = Shape<RPW, 32>;
     static_assert(Sn::TH == 4 * Sw::TH && Sn::TW == Sw::TW / 4, "a tall tile covers four wide tiles' rows");
-    constexpr int kWide = Sw::L0 + Sw::L1, kTall = Sn::L0 + Sn::L1;
-    __shared__ __align__(16) T lds[kWide > kTall ? kWide : kTall];
+    static_assert(Sh::TH == 2 * Sw::TH && Sh::TW == Sw::TW / 2, "a 128-column tile covers two wide tiles' rows");
+    constexpr int kWide = Sw::L0 + Sw::L1, kTall = Sn::L0 + Sn::L1, kHalf = Sh::L0 + Sh::L1;
+    static_assert(kTall <= kWide && kHalf <= kWide, "the tall shapes fit the wide tile's LDS");
+    __shared__ __align__(16) T lds[kWide];
     const int b = xcd_block((int)blockIdx.x, (int)gridDim.x);
     const int4 q = tiles[b];
     TileInfo t;
@@ -1545,10 +1549,14 @@ __global__ __launch_bounds__(256) void k_iq_mix(const InstDesc *descs, const int
     t.ty0 = q.z * Sw::TH;
     const FoldArgs none{nullptr, nullptr, 0, 0.0};
     if (q.w) {
-        iq_body<T, RPW, 16, STAGGER, false, true>(descs, nullptr, &t, lds, lds + Sn::L0, pl, scal, partials,
-                                                  parity, start, none);
-        const int k = (int)threadIdx.x;
-        if (k >= 1 && k < 4 && q.z + k < t.d.tiles_y)
+        if (q.w == 1)
+            iq_body<T, RPW, 16, STAGGER, false, true>(descs, nullptr, &t, lds, lds + Sn::L0, pl, scal, partials,
+                                                      parity, start, none);
+        else
+            iq_body<T, RPW, 32, STAGGER, false, true>(descs, nullptr, &t, lds, lds + Sh::L0, pl, scal, partials,
+                                                      parity, start, none);
+        const int k = (int)threadIdx.x, cover = q.w == 1 ? 4 : 2;
+        if (k >= 1 && k < cover && q.z + k < t.d.tiles_y)
             partials[t.blk + k * t.d.tiles_x] = make_double4(0.0, 0.0, 0.0, 0.0);
     } else {
         iq_body<T, RPW, 64, STAGGER, false, true>(descs, nullptr, &t, lds, lds + Sw::L0, pl, scal, partials,

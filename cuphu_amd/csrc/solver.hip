@@ -930,9 +930,14 @@ static void retile(Layout &L, TileShape sh) {
 // On 256-column tiles, an instance whose last tile column holds 64 work
 // columns (wcols = 256 k + 64, as the reference tile formula's 1843-column
 // tiles have) runs that column in 64 x 4TH tall tiles: one workgroup instead
-// of four quarter-filled ones.  CUPHU_LAPLACE_MERGE=0 turns it off.
-static bool tall_column(const InstDesc &d, TileShape sh) {
-    return sh.lpr == 64 && d.wcols - (d.tiles_x - 1) * sh.tw() == 64;
+// of four quarter-filled ones; with 128 work columns, in 128 x 2TH tiles, one
+// instead of two half-filled ones.  CUPHU_LAPLACE_MERGE=0 turns it off.
+// Returns the last tile column's kind: 0 (256 x TH), 1 (64 x 4TH), 2
+// (128 x 2TH).
+static int tall_column(const InstDesc &d, TileShape sh) {
+    if (sh.lpr != 64) return 0;
+    const int rem = d.wcols - (d.tiles_x - 1) * sh.tw();
+    return rem == 64 ? 1 : (rem == 128 ? 2 : 0);
 }
 static bool merge_allowed(TileShape sh) {
     int v = 1;
@@ -941,9 +946,10 @@ static bool merge_allowed(TileShape sh) {
 }
 // Appends instance i's workgroups (descriptor d, tile coordinates in
 // 256 x TH units) in the launch order of get_tile (raster, or band by band),
-// with the tall column's four-tile groups at the place of their first tile.
+// with the tall column's groups of four (or two) tiles at the place of their
+// first tile.
 static void mix_tiles(const InstDesc &d, int i, TileShape sh, std::vector<int4> &out) {
-    const bool tall = tall_column(d, sh);
+    const int kind = tall_column(d, sh), cover = kind == 1 ? 4 : 2;
     const int n = d.tiles_x * d.tiles_y;
     for (int local = 0; local < n; ++local) {
         int tx, ty;
@@ -958,8 +964,8 @@ static void mix_tiles(const InstDesc &d, int i, TileShape sh, std::vector<int4> 
             tx = local % d.tiles_x;
             ty = local / d.tiles_x;
         }
-        if (tall && tx == d.tiles_x - 1) {
-            if (ty % 4 == 0) out.push_back(make_int4(i, tx, ty, 1));
+        if (kind && tx == d.tiles_x - 1) {
+            if (ty % cover == 0) out.push_back(make_int4(i, tx, ty, kind));
         } else {
             out.push_back(make_int4(i, tx, ty, 0));
         }
@@ -1219,7 +1225,7 @@ static void run_batch(const std::vector<std::pair<int, int>> &dims, const CuPhuL
             for (int k = 0; k < d.tiles_x * d.tiles_y; ++k) blk.push_back(i - a);
             if (merge) {
                 mix_tiles(d, i - a, sh, mix);
-                merges = merges || (tall_column(d, sh) && d.tiles_y > 1);
+                merges = merges || (tall_column(d, sh) != 0 && d.tiles_y > 1);
             }
             gd.push_back(d);
         }
@@ -2763,7 +2769,7 @@ int cuphu_test_pcg_fused_step(const float *w, const float *dinv, int nrow, int n
         // the merged tiling wherever the solver would take it (also on a
         // single tile row)
         std::vector<int4> mix;
-        if (!stored_q && merge_allowed(sh) && tall_column(d, sh)) {
+        if (!stored_q && merge_allowed(sh) && tall_column(d, sh) != 0) {
             mix_tiles(d, 0, sh, mix);
             h.g.mix = ws<int4>(res, "grpmix", mix.size());
             h.g.nmix = (int)mix.size();

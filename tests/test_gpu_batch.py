@@ -99,13 +99,14 @@ def test_tile_widths_bit_identical(gpu_ready, monkeypatch, dims):
 
 @pytest.mark.parametrize("precision", [32, 64])
 def test_merged_tiling_bit_identical(gpu_ready, monkeypatch, precision):
-    """Instances 256 k + 64 columns wide run their last 64 columns in tall
-    tiles (one workgroup for four 256-column tiles, CUPHU_LAPLACE_MERGE=0
-    turns it off): the same unwrapped phase, components and iteration
-    counts as the plain 256-column tiling, beside instances it leaves alone.
-    The tall tiles group their rows' FP64 partial sums differently; the
-    sums round to the same FP32 step lengths."""
-    dims = [(300, 1843), (130, 4160), (77, 50), (200, 700), (40, 257)]
+    """Instances 256 k + 64 (128) columns wide run their last 64 (128)
+    columns in tall tiles (one workgroup for four (two) 256-column tiles,
+    CUPHU_LAPLACE_MERGE=0 turns it off): the same unwrapped phase,
+    components and iteration counts as the plain 256-column tiling, beside
+    instances it leaves alone.  The tall tiles group their rows' FP64
+    partial sums differently; the sums round to the same FP32 step
+    lengths."""
+    dims = [(300, 1843), (130, 4160), (77, 50), (200, 700), (40, 257), (150, 1100), (90, 100)]
     scenes = [synth.make_scene(nr, nc, nbumps=2, npatches=1, nlooks=10.0, seed=90 + i,
                                patch=max(1, min(20, nr // 3, nc // 3)))
               for i, (nr, nc) in enumerate(dims)]

@@ -144,10 +144,11 @@ def test_pcg_step_kernels_bitexact(gpu_ready, shape, rpw, beta):
     assert rz.value == pytest.approx(ref[6], rel=1e-12)
 
 
-# widths of 256 k + 64 columns: the q-free kernel on 256-column tiles runs
-# the last 64 columns in tall tiles (k_iq_mix); several tall tiles, and a
-# width over 4096 columns whose tiles are walked band by band
-MERGED_SHAPES = [(261, 1843), (70, 4160)]
+# widths of 256 k + 64 (or 128) columns: the q-free kernel on 256-column
+# tiles runs the last 64 (128) columns in 64 x 64 (128 x 32) tiles
+# (k_iq_mix); several such tiles, and widths over 4096 columns whose tiles are
+# walked band by band
+MERGED_SHAPES = [(261, 1843), (70, 4160), (133, 1900), (45, 4224)]
 
 
 @pytest.mark.parametrize("shape", SHAPES + MERGED_SHAPES)
@@ -159,7 +160,8 @@ def test_pcg_fused_step_kernel_bitexact(gpu_ready, shape, rpw, mode, stored_q):
     kernel recomputes q = L p, so q is passed as the oracle's L p.  rpw < 0:
     the q-free kernel on 128-column tiles of 8|rpw| rows (the stored-q
     kernel has 256-column tiles only).  Widths of 256 k + 64 columns (1, 2,
-    7, 301, 257, 1843, 4160 here) take the merged tiling with rpw 2 and 4."""
+    7, 301, 257, 1843, 4160 here) and 256 k + 128 (70, 1900, 4224) take the
+    merged tiling with rpw 2 and 4."""
     rng = np.random.default_rng(shape[0] + 5 * shape[1] + rpw)
     nrow, ncol = shape
     na = G.narcs(nrow, ncol)
