@@ -3,7 +3,8 @@ through the device-resident scene path at full size, checked by properties
 that do not depend on size: congruence with the wrapped phase, the anchor
 at pixel (0,0), label range, tile-stitch consistency in the overlaps, and
 bitwise repeatability; and the reference tile formula's 1843-column tiles
-bit-identical on 64- and 256-element row strides; and a 16384x16384 scene
+bit-identical on 64- and 256-element row strides and in the merged and
+plain iteration tilings; and a 16384x16384 scene
 whose pooled planes pass 2^31 bytes.  (Full-size parity against the FP64
 solve is bench.py's accuracy leg.)"""
 
@@ -64,7 +65,9 @@ def test_scene8192_reference_tiling_strides_bit_identical(gpu_ready, monkeypatch
     Python API's default tiling): row strides rounded to 256 elements (the
     fused solver's default; the iteration kernel covers the 1856 work
     columns of a 2048-element stride) give the same unwrapped phase and
-    labels, bit for bit, as strides rounded to 64."""
+    labels, bit for bit, as strides rounded to 64; and so does the plain
+    256-column tiling against the merged one (the last 64 of the 1856 work
+    columns in 64 x 64 tiles, the default)."""
     torch = G.torch_mod()
     s = synth.config_scene("scene8192")
     nrow, ncol = s.corr.shape
@@ -74,8 +77,9 @@ def test_scene8192_reference_tiling_strides_bit_identical(gpu_ready, monkeypatch
     prm.nlooks = prm.ncorrlooks = s.nlooks
     opts = _capi.default_options(n_irls=1, max_iter=60, tol=1e-6)
     outs = {}
-    for stride in ("64", "256"):
+    for stride, merge in (("64", "1"), ("256", "1"), ("256", "0")):
         monkeypatch.setenv("CUPHU_LAPLACE_STRIDE", stride)
+        monkeypatch.setenv("CUPHU_LAPLACE_MERGE", merge)
         du = torch.empty((nrow, ncol), dtype=torch.float32, device="cuda")
         dl = torch.empty((nrow, ncol), dtype=torch.int32, device="cuda")
         stats = (_capi.CuPhuSolveStats * 64)()
@@ -85,11 +89,13 @@ def test_scene8192_reference_tiling_strides_bit_identical(gpu_ready, monkeypatch
                                                      G.p(du), G.p(dl), stats, C.byref(nt), G.stream()), "scene")
         torch.cuda.synchronize()
         assert nt.value == 25
-        outs[stride] = (du.cpu().numpy(), dl.cpu().numpy(), [stats[i].as_dict()["iters"] for i in range(25)])
-    np.testing.assert_array_equal(outs["64"][0], outs["256"][0])
-    np.testing.assert_array_equal(outs["64"][1], outs["256"][1])
-    assert outs["64"][2] == outs["256"][2]
-    assert np.isfinite(outs["256"][0]).all()
+        outs[stride, merge] = (du.cpu().numpy(), dl.cpu().numpy(), [stats[i].as_dict()["iters"] for i in range(25)])
+    ref = outs["256", "1"]
+    for key in (("64", "1"), ("256", "0")):
+        np.testing.assert_array_equal(outs[key][0], ref[0], err_msg=str(key))
+        np.testing.assert_array_equal(outs[key][1], ref[1], err_msg=str(key))
+        assert outs[key][2] == ref[2]
+    assert np.isfinite(ref[0]).all()
 
 
 def test_scene16384_beyond_2gb_plane_offsets(gpu_ready):
