@@ -479,7 +479,8 @@ def iq_kernel(widths):
     multiple of 64) to 256 k + 64 or 256 k + 128 columns, whose last 64 (128)
     columns then run in 64 x 64 (128 x 32) tiles (off with
     CUPHU_LAPLACE_MERGE=0 or 128-column tiles forced by
-    CUPHU_LAPLACE_LPR=32), else k_iq."""
+    CUPHU_LAPLACE_LPR=32, and by default on grids of at most 768
+    workgroups, which none of these workloads is), else k_iq."""
     if os.environ.get("CUPHU_LAPLACE_MERGE", "1") == "0" or os.environ.get("CUPHU_LAPLACE_LPR") == "32":
         return "k_iq"
     return "k_iq_mix" if any((-(-w // 64) * 64) % 256 in (64, 128) for w in widths) else "k_iq"

@@ -129,7 +129,7 @@ template <class T>
 void launch_iq(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, const SolveScal *scal,
                double4 *partials, TileShape sh, int parity, int start, hipStream_t st);
 // The same iteration in the merged tiling (k_iq_mix): d_tiles lists the
-// launch's ntiles workgroups as {instance, tile column, tile row, tall}.
+// launch's ntiles workgroups as {instance, tile column, tile row, kind}.
 template <class T>
 void launch_iq_mix(const InstDesc *d_desc, const int4 *d_tiles, int ntiles, const Planes<T> &pl,
                    const SolveScal *scal, double4 *partials, int rpw, int parity, int start, hipStream_t st);
@@ -142,6 +142,11 @@ template <class T>
 void launch_iq_fold(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl,
                     const SolveScal *scal_prev, SolveScal *scal_next, const double4 *part_prev, double4 *partials,
                     TileShape sh, int parity, int max_iter, double tol2, hipStream_t st);
+// The folded iteration in the merged tiling (k_iq_fold_mix).
+template <class T>
+void launch_iq_fold_mix(const InstDesc *d_desc, const int4 *d_tiles, int ntiles, const Planes<T> &pl,
+                        const SolveScal *scal_prev, SolveScal *scal_next, const double4 *part_prev,
+                        double4 *partials, int rpw, int parity, int max_iter, double tol2, hipStream_t st);
 template <class T>
 void launch_rr_iq(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl, const SolveScal *scal,
                   double4 *partials, TileShape sh, hipStream_t st);

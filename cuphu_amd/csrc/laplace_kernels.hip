@@ -1528,9 +1528,10 @@ __global__ __launch_bounds__(256, (RPW == 4 && sizeof(T) == 4) ? 3 : 1) void k_i
 // units, kind 0 = 256 x TH, 1 = 64 x 4TH, 2 = 128 x 2TH.  A tall tile's sums
 // go to the raster slot of the first 256 x TH tile it covers and the slots of
 // the others hold zeros, so the reductions read the same slots as after k_iq.
-template <class T, int RPW, bool STAGGER>
-__global__ __launch_bounds__(256) void k_iq_mix(const InstDesc *descs, const int4 *tiles, Planes<T> pl,
-                                                const SolveScal *scal, double4 *partials, int parity, int start) {
+template <class T, int RPW, bool STAGGER, bool FOLD>
+__device__ __forceinline__ void iq_mix_body(const InstDesc *descs, const int4 *tiles, const Planes<T> &pl,
+                                            const SolveScal *scal, double4 *partials, int parity, int start,
+                                            const FoldArgs &fa) {
     using Sw = Shape<RPW, 64>;
     using Sn = Shape<RPW, 16>;
     using Sh = Shape<RPW, 32>;
@@ -1547,21 +1548,35 @@ __global__ __launch_bounds__(256) void k_iq_mix(const InstDesc *descs, const int
     t.blk = t.d.block_begin + q.z * t.d.tiles_x + q.y;
     t.tx0 = q.y * Sw::TW;
     t.ty0 = q.z * Sw::TH;
-    const FoldArgs none{nullptr, nullptr, 0, 0.0};
     if (q.w) {
         if (q.w == 1)
-            iq_body<T, RPW, 16, STAGGER, false, true>(descs, nullptr, &t, lds, lds + Sn::L0, pl, scal, partials,
-                                                      parity, start, none);
+            iq_body<T, RPW, 16, STAGGER, FOLD, true>(descs, nullptr, &t, lds, lds + Sn::L0, pl, scal, partials,
+                                                     parity, start, fa);
         else
-            iq_body<T, RPW, 32, STAGGER, false, true>(descs, nullptr, &t, lds, lds + Sh::L0, pl, scal, partials,
-                                                      parity, start, none);
+            iq_body<T, RPW, 32, STAGGER, FOLD, true>(descs, nullptr, &t, lds, lds + Sh::L0, pl, scal, partials,
+                                                     parity, start, fa);
         const int k = (int)threadIdx.x, cover = q.w == 1 ? 4 : 2;
         if (k >= 1 && k < cover && q.z + k < t.d.tiles_y)
             partials[t.blk + k * t.d.tiles_x] = make_double4(0.0, 0.0, 0.0, 0.0);
     } else {
-        iq_body<T, RPW, 64, STAGGER, false, true>(descs, nullptr, &t, lds, lds + Sw::L0, pl, scal, partials,
-                                                  parity, start, none);
+        iq_body<T, RPW, 64, STAGGER, FOLD, true>(descs, nullptr, &t, lds, lds + Sw::L0, pl, scal, partials, parity,
+                                                 start, fa);
     }
+}
+template <class T, int RPW, bool STAGGER>
+__global__ __launch_bounds__(256) void k_iq_mix(const InstDesc *descs, const int4 *tiles, Planes<T> pl,
+                                                const SolveScal *scal, double4 *partials, int parity, int start) {
+    const FoldArgs none{nullptr, nullptr, 0, 0.0};
+    iq_mix_body<T, RPW, STAGGER, false>(descs, tiles, pl, scal, partials, parity, start, none);
+}
+// The merged tiling with the previous launch's reduction folded in (grids of
+// at most kFoldMaxBlocks workgroups of the plain tiling).  Every instance's
+// first workgroup is its tile (0, 0), in either tiling, and stores the state.
+template <class T, int RPW>
+__global__ __launch_bounds__(256, (RPW == 4 && sizeof(T) == 4) ? 3 : 1) void k_iq_fold_mix(
+    const InstDesc *descs, const int4 *tiles, Planes<T> pl, const SolveScal *scal_prev, double4 *partials,
+    int parity, FoldArgs fa) {
+    iq_mix_body<T, RPW, false, true>(descs, tiles, pl, scal_prev, partials, parity, 0, fa);
 }
 
 // True residual for the q-free fused solver at the end of an even chunk
@@ -2069,6 +2084,18 @@ void launch_iq_mix(const InstDesc *d_desc, const int4 *d_tiles, int ntiles, cons
     CUPHU_LAUNCH_CHECK();
 }
 template <class T>
+void launch_iq_fold_mix(const InstDesc *d_desc, const int4 *d_tiles, int ntiles, const Planes<T> &pl,
+                        const SolveScal *scal_prev, SolveScal *scal_next, const double4 *part_prev,
+                        double4 *partials, int rpw, int parity, int max_iter, double tol2, hipStream_t st) {
+    CUPHU_REQUIRE(rpw == 2 || rpw == 4, "the merged iteration launch runs 2 or 4 rows per lane group");
+    const FoldArgs fa{part_prev, scal_next, max_iter, tol2};
+    if (rpw == 2) hipLaunchKernelGGL((k_iq_fold_mix<T, 2>), dim3(ntiles), dim3(256), 0, st, d_desc, d_tiles, pl,
+                                     scal_prev, partials, parity, fa);
+    else hipLaunchKernelGGL((k_iq_fold_mix<T, 4>), dim3(ntiles), dim3(256), 0, st, d_desc, d_tiles, pl, scal_prev,
+                            partials, parity, fa);
+    CUPHU_LAUNCH_CHECK();
+}
+template <class T>
 void launch_iq_fold(const InstDesc *d_desc, const int *d_blk, int nblk, const Planes<T> &pl,
                     const SolveScal *scal_prev, SolveScal *scal_next, const double4 *part_prev, double4 *partials,
                     TileShape sh, int parity, int max_iter, double tol2, hipStream_t st) {
@@ -2165,6 +2192,9 @@ void launch_plain_dot(const T *a, const T *b, size_t n, double *partials, double
                                double4 *, TileShape, int, int, hipStream_t);                                     \
     template void launch_iq_mix<T>(const InstDesc *, const int4 *, int, const Planes<T> &, const SolveScal *,    \
                                    double4 *, int, int, int, hipStream_t);                                       \
+    template void launch_iq_fold_mix<T>(const InstDesc *, const int4 *, int, const Planes<T> &,                \
+                                        const SolveScal *, SolveScal *, const double4 *, double4 *, int, int,   \
+                                        int, double, hipStream_t);                                               \
     template void launch_iq_fold<T>(const InstDesc *, const int *, int, const Planes<T> &, const SolveScal *,    \
                                     SolveScal *, const double4 *, double4 *, TileShape, int, int, double,        \
                                     hipStream_t);                                                                \

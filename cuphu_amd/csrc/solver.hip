@@ -598,13 +598,17 @@ static void solve_group_fused(const Planes<T> &pl, const GroupDev &g, const CuPh
     auto folded_steps = [&](int first, int last) {
         if (first >= last) return;
         double4 *pb[2] = {g.part4, g.part4b};
-        launch_iq<T>(g.desc, g.blk, g.nblk, pl, g.scal, pb[0], sh, 1 - (first & 1), 0, st);
+        it(1 - (first & 1), 0);   // into pb[0] = g.part4
         for (int i = first + 1; i < last; ++i) {
             const int k = i - first;
             const SolveScal *sin = (k & 1) ? g.scal : g.scal2;
             SolveScal *sout = (k & 1) ? g.scal2 : g.scal;
-            launch_iq_fold<T>(g.desc, g.blk, g.nblk, pl, sin, sout, pb[(k - 1) & 1], pb[k & 1], sh, 1 - (i & 1),
-                              o.max_iter, tol2, st);
+            if (g.nmix)
+                launch_iq_fold_mix<T>(g.desc, g.mix, g.nmix, pl, sin, sout, pb[(k - 1) & 1], pb[k & 1], rpw,
+                                      1 - (i & 1), o.max_iter, tol2, st);
+            else
+                launch_iq_fold<T>(g.desc, g.blk, g.nblk, pl, sin, sout, pb[(k - 1) & 1], pb[k & 1], sh,
+                                  1 - (i & 1), o.max_iter, tol2, st);
         }
         const int k = last - 1 - first;   // the last step: state after step k-1 is in (k odd ? scal2 : scal)
         launch_reduce_it(g.desc, g.ninst, pb[k & 1], g.scal, RED_IT, o.max_iter, tol2, st, 0,
@@ -931,7 +935,8 @@ static void retile(Layout &L, TileShape sh) {
 // columns (wcols = 256 k + 64, as the reference tile formula's 1843-column
 // tiles have) runs that column in 64 x 4TH tall tiles: one workgroup instead
 // of four quarter-filled ones; with 128 work columns, in 128 x 2TH tiles, one
-// instead of two half-filled ones.  CUPHU_LAPLACE_MERGE=0 turns it off.
+// instead of two half-filled ones.  CUPHU_LAPLACE_MERGE=0 turns it off, =2
+// takes it on grids of any size (default 1: beyond one resident round).
 // Returns the last tile column's kind: 0 (256 x TH), 1 (64 x 4TH), 2
 // (128 x 2TH).
 static int tall_column(const InstDesc &d, TileShape sh) {
@@ -939,10 +944,20 @@ static int tall_column(const InstDesc &d, TileShape sh) {
     const int rem = d.wcols - (d.tiles_x - 1) * sh.tw();
     return rem == 64 ? 1 : (rem == 128 ? 2 : 0);
 }
-static bool merge_allowed(TileShape sh) {
+// Grids that fit in one resident round (3 workgroups x 256 CUs) keep the
+// plain tiling: every workgroup is resident at once there, so fewer of them
+// save nothing and the launch waits for the slower tall tiles.  Measured on
+// single images, folded reductions, merged against plain: 1100 x 1300 (414
+// workgroups) 0.7 % slower, 1500 x 1400 (528) 0.8 % faster; 1843 x 1843
+// (928) 3.5 % and 2048 x 1600 (896) 6.5 % faster.
+constexpr size_t kMergeMinBlocks = 768;
+static int merge_mode() {
     int v = 1;
     env_int("CUPHU_LAPLACE_MERGE", &v);
-    return v != 0 && sh.lpr == 64 && (sh.rpw == 2 || sh.rpw == 4) && !fused_stored_q();
+    return v;
+}
+static bool merge_allowed(TileShape sh) {
+    return merge_mode() != 0 && sh.lpr == 64 && (sh.rpw == 2 || sh.rpw == 4) && !fused_stored_q();
 }
 // Appends instance i's workgroups (descriptor d, tile coordinates in
 // 256 x TH units) in the launch order of get_tile (raster, or band by band),
@@ -1231,7 +1246,7 @@ static void run_batch(const std::vector<std::pair<int, int>> &dims, const CuPhuL
         }
         CUPHU_HIP_CHECK(hipMemcpyAsync(g.desc, gd.data(), gd.size() * sizeof(InstDesc), hipMemcpyHostToDevice, st));
         CUPHU_HIP_CHECK(hipMemcpyAsync(g.blk, blk.data(), blk.size() * sizeof(int), hipMemcpyHostToDevice, st));
-        g.nmix = merges ? (int)mix.size() : 0;
+        g.nmix = merges && (blk.size() > kMergeMinBlocks || merge_mode() == 2) ? (int)mix.size() : 0;
         if (g.nmix)
             CUPHU_HIP_CHECK(hipMemcpyAsync(g.mix, mix.data(), mix.size() * sizeof(int4), hipMemcpyHostToDevice, st));
         g.ninst = b - a;

@@ -100,8 +100,9 @@ def test_tile_widths_bit_identical(gpu_ready, monkeypatch, dims):
 @pytest.mark.parametrize("precision", [32, 64])
 def test_merged_tiling_bit_identical(gpu_ready, monkeypatch, precision):
     """Instances 256 k + 64 (128) columns wide run their last 64 (128)
-    columns in tall tiles (one workgroup for four (two) 256-column tiles,
-    CUPHU_LAPLACE_MERGE=0 turns it off): the same unwrapped phase,
+    columns in tall tiles (one workgroup for four (two) 256-column tiles;
+    CUPHU_LAPLACE_MERGE=0 turns it off, =2 takes it on this small grid,
+    which the default leaves plain): the same unwrapped phase,
     components and iteration counts as the plain 256-column tiling, beside
     instances it leaves alone.  The tall tiles group their rows' FP64
     partial sums differently; the sums round to the same FP32 step
@@ -113,14 +114,14 @@ def test_merged_tiling_bit_identical(gpu_ready, monkeypatch, precision):
     opts = dict(n_irls=2, max_iter=300, tol=1e-6, precision=precision)
     monkeypatch.setenv("CUPHU_LAPLACE_LPR", "64")
     res = {}
-    for merge in ("0", "1"):
+    for merge in ("0", "2"):
         monkeypatch.setenv("CUPHU_LAPLACE_MERGE", merge)
         res[merge] = cuphu.unwrap_batch([s.igram for s in scenes], [s.corr for s in scenes], 10.0, "defo",
                                         laplace_options=opts, return_stats=True)
     for i in range(len(scenes)):
-        np.testing.assert_array_equal(res["0"][0][i], res["1"][0][i], err_msg=str(dims[i]))
-        np.testing.assert_array_equal(res["0"][1][i], res["1"][1][i], err_msg=str(dims[i]))
-        a, b = res["0"][2][i], res["1"][2][i]
+        np.testing.assert_array_equal(res["0"][0][i], res["2"][0][i], err_msg=str(dims[i]))
+        np.testing.assert_array_equal(res["0"][1][i], res["2"][1][i], err_msg=str(dims[i]))
+        a, b = res["0"][2][i], res["2"][2][i]
         assert (a["status"], a["iters"], a["pass_status"]) == (b["status"], b["iters"], b["pass_status"])
         np.testing.assert_allclose(a["relres"], b["relres"], rtol=1e-9)
 

@@ -429,27 +429,28 @@ def test_folded_reductions_bit_identical(gpu_ready, monkeypatch, opts):
     """The fused solver's iteration kernel with the previous launch's
     reduction folded in (small grids) gives the states, iteration counts and
     unwrapped phase of one reduction launch per iteration, bit for bit;
-    alone and batched, with the live kernel timing on and off.  Both
-    schedules run the same tiling: the 520-column scene (512 + 64 columns)
-    would otherwise run its last columns in tall tiles when unfolded, which
-    groups the FP64 sums differently (that tiling is compared with the plain
-    one in test_merged_tiling_bit_identical)."""
-    monkeypatch.setenv("CUPHU_LAPLACE_MERGE", "0")
+    alone and batched, with the live kernel timing on and off; in the plain
+    tiling and in the merged one, forced on these small grids with
+    CUPHU_LAPLACE_MERGE=2 (folded or not, the 520-column scene, 512 + 64
+    work columns, runs its last 64 columns in 64 x 64 tiles and the
+    90-column one its 128 work columns in 128 x 32 tiles)."""
     scenes = [synth.make_scene(70, 90, nbumps=2, npatches=1, nlooks=10.0, seed=41, patch=15),
               synth.make_scene(300, 520, nbumps=3, npatches=1, nlooks=5.0, seed=42, patch=60),
               synth.make_scene(33, 200, nbumps=1, npatches=0, nlooks=20.0, seed=43)]
     o = _capi.default_options(solver="fused", **opts)
     res = {}
-    for fold in ("0", "1"):
+    for fold, merge in (("0", "2"), ("1", "2"), ("0", "0"), ("1", "0")):
         monkeypatch.setenv("CUPHU_FUSED_FOLD", fold)
-        res[fold] = [G.batch_solve(scenes, o)]
-        res[fold] += [G.batch_solve([sc], o) for sc in scenes[:2]]
+        monkeypatch.setenv("CUPHU_LAPLACE_MERGE", merge)
+        key = fold + merge
+        res[key] = [G.batch_solve(scenes, o)]
+        res[key] += [G.batch_solve([sc], o) for sc in scenes[:2]]
         _capi.lib.cuphu_set_profiling(1)
         try:
-            res[fold].append(G.batch_solve(scenes[1:2], o))
+            res[key].append(G.batch_solve(scenes[1:2], o))
         finally:
             _capi.lib.cuphu_set_profiling(0)
-    for a, b in zip(res["0"], res["1"]):
+    for a, b in list(zip(res["02"], res["12"])) + list(zip(res["00"], res["10"])):
         for ua, ub in zip(a[0], b[0]):
             np.testing.assert_array_equal(ua, ub)
         for ca, cb in zip(a[1], b[1]):
