@@ -68,8 +68,12 @@ class _Case:
         assert _hip().hipFree(self.dcost.value) == 0
 
 
-def test_noalloc_equals_allocating_entry_on_two_calls(gpu_ready, monkeypatch):
+@pytest.mark.parametrize("merge", ["1", "2"])
+def test_noalloc_equals_allocating_entry_on_two_calls(gpu_ready, monkeypatch, merge):
+    """merge 2: the 517-column instance (512 + 64 work columns) runs its last
+    64 columns in 64 x 64 tiles, whose tile list also lives in the scratch."""
     monkeypatch.setenv("CUPHU_N_IRLS", "3")
+    monkeypatch.setenv("CUPHU_LAPLACE_MERGE", merge)
     big, small = _Case(300, 517, 61), _Case(131, 170, 62)
     try:
         for c in (big, small):
